@@ -1,0 +1,265 @@
+"""ctypes bindings of libapg_sgm_amd.so (the gfx950 build of the apg-sgm
+dense-stereo hot path).
+
+Only the in-tree shared library is loaded; there is no fallback.  When the
+library or a GPU is missing every compute call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "libapg_sgm_amd.so")
+
+STATUS = {
+    0: "Ok", 1: "EmptyInput", 2: "SizeMismatch", 3: "InvalidDisparityRange", 4: "InvalidConfig",
+    5: "UnsupportedBackend", 6: "ResourceLimitExceeded", 7: "AllocationFailure", 8: "InternalError",
+}
+
+
+class Params(C.Structure):
+    """apg_hip_params (include/apg_sgm_hip.h)."""
+    _fields_ = [
+        ("mode", C.c_int), ("min_disparity", C.c_int), ("max_disparity", C.c_int),
+        ("census_type", C.c_int),
+        ("lambda_census", C.c_float), ("lambda_ad", C.c_float), ("lambda_grad", C.c_float),
+        ("eta_ad", C.c_float), ("mu_grad", C.c_float),
+        ("use_ad", C.c_int), ("use_grad", C.c_int), ("cost_max", C.c_int),
+        ("agg_enable", C.c_int), ("max_arm_length", C.c_int), ("color_threshold", C.c_int),
+        ("agg_iterations", C.c_int),
+        ("paths", C.c_int), ("P1", C.c_int), ("P2_base", C.c_int), ("adaptive_penalties", C.c_int),
+        ("grad_t1", C.c_int), ("grad_t2", C.c_int), ("p2_alpha", C.c_float), ("piecewise_p2", C.c_int),
+        ("conf_uniqueness_ratio", C.c_float), ("conf_lr_max_diff", C.c_int),
+        ("conf_min_texture", C.c_float), ("conf_reliable_threshold", C.c_float),
+        ("post_subpixel", C.c_int), ("post_lr_check", C.c_int), ("post_lr_max_diff", C.c_int),
+        ("post_median_radius", C.c_int), ("post_max_fill_gap", C.c_int),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Timing(C.Structure):
+    _fields_ = [(n, C.c_double) for n in (
+        "aux_ms", "prior_ms", "cost_ms", "right_wta_ms", "cross_ms", "sgm_ms", "wta_ms",
+        "confidence_ms", "post_ms", "total_ms")]
+
+
+class Intermediates(C.Structure):
+    _fields_ = [("d_cost", C.c_void_p), ("d_cost_agg", C.c_void_p), ("d_sum32", C.c_void_p),
+                ("d_disp_right", C.c_void_p), ("d_disp_wta", C.c_void_p), ("d_best", C.c_void_p),
+                ("d_second", C.c_void_p)]
+
+
+class ApgError(RuntimeError):
+    def __init__(self, code, msg=""):
+        super().__init__(f"{STATUS.get(code, code)} ({code}): {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load():
+    """Load the in-tree library (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FileNotFoundError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+    lib = C.CDLL(LIB_PATH)
+    vp, i, sz = C.c_void_p, C.c_int, C.c_size_t
+    P = C.POINTER(Params)
+    sigs = {
+        "apg_hip_params_from_mode": (None, [i, i, P]),
+        "apg_hip_validate_params": (i, [P]),
+        "apg_hip_ctx_create": (i, [i, C.POINTER(vp)]),
+        "apg_hip_ctx_destroy": (None, [vp]),
+        "apg_hip_synchronize": (i, [vp]),
+        "apg_hip_stream": (vp, [vp]),
+        "apg_hip_workspace_bytes": (sz, [vp]),
+        "apg_hip_release_workspace": (None, [vp]),
+        "apg_hip_enable_stage_timing": (None, [vp, i]),
+        "apg_hip_last_timing": (i, [vp, C.POINTER(Timing)]),
+        "apg_hip_last_error": (C.c_char_p, [vp]),
+        "apg_hip_malloc": (i, [vp, sz, C.POINTER(vp)]),
+        "apg_hip_free": (i, [vp, vp]),
+        "apg_hip_upload": (i, [vp, vp, vp, sz]),
+        "apg_hip_download": (i, [vp, vp, vp, sz]),
+        "apg_hip_memset": (i, [vp, vp, i, sz]),
+        "apg_hip_aux": (i, [vp, P, vp, i, i, i, i, vp, vp, vp, vp, vp]),
+        "apg_hip_range_init": (i, [vp, P, i, i, vp, vp]),
+        "apg_hip_cost_volume": (i, [vp, P, i, i] + [vp] * 13),
+        "apg_hip_right_wta": (i, [vp, P, i, i, vp, vp]),
+        "apg_hip_cross_arms": (i, [vp, P, vp, i, i, vp]),
+        "apg_hip_cross_aggregate": (i, [vp, P, i, i, vp, vp, vp, vp, vp]),
+        "apg_hip_sgm_optimize": (i, [vp, P, i, i, vp, vp, vp, vp]),
+        "apg_hip_sgm_path": (i, [vp, P, i, i, i, vp, vp, vp, i, i]),
+        "apg_hip_wta": (i, [vp, P, i, i, vp, i, vp, vp, vp, vp, vp]),
+        "apg_hip_confidence": (i, [vp, P, i, i] + [vp] * 9),
+        "apg_hip_lr_check": (i, [vp, P, i, i, vp, vp, vp]),
+        "apg_hip_fill_median": (i, [vp, P, i, i, vp, vp, vp, vp, vp]),
+        "apg_hip_pipeline": (i, [vp, P, vp, vp, i, i, i, i, vp, vp, vp, vp, C.POINTER(Intermediates)]),
+        "apg_hip_bench_pipeline": (i, [vp, P, vp, vp, i, i, i, i, vp, vp, vp, vp, i, i, C.POINTER(C.c_double),
+                                       C.POINTER(Timing)]),
+        "apg_synth_pair": (i, [i, i, i, i, C.c_uint64, vp, vp, vp]),
+        "apg_hip_version_string": (C.c_char_p, []),
+    }
+    for name, (res, args) in sigs.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def params(mode=1, max_disparity=128, **over) -> Params:
+    p = Params()
+    load().apg_hip_params_from_mode(mode, max_disparity, C.byref(p))
+    for k, v in over.items():
+        setattr(p, k, v)
+    return p
+
+
+def synth_pair(kind, w, h, max_disp, seed):
+    """Seeded synthetic pair -> (left u8 HxW, right u8 HxW, gt f32 HxW)."""
+    left = np.empty((h, w), np.uint8)
+    right = np.empty((h, w), np.uint8)
+    gt = np.empty((h, w), np.float32)
+    rc = load().apg_synth_pair(kind, w, h, max_disp, seed, left.ctypes.data, right.ctypes.data, gt.ctypes.data)
+    if rc:
+        raise ApgError(rc, "apg_synth_pair")
+    return left, right, gt
+
+
+class DeviceArray:
+    """A device allocation owned by a Context (freed with it or on del)."""
+
+    def __init__(self, ctx, shape, dtype):
+        self.ctx, self.shape, self.dtype = ctx, tuple(shape), np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        ptr = C.c_void_p()
+        ctx._check(ctx.lib.apg_hip_malloc(ctx.h, max(self.nbytes, 1), C.byref(ptr)), "malloc")
+        self.ptr = ptr.value
+
+    def upload(self, arr):
+        a = np.ascontiguousarray(arr, dtype=self.dtype)
+        assert a.nbytes == self.nbytes, (a.nbytes, self.nbytes)
+        self.ctx._check(self.ctx.lib.apg_hip_upload(self.ctx.h, self.ptr, a.ctypes.data, a.nbytes), "upload")
+        return self
+
+    def download(self):
+        out = np.empty(self.shape, self.dtype)
+        self.ctx._check(self.ctx.lib.apg_hip_download(self.ctx.h, out.ctypes.data, self.ptr, self.nbytes), "download")
+        return out
+
+    def fill(self, byte):
+        self.ctx._check(self.ctx.lib.apg_hip_memset(self.ctx.h, self.ptr, byte, self.nbytes), "memset")
+        return self
+
+    def free(self):
+        if self.ptr and self.ctx.h:
+            self.ctx.lib.apg_hip_free(self.ctx.h, self.ptr)
+        self.ptr = None
+
+
+class Context:
+    """apg_hip_ctx: a HIP stream + device workspace on one GPU."""
+
+    def __init__(self, device=0):
+        self.lib = load()
+        h = C.c_void_p()
+        rc = self.lib.apg_hip_ctx_create(device, C.byref(h))
+        if rc:
+            raise ApgError(rc, "apg_hip_ctx_create (is a GPU visible?)")
+        self.h = h.value
+        self._arrays = []
+
+    def _check(self, rc, what):
+        if rc:
+            msg = self.lib.apg_hip_last_error(self.h)
+            raise ApgError(rc, f"{what}: {msg.decode() if msg else ''}")
+
+    def array(self, shape, dtype):
+        a = DeviceArray(self, shape, dtype)
+        self._arrays.append(a)
+        return a
+
+    def to_device(self, arr):
+        arr = np.ascontiguousarray(arr)
+        return self.array(arr.shape, arr.dtype).upload(arr)
+
+    def sync(self):
+        self._check(self.lib.apg_hip_synchronize(self.h), "sync")
+
+    def close(self):
+        if self.h:
+            for a in self._arrays:
+                a.free()
+            self._arrays = []
+            self.lib.apg_hip_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+@dataclass
+class PipelineOutputs:
+    disparity: np.ndarray
+    confidence: np.ndarray
+    invalid_reason: np.ndarray
+    reliable: np.ndarray
+    cost: np.ndarray | None = None
+    cost_agg: np.ndarray | None = None
+    sgm: np.ndarray | None = None
+    disp_right: np.ndarray | None = None
+    disp_wta: np.ndarray | None = None
+    best: np.ndarray | None = None
+    second: np.ndarray | None = None
+
+
+def run_pipeline(ctx: Context, p: Params, left: np.ndarray, right: np.ndarray, intermediates=False) -> PipelineOutputs:
+    """Upload a pair, run apg_hip_pipeline, download results (and, optionally,
+    the materialised intermediates C, C', S, d_R, WTA output, best, second)."""
+    h, w = left.shape[:2]
+    ch = 1 if left.ndim == 2 else left.shape[2]
+    D = p.max_disparity - p.min_disparity
+    dl, dr = ctx.to_device(left), ctx.to_device(right)
+    out_d = ctx.array((h, w), np.float32)
+    out_c = ctx.array((h, w), np.float32)
+    out_r = ctx.array((h, w), np.uint8)
+    out_m = ctx.array((h, w), np.uint8)
+    inter = None
+    arrs = {}
+    if intermediates:
+        arrs = {
+            "cost": ctx.array((h, w, D), np.uint16), "cost_agg": ctx.array((h, w, D), np.uint16),
+            "sgm": ctx.array((h, w, D), np.uint32), "disp_right": ctx.array((h, w), np.float32),
+            "disp_wta": ctx.array((h, w), np.float32), "best": ctx.array((h, w), np.float32),
+            "second": ctx.array((h, w), np.float32),
+        }
+        inter = Intermediates(arrs["cost"].ptr, arrs["cost_agg"].ptr, arrs["sgm"].ptr, arrs["disp_right"].ptr,
+                              arrs["disp_wta"].ptr, arrs["best"].ptr, arrs["second"].ptr)
+    rc = ctx.lib.apg_hip_pipeline(ctx.h, C.byref(p), dl.ptr, dr.ptr, w, h, w * ch, ch, out_d.ptr, out_c.ptr,
+                                  out_r.ptr, out_m.ptr, C.byref(inter) if inter is not None else None)
+    ctx._check(rc, "apg_hip_pipeline")
+    ctx.sync()
+    res = PipelineOutputs(out_d.download(), out_c.download(), out_r.download(), out_m.download(),
+                          **{k: v.download() for k, v in arrs.items()})
+    for a in [dl, dr, out_d, out_c, out_r, out_m] + list(arrs.values()):
+        a.free()
+    return res

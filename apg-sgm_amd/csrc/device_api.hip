@@ -1,0 +1,533 @@
+// extern "C" device layer (include/apg_sgm_hip.h): context, workspace,
+// stage entry points and the device pipeline schedule.
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "device_common.hpp"
+#include "kernels.hpp"
+
+using namespace apgk;
+
+namespace {
+
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+
+enum BufId {
+    B_SRC_L, B_SRC_R, B_GRAY_L, B_GX_L, B_GY_L, B_GRAY_R, B_GX_R, B_GY_R, B_CEN_L, B_CEN_R,
+    B_DMIN, B_DMAX, B_ARMS, B_COST, B_TMP, B_SUM, B_DR, B_BEST, B_SECOND, B_SCRATCH, B_LUT, B_COUNT
+};
+
+enum StageId { S_AUX, S_PRIOR, S_COST, S_RWTA, S_CROSS, S_SGM, S_WTA, S_CONF, S_POST, S_COUNT };
+
+}  // namespace
+
+struct apg_hip_ctx_s {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    DevBuf bufs[B_COUNT];
+    std::string last_error;
+    bool timing = false;
+    hipEvent_t ev[S_COUNT + 1] = {};
+    apg_hip_timing last{};
+    CostLut lut_host{};
+    bool lut_valid = false;
+    apg_hip_params lut_params{};
+};
+
+namespace {
+
+int fail(apg_hip_ctx ctx, int code, const std::string& msg) {
+    if (ctx) ctx->last_error = msg;
+    return code;
+}
+
+int hip_fail(apg_hip_ctx ctx, hipError_t e, const char* where) {
+    if (e == hipSuccess) return 0;
+    const int code = (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? 7 : 8;
+    return fail(ctx, code, std::string(where) + ": " + hipGetErrorString(e));
+}
+
+#define HIPCHK(expr, where)                              \
+    do {                                                 \
+        hipError_t _e = (expr);                          \
+        if (_e != hipSuccess) return hip_fail(ctx, _e, where); \
+    } while (0)
+
+// Grow-only device buffer of the workspace.
+hipError_t ensure(apg_hip_ctx ctx, BufId id, size_t bytes) {
+    DevBuf& b = ctx->bufs[id];
+    if (b.bytes >= bytes && b.ptr) return hipSuccess;
+    if (b.ptr) {
+        hipError_t e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) return e;
+        (void)hipFree(b.ptr);
+        b.ptr = nullptr;
+        b.bytes = 0;
+    }
+    hipError_t e = hipMalloc(&b.ptr, bytes < 256 ? 256 : bytes);
+    if (e == hipSuccess) b.bytes = bytes;
+    return e;
+}
+
+template <typename T>
+T* buf(apg_hip_ctx ctx, BufId id) { return (T*)ctx->bufs[id].ptr; }
+
+int ensure_lut(apg_hip_ctx ctx, const apg_hip_params* p) {
+    if (ctx->lut_valid && memcmp(&ctx->lut_params, p, sizeof(*p)) == 0) return 0;
+    build_cost_lut(*p, &ctx->lut_host);
+    HIPCHK(ensure(ctx, B_LUT, sizeof(CostLut)), "lut alloc");
+    HIPCHK(hipMemcpyAsync(ctx->bufs[B_LUT].ptr, &ctx->lut_host, sizeof(CostLut), hipMemcpyHostToDevice, ctx->stream),
+           "lut upload");
+    ctx->lut_valid = true;
+    ctx->lut_params = *p;
+    return 0;
+}
+
+void mark(apg_hip_ctx ctx, int stage) {
+    if (ctx->timing) (void)hipEventRecord(ctx->ev[stage], ctx->stream);
+}
+
+int check_ctx(apg_hip_ctx ctx, const apg_hip_params* p) {
+    if (!ctx) return 8;
+    if (!p) return fail(ctx, 4, "null params");
+    const int s = apg_hip_validate_params(p);
+    if (s) return s;
+    HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+void apg_hip_params_from_mode(int mode, int max_disparity, apg_hip_params* c) {
+    memset(c, 0, sizeof(*c));
+    c->mode = mode;
+    c->min_disparity = 0;
+    c->max_disparity = max_disparity;
+    c->census_type = 1;
+    c->lambda_census = 16.f; c->lambda_ad = 30.f; c->lambda_grad = 20.f;
+    c->eta_ad = 0.4f; c->mu_grad = 0.3f;
+    c->use_ad = 1; c->use_grad = 1; c->cost_max = 255;
+    c->agg_enable = 1; c->max_arm_length = 17; c->color_threshold = 12; c->agg_iterations = 1;
+    c->paths = 4; c->P1 = 10; c->P2_base = 120; c->adaptive_penalties = 1;
+    c->grad_t1 = 8; c->grad_t2 = 24; c->p2_alpha = 0.15f; c->piecewise_p2 = 1;
+    c->conf_uniqueness_ratio = 0.15f; c->conf_lr_max_diff = 1;
+    c->conf_min_texture = 2.f; c->conf_reliable_threshold = 0.25f;
+    c->post_subpixel = 1; c->post_lr_check = 1; c->post_lr_max_diff = 1;
+    c->post_median_radius = 1; c->post_max_fill_gap = 32;
+    if (mode == 0) {
+        c->use_ad = 0; c->use_grad = 0; c->agg_enable = 0; c->paths = 4; c->adaptive_penalties = 0;
+    } else if (mode == 1) {
+        c->agg_iterations = 1; c->paths = 4; c->adaptive_penalties = 1;
+    } else {
+        c->paths = 8; c->adaptive_penalties = 1;
+    }
+}
+
+int apg_hip_validate_params(const apg_hip_params* c) {
+    if (!c) return 4;
+    // validate_config (reference src/pipeline.cpp:171-186)
+    if (c->max_disparity <= c->min_disparity) return 3;
+    if (c->cost_max < 0 || c->cost_max > 65535) return 4;
+    if (c->P1 < 0) return 4;
+    if (c->P2_base <= c->P1) return 4;
+    if (c->grad_t1 < 0 || c->grad_t2 < c->grad_t1) return 4;
+    // device representability: u16 path sums need the reference's own
+    // can_use_u16_sgm_accumulator bound (include/apg_sgm/sgm_optimizer.hpp:9-18);
+    // Cross windows and D must fit the LDS layouts.
+    const long D = (long)c->max_disparity - c->min_disparity;
+    const long p2max = c->P2_base > c->P1 + 1 ? c->P2_base : c->P1 + 1;
+    const long npath = c->paths == 8 ? 8 : 4;
+    if (npath * ((long)c->cost_max + p2max) >= 65535) return 5;
+    if (D > kMaxD || D > 2 * 64 * kMaxSgmK) return 5;
+    if (c->agg_enable && c->max_arm_length > kMaxArm) return 5;
+    if (c->post_median_radius > 3) return 5;
+    if (c->paths != 4 && c->paths != 8) return 4;
+    return 0;
+}
+
+int apg_hip_ctx_create(int device, apg_hip_ctx* out) {
+    if (!out) return 8;
+    *out = nullptr;
+    apg_hip_ctx ctx = new (std::nothrow) apg_hip_ctx_s();
+    if (!ctx) return 7;
+    ctx->device = device;
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    for (int i = 0; e == hipSuccess && i <= S_COUNT; ++i) e = hipEventCreate(&ctx->ev[i]);
+    if (e != hipSuccess) {
+        delete ctx;
+        return (e == hipErrorOutOfMemory) ? 7 : 8;
+    }
+    *out = ctx;
+    return 0;
+}
+
+void apg_hip_ctx_destroy(apg_hip_ctx ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (auto& b : ctx->bufs)
+        if (b.ptr) (void)hipFree(b.ptr);
+    for (int i = 0; i <= S_COUNT; ++i)
+        if (ctx->ev[i]) (void)hipEventDestroy(ctx->ev[i]);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+int apg_hip_synchronize(apg_hip_ctx ctx) {
+    if (!ctx) return 8;
+    HIPCHK(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    return 0;
+}
+
+void* apg_hip_stream(apg_hip_ctx ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+size_t apg_hip_workspace_bytes(apg_hip_ctx ctx) {
+    if (!ctx) return 0;
+    size_t t = 0;
+    for (auto& b : ctx->bufs) t += b.bytes;
+    return t;
+}
+
+void apg_hip_release_workspace(apg_hip_ctx ctx) {
+    if (!ctx) return;
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto& b : ctx->bufs) {
+        if (b.ptr) (void)hipFree(b.ptr);
+        b.ptr = nullptr;
+        b.bytes = 0;
+    }
+    ctx->lut_valid = false;
+}
+
+void apg_hip_enable_stage_timing(apg_hip_ctx ctx, int on) {
+    if (ctx) ctx->timing = on != 0;
+}
+
+int apg_hip_last_timing(apg_hip_ctx ctx, apg_hip_timing* out) {
+    if (!ctx || !out) return 8;
+    *out = ctx->last;
+    return 0;
+}
+
+const char* apg_hip_last_error(apg_hip_ctx ctx) { return ctx ? ctx->last_error.c_str() : "invalid context"; }
+
+int apg_hip_malloc(apg_hip_ctx ctx, size_t bytes, void** d_out) {
+    if (!ctx || !d_out) return 8;
+    HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
+    HIPCHK(hipMalloc(d_out, bytes ? bytes : 1), "hipMalloc");
+    return 0;
+}
+
+int apg_hip_free(apg_hip_ctx ctx, void* d_ptr) {
+    if (!ctx) return 8;
+    HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
+    HIPCHK(hipFree(d_ptr), "hipFree");
+    return 0;
+}
+
+int apg_hip_upload(apg_hip_ctx ctx, void* d_dst, const void* h_src, size_t bytes) {
+    if (!ctx) return 8;
+    HIPCHK(hipMemcpyAsync(d_dst, h_src, bytes, hipMemcpyHostToDevice, ctx->stream), "upload");
+    HIPCHK(hipStreamSynchronize(ctx->stream), "upload sync");
+    return 0;
+}
+
+int apg_hip_download(apg_hip_ctx ctx, void* h_dst, const void* d_src, size_t bytes) {
+    if (!ctx) return 8;
+    HIPCHK(hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, ctx->stream), "download");
+    HIPCHK(hipStreamSynchronize(ctx->stream), "download sync");
+    return 0;
+}
+
+int apg_hip_memset(apg_hip_ctx ctx, void* d_dst, int value, size_t bytes) {
+    if (!ctx) return 8;
+    HIPCHK(hipMemsetAsync(d_dst, value, bytes, ctx->stream), "memset");
+    HIPCHK(hipStreamSynchronize(ctx->stream), "memset sync");
+    return 0;
+}
+
+// ---------------------------------------------------------------- stages ----
+
+int apg_hip_aux(apg_hip_ctx ctx, const apg_hip_params* p, const uint8_t* d_src, int w, int h, int stride,
+                int channels, uint8_t* d_gray, uint8_t* d_gx, uint8_t* d_gy, uint32_t* d_census32,
+                uint64_t* d_census64) {
+    if (int s = check_ctx(ctx, p)) return s;
+    if (!d_src || w <= 0 || h <= 0) return fail(ctx, 1, "empty input");
+    if ((channels != 1 && channels != 3) || stride < w * channels) return fail(ctx, 2, "bad stride/channels");
+    if ((p->census_type == 1 && !d_census32) || (p->census_type != 1 && !d_census64)) return fail(ctx, 4, "census buffer");
+    HIPCHK(launch_aux(ctx->stream, *p, d_src, w, h, stride, channels, d_gray, d_gx, d_gy, d_census32, d_census64), "aux");
+    return 0;
+}
+
+int apg_hip_range_init(apg_hip_ctx ctx, const apg_hip_params* p, int w, int h, int16_t* d_dmin, int16_t* d_dmax) {
+    if (int s = check_ctx(ctx, p)) return s;
+    HIPCHK(launch_range_init(ctx->stream, *p, w, h, d_dmin, d_dmax), "range");
+    return 0;
+}
+
+int apg_hip_cost_volume(apg_hip_ctx ctx, const apg_hip_params* p, int w, int h, const uint8_t* gl, const uint8_t* gxl,
+                        const uint8_t* gyl, const uint8_t* gr, const uint8_t* gxr, const uint8_t* gyr,
+                        const uint32_t* c32l, const uint32_t* c32r, const uint64_t* c64l, const uint64_t* c64r,
+                        const int16_t* dmin, const int16_t* dmax, uint16_t* d_cost) {
+    if (int s = check_ctx(ctx, p)) return s;
+    if (int s = ensure_lut(ctx, p)) return s;
+    HIPCHK(launch_cost(ctx->stream, *p, buf<CostLut>(ctx, B_LUT), w, h, gl, gxl, gyl, gr, gxr, gyr, c32l, c32r, c64l,
+                       c64r, dmin, dmax, d_cost),
+           "cost");
+    return 0;
+}
+
+int apg_hip_right_wta(apg_hip_ctx ctx, const apg_hip_params* p, int w, int h, const uint16_t* d_cost, float* d_dr) {
+    if (int s = check_ctx(ctx, p)) return s;
+    HIPCHK(launch_right_wta(ctx->stream, *p, w, h, d_cost, d_dr), "right_wta");
+    return 0;
+}
+
+int apg_hip_cross_arms(apg_hip_ctx ctx, const apg_hip_params* p, const uint8_t* d_gray, int w, int h, uint32_t* d_arms) {
+    if (int s = check_ctx(ctx, p)) return s;
+    HIPCHK(launch_cross_arms(ctx->stream, *p, d_gray, w, h, d_arms), "arms");
+    return 0;
+}
+
+int apg_hip_cross_aggregate(apg_hip_ctx ctx, const apg_hip_params* p, int w, int h, const uint8_t* d_gray,
+                            const int16_t* dmin, const int16_t* dmax, uint16_t* d_cost, uint16_t* d_tmp) {
+    if (int s = check_ctx(ctx, p)) return s;
+    if (!p->agg_enable) return 0;
+    HIPCHK(ensure(ctx, B_ARMS, (size_t)w * h * 4), "arms alloc");
+    uint32_t* arms = buf<uint32_t>(ctx, B_ARMS);
+    HIPCHK(launch_cross_arms(ctx->stream, *p, d_gray, w, h, arms), "arms");
+    const int iters = p->agg_iterations > 1 ? p->agg_iterations : 1;
+    for (int i = 0; i < iters; ++i) {
+        HIPCHK(launch_cross_h(ctx->stream, *p, w, h, dmin, dmax, arms, d_cost, d_tmp), "cross_h");
+        HIPCHK(launch_cross_v(ctx->stream, *p, w, h, dmin, dmax, arms, d_tmp, d_cost), "cross_v");
+    }
+    return 0;
+}
+
+static const int kDirs[8][2] = {{1, 0}, {-1, 0}, {0, 1}, {0, -1}, {1, 1}, {1, -1}, {-1, 1}, {-1, -1}};
+
+int apg_hip_sgm_optimize(apg_hip_ctx ctx, const apg_hip_params* p, int w, int h, const uint8_t* d_gray,
+                         const uint16_t* d_cost, uint16_t* d_sum16, uint32_t* d_sum32) {
+    if (int s = check_ctx(ctx, p)) return s;
+    const int D = p->max_disparity - p->min_disparity;
+    const int npath = p->paths == 8 ? 8 : 4;
+    for (int k = 0; k < npath; ++k)
+        HIPCHK(launch_sgm_dir(ctx->stream, *p, w, h, D, d_gray, d_cost, d_sum16, kDirs[k][0], kDirs[k][1], k == 0 ? 0 : 1),
+               "sgm");
+    if (d_sum32) HIPCHK(launch_widen16(ctx->stream, (size_t)w * h * D, d_sum16, d_sum32), "widen");
+    return 0;
+}
+
+int apg_hip_sgm_path(apg_hip_ctx ctx, const apg_hip_params* p, int w, int h, int D, const uint8_t* d_gray,
+                     const uint16_t* d_cost, uint32_t* d_acc, int dx, int dy) {
+    if (!ctx) return 8;
+    if (!p) return fail(ctx, 4, "null params");
+    if (D <= 0 || D > 2 * 64 * kMaxSgmK) return fail(ctx, 5, "unsupported D");
+    if (dx < -1 || dx > 1 || dy < -1 || dy > 1 || (dx == 0 && dy == 0)) return fail(ctx, 4, "bad direction");
+    HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
+    HIPCHK(launch_sgm_dir32(ctx->stream, *p, w, h, D, d_gray, d_cost, d_acc, dx, dy), "sgm_path");
+    return 0;
+}
+
+int apg_hip_wta(apg_hip_ctx ctx, const apg_hip_params* p, int w, int h, const void* d_vol, int vol_bits,
+                const int16_t* dmin, const int16_t* dmax, float* d_disp, float* d_best, float* d_second) {
+    if (int s = check_ctx(ctx, p)) return s;
+    if (vol_bits != 16 && vol_bits != 32) return fail(ctx, 4, "vol_bits must be 16 or 32");
+    HIPCHK(launch_wta(ctx->stream, *p, w, h, d_vol, vol_bits, dmin, dmax, d_disp, d_best, d_second), "wta");
+    return 0;
+}
+
+int apg_hip_confidence(apg_hip_ctx ctx, const apg_hip_params* p, int w, int h, const float* d_disp,
+                       const float* d_dr, const uint8_t* d_gx, const uint8_t* d_gy, const float* d_best,
+                       const float* d_second, float* d_conf, uint8_t* d_rel, uint8_t* d_reason) {
+    if (int s = check_ctx(ctx, p)) return s;
+    HIPCHK(launch_confidence(ctx->stream, *p, w, h, d_disp, d_dr, d_gx, d_gy, d_best, d_second, d_conf, d_rel, d_reason),
+           "confidence");
+    return 0;
+}
+
+int apg_hip_lr_check(apg_hip_ctx ctx, const apg_hip_params* p, int w, int h, float* d_disp, const float* d_dr,
+                     uint8_t* d_reason) {
+    if (int s = check_ctx(ctx, p)) return s;
+    HIPCHK(launch_lr_check(ctx->stream, *p, w, h, d_disp, d_dr, d_reason), "lr_check");
+    return 0;
+}
+
+int apg_hip_fill_median(apg_hip_ctx ctx, const apg_hip_params* p, int w, int h, float* d_disp,
+                        const uint8_t* d_reason, const uint8_t* d_gray, const float* d_prior, float* d_scratch) {
+    if (int s = check_ctx(ctx, p)) return s;
+    HIPCHK(launch_fill_median(ctx->stream, *p, w, h, d_disp, d_reason, d_gray, d_prior, d_scratch), "fill_median");
+    return 0;
+}
+
+// -------------------------------------------------------------- pipeline ----
+
+// Dense branch of StereoMatcher::compute_impl (reference src/pipeline.cpp:435-605)
+// on the device: aux -> range -> cost -> right WTA (raw volume) -> Cross ->
+// SGM -> WTA -> confidence -> LR -> fill -> median.
+int apg_hip_pipeline(apg_hip_ctx ctx, const apg_hip_params* p, const uint8_t* d_left, const uint8_t* d_right, int w,
+                     int h, int stride, int channels, float* d_disparity, float* d_confidence, uint8_t* d_reason,
+                     uint8_t* d_reliable, const apg_hip_intermediates* inter) {
+    if (!ctx) return 8;
+    if (!d_left || !d_right || w <= 0 || h <= 0) return fail(ctx, 1, "empty input image(left or right)");
+    if (stride < w * channels || (channels != 1 && channels != 3))
+        return fail(ctx, 2, "invalid row stride (stride less than row byte length)");
+    if (int s = check_ctx(ctx, p)) return s;
+    if (int s = ensure_lut(ctx, p)) return s;
+    hipStream_t st = ctx->stream;
+    const size_t n = (size_t)w * h;
+    const int D = p->max_disparity - p->min_disparity;
+    const size_t nv = n * (size_t)D;
+    const size_t cen = p->census_type == 1 ? 4 : 8;
+    HIPCHK(ensure(ctx, B_GRAY_L, n), "alloc");
+    HIPCHK(ensure(ctx, B_GX_L, n), "alloc");
+    HIPCHK(ensure(ctx, B_GY_L, n), "alloc");
+    HIPCHK(ensure(ctx, B_GRAY_R, n), "alloc");
+    HIPCHK(ensure(ctx, B_GX_R, n), "alloc");
+    HIPCHK(ensure(ctx, B_GY_R, n), "alloc");
+    HIPCHK(ensure(ctx, B_CEN_L, n * cen), "alloc");
+    HIPCHK(ensure(ctx, B_CEN_R, n * cen), "alloc");
+    HIPCHK(ensure(ctx, B_DMIN, n * 2), "alloc");
+    HIPCHK(ensure(ctx, B_DMAX, n * 2), "alloc");
+    HIPCHK(ensure(ctx, B_ARMS, n * 4), "alloc");
+    HIPCHK(ensure(ctx, B_COST, nv * 2), "alloc");
+    HIPCHK(ensure(ctx, B_TMP, nv * 2), "alloc");
+    HIPCHK(ensure(ctx, B_SUM, nv * 2), "alloc");
+    HIPCHK(ensure(ctx, B_DR, n * 4), "alloc");
+    HIPCHK(ensure(ctx, B_BEST, n * 4), "alloc");
+    HIPCHK(ensure(ctx, B_SECOND, n * 4), "alloc");
+    HIPCHK(ensure(ctx, B_SCRATCH, n * 4), "alloc");
+    uint8_t *gl = buf<uint8_t>(ctx, B_GRAY_L), *gxl = buf<uint8_t>(ctx, B_GX_L), *gyl = buf<uint8_t>(ctx, B_GY_L);
+    uint8_t *gr = buf<uint8_t>(ctx, B_GRAY_R), *gxr = buf<uint8_t>(ctx, B_GX_R), *gyr = buf<uint8_t>(ctx, B_GY_R);
+    void *cl = ctx->bufs[B_CEN_L].ptr, *cr = ctx->bufs[B_CEN_R].ptr;
+    int16_t *dmin = buf<int16_t>(ctx, B_DMIN), *dmax = buf<int16_t>(ctx, B_DMAX);
+    uint32_t* arms = buf<uint32_t>(ctx, B_ARMS);
+    uint16_t *cost = buf<uint16_t>(ctx, B_COST), *tmp = buf<uint16_t>(ctx, B_TMP), *sum = buf<uint16_t>(ctx, B_SUM);
+    float *dr = buf<float>(ctx, B_DR), *best = buf<float>(ctx, B_BEST), *second = buf<float>(ctx, B_SECOND);
+    float* scratch = buf<float>(ctx, B_SCRATCH);
+    const bool sym = p->census_type == 1;
+
+    mark(ctx, S_AUX);
+    HIPCHK(launch_aux(st, *p, d_left, w, h, stride, channels, gl, gxl, gyl, sym ? (uint32_t*)cl : nullptr,
+                      sym ? nullptr : (uint64_t*)cl),
+           "aux L");
+    HIPCHK(launch_aux(st, *p, d_right, w, h, stride, channels, gr, gxr, gyr, sym ? (uint32_t*)cr : nullptr,
+                      sym ? nullptr : (uint64_t*)cr),
+           "aux R");
+    mark(ctx, S_PRIOR);
+    HIPCHK(launch_range_init(st, *p, w, h, dmin, dmax), "range");
+    mark(ctx, S_COST);
+    HIPCHK(launch_cost(st, *p, buf<CostLut>(ctx, B_LUT), w, h, gl, gxl, gyl, gr, gxr, gyr,
+                       sym ? (const uint32_t*)cl : nullptr, sym ? (const uint32_t*)cr : nullptr,
+                       sym ? nullptr : (const uint64_t*)cl, sym ? nullptr : (const uint64_t*)cr, dmin, dmax, cost),
+           "cost");
+    if (inter && inter->d_cost) HIPCHK(hipMemcpyAsync(inter->d_cost, cost, nv * 2, hipMemcpyDeviceToDevice, st), "copy C");
+    mark(ctx, S_RWTA);
+    HIPCHK(launch_right_wta(st, *p, w, h, cost, dr), "right_wta");
+    mark(ctx, S_CROSS);
+    if (p->agg_enable) {
+        HIPCHK(launch_cross_arms(st, *p, gl, w, h, arms), "arms");
+        const int iters = p->agg_iterations > 1 ? p->agg_iterations : 1;
+        for (int i = 0; i < iters; ++i) {
+            HIPCHK(launch_cross_h(st, *p, w, h, dmin, dmax, arms, cost, tmp), "cross_h");
+            HIPCHK(launch_cross_v(st, *p, w, h, dmin, dmax, arms, tmp, cost), "cross_v");
+        }
+    }
+    if (inter && inter->d_cost_agg)
+        HIPCHK(hipMemcpyAsync(inter->d_cost_agg, cost, nv * 2, hipMemcpyDeviceToDevice, st), "copy C'");
+    mark(ctx, S_SGM);
+    const int npath = p->paths == 8 ? 8 : 4;
+    for (int k = 0; k < npath; ++k)
+        HIPCHK(launch_sgm_dir(st, *p, w, h, D, gl, cost, sum, kDirs[k][0], kDirs[k][1], k == 0 ? 0 : 1), "sgm");
+    if (inter && inter->d_sum32) HIPCHK(launch_widen16(st, nv, sum, inter->d_sum32), "widen");
+    mark(ctx, S_WTA);
+    HIPCHK(launch_wta(st, *p, w, h, sum, 16, dmin, dmax, d_disparity, best, second), "wta");
+    if (inter && inter->d_disp_wta)
+        HIPCHK(hipMemcpyAsync(inter->d_disp_wta, d_disparity, n * 4, hipMemcpyDeviceToDevice, st), "copy wta");
+    if (inter && inter->d_disp_right)
+        HIPCHK(hipMemcpyAsync(inter->d_disp_right, dr, n * 4, hipMemcpyDeviceToDevice, st), "copy dr");
+    if (inter && inter->d_best) HIPCHK(hipMemcpyAsync(inter->d_best, best, n * 4, hipMemcpyDeviceToDevice, st), "copy best");
+    if (inter && inter->d_second)
+        HIPCHK(hipMemcpyAsync(inter->d_second, second, n * 4, hipMemcpyDeviceToDevice, st), "copy second");
+    mark(ctx, S_CONF);
+    HIPCHK(launch_confidence(st, *p, w, h, d_disparity, dr, gxl, gyl, best, second, d_confidence, d_reliable, d_reason),
+           "confidence");
+    mark(ctx, S_POST);
+    HIPCHK(launch_lr_check(st, *p, w, h, d_disparity, dr, d_reason), "lr");
+    HIPCHK(launch_fill_median(st, *p, w, h, d_disparity, d_reason, gl, nullptr, scratch), "fill_median");
+    mark(ctx, S_COUNT);
+    if (ctx->timing) {
+        HIPCHK(hipEventSynchronize(ctx->ev[S_COUNT]), "event sync");
+        float ms[S_COUNT];
+        for (int i = 0; i < S_COUNT; ++i) (void)hipEventElapsedTime(&ms[i], ctx->ev[i], ctx->ev[i + 1]);
+        float tot = 0.f;
+        (void)hipEventElapsedTime(&tot, ctx->ev[0], ctx->ev[S_COUNT]);
+        apg_hip_timing& t = ctx->last;
+        t.aux_ms = ms[S_AUX]; t.prior_ms = ms[S_PRIOR]; t.cost_ms = ms[S_COST]; t.right_wta_ms = ms[S_RWTA];
+        t.cross_ms = ms[S_CROSS]; t.sgm_ms = ms[S_SGM]; t.wta_ms = ms[S_WTA]; t.confidence_ms = ms[S_CONF];
+        t.post_ms = ms[S_POST]; t.total_ms = tot;
+    }
+    return 0;
+}
+
+int apg_hip_bench_pipeline(apg_hip_ctx ctx, const apg_hip_params* p, const uint8_t* d_left, const uint8_t* d_right,
+                           int w, int h, int stride, int channels, float* d_disp, float* d_conf, uint8_t* d_reason,
+                           uint8_t* d_rel, int warmup, int steps, double* ms_total, apg_hip_timing* stage_avg) {
+    if (!ctx) return 8;
+    const bool was_timing = ctx->timing;
+    ctx->timing = false;
+    for (int i = 0; i < warmup; ++i) {
+        const int s = apg_hip_pipeline(ctx, p, d_left, d_right, w, h, stride, channels, d_disp, d_conf, d_reason, d_rel,
+                                       nullptr);
+        if (s) { ctx->timing = was_timing; return s; }
+    }
+    HIPCHK(hipStreamSynchronize(ctx->stream), "bench sync");
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0), "event");
+    HIPCHK(hipEventCreate(&e1), "event");
+    (void)hipEventRecord(e0, ctx->stream);
+    int status = 0;
+    for (int i = 0; i < steps && !status; ++i)
+        status = apg_hip_pipeline(ctx, p, d_left, d_right, w, h, stride, channels, d_disp, d_conf, d_reason, d_rel,
+                                  nullptr);
+    (void)hipEventRecord(e1, ctx->stream);
+    (void)hipEventSynchronize(e1);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (ms_total) *ms_total = ms;
+    if (!status && stage_avg) {
+        apg_hip_timing acc{};
+        ctx->timing = true;
+        for (int i = 0; i < steps && !status; ++i) {
+            status = apg_hip_pipeline(ctx, p, d_left, d_right, w, h, stride, channels, d_disp, d_conf, d_reason, d_rel,
+                                      nullptr);
+            const apg_hip_timing& t = ctx->last;
+            acc.aux_ms += t.aux_ms; acc.prior_ms += t.prior_ms; acc.cost_ms += t.cost_ms;
+            acc.right_wta_ms += t.right_wta_ms; acc.cross_ms += t.cross_ms; acc.sgm_ms += t.sgm_ms;
+            acc.wta_ms += t.wta_ms; acc.confidence_ms += t.confidence_ms; acc.post_ms += t.post_ms;
+            acc.total_ms += t.total_ms;
+        }
+        const double k = steps > 0 ? 1.0 / steps : 0.0;
+        stage_avg->aux_ms = acc.aux_ms * k; stage_avg->prior_ms = acc.prior_ms * k; stage_avg->cost_ms = acc.cost_ms * k;
+        stage_avg->right_wta_ms = acc.right_wta_ms * k; stage_avg->cross_ms = acc.cross_ms * k;
+        stage_avg->sgm_ms = acc.sgm_ms * k; stage_avg->wta_ms = acc.wta_ms * k;
+        stage_avg->confidence_ms = acc.confidence_ms * k; stage_avg->post_ms = acc.post_ms * k;
+        stage_avg->total_ms = acc.total_ms * k;
+    }
+    ctx->timing = was_timing;
+    return status;
+}
+
+const char* apg_hip_version_string(void) { return "1.0.0-gfx950"; }
+
+}  // extern "C"

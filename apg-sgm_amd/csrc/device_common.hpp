@@ -1,0 +1,81 @@
+// Shared device helpers for the gfx950 kernels: packed u16 arithmetic
+// (v_pk_*_u16), wave64 DPP reductions and the stage-parameter block that is
+// passed by value to every kernel.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/apg_sgm_hip.h"
+
+namespace apgk {
+
+constexpr uint16_t kInv16 = 0xFFFFu;          // kInvalidCost (reference include/apg_sgm/types.hpp:44)
+constexpr uint32_t kInv32 = 0xFFFFFFFFu;      // kInvalidCost32 (reference include/apg_sgm/types.hpp:45)
+constexpr int kWave = 64;
+
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ uint32_t as_u32(u16x2 v) { return __builtin_bit_cast(uint32_t, v); }
+__device__ __forceinline__ u16x2 as_v2(uint32_t v) { return __builtin_bit_cast(u16x2, v); }
+__device__ __forceinline__ u16x2 splat2(uint32_t s) { return (u16x2)((unsigned short)s); }
+__device__ __forceinline__ u16x2 add_sat2(u16x2 a, u16x2 b) { return __builtin_elementwise_add_sat(a, b); }
+__device__ __forceinline__ u16x2 min2(u16x2 a, u16x2 b) { return __builtin_elementwise_min(a, b); }
+
+// DPP control words (CDNA3/CDNA4 ISA, DPP_CTRL enumeration)
+constexpr int kDppRowShr1 = 0x111, kDppRowShr2 = 0x112, kDppRowShr4 = 0x114, kDppRowShr8 = 0x118;
+constexpr int kDppWaveShl1 = 0x130, kDppWaveShr1 = 0x138;
+constexpr int kDppRowBcast15 = 0x142, kDppRowBcast31 = 0x143;
+
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ uint32_t dpp(uint32_t old, uint32_t src) {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)src, CTRL, ROW_MASK, 0xF, false);
+}
+
+// lane l receives lane l-1's value; lane 0 receives `fill`.
+__device__ __forceinline__ uint32_t wave_shr1(uint32_t v, uint32_t fill) { return dpp<kDppWaveShr1>(fill, v); }
+// lane l receives lane l+1's value; lane 63 receives `fill`.
+__device__ __forceinline__ uint32_t wave_shl1(uint32_t v, uint32_t fill) { return dpp<kDppWaveShl1>(fill, v); }
+
+// Minimum over the 64 lanes, returned wave-uniform. Requires all lanes active.
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
+    v = min(v, dpp<kDppRowShr1>(v, v));
+    v = min(v, dpp<kDppRowShr2>(v, v));
+    v = min(v, dpp<kDppRowShr4>(v, v));
+    v = min(v, dpp<kDppRowShr8>(v, v));
+    v = min(v, dpp<kDppRowBcast15, 0xA>(v, v));
+    v = min(v, dpp<kDppRowBcast31, 0xC>(v, v));
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// detail::adaptive_p2 (reference src/sgm_common.hpp:25-39)
+__device__ __forceinline__ int adaptive_p2(const apg_hip_params& p, int dI) {
+    dI = dI < 0 ? -dI : dI;
+    int P2 = p.P2_base;
+    if (!p.adaptive_penalties) return max(P2, p.P1 + 1);
+    if (p.piecewise_p2) {
+        if (dI >= p.grad_t2) P2 = p.P2_base / 4;
+        else if (dI >= p.grad_t1) P2 = p.P2_base / 2;
+        else P2 = p.P2_base;
+    } else {
+        P2 = (int)((float)p.P2_base / (1.f + p.p2_alpha * (float)dI) + 0.5f);
+    }
+    return max(P2, p.P1 + 1);
+}
+
+// SubpixelEstimator::compute_legacy_delta (reference src/subpixel_estimator.hpp:115-124).
+// Built with -ffp-contract=off and IEEE division, so it rounds as the host does.
+__device__ __forceinline__ float legacy_delta(float cm, float c0, float cp) {
+    const float denom = cm - 2.0f * c0 + cp;
+    if (fabsf(denom) > 1e-6f) {
+        const float delta = 0.5f * (cm - cp) / denom;
+        if (delta < -0.5f) return -0.5f;
+        if (delta > 0.5f) return 0.5f;
+        return delta;
+    }
+    return 0.0f;
+}
+
+}  // namespace apgk

@@ -1,0 +1,372 @@
+// gfx950 kernels: Cross H and V aggregation (integer windowed means via packed
+// (sum << 8 | count) prefix sums) and the SGM path scans (one wavefront per
+// scanline, disparities across lanes as packed u16 pairs, DPP neighbour
+// exchange and wave-min reduction).
+#include "device_common.hpp"
+#include "kernels.hpp"
+
+namespace apgk {
+
+// Exact floor(n / c) for 1 <= c <= 2*kMaxArm+1 and n <= (2*kMaxArm+1)*65535:
+// m = ceil(2^32 / c), q = umulhi(n, m) is exact while n*(m*c - 2^32) < 2^32.
+__device__ __forceinline__ uint32_t div_small(uint32_t n, uint32_t c, uint32_t m) {
+    return c == 1 ? n : __umulhi(n, m);
+}
+__device__ __forceinline__ uint32_t magic_for(uint32_t c) {
+    return c <= 1 ? 0u : (uint32_t)((0x100000000ull + c - 1) / c);
+}
+
+constexpr int CH_TX = 64;
+
+// Horizontal pass of CostAggregator::aggregate_hv (reference
+// src/cost_aggregator.cpp:74-105): T(x,d) = floor(sum / count) over the valid
+// C(xx,d), xx in [x-L, x+R], d inside xx's range; 65535 when d is outside
+// x's range or count == 0.  Block = one row segment of CH_TX pixels and one
+// 64-wide d chunk (lane = d); the row's packed prefix sums over the segment
+// plus arm halo live in LDS.
+__global__ __launch_bounds__(256) void cross_h_kernel(int w, int h, int d0, int D, int Lmax,
+                                                      const int16_t* __restrict__ dmin, const int16_t* __restrict__ dmax,
+                                                      const uint32_t* __restrict__ arms,
+                                                      const uint16_t* __restrict__ cost, uint16_t* __restrict__ tmp) {
+    extern __shared__ uint32_t sh[];
+    uint32_t* magic = sh;                 // [2*kMaxArm + 2]
+    uint32_t* P = sh + 2 * kMaxArm + 2;   // [(NCOL + 1) * 64]
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int y = blockIdx.y;
+    const int x0 = blockIdx.x * CH_TX;
+    const int dc = blockIdx.z * 64;
+    const int di = dc + lane;
+    const int d = d0 + di;
+    const int base = x0 - Lmax;
+    const int NCOL = CH_TX + 2 * Lmax;
+    for (int c = threadIdx.x; c < 2 * kMaxArm + 2; c += 256) magic[c] = magic_for((uint32_t)c);
+    const int seg = (NCOL + 3) / 4;
+    const int c_beg = wave * seg, c_end = min(NCOL, c_beg + seg);
+    uint32_t acc = 0;
+    if (wave == 0) P[lane] = 0;
+    for (int col = c_beg; col < c_end; ++col) {
+        const int xx = base + col;
+        uint32_t v = 0;
+        if (xx >= 0 && xx < w && di < D) {
+            const size_t q = (size_t)y * w + xx;
+            if (d >= dmin[q] && d < dmax[q]) {
+                const uint16_t c = cost[q * D + di];
+                if (c != kInv16) v = ((uint32_t)c << 8) | 1u;
+            }
+        }
+        acc += v;
+        P[(col + 1) * 64 + lane] = acc;
+    }
+    __syncthreads();
+    uint32_t off = 0;
+    for (int ww = 0; ww < wave; ++ww) {
+        const int e = min(NCOL, (ww + 1) * seg);
+        if (e > ww * seg) off += P[e * 64 + lane];
+    }
+    __syncthreads();
+    if (off)
+        for (int col = c_beg; col < c_end; ++col) P[(col + 1) * 64 + lane] += off;
+    __syncthreads();
+    if (di >= D) return;
+    for (int px = wave; px < CH_TX; px += 4) {
+        const int x = x0 + px;
+        if (x >= w) break;
+        const size_t q = (size_t)y * w + x;
+        const uint32_t a = arms[q];
+        const int L = a & 0xFF, R = (a >> 8) & 0xFF;
+        uint16_t out = kInv16;
+        if (d >= dmin[q] && d < dmax[q]) {
+            const uint32_t s = P[(x + R + 1 - base) * 64 + lane] - P[(x - L - base) * 64 + lane];
+            const uint32_t cnt = s & 0xFF;
+            if (cnt) out = (uint16_t)div_small(s >> 8, cnt, magic[cnt]);
+        }
+        tmp[q * D + di] = out;
+    }
+}
+
+hipError_t launch_cross_h(hipStream_t s, const apg_hip_params& p, int w, int h, const int16_t* dmin,
+                          const int16_t* dmax, const uint32_t* arms, const uint16_t* cost, uint16_t* tmp) {
+    const int D = p.max_disparity - p.min_disparity;
+    const int Lmax = p.max_arm_length > 1 ? p.max_arm_length : 1;
+    const size_t shm = (2 * kMaxArm + 2 + (size_t)(CH_TX + 2 * Lmax + 1) * 64) * sizeof(uint32_t);
+    dim3 grid((w + CH_TX - 1) / CH_TX, h, (D + 63) / 64);
+    hipLaunchKernelGGL(cross_h_kernel, grid, dim3(256), shm, s, w, h, p.min_disparity, D, Lmax, dmin, dmax, arms,
+                       cost, tmp);
+    return hipGetLastError();
+}
+
+constexpr int CV_PF = 16;   // rows fetched per batch
+
+// Vertical pass of CostAggregator::aggregate_hv (reference
+// src/cost_aggregator.cpp:107-138): same windowed mean over y in [y-U, y+D].
+// One wavefront per (column, 64-wide d chunk) slides down the column keeping
+// the running packed prefix P(k) = sum over rows < k in an LDS ring of
+// RS >= 2*Lmax + 2 entries; rows are fetched CV_PF at a time.
+__global__ __launch_bounds__(64) void cross_v_kernel(int w, int h, int d0, int D, int Lmax, int rs_mask,
+                                                     const int16_t* __restrict__ dmin, const int16_t* __restrict__ dmax,
+                                                     const uint32_t* __restrict__ arms,
+                                                     const uint16_t* __restrict__ tmp, uint16_t* __restrict__ cost) {
+    extern __shared__ uint32_t sh[];
+    uint32_t* magic = sh;                  // [2*kMaxArm + 2]
+    uint32_t* ring = sh + 2 * kMaxArm + 2; // [RS * 64]
+    const int lane = threadIdx.x;
+    const int x = blockIdx.x;
+    const int di = blockIdx.y * 64 + lane;
+    const int d = d0 + di;
+    const bool act = di < D;
+    for (int c = lane; c < 2 * kMaxArm + 2; c += 64) magic[c] = magic_for((uint32_t)c);
+    ring[lane] = 0;
+    uint32_t acc = 0;
+    const int total = h + Lmax;   // fill index k runs over [0, h), output row y = k - Lmax
+    for (int kb = 0; kb < total; kb += CV_PF) {
+        uint32_t v[CV_PF];
+#pragma unroll
+        for (int i = 0; i < CV_PF; ++i) {
+            const int k = kb + i;
+            v[i] = 0;
+            if (k < h && act) {
+                const size_t q = (size_t)k * w + x;
+                const uint16_t c = tmp[q * D + di];
+                if (c != kInv16 && d >= dmin[q] && d < dmax[q]) v[i] = ((uint32_t)c << 8) | 1u;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < CV_PF; ++i) {
+            const int k = kb + i;
+            if (k < h) {
+                acc += v[i];
+                ring[((k + 1) & rs_mask) * 64 + lane] = acc;
+            }
+            const int y = k - Lmax;
+            if (y >= 0 && y < h && act) {
+                const size_t q = (size_t)y * w + x;
+                const uint32_t a = arms[q];
+                const int U = (a >> 16) & 0xFF, Dn = a >> 24;
+                uint16_t out = kInv16;
+                if (d >= dmin[q] && d < dmax[q]) {
+                    const uint32_t s = ring[((y + Dn + 1) & rs_mask) * 64 + lane] - ring[((y - U) & rs_mask) * 64 + lane];
+                    const uint32_t cnt = s & 0xFF;
+                    if (cnt) out = (uint16_t)div_small(s >> 8, cnt, magic[cnt]);
+                }
+                cost[q * D + di] = out;
+            }
+        }
+    }
+}
+
+hipError_t launch_cross_v(hipStream_t s, const apg_hip_params& p, int w, int h, const int16_t* dmin,
+                          const int16_t* dmax, const uint32_t* arms, const uint16_t* tmp, uint16_t* cost) {
+    const int D = p.max_disparity - p.min_disparity;
+    const int Lmax = p.max_arm_length > 1 ? p.max_arm_length : 1;
+    int rs = 1;
+    while (rs < 2 * Lmax + 2) rs <<= 1;
+    const size_t shm = (2 * kMaxArm + 2 + (size_t)rs * 64) * sizeof(uint32_t);
+    dim3 grid(w, (D + 63) / 64);
+    hipLaunchKernelGGL(cross_v_kernel, grid, dim3(64), shm, s, w, h, p.min_disparity, D, Lmax, rs - 1, dmin, dmax,
+                       arms, tmp, cost);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ SGM ----
+// Scanline geometry of direction (dx, dy) (reference src/sgm_optimizer.cpp:186-259):
+// rows for horizontal paths, columns for vertical ones and, for diagonals, the
+// W + H - 1 rays that start on the border they enter from.
+__device__ __forceinline__ void scan_start(int s, int w, int h, int dx, int dy, int& sx, int& sy, int& n) {
+    if (dy == 0) {
+        sy = s; sx = dx > 0 ? 0 : w - 1; n = w;
+    } else if (dx == 0) {
+        sx = s; sy = dy > 0 ? 0 : h - 1; n = h;
+    } else {
+        if (s < w) { sx = s; sy = dy > 0 ? 0 : h - 1; }
+        else { const int k = s - w + 1; sx = dx > 0 ? 0 : w - 1; sy = dy > 0 ? k : h - 1 - k; }
+        const int nx = dx > 0 ? w - sx : sx + 1;
+        const int ny = dy > 0 ? h - sy : sy + 1;
+        n = min(nx, ny);
+    }
+}
+
+constexpr int SG_PF = 8;   // scanline steps fetched per batch
+
+// Loads the packed pair (d = 2*m, 2*m+1) of a D-vector; out-of-range halves
+// read as 65535 (invalid).
+template <bool EVEN_D>
+__device__ __forceinline__ uint32_t load_pair(const uint16_t* __restrict__ v, int m, int D) {
+    const int d = 2 * m;
+    if (EVEN_D) return d < D ? *(const uint32_t*)(v + d) : 0xFFFFFFFFu;
+    const uint32_t lo = d < D ? v[d] : 0xFFFFu;
+    const uint32_t hi = d + 1 < D ? v[d + 1] : 0xFFFFu;
+    return lo | (hi << 16);
+}
+template <bool EVEN_D>
+__device__ __forceinline__ void store_pair(uint16_t* __restrict__ v, int m, int D, uint32_t val) {
+    const int d = 2 * m;
+    if (EVEN_D) {
+        if (d < D) *(uint32_t*)(v + d) = val;
+    } else {
+        if (d < D) v[d] = (uint16_t)val;
+        if (d + 1 < D) v[d + 1] = (uint16_t)(val >> 16);
+    }
+}
+
+// One wavefront runs the recurrence of process_pixel (reference
+// src/sgm_optimizer.cpp:21-81) along one scanline.  Lane l holds K packed
+// u16 pairs covering d = 2*(l*K + j) + {0,1}.  L and S use 65535 as the
+// path-infinity / invalid value: saturating adds keep invalid states at 65535
+// and (host-checked) npath*(cost_max + P2max) < 65535 keeps valid sums exact.
+//   L(d) = C(d) + min(L'(d), L'(d-1)+P1, L'(d+1)+P1, minL'+P2) - minL'
+//   L(d) = C(d) when the previous pixel has no valid state (or at a start).
+// MODE 0 writes S = L, MODE 1 writes S = sat(S + L).
+template <int K, bool EVEN_D, int MODE>
+__global__ __launch_bounds__(64) void sgm_dir_kernel(apg_hip_params p, int w, int h, int D, int dx, int dy,
+                                                     const uint8_t* __restrict__ gray, const uint16_t* __restrict__ cost,
+                                                     uint16_t* __restrict__ sum) {
+    const int lane = threadIdx.x;
+    int sx, sy, n;
+    scan_start(blockIdx.x, w, h, dx, dy, sx, sy, n);
+    const long step = (long)dy * w + dx;
+    const long p0 = (long)sy * w + sx;
+    const uint32_t P1 = (uint32_t)p.P1;
+    uint32_t prev[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) prev[j] = 0xFFFFFFFFu;
+    uint32_t minprev = 0xFFFFu;   // wave-uniform; 0xFFFF = no valid state
+    int gprev = 0;
+    for (int tb = 0; tb < n; tb += 64) {
+        // gray of the next 64 pixels of the scanline, one per lane
+        const int tl = tb + lane;
+        const uint32_t gchunk = tl < n ? gray[p0 + step * tl] : 0u;
+        for (int t0 = tb; t0 < min(n, tb + 64); t0 += SG_PF) {
+            uint32_t c[SG_PF][K];
+            uint32_t so[SG_PF][K];
+#pragma unroll
+            for (int i = 0; i < SG_PF; ++i) {
+                const int t = t0 + i;
+                if (t < n) {
+                    const uint16_t* cv = cost + (size_t)(p0 + step * t) * D;
+#pragma unroll
+                    for (int j = 0; j < K; ++j) c[i][j] = load_pair<EVEN_D>(cv, lane * K + j, D);
+                    if (MODE == 1) {
+                        const uint16_t* sv = sum + (size_t)(p0 + step * t) * D;
+#pragma unroll
+                        for (int j = 0; j < K; ++j) so[i][j] = load_pair<EVEN_D>(sv, lane * K + j, D);
+                    }
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < SG_PF; ++i) {
+                const int t = t0 + i;
+                if (t >= n) break;
+                const int g = __builtin_amdgcn_readlane((int)gchunk, t - tb);
+                uint32_t L[K];
+                if (t == 0 || minprev == 0xFFFFu) {
+#pragma unroll
+                    for (int j = 0; j < K; ++j) L[j] = c[i][j];
+                } else {
+                    const uint32_t P2 = (uint32_t)adaptive_p2(p, g - gprev);
+                    const u16x2 mp = splat2(minprev);
+                    const u16x2 mp2 = add_sat2(mp, splat2(P2));
+                    const u16x2 p1 = splat2(P1);
+                    const uint32_t left_hi = wave_shr1(prev[K - 1], 0xFFFFFFFFu);   // lane-1's last pair
+                    const uint32_t right_lo = wave_shl1(prev[0], 0xFFFFFFFFu);      // lane+1's first pair
+#pragma unroll
+                    for (int j = 0; j < K; ++j) {
+                        const uint32_t pj = prev[j];
+                        const uint32_t before = j == 0 ? left_hi : prev[j - 1];
+                        const uint32_t after = j == K - 1 ? right_lo : prev[j + 1];
+                        const u16x2 dm1 = as_v2(__builtin_amdgcn_alignbit(pj, before, 16));
+                        const u16x2 dp1 = as_v2(__builtin_amdgcn_alignbit(after, pj, 16));
+                        u16x2 t2 = min2(as_v2(pj), min2(add_sat2(dm1, p1), add_sat2(dp1, p1)));
+                        t2 = min2(t2, mp2);
+                        L[j] = as_u32(add_sat2(as_v2(c[i][j]), t2 - mp));
+                    }
+                }
+                uint32_t lm = 0xFFFFu;
+#pragma unroll
+                for (int j = 0; j < K; ++j) {
+                    prev[j] = L[j];
+                    lm = min(lm, min(L[j] & 0xFFFFu, L[j] >> 16));
+                }
+                minprev = wave_min_u32(lm);
+                gprev = g;
+                uint16_t* sv = sum + (size_t)(p0 + step * t) * D;
+#pragma unroll
+                for (int j = 0; j < K; ++j) {
+                    const uint32_t out = MODE == 0 ? L[j] : as_u32(add_sat2(as_v2(so[i][j]), as_v2(L[j])));
+                    store_pair<EVEN_D>(sv, lane * K + j, D, out);
+                }
+            }
+        }
+    }
+}
+
+template <int K, bool EVEN_D>
+static hipError_t launch_sgm_k(hipStream_t s, const apg_hip_params& p, int w, int h, int D, const uint8_t* gray,
+                               const uint16_t* cost, uint16_t* sum, int dx, int dy, int mode) {
+    const int nscan = dy == 0 ? h : (dx == 0 ? w : w + h - 1);
+    if (mode == 0)
+        hipLaunchKernelGGL((sgm_dir_kernel<K, EVEN_D, 0>), dim3(nscan), dim3(64), 0, s, p, w, h, D, dx, dy, gray, cost, sum);
+    else
+        hipLaunchKernelGGL((sgm_dir_kernel<K, EVEN_D, 1>), dim3(nscan), dim3(64), 0, s, p, w, h, D, dx, dy, gray, cost, sum);
+    return hipGetLastError();
+}
+
+hipError_t launch_sgm_dir(hipStream_t s, const apg_hip_params& p, int w, int h, int D, const uint8_t* gray,
+                          const uint16_t* cost, uint16_t* sum, int dx, int dy, int mode) {
+    const int K = (D + 127) / 128;
+    const bool even = (D % 2) == 0;
+#define APG_SGM_CASE(KK)                                                                           \
+    case KK:                                                                                       \
+        return even ? launch_sgm_k<KK, true>(s, p, w, h, D, gray, cost, sum, dx, dy, mode)         \
+                    : launch_sgm_k<KK, false>(s, p, w, h, D, gray, cost, sum, dx, dy, mode);
+    switch (K) {
+        APG_SGM_CASE(1)
+        APG_SGM_CASE(2)
+        APG_SGM_CASE(3)
+        APG_SGM_CASE(4)
+        default: return hipErrorInvalidValue;
+    }
+#undef APG_SGM_CASE
+}
+
+// u32 accumulator variant of one path (SgmOptimizer::aggregate_path with a
+// caller accumulator, reference src/sgm_optimizer.cpp:21-81): acc == INV32
+// stays INV32, an invalid C sets INV32, otherwise acc += L.  The path L is
+// recomputed by a MODE-0 scan into a scratch u16 volume first.
+__global__ void add_path32_kernel(size_t n, const uint16_t* __restrict__ L, uint32_t* __restrict__ acc) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint16_t l = L[i];
+    if (l == kInv16) acc[i] = kInv32;
+    else if (acc[i] != kInv32) acc[i] += l;
+}
+
+__global__ void widen16_kernel(size_t n, const uint16_t* __restrict__ in, uint32_t* __restrict__ out) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint16_t v = in[i];
+    out[i] = v == kInv16 ? kInv32 : (uint32_t)v;
+}
+
+hipError_t launch_widen16(hipStream_t s, size_t n, const uint16_t* in, uint32_t* out) {
+    hipLaunchKernelGGL(widen16_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, in, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_sgm_dir32(hipStream_t s, const apg_hip_params& p, int w, int h, int D, const uint8_t* gray,
+                            const uint16_t* cost, uint32_t* acc, int dx, int dy) {
+    // scratch L volume owned by the caller via acc's tail is not available;
+    // allocate on the stream (stream-ordered allocator keeps it on device).
+    const size_t n = (size_t)w * h * D;
+    uint16_t* L = nullptr;
+    hipError_t e = hipMallocAsync((void**)&L, n * sizeof(uint16_t), s);
+    if (e != hipSuccess) return e;
+    e = launch_sgm_dir(s, p, w, h, D, gray, cost, L, dx, dy, 0);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(add_path32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, L, acc);
+        e = hipGetLastError();
+    }
+    hipError_t e2 = hipFreeAsync(L, s);
+    return e != hipSuccess ? e : e2;
+}
+
+}  // namespace apgk

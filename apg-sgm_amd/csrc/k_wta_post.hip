@@ -1,0 +1,277 @@
+// gfx950 kernels: winner-take-all with second best and legacy subpixel,
+// confidence, left-right check, legacy hole fill and valid-only median.
+#include <math.h>
+
+#include "device_common.hpp"
+#include "kernels.hpp"
+
+namespace apgk {
+
+constexpr int WTA_PB = 64;   // pixels per block (one per lane)
+constexpr int WTA_DC = 64;   // disparities staged per chunk
+
+// wta_impl (reference src/sgm_optimizer.cpp:436-555) as a single streaming
+// pass per pixel.  Scanning d upward over [lo, hi):
+//   best: earliest strict minimum (the reference's `c < best`);
+//   second: min over valid d with |d - best_d| > 1.  When a new best appears
+//   at d, every earlier index except d-1 is eligible, i.e. the prefix minimum
+//   over indices <= d-2 (kept one step delayed); later indices d' >= best+2
+//   feed a right-hand minimum.  second = min(left part, right part).
+// Invalid entries (the volume's sentinel) and d outside [lo, hi) are skipped.
+// The 64 pixels of a block are consecutive in memory, so each d-chunk is
+// staged into LDS with row-contiguous loads and read back one pixel per lane.
+template <typename T>
+__global__ __launch_bounds__(64) void wta_kernel(int w, int h, int d0, int D, int subpixel, const T* __restrict__ vol,
+                                                 const int16_t* __restrict__ dmin, const int16_t* __restrict__ dmax,
+                                                 float* __restrict__ disp, float* __restrict__ best_out,
+                                                 float* __restrict__ second_out) {
+    __shared__ uint32_t st[WTA_PB][WTA_DC + 1];
+    const T INV = (T)~(T)0;
+    const int lane = threadIdx.x;
+    const size_t npx = (size_t)w * h;
+    const size_t p_base = (size_t)blockIdx.x * WTA_PB;
+    const size_t p = p_base + lane;
+    const bool act = p < npx;
+    const int lo = act ? dmin[p] : 0, hi = act ? dmax[p] : 0;
+    uint64_t bv = ~0ull;          // best value
+    int bd = -1;                  // best d
+    uint64_t pm = ~0ull;          // prefix min over scanned indices < d
+    uint64_t pm_lag = ~0ull;      // prefix min over scanned indices < d-1
+    uint64_t secL = ~0ull, secR = ~0ull;
+    uint64_t vprev = ~0ull;       // value at d-1 (INF when skipped)
+    uint64_t v_bm1 = ~0ull, v_bp1 = ~0ull;   // raw values at best-1 / best+1
+    uint64_t raw_prev = ~0ull;
+    for (int dc = 0; dc < D; dc += WTA_DC) {
+        const int nd = min(WTA_DC, D - dc);
+        for (int i = 0; i < WTA_PB; ++i) {
+            const size_t q = p_base + i;
+            if (q < npx && lane < nd) st[i][lane] = (uint32_t)vol[q * D + dc + lane];
+        }
+        __syncthreads();
+        if (act) {
+            for (int j = 0; j < nd; ++j) {
+                const int di = dc + j;
+                const int d = d0 + di;
+                const T raw = (T)st[lane][j];
+                // raw neighbour for the subpixel step is taken regardless of range
+                if (bd >= 0 && d == bd + 1) v_bp1 = (uint64_t)raw;
+                const bool valid = d >= lo && d < hi && raw != INV;
+                const uint64_t c = valid ? (uint64_t)raw : ~0ull;
+                const uint64_t pm_new = pm < c ? pm : c;
+                if (valid && c < bv) {
+                    secL = pm_lag;        // min over indices <= d-2
+                    secR = ~0ull;
+                    bv = c; bd = d;
+                    v_bm1 = raw_prev;
+                    v_bp1 = ~0ull;
+                } else if (valid && bd >= 0 && d >= bd + 2) {
+                    secR = secR < c ? secR : c;
+                }
+                pm_lag = pm;
+                pm = pm_new;
+                vprev = c;
+                raw_prev = (uint64_t)raw;
+            }
+        }
+        __syncthreads();
+    }
+    (void)vprev;
+    if (!act) return;
+    float dv = -1.f, b = INFINITY, sc = INFINITY;
+    if (hi > lo && bd >= 0) {
+        const uint64_t sec = secL < secR ? secL : secR;
+        float dval = (float)bd;
+        if (subpixel && bd - 1 >= lo && bd + 1 < hi) {
+            const int di = bd - d0;
+            if (di > 0 && di + 1 < D) {
+                const uint64_t INV64 = (uint64_t)INV;
+                if (v_bm1 != INV64 && v_bp1 != INV64) dval += legacy_delta((float)v_bm1, (float)bv, (float)v_bp1);
+            }
+        }
+        dv = dval;
+        b = (float)bv;
+        sc = sec != ~0ull ? (float)sec : INFINITY;
+    }
+    disp[p] = dv;
+    if (best_out) best_out[p] = b;
+    if (second_out) second_out[p] = sc;
+}
+
+hipError_t launch_wta(hipStream_t s, const apg_hip_params& p, int w, int h, const void* vol, int bits,
+                      const int16_t* dmin, const int16_t* dmax, float* disp, float* best, float* second) {
+    const size_t npx = (size_t)w * h;
+    dim3 grid((unsigned)((npx + WTA_PB - 1) / WTA_PB));
+    const int D = p.max_disparity - p.min_disparity;
+    if (bits == 16)
+        hipLaunchKernelGGL(wta_kernel<uint16_t>, grid, dim3(64), 0, s, w, h, p.min_disparity, D, p.post_subpixel,
+                           (const uint16_t*)vol, dmin, dmax, disp, best, second);
+    else
+        hipLaunchKernelGGL(wta_kernel<uint32_t>, grid, dim3(64), 0, s, w, h, p.min_disparity, D, p.post_subpixel,
+                           (const uint32_t*)vol, dmin, dmax, disp, best, second);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------ confidence ----
+enum { R_VALID = 0, R_OCCLUSION = 1, R_MISMATCH = 2, R_LOWCONF = 3 };
+
+__device__ __forceinline__ float clampf(float v, float lo, float hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// ConfidenceEstimator::estimate (reference src/confidence_estimator.cpp:7-63)
+__global__ void confidence_kernel(apg_hip_params p, int w, int h, const float* __restrict__ disp,
+                                  const float* __restrict__ dr, const uint8_t* __restrict__ gx,
+                                  const uint8_t* __restrict__ gy, const float* __restrict__ best,
+                                  const float* __restrict__ second, float* __restrict__ conf_out,
+                                  uint8_t* __restrict__ reliable, uint8_t* __restrict__ reason) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= w) return;
+    const size_t q = (size_t)y * w + x;
+    const float c1 = best[q], c2 = second[q];
+    float uniq = 0.f;
+    if (isfinite(c1) && isfinite(c2)) {
+        const float den = c2 < 1e-3f ? 1e-3f : c2;
+        uniq = clampf((c2 - c1) / den, 0.f, 1.f);
+    }
+    float lr = 1.f;
+    const float d = disp[q];
+    if (dr) {
+        const int xr = (int)roundf((float)x - d);
+        if (xr < 0 || xr >= w || d < 0.f) lr = 0.f;
+        else {
+            const float drv = dr[(size_t)y * w + xr];
+            if (drv < 0.f || fabsf(d - drv) > (float)p.conf_lr_max_diff + 0.5f) lr = 0.f;
+        }
+    }
+    const float tex = (float)((int)gx[q] + (int)gy[q]);
+    const float tex_term = clampf(tex / 16.f, 0.f, 1.f);
+    float conf = clampf(uniq, 0.f, 4.f);
+    conf = conf * (0.5f + 0.5f * tex_term);
+    if (lr < 0.5f) conf *= 0.25f;
+    conf_out[q] = conf;
+    const bool lr_ok = lr >= 0.5f;
+    const bool rel = lr_ok && conf >= p.conf_reliable_threshold && uniq >= p.conf_uniqueness_ratio &&
+                     tex >= p.conf_min_texture && d >= 0.f;
+    reliable[q] = rel ? 1 : 0;
+    reason[q] = rel ? R_VALID : (lr_ok ? R_LOWCONF : R_MISMATCH);
+}
+
+hipError_t launch_confidence(hipStream_t s, const apg_hip_params& p, int w, int h, const float* disp,
+                             const float* dr, const uint8_t* gx, const uint8_t* gy, const float* best,
+                             const float* second, float* conf, uint8_t* reliable, uint8_t* reason) {
+    hipLaunchKernelGGL(confidence_kernel, dim3((w + 127) / 128, h), dim3(128), 0, s, p, w, h, disp, dr, gx, gy, best,
+                       second, conf, reliable, reason);
+    return hipGetLastError();
+}
+
+// PostProcessor::left_right_check (reference src/postprocess.cpp:10-40)
+__global__ void lr_kernel(int w, int h, int lr_max_diff, float* __restrict__ disp, const float* __restrict__ dr,
+                          uint8_t* __restrict__ reason) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= w) return;
+    const size_t q = (size_t)y * w + x;
+    const float tol = (float)lr_max_diff + 0.5f;
+    const float d = disp[q];
+    if (d < 0.f) return;
+    const int xr = (int)roundf((float)x - d);
+    if (xr < 0 || xr >= w) { disp[q] = -1.f; reason[q] = R_OCCLUSION; return; }
+    const float drv = dr[(size_t)y * w + xr];
+    if (drv < 0.f || fabsf(d - drv) > tol) {
+        disp[q] = -1.f;
+        reason[q] = (drv > d + tol) ? R_OCCLUSION : R_MISMATCH;
+    }
+}
+
+hipError_t launch_lr_check(hipStream_t s, const apg_hip_params& p, int w, int h, float* disp, const float* dr,
+                           uint8_t* reason) {
+    if (!p.post_lr_check || !dr) return hipSuccess;
+    hipLaunchKernelGGL(lr_kernel, dim3((w + 127) / 128, h), dim3(128), 0, s, w, h, p.post_lr_max_diff, disp, dr, reason);
+    return hipGetLastError();
+}
+
+// PostProcessor::fill_holes, Legacy route (reference src/postprocess.cpp:42-168):
+// reads `in` (the LR-checked disparity), writes `out`.
+__global__ void fill_kernel(int w, int h, int max_gap, const float* __restrict__ in, const uint8_t* __restrict__ reason,
+                            const uint8_t* __restrict__ gray, const float* __restrict__ prior, float* __restrict__ out) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= w) return;
+    const float* row = in + (size_t)y * w;
+    const size_t q = (size_t)y * w + x;
+    float o = row[x];
+    if (o < 0.f) {
+        int xl = x - 1;
+        while (xl >= 0 && xl >= x - max_gap && row[xl] < 0.f) --xl;
+        int xr = x + 1;
+        while (xr < w && xr <= x + max_gap && row[xr] < 0.f) ++xr;
+        const float dl = (xl >= 0 && row[xl] >= 0.f) ? row[xl] : -1.f;
+        const float drr = (xr < w && row[xr] >= 0.f) ? row[xr] : -1.f;
+        if (reason[q] == R_OCCLUSION) {
+            if (dl >= 0.f && drr >= 0.f) o = drr < dl ? drr : dl;
+            else if (dl >= 0.f) o = dl;
+            else if (drr >= 0.f) o = drr;
+        } else {
+            const int c0 = gray[q];
+            float best_d = -1.f;
+            int best_abs = 999;
+            const int cx[4] = {xl, xr, x, x};
+            const int cy[4] = {y, y, y - 1, y + 1};
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int xx = cx[k], yy = cy[k];
+                if (xx < 0 || yy < 0 || xx >= w || yy >= h) continue;
+                const float dd = in[(size_t)yy * w + xx];
+                if (dd < 0.f) continue;
+                const int ad = abs((int)gray[(size_t)yy * w + xx] - c0);
+                if (ad < best_abs) { best_abs = ad; best_d = dd; }
+            }
+            if (best_d >= 0.f && best_abs <= 18) o = best_d;
+            else if (prior && prior[q] >= 0.f) o = prior[q];
+            else if (dl >= 0.f && drr >= 0.f) {
+                const float t = (float)(x - xl) / (float)(xr - xl);
+                o = dl * (1.f - t) + drr * t;
+            } else if (dl >= 0.f) o = dl;
+            else if (drr >= 0.f) o = drr;
+        }
+    }
+    out[q] = o;
+}
+
+// PostProcessor::median (reference src/postprocess.cpp:255-309): the element
+// at position size/2 of the sorted valid window.
+constexpr int kMaxMedianR = 3;
+__global__ void median_kernel(int w, int h, int r, const float* __restrict__ in, float* __restrict__ out) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= w) return;
+    float win[(2 * kMaxMedianR + 1) * (2 * kMaxMedianR + 1)];
+    int n = 0;
+    for (int oy = -r; oy <= r; ++oy)
+        for (int ox = -r; ox <= r; ++ox) {
+            const int xx = x + ox, yy = y + oy;
+            if (xx < 0 || yy < 0 || xx >= w || yy >= h) continue;
+            const float d = in[(size_t)yy * w + xx];
+            if (d >= 0.f) win[n++] = d;
+        }
+    const size_t q = (size_t)y * w + x;
+    if (n == 0) { out[q] = in[q]; return; }
+    for (int i = 1; i < n; ++i) {
+        const float v = win[i];
+        int j = i - 1;
+        while (j >= 0 && win[j] > v) { win[j + 1] = win[j]; --j; }
+        win[j + 1] = v;
+    }
+    out[q] = win[n / 2];
+}
+
+hipError_t launch_fill_median(hipStream_t s, const apg_hip_params& p, int w, int h, float* disp,
+                              const uint8_t* reason, const uint8_t* gray, const float* prior, float* scratch) {
+    dim3 grid((w + 127) / 128, h), block(128);
+    hipLaunchKernelGGL(fill_kernel, grid, block, 0, s, w, h, p.post_max_fill_gap, disp, reason, gray, prior, scratch);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if (p.post_median_radius > 0) {
+        hipLaunchKernelGGL(median_kernel, grid, block, 0, s, w, h, p.post_median_radius, scratch, disp);
+    } else {
+        (void)hipMemcpyAsync(disp, scratch, sizeof(float) * (size_t)w * h, hipMemcpyDeviceToDevice, s);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace apgk

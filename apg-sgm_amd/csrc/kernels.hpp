@@ -1,0 +1,63 @@
+// Host-side launchers of the gfx950 kernels (internal; the exported C ABI is
+// include/apg_sgm_hip.h). All launchers enqueue on `s` and return
+// hipSuccess or the launch error.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/apg_sgm_hip.h"
+
+namespace apgk {
+
+// Cost LUTs built on the host with the same libm expf as the reference
+// (reference src/discrete_cost_lut.hpp:10-38): census[65], ad[256], grad[511].
+struct CostLut {
+    float census[65];
+    float ad[256];
+    float grad[511];
+    float scale;
+    int pad[1];
+};
+
+void build_cost_lut(const apg_hip_params& p, CostLut* out);
+
+// Device-side limits of this build (checked by apg_hip_validate_params).
+constexpr int kMaxArm = 62;           // Cross arm length; window <= 125 columns/rows
+constexpr int kMaxD = 1024;           // disparity levels
+constexpr int kMaxSgmK = 4;           // packed u16 pairs per lane in the SGM scan (D <= 512)
+
+hipError_t launch_aux(hipStream_t s, const apg_hip_params& p, const uint8_t* src, int w, int h, int stride,
+                      int ch, uint8_t* gray, uint8_t* gx, uint8_t* gy, uint32_t* c32, uint64_t* c64);
+hipError_t launch_range_init(hipStream_t s, const apg_hip_params& p, int w, int h, int16_t* dmin, int16_t* dmax);
+hipError_t launch_cost(hipStream_t s, const apg_hip_params& p, const CostLut* d_lut, int w, int h,
+                       const uint8_t* gl, const uint8_t* gxl, const uint8_t* gyl,
+                       const uint8_t* gr, const uint8_t* gxr, const uint8_t* gyr,
+                       const uint32_t* c32l, const uint32_t* c32r, const uint64_t* c64l, const uint64_t* c64r,
+                       const int16_t* dmin, const int16_t* dmax, uint16_t* cost);
+hipError_t launch_right_wta(hipStream_t s, const apg_hip_params& p, int w, int h, const uint16_t* cost, float* dr);
+hipError_t launch_cross_arms(hipStream_t s, const apg_hip_params& p, const uint8_t* gray, int w, int h, uint32_t* arms);
+hipError_t launch_cross_h(hipStream_t s, const apg_hip_params& p, int w, int h, const int16_t* dmin,
+                          const int16_t* dmax, const uint32_t* arms, const uint16_t* cost, uint16_t* tmp);
+hipError_t launch_cross_v(hipStream_t s, const apg_hip_params& p, int w, int h, const int16_t* dmin,
+                          const int16_t* dmax, const uint32_t* arms, const uint16_t* tmp, uint16_t* cost);
+// One SGM direction over every scanline of that direction. mode 0: S = L
+// (first path), 1: S = sat(S + L).
+hipError_t launch_sgm_dir(hipStream_t s, const apg_hip_params& p, int w, int h, int D, const uint8_t* gray,
+                          const uint16_t* cost, uint16_t* sum, int dx, int dy, int mode);
+// One SGM direction added into a u32 accumulator with the reference's
+// invalid rule (used by the per-path class API).
+hipError_t launch_sgm_dir32(hipStream_t s, const apg_hip_params& p, int w, int h, int D, const uint8_t* gray,
+                            const uint16_t* cost, uint32_t* acc, int dx, int dy);
+hipError_t launch_widen16(hipStream_t s, size_t n, const uint16_t* in, uint32_t* out);
+hipError_t launch_wta(hipStream_t s, const apg_hip_params& p, int w, int h, const void* vol, int bits,
+                      const int16_t* dmin, const int16_t* dmax, float* disp, float* best, float* second);
+hipError_t launch_confidence(hipStream_t s, const apg_hip_params& p, int w, int h, const float* disp,
+                             const float* dr, const uint8_t* gx, const uint8_t* gy, const float* best,
+                             const float* second, float* conf, uint8_t* reliable, uint8_t* reason);
+hipError_t launch_lr_check(hipStream_t s, const apg_hip_params& p, int w, int h, float* disp, const float* dr,
+                           uint8_t* reason);
+hipError_t launch_fill_median(hipStream_t s, const apg_hip_params& p, int w, int h, float* disp,
+                              const uint8_t* reason, const uint8_t* gray, const float* prior, float* scratch);
+
+}  // namespace apgk

@@ -1,0 +1,229 @@
+/*
+ * apg_sgm_hip.h -- thin C-ABI device layer of the MI355X (gfx950) build of the
+ * apg-sgm dense-stereo hot path.
+ *
+ * Every entry point takes plain pointers and sizes, returns an
+ * apg_status_code_t (0 = OK), never throws, and is stream-ordered on the
+ * context's HIP stream.  Pointers named d_* are device (HBM) pointers owned
+ * by the caller or by the context's workspace.  Volumes use the reference's
+ * [y][x][d] layout (include/apg_sgm/buffers.hpp:45-47 in the reference):
+ * element (x, y, d) lives at ((y * w + x) * D + (d - min_disparity)).
+ *
+ * Each stage function cites the reference interface it replaces; the C++
+ * stage classes (include/apg_sgm/ headers) and the session C ABI
+ * (include/apg_sgm_c_api.h) are built on top of these calls.
+ */
+#ifndef APG_SGM_HIP_H
+#define APG_SGM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#if defined(__GNUC__) && __GNUC__ >= 4
+#define APG_HIP_API __attribute__((visibility("default")))
+#else
+#define APG_HIP_API
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Stage parameters: the fields of PipelineConfig (reference
+ * include/apg_sgm/config.hpp:7-112) that the hot path reads, as plain ints
+ * and floats. Fill with apg_hip_params_from_mode() and override. */
+typedef struct {
+    int mode;               /* 0 Fast, 1 Balanced, 2 HighQuality */
+    int min_disparity;
+    int max_disparity;
+    int census_type;        /* 0 Census9x7 (64-bit), 1 SymmetricCensus9x7 */
+    float lambda_census, lambda_ad, lambda_grad, eta_ad, mu_grad;
+    int use_ad, use_grad, cost_max;
+    int agg_enable, max_arm_length, color_threshold, agg_iterations;
+    int paths;              /* 4 or 8 */
+    int P1, P2_base, adaptive_penalties, grad_t1, grad_t2;
+    float p2_alpha;
+    int piecewise_p2;
+    float conf_uniqueness_ratio;
+    int conf_lr_max_diff;
+    float conf_min_texture, conf_reliable_threshold;
+    int post_subpixel, post_lr_check, post_lr_max_diff, post_median_radius, post_max_fill_gap;
+} apg_hip_params;
+
+typedef struct apg_hip_ctx_s* apg_hip_ctx;
+
+/* Per-stage device time of the last pipeline call, milliseconds (HIP events
+ * on the context stream; zero unless stage timing is enabled). Field names
+ * follow StageTiming (reference include/apg_sgm/pipeline_stats.hpp:10-22). */
+typedef struct {
+    double aux_ms, prior_ms, cost_ms, right_wta_ms, cross_ms, sgm_ms, wta_ms,
+           confidence_ms, post_ms, total_ms;
+} apg_hip_timing;
+
+/* PipelineConfig::from_mode (reference src/config.cpp:5-45). */
+APG_HIP_API void apg_hip_params_from_mode(int mode, int max_disparity, apg_hip_params* out);
+
+/* validate_config (reference src/pipeline.cpp:171-213) plus the device
+ * path's own representability limits; returns a status code. */
+APG_HIP_API int apg_hip_validate_params(const apg_hip_params* p);
+
+/* Context: one HIP stream at default priority on `device`, a device
+ * workspace and the LUTs.  No counterpart in the reference (its state lives
+ * in PipelineBuffers / StereoWorkspace, reference include/apg_sgm/workspace.hpp). */
+APG_HIP_API int apg_hip_ctx_create(int device, apg_hip_ctx* out);
+APG_HIP_API void apg_hip_ctx_destroy(apg_hip_ctx ctx);
+APG_HIP_API int apg_hip_synchronize(apg_hip_ctx ctx);
+APG_HIP_API void* apg_hip_stream(apg_hip_ctx ctx);          /* hipStream_t */
+APG_HIP_API size_t apg_hip_workspace_bytes(apg_hip_ctx ctx);
+APG_HIP_API void apg_hip_release_workspace(apg_hip_ctx ctx);
+APG_HIP_API void apg_hip_enable_stage_timing(apg_hip_ctx ctx, int on);
+APG_HIP_API int apg_hip_last_timing(apg_hip_ctx ctx, apg_hip_timing* out);
+APG_HIP_API const char* apg_hip_last_error(apg_hip_ctx ctx);
+
+/* Device memory plumbing (synchronous on the context stream). */
+APG_HIP_API int apg_hip_malloc(apg_hip_ctx ctx, size_t bytes, void** d_out);
+APG_HIP_API int apg_hip_free(apg_hip_ctx ctx, void* d_ptr);
+APG_HIP_API int apg_hip_upload(apg_hip_ctx ctx, void* d_dst, const void* h_src, size_t bytes);
+APG_HIP_API int apg_hip_download(apg_hip_ctx ctx, void* h_dst, const void* d_src, size_t bytes);
+APG_HIP_API int apg_hip_memset(apg_hip_ctx ctx, void* d_dst, int value, size_t bytes);
+
+/* ---- stages --------------------------------------------------------- */
+
+/* gray, |gx|, |gy| and census of one image: replaces
+ * CostComputer::build_gray_and_grad + build_census (reference
+ * src/cost_computer.cpp:63-122) as called by compute_aux (:124-130).
+ * d_src: strided u8 image (stride >= w*channels, channels 1 or 3).
+ * d_census32 (census_type 1) or d_census64 (census_type 0); the other may be NULL. */
+APG_HIP_API int apg_hip_aux(apg_hip_ctx ctx, const apg_hip_params* p,
+                            const uint8_t* d_src, int w, int h, int stride, int channels,
+                            uint8_t* d_gray, uint8_t* d_gx, uint8_t* d_gy,
+                            uint32_t* d_census32, uint64_t* d_census64);
+
+/* search range with the prior off: replaces PriorEstimator::estimate's range
+ * init (reference src/prior_estimator.cpp:414-424). */
+APG_HIP_API int apg_hip_range_init(apg_hip_ctx ctx, const apg_hip_params* p, int w, int h,
+                                   int16_t* d_dmin, int16_t* d_dmax);
+
+/* dense u16 cost volume: replaces CostComputer::compute_volume (reference
+ * src/cost_computer.cpp:139-203). */
+APG_HIP_API int apg_hip_cost_volume(apg_hip_ctx ctx, const apg_hip_params* p, int w, int h,
+                                    const uint8_t* d_gray_l, const uint8_t* d_gx_l, const uint8_t* d_gy_l,
+                                    const uint8_t* d_gray_r, const uint8_t* d_gx_r, const uint8_t* d_gy_r,
+                                    const uint32_t* d_census32_l, const uint32_t* d_census32_r,
+                                    const uint64_t* d_census64_l, const uint64_t* d_census64_r,
+                                    const int16_t* d_dmin, const int16_t* d_dmax, uint16_t* d_cost);
+
+/* right-view WTA on the raw volume: replaces wta_right_from_left_volume
+ * (reference src/pipeline.cpp:24-85). */
+APG_HIP_API int apg_hip_right_wta(apg_hip_ctx ctx, const apg_hip_params* p, int w, int h,
+                                  const uint16_t* d_cost, float* d_disp_right);
+
+/* Cross arms from the left gray: replaces CostAggregator::build_cross_arms
+ * (reference src/cost_aggregator.cpp:17-63). Arms are packed one byte each:
+ * left | right << 8 | up << 16 | down << 24. */
+APG_HIP_API int apg_hip_cross_arms(apg_hip_ctx ctx, const apg_hip_params* p,
+                                   const uint8_t* d_gray, int w, int h, uint32_t* d_arms);
+
+/* Cross H+V, in place on d_cost: replaces CostAggregator::aggregate
+ * (reference src/cost_aggregator.cpp:141-149 -> aggregate_hv :65-139).
+ * d_tmp: caller scratch of w*h*D u16 (the H-pass output T). No-op when
+ * p->agg_enable == 0. */
+APG_HIP_API int apg_hip_cross_aggregate(apg_hip_ctx ctx, const apg_hip_params* p, int w, int h,
+                                        const uint8_t* d_gray, const int16_t* d_dmin, const int16_t* d_dmax,
+                                        uint16_t* d_cost, uint16_t* d_tmp);
+
+/* SGM path sums: replaces SgmOptimizer::optimize (reference
+ * src/sgm_optimizer.cpp:261-274). Writes a u16 sum volume (d_sum16, invalid =
+ * 65535) and, when d_sum32 is not NULL, the reference's u32 layout
+ * (invalid = 0xFFFFFFFF) as well. */
+APG_HIP_API int apg_hip_sgm_optimize(apg_hip_ctx ctx, const apg_hip_params* p, int w, int h,
+                                     const uint8_t* d_gray, const uint16_t* d_cost,
+                                     uint16_t* d_sum16, uint32_t* d_sum32);
+
+/* one SGM path added into a u32 accumulator: replaces
+ * SgmOptimizer::aggregate_path (reference src/sgm_optimizer.cpp:186-259).
+ * D is the volume depth (the accumulator must already hold the previous paths). */
+APG_HIP_API int apg_hip_sgm_path(apg_hip_ctx ctx, const apg_hip_params* p, int w, int h, int D,
+                                 const uint8_t* d_gray, const uint16_t* d_cost, uint32_t* d_acc,
+                                 int dx, int dy);
+
+/* WTA + second best + legacy subpixel: replaces SgmOptimizer::winner_take_all
+ * (reference src/sgm_optimizer.cpp:436-555). vol_bits = 16 or 32.
+ * d_best / d_second may be NULL. */
+APG_HIP_API int apg_hip_wta(apg_hip_ctx ctx, const apg_hip_params* p, int w, int h,
+                            const void* d_vol, int vol_bits,
+                            const int16_t* d_dmin, const int16_t* d_dmax,
+                            float* d_disp, float* d_best, float* d_second);
+
+/* ConfidenceEstimator::estimate (reference src/confidence_estimator.cpp:7-63). */
+APG_HIP_API int apg_hip_confidence(apg_hip_ctx ctx, const apg_hip_params* p, int w, int h,
+                                   const float* d_disp, const float* d_disp_right,
+                                   const uint8_t* d_gx, const uint8_t* d_gy,
+                                   const float* d_best, const float* d_second,
+                                   float* d_confidence, uint8_t* d_reliable, uint8_t* d_invalid_reason);
+
+/* PostProcessor::left_right_check, in place (reference src/postprocess.cpp:10-40). */
+APG_HIP_API int apg_hip_lr_check(apg_hip_ctx ctx, const apg_hip_params* p, int w, int h,
+                                 float* d_disp, const float* d_disp_right, uint8_t* d_invalid_reason);
+
+/* PostProcessor::fill_holes (Legacy route) + median, in place (reference
+ * src/postprocess.cpp:42-309). d_prior may be NULL; d_scratch: w*h floats. */
+APG_HIP_API int apg_hip_fill_median(apg_hip_ctx ctx, const apg_hip_params* p, int w, int h,
+                                    float* d_disp, const uint8_t* d_invalid_reason,
+                                    const uint8_t* d_gray, const float* d_prior, float* d_scratch);
+
+/* ---- whole pipeline -------------------------------------------------- */
+
+/* Optional device intermediates of the pipeline (any may be NULL). Setting
+ * d_cost / d_cost_agg / d_sum32 selects the stage-by-stage (materialising)
+ * schedule; otherwise the fused schedule runs. */
+typedef struct {
+    uint16_t* d_cost;        /* raw C  (w*h*D) */
+    uint16_t* d_cost_agg;    /* C' after Cross (w*h*D) */
+    uint32_t* d_sum32;       /* S in the reference's u32 layout (w*h*D) */
+    float* d_disp_right;     /* w*h */
+    float* d_disp_wta;       /* w*h, WTA output before post-processing */
+    float* d_best;           /* w*h */
+    float* d_second;         /* w*h */
+} apg_hip_intermediates;
+
+/* StereoMatcher::compute_impl, dense branch (reference src/pipeline.cpp:435-605)
+ * on device inputs and outputs.  Intermediate buffers come from the context
+ * workspace.  d_left / d_right: strided u8 images. */
+APG_HIP_API int apg_hip_pipeline(apg_hip_ctx ctx, const apg_hip_params* p,
+                                 const uint8_t* d_left, const uint8_t* d_right,
+                                 int w, int h, int stride, int channels,
+                                 float* d_disparity, float* d_confidence,
+                                 uint8_t* d_invalid_reason, uint8_t* d_reliable,
+                                 const apg_hip_intermediates* inter);
+
+/* Timing harness (no counterpart in the reference; its benchmark times
+ * StereoMatcher::compute with steady_clock, apps/benchmark_stereo.cpp:537-575):
+ * `warmup` untimed pipeline calls, then `steps` calls bracketed by HIP events
+ * on the context stream.  *ms_total = device time of the timed steps.  When
+ * stage_avg is not NULL, a further `steps` calls run with per-stage events and
+ * their per-stage averages are returned there. */
+APG_HIP_API int apg_hip_bench_pipeline(apg_hip_ctx ctx, const apg_hip_params* p,
+                                       const uint8_t* d_left, const uint8_t* d_right,
+                                       int w, int h, int stride, int channels,
+                                       float* d_disparity, float* d_confidence,
+                                       uint8_t* d_invalid_reason, uint8_t* d_reliable,
+                                       int warmup, int steps, double* ms_total,
+                                       apg_hip_timing* stage_avg);
+
+/* ---- synthetic inputs ------------------------------------------------ */
+
+/* Seeded synthetic stereo pair (host memory).  kind 0: noise texture, flat
+ * background + 4 rectangles; kind 1: + background ramp, 8 rectangles and a
+ * right-image gain/bias; kind 2: smooth slanted planes + sinusoid.
+ * gt may be NULL. Integer-only PRNG (splitmix64). */
+APG_HIP_API int apg_synth_pair(int kind, int w, int h, int max_disparity, uint64_t seed,
+                               uint8_t* left, uint8_t* right, float* gt);
+
+APG_HIP_API const char* apg_hip_version_string(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* APG_SGM_HIP_H */
