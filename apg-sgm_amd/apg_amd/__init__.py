@@ -102,7 +102,8 @@ def load():
         "apg_hip_wta": (i, [vp, P, i, i, vp, i, vp, vp, vp, vp, vp]),
         "apg_hip_confidence": (i, [vp, P, i, i] + [vp] * 9),
         "apg_hip_lr_check": (i, [vp, P, i, i, vp, vp, vp]),
-        "apg_hip_fill_median": (i, [vp, P, i, i, vp, vp, vp, vp, vp]),
+        "apg_hip_fill_holes": (i, [vp, P, i, i, vp, vp, vp, vp, vp]),
+        "apg_hip_median": (i, [vp, P, i, i, vp, vp]),
         "apg_hip_pipeline": (i, [vp, P, vp, vp, i, i, i, i, vp, vp, vp, vp, C.POINTER(Intermediates)]),
         "apg_hip_bench_pipeline": (i, [vp, P, vp, vp, i, i, i, i, vp, vp, vp, vp, i, i, C.POINTER(C.c_double),
                                        C.POINTER(Timing)]),

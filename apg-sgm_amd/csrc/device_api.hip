@@ -361,10 +361,16 @@ int apg_hip_lr_check(apg_hip_ctx ctx, const apg_hip_params* p, int w, int h, flo
     return 0;
 }
 
-int apg_hip_fill_median(apg_hip_ctx ctx, const apg_hip_params* p, int w, int h, float* d_disp,
-                        const uint8_t* d_reason, const uint8_t* d_gray, const float* d_prior, float* d_scratch) {
+int apg_hip_fill_holes(apg_hip_ctx ctx, const apg_hip_params* p, int w, int h, const float* d_disp,
+                       const uint8_t* d_reason, const uint8_t* d_gray, const float* d_prior, float* d_out) {
     if (int s = check_ctx(ctx, p)) return s;
-    HIPCHK(launch_fill_median(ctx->stream, *p, w, h, d_disp, d_reason, d_gray, d_prior, d_scratch), "fill_median");
+    HIPCHK(launch_fill(ctx->stream, *p, w, h, d_disp, d_reason, d_gray, d_prior, d_out), "fill_holes");
+    return 0;
+}
+
+int apg_hip_median(apg_hip_ctx ctx, const apg_hip_params* p, int w, int h, const float* d_disp, float* d_out) {
+    if (int s = check_ctx(ctx, p)) return s;
+    HIPCHK(launch_median(ctx->stream, *p, w, h, d_disp, d_out), "median");
     return 0;
 }
 
@@ -462,7 +468,8 @@ int apg_hip_pipeline(apg_hip_ctx ctx, const apg_hip_params* p, const uint8_t* d_
            "confidence");
     mark(ctx, S_POST);
     HIPCHK(launch_lr_check(st, *p, w, h, d_disparity, dr, d_reason), "lr");
-    HIPCHK(launch_fill_median(st, *p, w, h, d_disparity, d_reason, gl, nullptr, scratch), "fill_median");
+    HIPCHK(launch_fill(st, *p, w, h, d_disparity, d_reason, gl, nullptr, scratch), "fill");
+    HIPCHK(launch_median(st, *p, w, h, scratch, d_disparity), "median");
     mark(ctx, S_COUNT);
     if (ctx->timing) {
         HIPCHK(hipEventSynchronize(ctx->ev[S_COUNT]), "event sync");

@@ -260,18 +260,20 @@ __global__ void median_kernel(int w, int h, int r, const float* __restrict__ in,
     out[q] = win[n / 2];
 }
 
-hipError_t launch_fill_median(hipStream_t s, const apg_hip_params& p, int w, int h, float* disp,
-                              const uint8_t* reason, const uint8_t* gray, const float* prior, float* scratch) {
-    dim3 grid((w + 127) / 128, h), block(128);
-    hipLaunchKernelGGL(fill_kernel, grid, block, 0, s, w, h, p.post_max_fill_gap, disp, reason, gray, prior, scratch);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    if (p.post_median_radius > 0) {
-        hipLaunchKernelGGL(median_kernel, grid, block, 0, s, w, h, p.post_median_radius, scratch, disp);
-    } else {
-        (void)hipMemcpyAsync(disp, scratch, sizeof(float) * (size_t)w * h, hipMemcpyDeviceToDevice, s);
-    }
+hipError_t launch_fill(hipStream_t s, const apg_hip_params& p, int w, int h, const float* disp, const uint8_t* reason,
+                       const uint8_t* gray, const float* prior, float* out) {
+    hipLaunchKernelGGL(fill_kernel, dim3((w + 127) / 128, h), dim3(128), 0, s, w, h, p.post_max_fill_gap, disp, reason,
+                       gray, prior, out);
     return hipGetLastError();
+}
+
+hipError_t launch_median(hipStream_t s, const apg_hip_params& p, int w, int h, const float* disp, float* out) {
+    if (p.post_median_radius > 0) {
+        hipLaunchKernelGGL(median_kernel, dim3((w + 127) / 128, h), dim3(128), 0, s, w, h, p.post_median_radius, disp,
+                           out);
+        return hipGetLastError();
+    }
+    return hipMemcpyAsync(out, disp, sizeof(float) * (size_t)w * h, hipMemcpyDeviceToDevice, s);
 }
 
 }  // namespace apgk

@@ -57,7 +57,9 @@ hipError_t launch_confidence(hipStream_t s, const apg_hip_params& p, int w, int 
                              const float* second, float* conf, uint8_t* reliable, uint8_t* reason);
 hipError_t launch_lr_check(hipStream_t s, const apg_hip_params& p, int w, int h, float* disp, const float* dr,
                            uint8_t* reason);
-hipError_t launch_fill_median(hipStream_t s, const apg_hip_params& p, int w, int h, float* disp,
-                              const uint8_t* reason, const uint8_t* gray, const float* prior, float* scratch);
+hipError_t launch_fill(hipStream_t s, const apg_hip_params& p, int w, int h, const float* disp, const uint8_t* reason,
+                       const uint8_t* gray, const float* prior, float* out);
+// median_radius 0 copies disp to out.
+hipError_t launch_median(hipStream_t s, const apg_hip_params& p, int w, int h, const float* disp, float* out);
 
 }  // namespace apgk

@@ -166,11 +166,16 @@ APG_HIP_API int apg_hip_confidence(apg_hip_ctx ctx, const apg_hip_params* p, int
 APG_HIP_API int apg_hip_lr_check(apg_hip_ctx ctx, const apg_hip_params* p, int w, int h,
                                  float* d_disp, const float* d_disp_right, uint8_t* d_invalid_reason);
 
-/* PostProcessor::fill_holes (Legacy route) + median, in place (reference
- * src/postprocess.cpp:42-309). d_prior may be NULL; d_scratch: w*h floats. */
-APG_HIP_API int apg_hip_fill_median(apg_hip_ctx ctx, const apg_hip_params* p, int w, int h,
-                                    float* d_disp, const uint8_t* d_invalid_reason,
-                                    const uint8_t* d_gray, const float* d_prior, float* d_scratch);
+/* PostProcessor::fill_holes, Legacy route (reference src/postprocess.cpp:42-253):
+ * reads d_disp, writes d_out (distinct buffers). d_prior may be NULL. */
+APG_HIP_API int apg_hip_fill_holes(apg_hip_ctx ctx, const apg_hip_params* p, int w, int h,
+                                   const float* d_disp, const uint8_t* d_invalid_reason,
+                                   const uint8_t* d_gray, const float* d_prior, float* d_out);
+
+/* PostProcessor::median, valid-only (reference src/postprocess.cpp:255-309):
+ * reads d_disp, writes d_out (distinct buffers). */
+APG_HIP_API int apg_hip_median(apg_hip_ctx ctx, const apg_hip_params* p, int w, int h,
+                               const float* d_disp, float* d_out);
 
 /* ---- whole pipeline -------------------------------------------------- */
 

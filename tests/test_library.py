@@ -1,0 +1,91 @@
+"""CPU checks of the shared library: it loads, exports every symbol that
+include/*.h declares, its presets equal the reference's (via the oracle's
+restatement), parameter validation and the synthetic generator."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import apg_amd as A
+import pyoracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = set()
+    for fn in ("apg_sgm_hip.h", "apg_sgm_c_api.h"):
+        with open(os.path.join(ROOT, "include", fn)) as f:
+            txt = f.read()
+        for m in re.finditer(r"APG_(?:HIP_)?API\s+[^;(]*?\b(apg_\w+)\s*\(", txt, re.S):
+            names.add(m.group(1))
+    return sorted(names)
+
+
+def test_library_exports_every_declared_symbol():
+    names = declared_symbols()
+    assert len(names) >= 40
+    out = subprocess.run(["nm", "-D", "--defined-only", A.LIB_PATH], capture_output=True, text=True, check=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if line.strip()}
+    missing = [n for n in names if n not in exported]
+    assert not missing, missing
+    lib = A.load()
+    for n in names:
+        assert getattr(lib, n) is not None
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+@pytest.mark.parametrize("dmax", [16, 128, 256])
+def test_presets_match_reference_from_mode(mode, dmax):
+    p = A.params(mode, dmax)
+    c = O.cfg(mode, dmax)
+    assert p.as_dict() == c.as_dict()
+
+
+def test_validate_params_codes():
+    lib = A.load()
+
+    def v(**over):
+        p = A.params(1, 128)
+        for k, val in over.items():
+            setattr(p, k, val)
+        return lib.apg_hip_validate_params(C.byref(p))
+    assert v() == 0
+    assert v(max_disparity=0) == 3
+    assert v(P2_base=5, P1=10) == 4
+    assert v(P1=-1) == 4
+    assert v(grad_t1=30, grad_t2=10) == 4
+    assert v(cost_max=70000) == 4
+    # u16 path sums need npath*(cost_max + P2max) < 65535 (the reference's own u16 criterion)
+    assert v(cost_max=20000) == 5
+    assert v(max_disparity=1024) == 5
+    assert v(max_arm_length=100) == 5
+
+
+def test_synth_deterministic_and_shaped():
+    a = A.synth_pair(1, 320, 240, 96, 7)
+    b = A.synth_pair(1, 320, 240, 96, 7)
+    c = A.synth_pair(1, 320, 240, 96, 8)
+    for x, y in zip(a, b):
+        np.testing.assert_array_equal(x, y)
+    assert not np.array_equal(a[0], c[0])
+    left, right, gt = a
+    assert left.shape == (240, 320) and gt.min() >= 4 and gt.max() <= 96 - 8
+    # the right image is the left shifted by the ground truth where visible
+    l2, r2, g2 = A.synth_pair(0, 200, 100, 64, 3)
+    y, x = 50, 150
+    d = int(g2[y, x])
+    assert abs(int(l2[y, x]) - int(r2[y, x - d])) <= 16
+
+
+def test_no_gpu_fails_loudly():
+    """Without a visible GPU the device entry points return an error (no CPU fallback)."""
+    try:
+        ctx = A.Context(0)
+    except A.ApgError:
+        return
+    ctx.close()
+    pytest.skip("a GPU is visible here")
