@@ -105,6 +105,7 @@ def load():
         "apg_hip_fill_holes": (i, [vp, P, i, i, vp, vp, vp, vp, vp]),
         "apg_hip_median": (i, [vp, P, i, i, vp, vp]),
         "apg_hip_pipeline": (i, [vp, P, vp, vp, i, i, i, i, vp, vp, vp, vp, C.POINTER(Intermediates)]),
+        "apg_hip_set_schedule": (None, [vp, i]),
         "apg_hip_bench_pipeline": (i, [vp, P, vp, vp, i, i, i, i, vp, vp, vp, vp, i, i, C.POINTER(C.c_double),
                                        C.POINTER(Timing)]),
         "apg_synth_pair": (i, [i, i, i, i, C.c_uint64, vp, vp, vp]),
@@ -233,9 +234,20 @@ class PipelineOutputs:
     second: np.ndarray | None = None
 
 
+ALL_INTERMEDIATES = ("cost", "cost_agg", "sgm", "disp_right", "disp_wta", "best", "second")
+# requesting the raw (cost) or post-Cross (cost_agg) volume selects the stage-by-stage schedule
+FUSED_INTERMEDIATES = ("sgm", "disp_right", "disp_wta", "best", "second")
+
+
+def set_schedule(ctx: "Context", schedule: int):
+    """0: fused schedule when eligible (default); 1: always stage by stage."""
+    ctx.lib.apg_hip_set_schedule(ctx.h, schedule)
+
+
 def run_pipeline(ctx: Context, p: Params, left: np.ndarray, right: np.ndarray, intermediates=False) -> PipelineOutputs:
-    """Upload a pair, run apg_hip_pipeline, download results (and, optionally,
-    the materialised intermediates C, C', S, d_R, WTA output, best, second)."""
+    """Upload a pair, run apg_hip_pipeline, download results and, optionally,
+    materialised intermediates: True = all of ALL_INTERMEDIATES, or an
+    iterable of names."""
     h, w = left.shape[:2]
     ch = 1 if left.ndim == 2 else left.shape[2]
     D = p.max_disparity - p.min_disparity
@@ -246,15 +258,13 @@ def run_pipeline(ctx: Context, p: Params, left: np.ndarray, right: np.ndarray, i
     out_m = ctx.array((h, w), np.uint8)
     inter = None
     arrs = {}
-    if intermediates:
-        arrs = {
-            "cost": ctx.array((h, w, D), np.uint16), "cost_agg": ctx.array((h, w, D), np.uint16),
-            "sgm": ctx.array((h, w, D), np.uint32), "disp_right": ctx.array((h, w), np.float32),
-            "disp_wta": ctx.array((h, w), np.float32), "best": ctx.array((h, w), np.float32),
-            "second": ctx.array((h, w), np.float32),
-        }
-        inter = Intermediates(arrs["cost"].ptr, arrs["cost_agg"].ptr, arrs["sgm"].ptr, arrs["disp_right"].ptr,
-                              arrs["disp_wta"].ptr, arrs["best"].ptr, arrs["second"].ptr)
+    want = ALL_INTERMEDIATES if intermediates is True else tuple(intermediates or ())
+    if want:
+        shapes = {"cost": ((h, w, D), np.uint16), "cost_agg": ((h, w, D), np.uint16), "sgm": ((h, w, D), np.uint32),
+                  "disp_right": ((h, w), np.float32), "disp_wta": ((h, w), np.float32), "best": ((h, w), np.float32),
+                  "second": ((h, w), np.float32)}
+        arrs = {k: ctx.array(*shapes[k]) for k in want}
+        inter = Intermediates(*[arrs[k].ptr if k in arrs else None for k in ALL_INTERMEDIATES])
     rc = ctx.lib.apg_hip_pipeline(ctx.h, C.byref(p), dl.ptr, dr.ptr, w, h, w * ch, ch, out_d.ptr, out_c.ptr,
                                   out_r.ptr, out_m.ptr, C.byref(inter) if inter is not None else None)
     ctx._check(rc, "apg_hip_pipeline")

@@ -19,7 +19,8 @@ struct DevBuf {
 
 enum BufId {
     B_SRC_L, B_SRC_R, B_GRAY_L, B_GX_L, B_GY_L, B_GRAY_R, B_GX_R, B_GY_R, B_CEN_L, B_CEN_R,
-    B_DMIN, B_DMAX, B_ARMS, B_COST, B_TMP, B_SUM, B_DR, B_BEST, B_SECOND, B_SCRATCH, B_LUT, B_COUNT
+    B_DMIN, B_DMAX, B_ARMS, B_COST, B_TMP, B_SUM, B_DR, B_BEST, B_SECOND, B_SCRATCH, B_LUT,
+    B_RKEYS, B_DS0, B_DS1, B_DS2, B_DS3, B_COUNT
 };
 
 enum StageId { S_AUX, S_PRIOR, S_COST, S_RWTA, S_CROSS, S_SGM, S_WTA, S_CONF, S_POST, S_COUNT };
@@ -37,6 +38,7 @@ struct apg_hip_ctx_s {
     CostLut lut_host{};
     bool lut_valid = false;
     apg_hip_params lut_params{};
+    int schedule = 0;   // 0 automatic (fused when eligible), 1 stage-by-stage
 };
 
 namespace {
@@ -90,6 +92,22 @@ int ensure_lut(apg_hip_ctx ctx, const apg_hip_params* p) {
 
 void mark(apg_hip_ctx ctx, int stage) {
     if (ctx->timing) (void)hipEventRecord(ctx->ev[stage], ctx->stream);
+}
+
+// Per-stage times from the stage events of the last pipeline call.
+int finish_timing(apg_hip_ctx ctx) {
+    if (ctx->timing) {
+        HIPCHK(hipEventSynchronize(ctx->ev[S_COUNT]), "event sync");
+        float ms[S_COUNT];
+        for (int i = 0; i < S_COUNT; ++i) (void)hipEventElapsedTime(&ms[i], ctx->ev[i], ctx->ev[i + 1]);
+        float tot = 0.f;
+        (void)hipEventElapsedTime(&tot, ctx->ev[0], ctx->ev[S_COUNT]);
+        apg_hip_timing& t = ctx->last;
+        t.aux_ms = ms[S_AUX]; t.prior_ms = ms[S_PRIOR]; t.cost_ms = ms[S_COST]; t.right_wta_ms = ms[S_RWTA];
+        t.cross_ms = ms[S_CROSS]; t.sgm_ms = ms[S_SGM]; t.wta_ms = ms[S_WTA]; t.confidence_ms = ms[S_CONF];
+        t.post_ms = ms[S_POST]; t.total_ms = tot;
+    }
+    return 0;
 }
 
 int check_ctx(apg_hip_ctx ctx, const apg_hip_params* p) {
@@ -430,6 +448,59 @@ int apg_hip_pipeline(apg_hip_ctx ctx, const apg_hip_params* p, const uint8_t* d_
            "aux R");
     mark(ctx, S_PRIOR);
     HIPCHK(launch_range_init(st, *p, w, h, dmin, dmax), "range");
+
+    // Fused schedule: the raw cost volume never reaches HBM, C' is stored as
+    // u8 when every reachable cost is <= 254, and the SGM keeps per-direction
+    // pair deltas (u8, needs 2*P2max <= 254) instead of u16 path sums.
+    const int p2max = p->P2_base > p->P1 + 1 ? p->P2_base : p->P1 + 1;
+    const bool want_raw = inter && (inter->d_cost || inter->d_cost_agg);
+    const bool fused = ctx->schedule != 1 && !want_raw && D % 2 == 0 && 2 * p2max <= 254 &&
+                       (!p->agg_enable || p->agg_iterations <= 1);
+    if (fused) {
+        const bool u8 = cost_upper_bound(ctx->lut_host, p->cost_max) <= 254;
+        const size_t esz = u8 ? 1 : 2;
+        const int nfam = p->paths == 8 ? 4 : 2;
+        HIPCHK(ensure(ctx, B_RKEYS, n * 4), "alloc");
+        for (int f = 0; f < nfam; ++f) HIPCHK(ensure(ctx, (BufId)(B_DS0 + f), nv), "alloc");
+        uint8_t* ds[4] = {buf<uint8_t>(ctx, B_DS0), buf<uint8_t>(ctx, B_DS1), buf<uint8_t>(ctx, B_DS2),
+                          buf<uint8_t>(ctx, B_DS3)};
+        uint32_t* rkeys = buf<uint32_t>(ctx, B_RKEYS);
+        void* cp = ctx->bufs[B_COST].ptr;   // C' (u8 or u16)
+        void* tt = ctx->bufs[B_TMP].ptr;    // T  (u8 or u16)
+        (void)esz;
+        if (p->agg_enable) HIPCHK(launch_cross_arms(st, *p, gl, w, h, arms), "arms");
+        HIPCHK(launch_fill_u32(st, n, 0xFFFFFFFFu, rkeys), "keys");
+        mark(ctx, S_COST);
+        HIPCHK(launch_cost_crossh(st, *p, buf<CostLut>(ctx, B_LUT), w, h, gl, gxl, gyl, gr, gxr, gyr,
+                                  sym ? (const uint32_t*)cl : nullptr, sym ? (const uint32_t*)cr : nullptr,
+                                  sym ? nullptr : (const uint64_t*)cl, sym ? nullptr : (const uint64_t*)cr, dmin,
+                                  dmax, arms, p->agg_enable ? tt : cp, u8, rkeys),
+               "cost_crossh");
+        mark(ctx, S_RWTA);
+        HIPCHK(launch_rwta_finalize(st, *p, buf<CostLut>(ctx, B_LUT), w, h, gl, gxl, gyl, gr, gxr, gyr,
+                                    sym ? (const uint32_t*)cl : nullptr, sym ? (const uint32_t*)cr : nullptr,
+                                    sym ? nullptr : (const uint64_t*)cl, sym ? nullptr : (const uint64_t*)cr, dmin,
+                                    dmax, rkeys, dr),
+               "rwta_finalize");
+        mark(ctx, S_CROSS);
+        if (p->agg_enable) HIPCHK(launch_cross_v_t(st, *p, w, h, dmin, dmax, arms, tt, cp, u8), "cross_v");
+        mark(ctx, S_SGM);
+        HIPCHK(launch_sgm_pairs(st, *p, w, h, D, gl, cp, u8, ds), "sgm_pairs");
+        mark(ctx, S_WTA);
+        HIPCHK(launch_wta_fused(st, *p, w, h, cp, u8, ds, dmin, dmax, dr, gxl, gyl, d_disparity, d_confidence, d_reason,
+                                d_reliable, inter ? inter->d_disp_wta : nullptr, inter ? inter->d_best : nullptr,
+                                inter ? inter->d_second : nullptr, inter ? inter->d_sum32 : nullptr),
+               "wta_fused");
+        if (inter && inter->d_disp_right)
+            HIPCHK(hipMemcpyAsync(inter->d_disp_right, dr, n * 4, hipMemcpyDeviceToDevice, st), "copy dr");
+        mark(ctx, S_CONF);
+        mark(ctx, S_POST);
+        HIPCHK(launch_fill(st, *p, w, h, d_disparity, d_reason, gl, nullptr, scratch), "fill");
+        HIPCHK(launch_median(st, *p, w, h, scratch, d_disparity), "median");
+        mark(ctx, S_COUNT);
+        return finish_timing(ctx);
+    }
+
     mark(ctx, S_COST);
     HIPCHK(launch_cost(st, *p, buf<CostLut>(ctx, B_LUT), w, h, gl, gxl, gyl, gr, gxr, gyr,
                        sym ? (const uint32_t*)cl : nullptr, sym ? (const uint32_t*)cr : nullptr,
@@ -471,18 +542,11 @@ int apg_hip_pipeline(apg_hip_ctx ctx, const apg_hip_params* p, const uint8_t* d_
     HIPCHK(launch_fill(st, *p, w, h, d_disparity, d_reason, gl, nullptr, scratch), "fill");
     HIPCHK(launch_median(st, *p, w, h, scratch, d_disparity), "median");
     mark(ctx, S_COUNT);
-    if (ctx->timing) {
-        HIPCHK(hipEventSynchronize(ctx->ev[S_COUNT]), "event sync");
-        float ms[S_COUNT];
-        for (int i = 0; i < S_COUNT; ++i) (void)hipEventElapsedTime(&ms[i], ctx->ev[i], ctx->ev[i + 1]);
-        float tot = 0.f;
-        (void)hipEventElapsedTime(&tot, ctx->ev[0], ctx->ev[S_COUNT]);
-        apg_hip_timing& t = ctx->last;
-        t.aux_ms = ms[S_AUX]; t.prior_ms = ms[S_PRIOR]; t.cost_ms = ms[S_COST]; t.right_wta_ms = ms[S_RWTA];
-        t.cross_ms = ms[S_CROSS]; t.sgm_ms = ms[S_SGM]; t.wta_ms = ms[S_WTA]; t.confidence_ms = ms[S_CONF];
-        t.post_ms = ms[S_POST]; t.total_ms = tot;
-    }
-    return 0;
+    return finish_timing(ctx);
+}
+
+void apg_hip_set_schedule(apg_hip_ctx ctx, int schedule) {
+    if (ctx) ctx->schedule = schedule;
 }
 
 int apg_hip_bench_pipeline(apg_hip_ctx ctx, const apg_hip_params* p, const uint8_t* d_left, const uint8_t* d_right,

@@ -38,13 +38,16 @@ __device__ __forceinline__ uint32_t wave_shr1(uint32_t v, uint32_t fill) { retur
 __device__ __forceinline__ uint32_t wave_shl1(uint32_t v, uint32_t fill) { return dpp<kDppWaveShl1>(fill, v); }
 
 // Minimum over the 64 lanes, returned wave-uniform. Requires all lanes active.
+// Lanes without a DPP source read the min identity, so each level compiles
+// to one v_min_u32_dpp (row_shr 1/2/4/8 within rows of 16, then the row
+// broadcasts 15 and 31 leave the total in lane 63).
 __device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
-    v = min(v, dpp<kDppRowShr1>(v, v));
-    v = min(v, dpp<kDppRowShr2>(v, v));
-    v = min(v, dpp<kDppRowShr4>(v, v));
-    v = min(v, dpp<kDppRowShr8>(v, v));
-    v = min(v, dpp<kDppRowBcast15, 0xA>(v, v));
-    v = min(v, dpp<kDppRowBcast31, 0xC>(v, v));
+    v = min(v, dpp<kDppRowShr1>(0xFFFFFFFFu, v));
+    v = min(v, dpp<kDppRowShr2>(0xFFFFFFFFu, v));
+    v = min(v, dpp<kDppRowShr4>(0xFFFFFFFFu, v));
+    v = min(v, dpp<kDppRowShr8>(0xFFFFFFFFu, v));
+    v = min(v, dpp<kDppRowBcast15, 0xA>(0xFFFFFFFFu, v));
+    v = min(v, dpp<kDppRowBcast31, 0xC>(0xFFFFFFFFu, v));
     return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
 }
 
@@ -76,6 +79,30 @@ __device__ __forceinline__ float legacy_delta(float cm, float c0, float cp) {
         return delta;
     }
     return 0.0f;
+}
+
+// Per-pixel cost descriptor: census (hi word only for the 64-bit census),
+// gray | gx << 8 | gy << 16.
+struct PixDesc {
+    uint32_t clo, chi;
+    uint32_t ggg;
+};
+
+// Quantised matching cost of one state (reference src/cost_computer.cpp:176-199):
+// c = census_term + ad_term + grad_term in that order, q = (int)(c*scale + 0.5f),
+// clamped to [0, cost_max].  lut_* hold the host-built float terms.
+template <bool SYM>
+__device__ __forceinline__ uint16_t state_cost(const PixDesc& L, const PixDesc& R, const float* lut_c,
+                                               const float* lut_a, const float* lut_g, float scale, int cost_max,
+                                               int use_ad, int use_grad) {
+    const int pop = SYM ? __popc(L.clo ^ R.clo) : (__popc(L.clo ^ R.clo) + __popc(L.chi ^ R.chi));
+    float c = lut_c[pop];
+    if (use_ad) c = c + lut_a[abs((int)(L.ggg & 0xFF) - (int)(R.ggg & 0xFF))];
+    if (use_grad)
+        c = c + lut_g[abs((int)((L.ggg >> 8) & 0xFF) - (int)((R.ggg >> 8) & 0xFF)) +
+                      abs((int)((L.ggg >> 16) & 0xFF) - (int)((R.ggg >> 16) & 0xFF))];
+    const int q = (int)(c * scale + 0.5f);
+    return (uint16_t)clampi(q, 0, cost_max);
 }
 
 }  // namespace apgk

@@ -25,6 +25,18 @@ void build_cost_lut(const apg_hip_params& p, CostLut* out) {
     out->pad[0] = 0;
 }
 
+// Upper bound of every quantised cost: float addition and the final
+// quantisation are monotone, so the sum of the three LUT maxima bounds c.
+int cost_upper_bound(const CostLut& lut, int cost_max) {
+    float mc = 0.f, ma = 0.f, mg = 0.f;
+    for (float v : lut.census) mc = v > mc ? v : mc;
+    for (float v : lut.ad) ma = v > ma ? v : ma;
+    for (float v : lut.grad) mg = v > mg ? v : mg;
+    const float c = mc + ma + mg;
+    const int q = (int)(c * lut.scale + 0.5f);
+    return q < 0 ? 0 : (q > cost_max ? cost_max : q);
+}
+
 // ------------------------------------------------------------------ aux ----
 constexpr int AUX_TX = 64, AUX_TY = 4;
 

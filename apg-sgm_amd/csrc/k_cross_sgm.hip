@@ -102,10 +102,12 @@ constexpr int CV_PF = 16;   // rows fetched per batch
 // One wavefront per (column, 64-wide d chunk) slides down the column keeping
 // the running packed prefix P(k) = sum over rows < k in an LDS ring of
 // RS >= 2*Lmax + 2 entries; rows are fetched CV_PF at a time.
+template <typename T>
 __global__ __launch_bounds__(64) void cross_v_kernel(int w, int h, int d0, int D, int Lmax, int rs_mask,
                                                      const int16_t* __restrict__ dmin, const int16_t* __restrict__ dmax,
                                                      const uint32_t* __restrict__ arms,
-                                                     const uint16_t* __restrict__ tmp, uint16_t* __restrict__ cost) {
+                                                     const T* __restrict__ tmp, T* __restrict__ cost) {
+    constexpr T INV = (T)~(T)0;
     extern __shared__ uint32_t sh[];
     uint32_t* magic = sh;                  // [2*kMaxArm + 2]
     uint32_t* ring = sh + 2 * kMaxArm + 2; // [RS * 64]
@@ -126,8 +128,8 @@ __global__ __launch_bounds__(64) void cross_v_kernel(int w, int h, int d0, int D
             v[i] = 0;
             if (k < h && act) {
                 const size_t q = (size_t)k * w + x;
-                const uint16_t c = tmp[q * D + di];
-                if (c != kInv16 && d >= dmin[q] && d < dmax[q]) v[i] = ((uint32_t)c << 8) | 1u;
+                const T c = tmp[q * D + di];
+                if (c != INV && d >= dmin[q] && d < dmax[q]) v[i] = ((uint32_t)c << 8) | 1u;
             }
         }
 #pragma unroll
@@ -142,11 +144,11 @@ __global__ __launch_bounds__(64) void cross_v_kernel(int w, int h, int d0, int D
                 const size_t q = (size_t)y * w + x;
                 const uint32_t a = arms[q];
                 const int U = (a >> 16) & 0xFF, Dn = a >> 24;
-                uint16_t out = kInv16;
+                T out = INV;
                 if (d >= dmin[q] && d < dmax[q]) {
                     const uint32_t s = ring[((y + Dn + 1) & rs_mask) * 64 + lane] - ring[((y - U) & rs_mask) * 64 + lane];
                     const uint32_t cnt = s & 0xFF;
-                    if (cnt) out = (uint16_t)div_small(s >> 8, cnt, magic[cnt]);
+                    if (cnt) out = (T)div_small(s >> 8, cnt, magic[cnt]);
                 }
                 cost[q * D + di] = out;
             }
@@ -154,17 +156,148 @@ __global__ __launch_bounds__(64) void cross_v_kernel(int w, int h, int d0, int D
     }
 }
 
-hipError_t launch_cross_v(hipStream_t s, const apg_hip_params& p, int w, int h, const int16_t* dmin,
-                          const int16_t* dmax, const uint32_t* arms, const uint16_t* tmp, uint16_t* cost) {
+// Two disparities of one pixel as u16 halves (invalid -> 65535).
+template <typename T>
+__device__ __forceinline__ uint32_t load_pair_any(const T* __restrict__ v, int d, int D);
+template <>
+__device__ __forceinline__ uint32_t load_pair_any<uint16_t>(const uint16_t* __restrict__ v, int d, int D) {
+    if (d + 1 < D) return *(const uint32_t*)(v + d);
+    return d < D ? (uint32_t)v[d] | 0xFFFF0000u : 0xFFFFFFFFu;
+}
+template <>
+__device__ __forceinline__ uint32_t load_pair_any<uint8_t>(const uint8_t* __restrict__ v, int d, int D) {
+    uint32_t b0, b1;
+    if (d + 1 < D && !(D & 1)) {   // even D: the pair is 2-byte aligned
+        const uint32_t b = *(const uint16_t*)(v + d);
+        b0 = b & 0xFF;
+        b1 = b >> 8;
+    } else {
+        b0 = d < D ? v[d] : 0xFFu;
+        b1 = d + 1 < D ? v[d + 1] : 0xFFu;
+    }
+    return (b0 == 0xFFu ? 0xFFFFu : b0) | ((b1 == 0xFFu ? 0xFFFFu : b1) << 16);
+}
+
+constexpr int CV2_PF = 16;
+
+// Vertical Cross pass, sliding form: one wavefront per (column, 128-wide d
+// chunk), two disparities per lane, packed prefix sums of the last
+// RS >= 2*Lmax+2 rows in an LDS ring, rows fetched CV2_PF at a time with the
+// next batch in flight while the current one is consumed.  T entries outside
+// the pixel's range are already invalid (written so by the H pass), so only
+// the output side applies the range.  D must be even when T is u16.
+template <typename T>
+__global__ __launch_bounds__(64) void cross_v2_kernel(int w, int h, int d0, int D, int Lmax, int rs_mask,
+                                                      const int16_t* __restrict__ dmin, const int16_t* __restrict__ dmax,
+                                                      const uint32_t* __restrict__ arms, const T* __restrict__ tmp,
+                                                      T* __restrict__ cost) {
+    constexpr uint32_t TINV = (T)~(T)0;
+    extern __shared__ uint32_t sh[];
+    uint32_t* magic = sh;                   // [2*kMaxArm + 2]
+    uint32_t* ring = sh + 2 * kMaxArm + 2;  // [RS][2][64]
+    const int lane = threadIdx.x;
+    const int x = blockIdx.x;
+    const int di = blockIdx.y * 128 + 2 * lane;   // first of the lane's two disparities
+    const int dA = d0 + di, dB = dA + 1;
+    for (int c = lane; c < 2 * kMaxArm + 2; c += 64) magic[c] = magic_for((uint32_t)c);
+    ring[lane] = 0;
+    ring[64 + lane] = 0;
+    uint32_t accA = 0, accB = 0;
+    const int total = h + Lmax;
+    uint32_t nxt[CV2_PF];
+    auto fetch = [&](int kb, uint32_t (&v)[CV2_PF]) {
+#pragma unroll
+        for (int i = 0; i < CV2_PF; ++i) {
+            const int k = kb + i;
+            v[i] = k < h ? load_pair_any<T>(tmp + ((size_t)k * w + x) * D, di, D) : 0xFFFFFFFFu;
+        }
+    };
+    fetch(0, nxt);
+    for (int kb = 0; kb < total; kb += CV2_PF) {
+        uint32_t cur[CV2_PF];
+#pragma unroll
+        for (int i = 0; i < CV2_PF; ++i) cur[i] = nxt[i];
+        if (kb + CV2_PF < h) fetch(kb + CV2_PF, nxt);
+        // row metadata of the outputs of this batch, one row per lane
+        const int yl = kb - Lmax + lane;
+        const bool yin = lane < CV2_PF && yl >= 0 && yl < h;
+        const uint32_t meta_arm = yin ? arms[(size_t)yl * w + x] : 0u;
+        const uint32_t meta_rng = yin ? ((uint32_t)(uint16_t)dmin[(size_t)yl * w + x] |
+                                         ((uint32_t)(uint16_t)dmax[(size_t)yl * w + x] << 16)) : 0u;
+#pragma unroll
+        for (int i = 0; i < CV2_PF; ++i) {
+            const int k = kb + i;
+            if (k < h) {
+                const uint32_t c = cur[i];
+                const uint32_t ca = c & 0xFFFFu, cb = c >> 16;
+                accA += ca != 0xFFFFu && ca != TINV ? (ca << 8) | 1u : 0u;
+                accB += cb != 0xFFFFu && cb != TINV ? (cb << 8) | 1u : 0u;
+                const int slot = ((k + 1) & rs_mask) * 128;
+                ring[slot + lane] = accA;
+                ring[slot + 64 + lane] = accB;
+            }
+            const int y = k - Lmax;
+            if (y >= 0 && y < h) {
+                const uint32_t a = (uint32_t)__builtin_amdgcn_readlane((int)meta_arm, i);
+                const uint32_t rg = (uint32_t)__builtin_amdgcn_readlane((int)meta_rng, i);
+                const int lo = (int16_t)(rg & 0xFFFF), hi = (int16_t)(rg >> 16);
+                const int U = (a >> 16) & 0xFF, Dn = a >> 24;
+                const int s1 = ((y + Dn + 1) & rs_mask) * 128, s0 = ((y - U) & rs_mask) * 128;
+                const uint32_t sa = ring[s1 + lane] - ring[s0 + lane];
+                const uint32_t sb = ring[s1 + 64 + lane] - ring[s0 + 64 + lane];
+                T oa = (T)TINV, ob = (T)TINV;
+                if (dA >= lo && dA < hi && (sa & 0xFF)) oa = (T)div_small(sa >> 8, sa & 0xFF, magic[sa & 0xFF]);
+                if (dB >= lo && dB < hi && (sb & 0xFF)) ob = (T)div_small(sb >> 8, sb & 0xFF, magic[sb & 0xFF]);
+                T* o = cost + ((size_t)y * w + x) * D + di;
+                if (di + 1 < D && !(D & 1)) {
+                    if (sizeof(T) == 1) *(uint16_t*)o = (uint16_t)((uint32_t)oa | ((uint32_t)ob << 8));
+                    else *(uint32_t*)o = (uint32_t)oa | ((uint32_t)ob << 16);
+                } else {
+                    if (di < D) o[0] = oa;
+                    if (di + 1 < D) o[1] = ob;
+                }
+            }
+        }
+    }
+}
+
+hipError_t launch_cross_v_t(hipStream_t s, const apg_hip_params& p, int w, int h, const int16_t* dmin,
+                            const int16_t* dmax, const uint32_t* arms, const void* tmp, void* cost, bool u8) {
+    {
+        const int D = p.max_disparity - p.min_disparity;
+        const int Lmax = p.max_arm_length > 1 ? p.max_arm_length : 1;
+        if (u8 || D % 2 == 0) {
+            int rs = 1;
+            while (rs < 2 * Lmax + 2) rs <<= 1;
+            const size_t shm = (2 * kMaxArm + 2 + (size_t)rs * 128) * sizeof(uint32_t);
+            dim3 grid(w, (D + 127) / 128);
+            if (u8)
+                hipLaunchKernelGGL(cross_v2_kernel<uint8_t>, grid, dim3(64), shm, s, w, h, p.min_disparity, D, Lmax,
+                                   rs - 1, dmin, dmax, arms, (const uint8_t*)tmp, (uint8_t*)cost);
+            else
+                hipLaunchKernelGGL(cross_v2_kernel<uint16_t>, grid, dim3(64), shm, s, w, h, p.min_disparity, D, Lmax,
+                                   rs - 1, dmin, dmax, arms, (const uint16_t*)tmp, (uint16_t*)cost);
+            return hipGetLastError();
+        }
+    }
     const int D = p.max_disparity - p.min_disparity;
     const int Lmax = p.max_arm_length > 1 ? p.max_arm_length : 1;
     int rs = 1;
     while (rs < 2 * Lmax + 2) rs <<= 1;
     const size_t shm = (2 * kMaxArm + 2 + (size_t)rs * 64) * sizeof(uint32_t);
     dim3 grid(w, (D + 63) / 64);
-    hipLaunchKernelGGL(cross_v_kernel, grid, dim3(64), shm, s, w, h, p.min_disparity, D, Lmax, rs - 1, dmin, dmax,
-                       arms, tmp, cost);
+    if (u8)
+        hipLaunchKernelGGL(cross_v_kernel<uint8_t>, grid, dim3(64), shm, s, w, h, p.min_disparity, D, Lmax, rs - 1, dmin,
+                           dmax, arms, (const uint8_t*)tmp, (uint8_t*)cost);
+    else
+        hipLaunchKernelGGL(cross_v_kernel<uint16_t>, grid, dim3(64), shm, s, w, h, p.min_disparity, D, Lmax, rs - 1,
+                           dmin, dmax, arms, (const uint16_t*)tmp, (uint16_t*)cost);
     return hipGetLastError();
+}
+
+hipError_t launch_cross_v(hipStream_t s, const apg_hip_params& p, int w, int h, const int16_t* dmin,
+                          const int16_t* dmax, const uint32_t* arms, const uint16_t* tmp, uint16_t* cost) {
+    return launch_cross_v_t(s, p, w, h, dmin, dmax, arms, tmp, cost, false);
 }
 
 // ------------------------------------------------------------------ SGM ----

@@ -62,4 +62,28 @@ hipError_t launch_fill(hipStream_t s, const apg_hip_params& p, int w, int h, con
 // median_radius 0 copies disp to out.
 hipError_t launch_median(hipStream_t s, const apg_hip_params& p, int w, int h, const float* disp, float* out);
 
+// ---- fused schedule ----
+// Largest quantised cost any state can take (bound from the LUT maxima);
+// <= 254 allows u8 storage of T and C' with 255 as the invalid value.
+int cost_upper_bound(const CostLut& lut, int cost_max);
+hipError_t launch_cross_v_t(hipStream_t s, const apg_hip_params& p, int w, int h, const int16_t* dmin,
+                            const int16_t* dmax, const uint32_t* arms, const void* tmp, void* cost, bool u8);
+hipError_t launch_fill_u32(hipStream_t s, size_t n, uint32_t v, uint32_t* p);
+hipError_t launch_cost_crossh(hipStream_t s, const apg_hip_params& p, const CostLut* d_lut, int w, int h,
+                              const uint8_t* gl, const uint8_t* gxl, const uint8_t* gyl, const uint8_t* gr,
+                              const uint8_t* gxr, const uint8_t* gyr, const uint32_t* c32l, const uint32_t* c32r,
+                              const uint64_t* c64l, const uint64_t* c64r, const int16_t* dmin, const int16_t* dmax,
+                              const uint32_t* arms, void* out, bool out_u8, uint32_t* rkeys);
+hipError_t launch_rwta_finalize(hipStream_t s, const apg_hip_params& p, const CostLut* d_lut, int w, int h,
+                                const uint8_t* gl, const uint8_t* gxl, const uint8_t* gyl, const uint8_t* gr,
+                                const uint8_t* gxr, const uint8_t* gyr, const uint32_t* c32l, const uint32_t* c32r,
+                                const uint64_t* c64l, const uint64_t* c64r, const int16_t* dmin, const int16_t* dmax,
+                                const uint32_t* rkeys, float* dr);
+hipError_t launch_sgm_pairs(hipStream_t s, const apg_hip_params& p, int w, int h, int D, const uint8_t* gray,
+                            const void* cost, bool cost_u8, uint8_t* const dsum[4]);
+hipError_t launch_wta_fused(hipStream_t s, const apg_hip_params& p, int w, int h, const void* cost, bool cost_u8,
+                            uint8_t* const dsum[4], const int16_t* dmin, const int16_t* dmax, const float* dr,
+                            const uint8_t* gx, const uint8_t* gy, float* disp, float* conf, uint8_t* reason,
+                            uint8_t* reliable, float* disp_wta, float* best, float* second, uint32_t* sum32);
+
 }  // namespace apgk

@@ -29,7 +29,11 @@ HBM_PEAK_GBS = 8000.0
 SEED = 20260101
 W, H, DMAX = 1280, 720, 128
 # Algorithmic HBM bytes per disparity state (compulsory stage-boundary volume
-# traffic): cost write 2 + Cross read/write 4 + SGM read C' / write S 4 + WTA read 2.
+# traffic of u16 volumes): cost write 2 + Cross read/write 4 + SGM read C' /
+# write S 4 + WTA read S 2 = 12 for the pipeline.  Per timed stage of the
+# fused schedule: "cost" = cost + right-WTA candidates + Cross H (writes T: 2),
+# "cross" = Cross V (reads T, writes C': 4), "sgm" = all path scans (reads C',
+# writes S: 4), "wta" = WTA + confidence + LR (reads S: 2).
 BYTES_PER_STATE = {"pipeline": 12.0, "sgm": 4.0, "cross": 4.0, "cost": 2.0, "wta": 2.0}
 
 
@@ -59,6 +63,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--schedule", type=int, default=0, help="0 fused (default), 1 stage by stage")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs one GPU per process; --gpus {args.gpus} is not supported "
@@ -70,6 +75,7 @@ def main():
     left, right, _ = A.synth_pair(1, W, H, DMAX, SEED)
     states = W * H * DMAX
     ctx = A.Context(0)
+    A.set_schedule(ctx, args.schedule)
     dl, dr = ctx.to_device(left), ctx.to_device(right)
     od = ctx.array((H, W), np.float32)
     oc = ctx.array((H, W), np.float32)

@@ -192,6 +192,13 @@ typedef struct {
     float* d_second;         /* w*h */
 } apg_hip_intermediates;
 
+/* Schedule of apg_hip_pipeline on this context: 0 (default) runs the fused
+ * schedule whenever it is eligible (no raw-volume intermediates requested,
+ * even D, 2*max(P2_base, P1+1) <= 254, at most one Cross iteration) and the
+ * stage-by-stage schedule otherwise; 1 always runs stage by stage.  Both
+ * produce identical results. */
+APG_HIP_API void apg_hip_set_schedule(apg_hip_ctx ctx, int schedule);
+
 /* StereoMatcher::compute_impl, dense branch (reference src/pipeline.cpp:435-605)
  * on device inputs and outputs.  Intermediate buffers come from the context
  * workspace.  d_left / d_right: strided u8 images. */

@@ -1,15 +1,19 @@
 """GPU parity of the device pipeline (apg_hip_pipeline, C ABI) against the CPU
-oracle on seeded synthetic pairs: every intermediate (C, C', S, right WTA, WTA
-output, best, second) and every output (disparity, confidence,
-invalid_reason, reliable mask) must be bit-exact."""
+oracle on seeded synthetic pairs, for both schedules:
+  stage-by-stage: every intermediate (C, C', S, right WTA, WTA output, best,
+                  second) and every output must be bit-exact;
+  fused:          S, right WTA, WTA output, best, second and every output
+                  (disparity, confidence, invalid_reason, reliable mask)
+                  must be bit-exact."""
 import numpy as np
 import pytest
 
 import apg_amd as A
 import pyoracle as O
 
-# (kind, w, h, max_disp, mode, overrides) -- odd sizes, odd D, D > 128,
-# negative min disparity, 64-bit census, 8 paths, non-piecewise P2.
+# (kind, w, h, D, mode, overrides) -- odd sizes, odd D, D > 128, negative
+# min disparity, 64-bit census, 8 paths, non-piecewise P2, u16 C' (cost_max
+# above 254), large P2 (stage-by-stage fallback).
 CASES = [
     (0, 64, 48, 16, 1, {}),
     (0, 97, 61, 24, 1, {}),
@@ -23,7 +27,15 @@ CASES = [
     (1, 120, 80, 32, 1, {"census_type": 0}),
     (2, 100, 70, 28, 1, {"piecewise_p2": 0, "agg_iterations": 2}),
     (0, 33, 27, 16, 1, {"max_arm_length": 5, "color_threshold": 30}),
+    (1, 150, 90, 64, 1, {"cost_max": 600}),
+    (2, 140, 70, 40, 1, {"P2_base": 200}),
+    (0, 1, 9, 4, 1, {}),
+    (1, 70, 1, 8, 0, {}),
 ]
+
+FUSED_KEYS = ["sgm", "disp_right", "disp_wta", "best", "second",
+              "disparity", "confidence", "invalid_reason", "reliable"]
+PLAIN_KEYS = ["cost", "cost_agg"] + FUSED_KEYS
 
 
 def _check(g, o, keys):
@@ -36,19 +48,39 @@ def _check(g, o, keys):
             raise AssertionError(f"{k}: {len(bad)} mismatches, first at {i}: gpu={a[i]} oracle={b[i]}")
 
 
-@pytest.mark.gpu
-@pytest.mark.parametrize("kind,w,h,D,mode,over", CASES)
-def test_pipeline_bit_exact(ctx, kind, w, h, D, mode, over):
-    print(f"case kind={kind} {w}x{h} D={D} mode={mode} {over}", flush=True)
+def _case(kind, w, h, D, mode, over):
     p = A.params(mode, D, **over)
     if "min_disparity" in over:
         p.max_disparity = over["min_disparity"] + D
     left, right, _ = A.synth_pair(kind, w, h, max(D, 2), 1234 + w)
-    g = A.run_pipeline(ctx, p, left, right, intermediates=True)
-    c = O.cfg_from(p.as_dict())
-    o = O.pipeline(c, left, right, intermediates=True)
-    _check(g, o, ["cost", "cost_agg", "sgm", "disp_right", "disp_wta", "best", "second",
-                  "disparity", "confidence", "invalid_reason", "reliable"])
+    o = O.pipeline(O.cfg_from(p.as_dict()), left, right, intermediates=True)
+    return p, left, right, o
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind,w,h,D,mode,over", CASES)
+def test_pipeline_stage_by_stage_bit_exact(ctx, kind, w, h, D, mode, over):
+    print(f"stage-by-stage kind={kind} {w}x{h} D={D} mode={mode} {over}", flush=True)
+    p, left, right, o = _case(kind, w, h, D, mode, over)
+    A.set_schedule(ctx, 1)
+    try:
+        g = A.run_pipeline(ctx, p, left, right, intermediates=True)
+    finally:
+        A.set_schedule(ctx, 0)
+    _check(g, o, PLAIN_KEYS)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind,w,h,D,mode,over", CASES)
+def test_pipeline_fused_bit_exact(ctx, kind, w, h, D, mode, over):
+    print(f"fused kind={kind} {w}x{h} D={D} mode={mode} {over}", flush=True)
+    p, left, right, o = _case(kind, w, h, D, mode, over)
+    A.set_schedule(ctx, 0)
+    g = A.run_pipeline(ctx, p, left, right, intermediates=A.FUSED_INTERMEDIATES)
+    _check(g, o, FUSED_KEYS)
+    # and without any intermediate requested (the production call)
+    g2 = A.run_pipeline(ctx, p, left, right)
+    _check(g2, o, ["disparity", "confidence", "invalid_reason", "reliable"])
 
 
 @pytest.mark.gpu
@@ -57,6 +89,20 @@ def test_pipeline_color_input(ctx):
     left = rng.integers(0, 256, (45, 70, 3), dtype=np.uint8)
     right = np.roll(left, -6, axis=1)
     p = A.params(1, 24)
-    g = A.run_pipeline(ctx, p, left, right, intermediates=True)
     o = O.pipeline(O.cfg_from(p.as_dict()), left, right, intermediates=True)
-    _check(g, o, ["cost", "cost_agg", "sgm", "disparity", "confidence", "invalid_reason", "reliable"])
+    for sched in (1, 0):
+        A.set_schedule(ctx, sched)
+        g = A.run_pipeline(ctx, p, left, right, intermediates=A.FUSED_INTERMEDIATES)
+        _check(g, o, FUSED_KEYS)
+    A.set_schedule(ctx, 0)
+
+
+@pytest.mark.gpu
+def test_pipeline_full_size_fused_vs_oracle(ctx):
+    """Balanced 1280x720 D=128 (the bench workload): fused outputs bit-exact."""
+    p = A.params(1, 128)
+    left, right, _ = A.synth_pair(1, 1280, 720, 128, 20260101)
+    O.set_threads(16)
+    o = O.pipeline(O.cfg_from(p.as_dict()), left, right, intermediates=True)
+    g = A.run_pipeline(ctx, p, left, right, intermediates=("disp_right", "disp_wta", "best", "second"))
+    _check(g, o, ["disp_right", "disp_wta", "best", "second", "disparity", "confidence", "invalid_reason", "reliable"])
