@@ -91,16 +91,21 @@ struct PixDesc {
 // Quantised matching cost of one state (reference src/cost_computer.cpp:176-199):
 // c = census_term + ad_term + grad_term in that order, q = (int)(c*scale + 0.5f),
 // clamped to [0, cost_max].  lut_* hold the host-built float terms.
+// The AD / gradient LUTs hold +0.0f when the term is disabled, and c >= 0,
+// so adding their entry unconditionally leaves c bit-identical to skipping
+// the term; the three table reads are independent and issue together.
 template <bool SYM>
 __device__ __forceinline__ uint16_t state_cost(const PixDesc& L, const PixDesc& R, const float* lut_c,
                                                const float* lut_a, const float* lut_g, float scale, int cost_max,
                                                int use_ad, int use_grad) {
+    (void)use_ad;
+    (void)use_grad;
     const int pop = SYM ? __popc(L.clo ^ R.clo) : (__popc(L.clo ^ R.clo) + __popc(L.chi ^ R.chi));
-    float c = lut_c[pop];
-    if (use_ad) c = c + lut_a[abs((int)(L.ggg & 0xFF) - (int)(R.ggg & 0xFF))];
-    if (use_grad)
-        c = c + lut_g[abs((int)((L.ggg >> 8) & 0xFF) - (int)((R.ggg >> 8) & 0xFF)) +
-                      abs((int)((L.ggg >> 16) & 0xFF) - (int)((R.ggg >> 16) & 0xFF))];
+    const int ad = abs((int)(L.ggg & 0xFF) - (int)(R.ggg & 0xFF));
+    const int g = abs((int)((L.ggg >> 8) & 0xFF) - (int)((R.ggg >> 8) & 0xFF)) +
+                  abs((int)((L.ggg >> 16) & 0xFF) - (int)((R.ggg >> 16) & 0xFF));
+    const float tc = lut_c[pop], ta = lut_a[ad], tg = lut_g[g];
+    const float c = (tc + ta) + tg;
     const int q = (int)(c * scale + 0.5f);
     return (uint16_t)clampi(q, 0, cost_max);
 }

@@ -156,28 +156,6 @@ __global__ __launch_bounds__(64) void cross_v_kernel(int w, int h, int d0, int D
     }
 }
 
-// Two disparities of one pixel as u16 halves (invalid -> 65535).
-template <typename T>
-__device__ __forceinline__ uint32_t load_pair_any(const T* __restrict__ v, int d, int D);
-template <>
-__device__ __forceinline__ uint32_t load_pair_any<uint16_t>(const uint16_t* __restrict__ v, int d, int D) {
-    if (d + 1 < D) return *(const uint32_t*)(v + d);
-    return d < D ? (uint32_t)v[d] | 0xFFFF0000u : 0xFFFFFFFFu;
-}
-template <>
-__device__ __forceinline__ uint32_t load_pair_any<uint8_t>(const uint8_t* __restrict__ v, int d, int D) {
-    uint32_t b0, b1;
-    if (d + 1 < D && !(D & 1)) {   // even D: the pair is 2-byte aligned
-        const uint32_t b = *(const uint16_t*)(v + d);
-        b0 = b & 0xFF;
-        b1 = b >> 8;
-    } else {
-        b0 = d < D ? v[d] : 0xFFu;
-        b1 = d + 1 < D ? v[d + 1] : 0xFFu;
-    }
-    return (b0 == 0xFFu ? 0xFFFFu : b0) | ((b1 == 0xFFu ? 0xFFFFu : b1) << 16);
-}
-
 constexpr int CV2_PF = 16;
 
 // Vertical Cross pass, sliding form: one wavefront per (column, 128-wide d
@@ -204,34 +182,49 @@ __global__ __launch_bounds__(64) void cross_v2_kernel(int w, int h, int d0, int 
     ring[64 + lane] = 0;
     uint32_t accA = 0, accB = 0;
     const int total = h + Lmax;
-    uint32_t nxt[CV2_PF];
-    auto fetch = [&](int kb, uint32_t (&v)[CV2_PF]) {
+    // Raw words are prefetched unconditionally (row index clamped) and decoded
+    // at use, so the next batch stays in flight while this one is consumed.
+    // D is even (launcher): one aligned word per lane; lanes past D read pair 0.
+    const int dl0 = di < D ? di : 0;
+    // Per batch: the metadata (arms, range) of the batch's output rows, one
+    // row per lane, then the T words of its input rows; issued in the order
+    // they are consumed so a wait for the former never waits for the latter.
+    uint32_t nxt[CV2_PF], nxt_arm, nxt_rng;
+    auto fetch = [&](int kb) {
+        const int yl = min(max(kb - Lmax + lane, 0), h - 1);
+        nxt_arm = arms[(size_t)yl * w + x];
+        nxt_rng = (uint32_t)(uint16_t)dmin[(size_t)yl * w + x] | ((uint32_t)(uint16_t)dmax[(size_t)yl * w + x] << 16);
 #pragma unroll
         for (int i = 0; i < CV2_PF; ++i) {
-            const int k = kb + i;
-            v[i] = k < h ? load_pair_any<T>(tmp + ((size_t)k * w + x) * D, di, D) : 0xFFFFFFFFu;
+            const int k = min(kb + i, h - 1);
+            const T* row = tmp + ((size_t)k * w + x) * D;
+            if (sizeof(T) == 2) nxt[i] = *(const uint32_t*)(row + dl0);
+            else nxt[i] = *(const uint16_t*)((const uint8_t*)row + dl0);
         }
     };
-    fetch(0, nxt);
+    auto decode = [&](uint32_t raw) -> uint32_t {   // -> u16 halves, invalid 65535
+        uint32_t a, b;
+        if (sizeof(T) == 2) { a = raw & 0xFFFFu; b = raw >> 16; }
+        else { a = raw & 0xFFu; b = (raw >> 8) & 0xFFu; a = a == 0xFFu ? 0xFFFFu : a; b = b == 0xFFu ? 0xFFFFu : b; }
+        if (di >= D) a = 0xFFFFu;
+        if (di + 1 >= D) b = 0xFFFFu;
+        return a | (b << 16);
+    };
+    fetch(0);
     for (int kb = 0; kb < total; kb += CV2_PF) {
         uint32_t cur[CV2_PF];
 #pragma unroll
         for (int i = 0; i < CV2_PF; ++i) cur[i] = nxt[i];
-        if (kb + CV2_PF < h) fetch(kb + CV2_PF, nxt);
-        // row metadata of the outputs of this batch, one row per lane
-        const int yl = kb - Lmax + lane;
-        const bool yin = lane < CV2_PF && yl >= 0 && yl < h;
-        const uint32_t meta_arm = yin ? arms[(size_t)yl * w + x] : 0u;
-        const uint32_t meta_rng = yin ? ((uint32_t)(uint16_t)dmin[(size_t)yl * w + x] |
-                                         ((uint32_t)(uint16_t)dmax[(size_t)yl * w + x] << 16)) : 0u;
+        const uint32_t meta_arm = nxt_arm, meta_rng = nxt_rng;
+        fetch(kb + CV2_PF);
 #pragma unroll
         for (int i = 0; i < CV2_PF; ++i) {
             const int k = kb + i;
             if (k < h) {
-                const uint32_t c = cur[i];
+                const uint32_t c = decode(cur[i]);
                 const uint32_t ca = c & 0xFFFFu, cb = c >> 16;
-                accA += ca != 0xFFFFu && ca != TINV ? (ca << 8) | 1u : 0u;
-                accB += cb != 0xFFFFu && cb != TINV ? (cb << 8) | 1u : 0u;
+                accA += ca != 0xFFFFu ? (ca << 8) | 1u : 0u;
+                accB += cb != 0xFFFFu ? (cb << 8) | 1u : 0u;
                 const int slot = ((k + 1) & rs_mask) * 128;
                 ring[slot + lane] = accA;
                 ring[slot + 64 + lane] = accB;
@@ -249,12 +242,9 @@ __global__ __launch_bounds__(64) void cross_v2_kernel(int w, int h, int d0, int 
                 if (dA >= lo && dA < hi && (sa & 0xFF)) oa = (T)div_small(sa >> 8, sa & 0xFF, magic[sa & 0xFF]);
                 if (dB >= lo && dB < hi && (sb & 0xFF)) ob = (T)div_small(sb >> 8, sb & 0xFF, magic[sb & 0xFF]);
                 T* o = cost + ((size_t)y * w + x) * D + di;
-                if (di + 1 < D && !(D & 1)) {
+                if (di < D) {
                     if (sizeof(T) == 1) *(uint16_t*)o = (uint16_t)((uint32_t)oa | ((uint32_t)ob << 8));
                     else *(uint32_t*)o = (uint32_t)oa | ((uint32_t)ob << 16);
-                } else {
-                    if (di < D) o[0] = oa;
-                    if (di + 1 < D) o[1] = ob;
                 }
             }
         }
@@ -266,7 +256,7 @@ hipError_t launch_cross_v_t(hipStream_t s, const apg_hip_params& p, int w, int h
     {
         const int D = p.max_disparity - p.min_disparity;
         const int Lmax = p.max_arm_length > 1 ? p.max_arm_length : 1;
-        if (u8 || D % 2 == 0) {
+        if (D % 2 == 0) {
             int rs = 1;
             while (rs < 2 * Lmax + 2) rs <<= 1;
             const size_t shm = (2 * kMaxArm + 2 + (size_t)rs * 128) * sizeof(uint32_t);

@@ -111,6 +111,11 @@ __global__ __launch_bounds__(256) void cost_crossh_kernel(
     const int seg = (NC + 3) / 4;
     const int c_beg = min(NC, wave * seg), c_end = min(NC, c_beg + seg);
 
+    // this wave's column segment cached one column per lane (seg <= 64), read
+    // back with readlane so the per-column values are scalars in the loop
+    const int cl = min(c_beg + lane, NC - 1);
+    const uint32_t col_clo = s_l[cl].clo, col_chi = s_l[cl].chi, col_ggg = s_l[cl].ggg;
+    const uint32_t col_rng = (uint32_t)(s_lo[cl] & 0xFFFF) | ((uint32_t)s_hi[cl] << 16);
     for (int dc = 0; dc < D; dc += 64) {
         const int di = dc + lane;
         const int d = d0 + di;
@@ -118,23 +123,26 @@ __global__ __launch_bounds__(256) void cost_crossh_kernel(
         // phase A: costs of this wave's column segment, local prefix, WTA keys
         uint32_t acc = 0;
         for (int c = c_beg; c < c_end; ++c) {
+            const int i = c - c_beg;
             const int x = xb + c;
             const int xr = x - d;
-            uint32_t v = 0;
-            if (dact && x >= 0 && x < w && d >= s_lo[c] && d < s_hi[c] && xr >= 0 && xr < w) {
-                const uint16_t cost = state_cost<SYM>(s_l[c], s_r[c + (d1 - 1 - d)], s_lc, s_la, s_lg, scale,
-                                                      cost_max, use_ad, use_grad);
-                // a cost equal to the sentinel (cost_max 65535) is invalid downstream
-                if (cost != kInv16) v = ((uint32_t)cost << 8) | 1u;
-                if (c >= Lmax && c < Lmax + F_TX) {
-                    if (cost != kInv16) atomicMin(&s_key[xr - kbase], ((uint32_t)cost << 16) | (uint32_t)di);
-                    if (!CROSS) out[(row + x) * D + di] = cost == kInv16 ? OINV : (TO)cost;
-                }
-            } else if (!CROSS && dact && c >= Lmax && c < Lmax + F_TX && x < w) {
-                out[(row + x) * D + di] = OINV;
-            }
+            PixDesc Lp;
+            Lp.clo = (uint32_t)__builtin_amdgcn_readlane((int)col_clo, i);
+            Lp.chi = SYM ? 0u : (uint32_t)__builtin_amdgcn_readlane((int)col_chi, i);
+            Lp.ggg = (uint32_t)__builtin_amdgcn_readlane((int)col_ggg, i);
+            const uint32_t rng = (uint32_t)__builtin_amdgcn_readlane((int)col_rng, i);
+            const int lo = (int16_t)(rng & 0xFFFF), hi = (int16_t)(rng >> 16);
+            const bool xin = x >= 0 && x < w;   // uniform
+            const bool valid = dact && xin && d >= lo && d < hi && xr >= 0 && xr < w;
+            const int ri = min(max(c + (d1 - 1 - d), 0), NC + D - 2);
+            const uint16_t cost = state_cost<SYM>(Lp, s_r[ri], s_lc, s_la, s_lg, scale, cost_max, use_ad, use_grad);
+            // a cost equal to the sentinel (cost_max 65535) is invalid downstream
+            const bool ok = valid && cost != kInv16;
+            const bool own = c >= Lmax && c < Lmax + F_TX;   // uniform
+            if (own && ok) atomicMin(&s_key[xr - kbase], ((uint32_t)cost << 16) | (uint32_t)di);
+            if (!CROSS && own && xin && dact) out[(row + x) * D + di] = ok ? (TO)cost : OINV;
             if (CROSS) {
-                acc += v;
+                acc += ok ? ((uint32_t)cost << 8) | 1u : 0u;
                 P[(c + 1) * 64 + lane] = acc;
             }
         }
@@ -239,7 +247,8 @@ __global__ void rwta_finalize_kernel(const CostLut* __restrict__ lut, int w, int
     const size_t row = (size_t)y * w;
     const uint32_t k = rkeys[row + xr];
     float dval = -1.f;
-    if (k != 0xFFFFFFFFu) {
+    // as in the reference, a negative best disparity counts as "no match" (best_d >= 0 test)
+    if (k != 0xFFFFFFFFu && d0 + (int)(k & 0xFFFF) >= 0) {
         const int di = (int)(k & 0xFFFF);
         const int bd = d0 + di;
         dval = (float)bd;

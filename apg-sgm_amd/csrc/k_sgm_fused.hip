@@ -20,23 +20,23 @@ namespace apgk {
 
 // C' element of the fused schedule: u8 with 255 = invalid when every
 // reachable cost is <= 254, else u16 with 65535 = invalid.
+// Loads are split from decoding so that prefetched words stay in flight
+// until they are used (decoding right after a load would force the wait).
+// raw_cpair: the lane's pair (d = 2m, 2m+1) as stored; lanes past D read the
+// pixel's first pair instead (always in bounds) and are masked by decode.
 template <typename TC>
-__device__ __forceinline__ uint32_t load_cpair(const TC* __restrict__ v, int m, int D);
-
-template <>
-__device__ __forceinline__ uint32_t load_cpair<uint16_t>(const uint16_t* __restrict__ v, int m, int D) {
-    const int d = 2 * m;
-    if (d + 1 < D) return *(const uint32_t*)(v + d);   // D even on this path (checked by the launcher)
-    return 0xFFFFFFFFu;
+__device__ __forceinline__ uint32_t raw_cpair(const TC* __restrict__ v, int m, int D) {
+    const int d = 2 * m < D ? 2 * m : 0;   // D even on this path (checked by the launcher)
+    if (sizeof(TC) == 1) return *(const uint16_t*)((const uint8_t*)v + d);
+    return *(const uint32_t*)((const uint16_t*)v + d);
 }
 
-template <>
-__device__ __forceinline__ uint32_t load_cpair<uint8_t>(const uint8_t* __restrict__ v, int m, int D) {
-    const int d = 2 * m;
-    if (d + 1 >= D) return 0xFFFFFFFFu;
-    const uint32_t b = *(const uint16_t*)(v + d);
-    // bytes -> u16 halves, 255 -> 65535
-    uint32_t x = (b & 0xFFu) | ((b & 0xFF00u) << 8);
+// raw pair -> u16 halves with the invalid value 65535 (u8: 255 -> 65535)
+template <typename TC>
+__device__ __forceinline__ uint32_t decode_cpair(uint32_t raw, bool in_range) {
+    if (!in_range) return 0xFFFFFFFFu;
+    if (sizeof(TC) == 2) return raw;
+    const uint32_t x = (raw & 0xFFu) | ((raw & 0xFF00u) << 8);
     const uint32_t inv = ((x + 0x00010001u) >> 8) & 0x00010001u;   // 1 in halves equal to 255
     return x | (inv * 0xFF00u);
 }
@@ -123,13 +123,18 @@ __global__ __launch_bounds__(64) void sgm_pairs_kernel(apg_hip_params p, int w, 
     for (int j = 0; j < K; ++j) prevA[j] = prevB[j] = 0xFFFFFFFFu;
     uint32_t minA = 0xFFFFu, minB = 0xFFFFu;
     int gA = 0, gB = 0;
-    const int mB = 2 * K;   // bytes per lane in the delta planes
-    // C' of both chains for one batch of SF_PF steps; the next batch is in
-    // flight while the current one is consumed.
-    // The other direction's deltas of a batch are fetched with it when their
-    // producing step (n-1-t) finished before the fetch was issued, i.e. lies
-    // before the batch then being processed (done_before); else at use time.
-    uint32_t nA[SF_PF][K], nB[SF_PF][K], nOA[SF_PF][K], nOB[SF_PF][K];
+    bool inr[K];   // lane's pair j lies inside D
+#pragma unroll
+    for (int j = 0; j < K; ++j) inr[j] = 2 * (lane * K + j) + 1 < D;
+    // lane byte offset of pair j in a delta plane row: 2*(lane*K + j); lanes past D
+    // write/read pair 0 of the pixel with an unchanged value (see below)
+    auto doff = [&](int j) { return inr[j] ? 2 * (lane * K + j) : 0; };
+    // Raw words of one batch of SF_PF steps for both chains: C' pairs and the
+    // other direction's delta.  The delta word is fetched from the delta plane
+    // when its producing step (n-1-t) finished before the fetch is issued
+    // (before step `done_before`), else from the C' pair just fetched (a cache
+    // hit, ignored).  The next batch is in flight while the current one runs.
+    uint32_t rA[SF_PF][K], rB[SF_PF][K], qA[SF_PF][K], qB[SF_PF][K];
     auto fetch = [&](int t0, int done_before) {
 #pragma unroll
         for (int i = 0; i < SF_PF; ++i) {
@@ -140,11 +145,12 @@ __global__ __launch_bounds__(64) void sgm_pairs_kernel(apg_hip_params p, int w, 
             const bool od = 2 * t > n - 1 && n - 1 - t < done_before;
 #pragma unroll
             for (int j = 0; j < K; ++j) {
-                nA[i][j] = load_cpair<TC>(va, lane * K + j, D);
-                nB[i][j] = load_cpair<TC>(vb, lane * K + j, D);
-                const bool inr = 2 * (lane * K + j) + 1 < D;
-                nOA[i][j] = od && inr ? unpack_bytes(*(const uint16_t*)(dsum + (size_t)pa * D + lane * mB + 2 * j)) : 0u;
-                nOB[i][j] = od && inr ? unpack_bytes(*(const uint16_t*)(dsum + (size_t)pb * D + lane * mB + 2 * j)) : 0u;
+                rA[i][j] = raw_cpair<TC>(va, lane * K + j, D);
+                rB[i][j] = raw_cpair<TC>(vb, lane * K + j, D);
+                const uint8_t* ea = od ? dsum + (size_t)pa * D + doff(j) : (const uint8_t*)va;
+                const uint8_t* eb = od ? dsum + (size_t)pb * D + doff(j) : (const uint8_t*)vb;
+                qA[i][j] = *(const uint16_t*)ea;
+                qB[i][j] = *(const uint16_t*)eb;
             }
         }
     };
@@ -153,22 +159,26 @@ __global__ __launch_bounds__(64) void sgm_pairs_kernel(apg_hip_params p, int w, 
     for (int t0 = 0; t0 < n; t0 += SF_PF) {
         if ((t0 & 63) == 0) {
             // gray of the next 64 pixels of both chains, one per lane
-            const int tl = t0 + lane;
-            gchA = tl < n ? gray[p0 + step * tl] : 0u;
-            gchB = tl < n ? gray[p0 + step * (n - 1 - tl)] : 0u;
+            const int tl = min(t0 + lane, n - 1);
+            gchA = gray[p0 + step * tl];
+            gchB = gray[p0 + step * (n - 1 - tl)];
         }
-        uint32_t cA[SF_PF][K], cB[SF_PF][K], oA[SF_PF][K], oB[SF_PF][K];
+        uint32_t cA[SF_PF][K], cB[SF_PF][K], xA[SF_PF][K], xB[SF_PF][K];
 #pragma unroll
         for (int i = 0; i < SF_PF; ++i)
 #pragma unroll
             for (int j = 0; j < K; ++j) {
-                cA[i][j] = nA[i][j];
-                cB[i][j] = nB[i][j];
-                oA[i][j] = nOA[i][j];
-                oB[i][j] = nOB[i][j];
+                cA[i][j] = rA[i][j];
+                cB[i][j] = rB[i][j];
+                xA[i][j] = qA[i][j];
+                xB[i][j] = qB[i][j];
             }
-        // this batch's deltas were prefetched iff their producer preceded t0 - SF_PF
+        // deltas of this batch were prefetched iff their producer preceded t0 - SF_PF
         const int pf_done = t0 - SF_PF;
+        // a batch that needs some delta not prefetched re-reads all its deltas at use time
+        const int tl_last = min(t0 + SF_PF - 1, n - 1);
+        bool slow = false;
+        for (int t = t0; t <= tl_last; ++t) slow |= (2 * t > n - 1) && !(n - 1 - t < pf_done);
         if (t0 + SF_PF < n) fetch(t0 + SF_PF, t0);
 #pragma unroll
         for (int i = 0; i < SF_PF; ++i) {
@@ -176,34 +186,35 @@ __global__ __launch_bounds__(64) void sgm_pairs_kernel(apg_hip_params p, int w, 
             if (t >= n) break;
             const int ga = __builtin_amdgcn_readlane((int)gchA, t & 63);
             const int gb = __builtin_amdgcn_readlane((int)gchB, t & 63);
-            uint32_t dA[K], dB[K];
-            minA = sgm_step<K>(p, prevA, cA[i], minA, ga, gA, t == 0, dA);
-            minB = sgm_step<K>(p, prevB, cB[i], minB, gb, gB, t == 0, dB);
+            uint32_t ca[K], cb[K], dA[K], dB[K];
+#pragma unroll
+            for (int j = 0; j < K; ++j) {
+                ca[j] = decode_cpair<TC>(cA[i][j], inr[j]);
+                cb[j] = decode_cpair<TC>(cB[i][j], inr[j]);
+            }
+            minA = sgm_step<K>(p, prevA, ca, minA, ga, gA, t == 0, dA);
+            minB = sgm_step<K>(p, prevB, cb, minB, gb, gB, t == 0, dB);
             gA = ga;
             gB = gb;
             const long pa = p0 + step * t, pb = p0 + step * (n - 1 - t);
-            uint8_t* da = dsum + (size_t)pa * D + lane * mB;
-            uint8_t* db = dsum + (size_t)pb * D + lane * mB;
+            uint8_t* da = dsum + (size_t)pa * D;
+            uint8_t* db = dsum + (size_t)pb * D;
             // Before the chains meet each stores its own delta; at the meeting
-            // pixel (odd n) the pair sum is stored once; afterwards each chain
-            // adds the delta the other direction stored here at step n-1-t
-            // (same lane, earlier in program order).
+            // pixel (odd n) both store the pair sum; afterwards each adds the
+            // delta the other direction stored here at step n-1-t.
+            const bool second = 2 * t > n - 1, middle = 2 * t == n - 1;
 #pragma unroll
             for (int j = 0; j < K; ++j) {
-                if (2 * (lane * K + j) + 1 >= D) continue;
-                if (2 * t < n - 1) {
-                    *(uint16_t*)(da + 2 * j) = (uint16_t)pack_bytes(dA[j]);
-                    *(uint16_t*)(db + 2 * j) = (uint16_t)pack_bytes(dB[j]);
-                } else if (2 * t == n - 1) {
-                    *(uint16_t*)(da + 2 * j) = (uint16_t)pack_bytes(dA[j] + dB[j]);
-                } else {
-                    uint32_t xA = oA[i][j], xB = oB[i][j];
-                    if (!(n - 1 - t < pf_done)) {
-                        xA = unpack_bytes(*(const uint16_t*)(da + 2 * j));
-                        xB = unpack_bytes(*(const uint16_t*)(db + 2 * j));
-                    }
-                    *(uint16_t*)(da + 2 * j) = (uint16_t)pack_bytes(dA[j] + xA);
-                    *(uint16_t*)(db + 2 * j) = (uint16_t)pack_bytes(dB[j] + xB);
+                uint32_t oA = unpack_bytes(xA[i][j]), oB = unpack_bytes(xB[i][j]);
+                if (slow && second) {
+                    oA = unpack_bytes(*(const uint16_t*)(da + doff(j)));
+                    oB = unpack_bytes(*(const uint16_t*)(db + doff(j)));
+                }
+                const uint32_t addA = second ? oA : (middle ? dB[j] : 0u);
+                const uint32_t addB = second ? oB : (middle ? dA[j] : 0u);
+                if (inr[j]) {
+                    *(uint16_t*)(da + doff(j)) = (uint16_t)pack_bytes(dA[j] + addA);
+                    *(uint16_t*)(db + doff(j)) = (uint16_t)pack_bytes(dB[j] + addB);
                 }
             }
         }

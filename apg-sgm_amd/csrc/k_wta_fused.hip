@@ -44,31 +44,44 @@ __global__ __launch_bounds__(64) void wta_fused_kernel(apg_hip_params p, int w, 
     for (int dc = 0; dc < D; dc += WF_DC) {
         const int nd = min(WF_DC, D - dc);   // even
         // stage S for the block's pixels: element (i, j) of the chunk at st[i*STRIDE + j/2] (u16 halves)
-        // one pixel's chunk per wave iteration: 64 lanes x 2 disparities, contiguous
+        // one pixel's chunk per wave iteration: 64 lanes x 2 disparities,
+        // contiguous; WF_G pixels of raw words are loaded before any is decoded
+        // (lanes past the chunk read its first pair and store nothing)
         const int jw = lane;
-        for (int i = 0; i < npb; ++i) {
-            if (jw >= nd / 2) continue;
-            const size_t q = (p_base + i) * (size_t)D + dc + 2 * jw;
-            uint32_t c0, c1;
-            if (sizeof(TC) == 1) {
-                const uint32_t b = *(const uint16_t*)((const uint8_t*)cost + q);
-                c0 = b & 0xFF; c1 = b >> 8;
-            } else {
-                const uint32_t b = *(const uint32_t*)((const uint16_t*)cost + q);
-                c0 = b & 0xFFFF; c1 = b >> 16;
+        const bool jin = jw < nd / 2;
+        const int jo = jin ? 2 * jw : 0;
+        const uint8_t* planes[4] = {ds0, ds1, ds2, ds3};
+        constexpr int WF_G = 8;
+        for (int i0 = 0; i0 < npb; i0 += WF_G) {
+            uint32_t rc[WF_G], rd[WF_G][4];
+#pragma unroll
+            for (int g = 0; g < WF_G; ++g) {
+                const size_t q = (p_base + min(i0 + g, npb - 1)) * (size_t)D + dc + jo;
+                rc[g] = sizeof(TC) == 1 ? (uint32_t) * (const uint16_t*)((const uint8_t*)cost + q)
+                                        : *(const uint32_t*)((const uint16_t*)cost + q);
+#pragma unroll
+                for (int f = 0; f < 4; ++f) rd[g][f] = f < nfam ? (uint32_t) * (const uint16_t*)(planes[f] + q) : 0u;
             }
-            uint32_t a0 = npath * c0, a1 = npath * c1;
-            const uint8_t* planes[4] = {ds0, ds1, ds2, ds3};
-            for (int f = 0; f < nfam; ++f) {
-                const uint32_t b = *(const uint16_t*)(planes[f] + q);
-                a0 += b & 0xFF;
-                a1 += b >> 8;
-            }
-            const uint32_t s0 = c0 == CINV ? 0xFFFFu : a0, s1 = c1 == CINV ? 0xFFFFu : a1;
-            st[i * WF_STRIDE + jw] = s0 | (s1 << 16);
-            if (sum32) {
-                sum32[q] = s0 == 0xFFFFu ? 0xFFFFFFFFu : s0;
-                sum32[q + 1] = s1 == 0xFFFFu ? 0xFFFFFFFFu : s1;
+#pragma unroll
+            for (int g = 0; g < WF_G; ++g) {
+                const int i = i0 + g;
+                if (i >= npb || !jin) continue;
+                const size_t q = (p_base + i) * (size_t)D + dc + 2 * jw;
+                uint32_t c0, c1;
+                if (sizeof(TC) == 1) { c0 = rc[g] & 0xFF; c1 = (rc[g] >> 8) & 0xFF; }
+                else { c0 = rc[g] & 0xFFFF; c1 = rc[g] >> 16; }
+                uint32_t a0 = npath * c0, a1 = npath * c1;
+#pragma unroll
+                for (int f = 0; f < 4; ++f) {
+                    a0 += rd[g][f] & 0xFF;
+                    a1 += (rd[g][f] >> 8) & 0xFF;
+                }
+                const uint32_t s0 = c0 == CINV ? 0xFFFFu : a0, s1 = c1 == CINV ? 0xFFFFu : a1;
+                st[i * WF_STRIDE + jw] = s0 | (s1 << 16);
+                if (sum32) {
+                    sum32[q] = s0 == 0xFFFFu ? 0xFFFFFFFFu : s0;
+                    sum32[q + 1] = s1 == 0xFFFFu ? 0xFFFFFFFFu : s1;
+                }
             }
         }
         __syncthreads();
