@@ -20,7 +20,7 @@ struct DevBuf {
 enum BufId {
     B_SRC_L, B_SRC_R, B_GRAY_L, B_GX_L, B_GY_L, B_GRAY_R, B_GX_R, B_GY_R, B_CEN_L, B_CEN_R,
     B_DMIN, B_DMAX, B_ARMS, B_COST, B_TMP, B_SUM, B_DR, B_BEST, B_SECOND, B_SCRATCH, B_LUT,
-    B_RKEYS, B_DS0, B_DS1, B_DS2, B_DS3, B_COUNT
+    B_RKEYS, B_DS0, B_DS1, B_DS2, B_DS3, B_DS4, B_DS5, B_DS6, B_DS7, B_COUNT
 };
 
 enum StageId { S_AUX, S_PRIOR, S_COST, S_RWTA, S_CROSS, S_SGM, S_WTA, S_CONF, S_POST, S_COUNT };
@@ -450,24 +450,27 @@ int apg_hip_pipeline(apg_hip_ctx ctx, const apg_hip_params* p, const uint8_t* d_
     HIPCHK(launch_range_init(st, *p, w, h, dmin, dmax), "range");
 
     // Fused schedule: the raw cost volume never reaches HBM, C' is stored as
-    // u8 when every reachable cost is <= 254, and the SGM keeps per-direction
-    // pair deltas (u8, needs 2*P2max <= 254) instead of u16 path sums.
+    // u8 when every reachable cost is <= 254, and the SGM keeps one u8 delta
+    // plane per direction (needs P2max <= 254) instead of u16 path sums.
     const int p2max = p->P2_base > p->P1 + 1 ? p->P2_base : p->P1 + 1;
     const bool want_raw = inter && (inter->d_cost || inter->d_cost_agg);
-    const bool fused = ctx->schedule != 1 && !want_raw && D % 2 == 0 && 2 * p2max <= 254 &&
-                       (!p->agg_enable || p->agg_iterations <= 1);
+    const int cbound = cost_upper_bound(ctx->lut_host, p->cost_max);
+    const bool u8 = cbound <= 254;
+    // the WTA stage rebuilds S in u16: npath * (C' + P2max) must stay below 65535
+    const int npaths = p->paths == 8 ? 8 : 4;
+    const bool fused = ctx->schedule != 1 && !want_raw && D % 2 == 0 && D <= 256 && p2max <= 254 &&
+                       npaths * (cbound + p2max) < 65535 && (!p->agg_enable || p->agg_iterations <= 1) &&
+                       nv * (u8 ? 1 : 2) < (1ull << 31);
     if (fused) {
-        const bool u8 = cost_upper_bound(ctx->lut_host, p->cost_max) <= 254;
-        const size_t esz = u8 ? 1 : 2;
-        const int nfam = p->paths == 8 ? 4 : 2;
         HIPCHK(ensure(ctx, B_RKEYS, n * 4), "alloc");
-        for (int f = 0; f < nfam; ++f) HIPCHK(ensure(ctx, (BufId)(B_DS0 + f), nv), "alloc");
-        uint8_t* ds[4] = {buf<uint8_t>(ctx, B_DS0), buf<uint8_t>(ctx, B_DS1), buf<uint8_t>(ctx, B_DS2),
-                          buf<uint8_t>(ctx, B_DS3)};
+        PlaneSet planes{};
+        for (int r = 0; r < npaths; ++r) {
+            HIPCHK(ensure(ctx, (BufId)(B_DS0 + r), nv), "alloc");
+            planes.p[r] = buf<uint8_t>(ctx, (BufId)(B_DS0 + r));
+        }
         uint32_t* rkeys = buf<uint32_t>(ctx, B_RKEYS);
         void* cp = ctx->bufs[B_COST].ptr;   // C' (u8 or u16)
         void* tt = ctx->bufs[B_TMP].ptr;    // T  (u8 or u16)
-        (void)esz;
         if (p->agg_enable) HIPCHK(launch_cross_arms(st, *p, gl, w, h, arms), "arms");
         HIPCHK(launch_fill_u32(st, n, 0xFFFFFFFFu, rkeys), "keys");
         mark(ctx, S_COST);
@@ -483,11 +486,11 @@ int apg_hip_pipeline(apg_hip_ctx ctx, const apg_hip_params* p, const uint8_t* d_
                                     dmax, rkeys, dr),
                "rwta_finalize");
         mark(ctx, S_CROSS);
-        if (p->agg_enable) HIPCHK(launch_cross_v_t(st, *p, w, h, dmin, dmax, arms, tt, cp, u8), "cross_v");
+        if (p->agg_enable) HIPCHK(launch_cross_v_t(st, *p, w, h, dmin, dmax, arms, tt, cp, u8, true), "cross_v");
         mark(ctx, S_SGM);
-        HIPCHK(launch_sgm_pairs(st, *p, w, h, D, gl, cp, u8, ds), "sgm_pairs");
+        HIPCHK(launch_sgm_lines(st, *p, w, h, D, gl, cp, u8, planes), "sgm_lines");
         mark(ctx, S_WTA);
-        HIPCHK(launch_wta_fused(st, *p, w, h, cp, u8, ds, dmin, dmax, dr, gxl, gyl, d_disparity, d_confidence, d_reason,
+        HIPCHK(launch_wta_fused(st, *p, w, h, cp, u8, planes, dmin, dmax, dr, gxl, gyl, d_disparity, d_confidence, d_reason,
                                 d_reliable, inter ? inter->d_disp_wta : nullptr, inter ? inter->d_best : nullptr,
                                 inter ? inter->d_second : nullptr, inter ? inter->d_sum32 : nullptr),
                "wta_fused");

@@ -51,7 +51,49 @@ __device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
     return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
 }
 
+// Minima over the 64 lanes of two independent values, levels interleaved so
+// each DPP read of one value follows the other value's write (no hazard NOPs).
+__device__ __forceinline__ void wave_min2_u32(uint32_t& a, uint32_t& b) {
+    a = min(a, dpp<kDppRowShr1>(0xFFFFFFFFu, a));
+    b = min(b, dpp<kDppRowShr1>(0xFFFFFFFFu, b));
+    a = min(a, dpp<kDppRowShr2>(0xFFFFFFFFu, a));
+    b = min(b, dpp<kDppRowShr2>(0xFFFFFFFFu, b));
+    a = min(a, dpp<kDppRowShr4>(0xFFFFFFFFu, a));
+    b = min(b, dpp<kDppRowShr4>(0xFFFFFFFFu, b));
+    a = min(a, dpp<kDppRowShr8>(0xFFFFFFFFu, a));
+    b = min(b, dpp<kDppRowShr8>(0xFFFFFFFFu, b));
+    a = min(a, dpp<kDppRowBcast15, 0xA>(0xFFFFFFFFu, a));
+    b = min(b, dpp<kDppRowBcast15, 0xA>(0xFFFFFFFFu, b));
+    a = min(a, dpp<kDppRowBcast31, 0xC>(0xFFFFFFFFu, a));
+    b = min(b, dpp<kDppRowBcast31, 0xC>(0xFFFFFFFFu, b));
+    a = (uint32_t)__builtin_amdgcn_readlane((int)a, 63);
+    b = (uint32_t)__builtin_amdgcn_readlane((int)b, 63);
+}
+
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// Raw (stride 0) buffer resource over a device allocation, so a load or store
+// addresses base + lane offset (VGPR) + wave offset (SGPR): per-step address
+// updates stay scalar and 32-bit.  Dword 3 is the gfx9 default format word.
+// Callers keep every offset inside the allocation (< 2^31 bytes).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t raw_rsrc(const void* p) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, 0x7FFFFFFF, 0x00020000);
+}
+__device__ __forceinline__ uint32_t buf_ld16(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    return __builtin_amdgcn_raw_buffer_load_b16(r, (int)voff, (int)soff, 0);
+}
+__device__ __forceinline__ void buf_st16(__amdgpu_buffer_rsrc_t r, uint32_t v, uint32_t voff, uint32_t soff) {
+    __builtin_amdgcn_raw_buffer_store_b16((unsigned short)v, r, (int)voff, (int)soff, 0);
+}
+__device__ __forceinline__ uint32_t buf_ld32(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    return __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0);
+}
+__device__ __forceinline__ void buf_st32(__amdgpu_buffer_rsrc_t r, uint32_t v, uint32_t voff, uint32_t soff) {
+    __builtin_amdgcn_raw_buffer_store_b32(v, r, (int)voff, (int)soff, 0);
+}
+
+// u8 pair (bytes 0, 1 of `raw`) -> u16 halves
+__device__ __forceinline__ uint32_t bytes_to_halves(uint32_t raw) { return __builtin_amdgcn_perm(0u, raw, 0x0c010c00u); }
 
 // detail::adaptive_p2 (reference src/sgm_common.hpp:25-39)
 __device__ __forceinline__ int adaptive_p2(const apg_hip_params& p, int dI) {

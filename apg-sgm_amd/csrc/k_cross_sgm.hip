@@ -251,8 +251,119 @@ __global__ __launch_bounds__(64) void cross_v2_kernel(int w, int h, int d0, int 
     }
 }
 
+// Vertical Cross pass over u8 T for ranges that depend on the column only
+// (the dense range init: [min_disparity, min(max_disparity, x+1))) and
+// min_disparity >= 0.  Then T(x, y, d) is valid iff d lies in column x's
+// range (the pixel itself is in its own H window and x - d < W), so the
+// count of valid entries in a vertical window is the window length
+// U + Dn + 1 or 0, and only sums need prefixes: u16 pairs (window sums
+// < 125 * 255 < 2^15, exact modulo 2^16).  Invalid entries (255) join the
+// sums of d outside the range and are masked at the output.
+// Grid: (column, row segment, 128-wide d chunk), one wavefront each; the
+// prefix of input rows [y0 - Lmax, y1 + Lmax) lives in an LDS ring of
+// 2*Lmax + 2 rows.  floor(s / n) = umulhi(2s, ceil(2^31 / n)), exact for
+// s * n < 2^31.
+constexpr int CV3_PF = 8;
+
+__global__ __launch_bounds__(64) void cross_v3_kernel(int w, int h, int d0, int D, int Lmax, int seg,
+                                                      const int16_t* __restrict__ dmin,
+                                                      const int16_t* __restrict__ dmax,
+                                                      const uint32_t* __restrict__ arms,
+                                                      const uint8_t* __restrict__ tmp, uint8_t* __restrict__ cost) {
+    extern __shared__ uint32_t sh[];
+    const int lane = threadIdx.x;
+    const int x = blockIdx.x;
+    const int y0 = blockIdx.y * seg, y1 = min(h, y0 + seg);
+    if (y0 >= h) return;
+    const int di = blockIdx.z * 128 + 2 * lane;
+    const int rs = 2 * Lmax + 2;
+    const int ya = max(0, y0 - Lmax), yb = min(h, y1 + Lmax), kend = y1 + Lmax, nk = kend - ya;
+    uint32_t* ring = sh;                       // [rs][64] u16-pair prefix sums
+    uint4* meta = (uint4*)(sh + rs * 64);      // [nk] per step: write slot, window slots, magic
+    // step k (input row k, output row y = k - Lmax): byte offsets of the ring
+    // rows P(k + 1), P(y + Dn + 1), P(y - U) and ceil(2^31 / (U + Dn + 1))
+    for (int r = lane; r < nk; r += 64) {
+        const int k = ya + r, y = k - Lmax;
+        uint4 m;
+        m.x = (uint32_t)((r + 1) % rs) * 256u;
+        m.y = m.z = m.w = 0u;
+        if (y >= y0) {
+            const uint32_t a = arms[(size_t)y * w + x];
+            const int U = (a >> 16) & 0xFF, Dn = a >> 24;
+            m.y = (uint32_t)((y + Dn + 1 - ya) % rs) * 256u;
+            m.z = (uint32_t)((y - U - ya) % rs) * 256u;
+            const uint32_t n = (uint32_t)(U + Dn + 1);
+            m.w = (0x80000000u + n - 1u) / n;
+        }
+        meta[r] = m;
+    }
+    ring[lane] = 0u;   // P(ya)
+    __syncthreads();
+    const int lo = dmin[(size_t)y0 * w + x], hi = dmax[(size_t)y0 * w + x];
+    const int dA = d0 + di;
+    const bool inA = di < D;   // D even: the pair is inside or outside as a whole
+    const uint32_t omask = ((inA && dA >= lo && dA < hi) ? 0u : 0xFFu) |
+                           ((inA && dA + 1 >= lo && dA + 1 < hi) ? 0u : 0xFF00u);
+    const uint32_t voff = inA ? (uint32_t)di : 0u;
+    const uint32_t lofs = 4u * (uint32_t)lane;
+    const auto rt = raw_rsrc(tmp);
+    const auto rc = raw_rsrc(cost);
+    const uint32_t rowb = (uint32_t)w * (uint32_t)D, colb = (uint32_t)x * (uint32_t)D;
+    const int kout = y0 + Lmax;   // first step with an output row
+    uint32_t acc = 0;
+    auto step = [&](int k, uint32_t raw, uint4 m) {
+        acc = as_u32(as_v2(acc) + as_v2(bytes_to_halves(raw)));
+        *(uint32_t*)((uint8_t*)ring + m.x + lofs) = acc;
+        if (k < kout) return;
+        const uint32_t p1 = *(const uint32_t*)((const uint8_t*)ring + m.y + lofs);
+        const uint32_t p0 = *(const uint32_t*)((const uint8_t*)ring + m.z + lofs);
+        const uint32_t s = as_u32(as_v2(p1) - as_v2(p0));
+        const uint32_t s2 = as_u32(as_v2(s) + as_v2(s));
+        const uint32_t qa = __umulhi(s2 & 0xFFFFu, m.w), qb = __umulhi(s2 >> 16, m.w);
+        const uint32_t out = (qa | (qb << 8)) | omask;
+        if (inA) buf_st16(rc, out, voff, (uint32_t)(k - Lmax) * rowb + colb);
+    };
+    uint32_t X[CV3_PF], Y[CV3_PF];
+    auto load = [&](uint32_t (&b)[CV3_PF], int kb) {
+#pragma unroll
+        for (int i = 0; i < CV3_PF; ++i) b[i] = buf_ld16(rt, voff, (uint32_t)min(kb + i, yb - 1) * rowb + colb);
+    };
+    auto batch = [&](const uint32_t (&b)[CV3_PF], int kb) {
+        uint4 m[CV3_PF];
+#pragma unroll
+        for (int i = 0; i < CV3_PF; ++i) m[i] = meta[min(kb + i - ya, nk - 1)];
+#pragma unroll
+        for (int i = 0; i < CV3_PF; ++i)
+            if (kb + i < kend) step(kb + i, b[i], m[i]);
+    };
+    load(X, ya);
+    load(Y, ya + CV3_PF);
+    for (int kb = ya; kb < kend; kb += 2 * CV3_PF) {
+        batch(X, kb);
+        load(X, kb + 2 * CV3_PF);
+        batch(Y, kb + CV3_PF);
+        load(Y, kb + 3 * CV3_PF);
+    }
+}
+
 hipError_t launch_cross_v_t(hipStream_t s, const apg_hip_params& p, int w, int h, const int16_t* dmin,
-                            const int16_t* dmax, const uint32_t* arms, const void* tmp, void* cost, bool u8) {
+                            const int16_t* dmax, const uint32_t* arms, const void* tmp, void* cost, bool u8,
+                            bool column_ranges) {
+    {
+        const int D = p.max_disparity - p.min_disparity;
+        const int Lmax = p.max_arm_length > 1 ? p.max_arm_length : 1;
+        if (u8 && column_ranges && p.min_disparity >= 0 && D % 2 == 0 && (size_t)w * h * D < (1ull << 31)) {
+            const int dch = (D + 127) / 128;
+            int nseg = (4096 + w * dch - 1) / (w * dch);
+            nseg = max(1, min(nseg, (h + 63) / 64));
+            const int seg = (h + nseg - 1) / nseg;
+            nseg = (h + seg - 1) / seg;
+            const size_t shm = (size_t)(2 * Lmax + 2) * 64 * sizeof(uint32_t) + (size_t)(seg + 2 * Lmax) * 16;
+            hipLaunchKernelGGL(cross_v3_kernel, dim3(w, nseg, dch), dim3(64), shm, s, w, h, p.min_disparity, D, Lmax,
+                               seg, dmin, dmax, arms, (const uint8_t*)tmp, (uint8_t*)cost);
+            return hipGetLastError();
+        }
+    }
     {
         const int D = p.max_disparity - p.min_disparity;
         const int Lmax = p.max_arm_length > 1 ? p.max_arm_length : 1;
@@ -287,7 +398,7 @@ hipError_t launch_cross_v_t(hipStream_t s, const apg_hip_params& p, int w, int h
 
 hipError_t launch_cross_v(hipStream_t s, const apg_hip_params& p, int w, int h, const int16_t* dmin,
                           const int16_t* dmax, const uint32_t* arms, const uint16_t* tmp, uint16_t* cost) {
-    return launch_cross_v_t(s, p, w, h, dmin, dmax, arms, tmp, cost, false);
+    return launch_cross_v_t(s, p, w, h, dmin, dmax, arms, tmp, cost, false, false);
 }
 
 // ------------------------------------------------------------------ SGM ----

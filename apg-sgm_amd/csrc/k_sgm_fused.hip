@@ -1,96 +1,61 @@
-// gfx950 SGM kernels of the fused schedule.
+// gfx950 SGM kernel of the fused schedule.
 //
 // Every path value is L_r(p,d) = C'(p,d) + delta_r(p,d) with
 // delta_r = min(L'(d), L'(d+-1)+P1, minL'+P2) - minL' in [0, P2] (0 at a scan
 // start or after a pixel without valid state), so the path sum is
-//   S(p,d) = npath * C'(p,d) + sum_r delta_r(p,d).
-// One wavefront owns one scanline family member (a row, a column or a
-// diagonal ray) and runs BOTH of its opposite directions at once: chain A
-// walks forward, chain B backward.  Until they meet each chain stores its
-// delta at its own pixel; after the meeting point each chain reads the other
-// direction's delta at its pixel and stores the pair sum Delta (u8, <= 2*P2
-// <= 254).  Traffic per state and direction pair: C' read twice, one delta
-// store, one delta load, one Delta store.  All families of one launch run
-// concurrently (H + W waves for 4 paths, + 2(W + H - 1) rays for 8 paths).
-// The u16 recurrence keeps 65535 as path infinity exactly as the plain scan.
+//   S(p,d) = npath * C'(p,d) + sum_r delta_r(p,d)
+// and only the deltas (u8, P2max <= 254) are stored, one plane per direction.
+// One wavefront owns one scanline (a row, a column or a diagonal ray) and runs
+// its two opposite directions as independent chains: chain A walks forward,
+// chain B backward.  Lane l holds disparity pairs (2(lK+j), 2(lK+j)+1),
+// j < K, as packed u16 with 65535 as path infinity.  Loads and stores use
+// buffer instructions with scalar per-step offsets; the C' words of the next
+// SL_PF steps of both chains are in flight while the current ones are used.
 #include "device_common.hpp"
 #include "kernels.hpp"
 
 namespace apgk {
 
-// C' element of the fused schedule: u8 with 255 = invalid when every
-// reachable cost is <= 254, else u16 with 65535 = invalid.
-// Loads are split from decoding so that prefetched words stay in flight
-// until they are used (decoding right after a load would force the wait).
-// raw_cpair: the lane's pair (d = 2m, 2m+1) as stored; lanes past D read the
-// pixel's first pair instead (always in bounds) and are masked by decode.
-template <typename TC>
-__device__ __forceinline__ uint32_t raw_cpair(const TC* __restrict__ v, int m, int D) {
-    const int d = 2 * m < D ? 2 * m : 0;   // D even on this path (checked by the launcher)
-    if (sizeof(TC) == 1) return *(const uint16_t*)((const uint8_t*)v + d);
-    return *(const uint32_t*)((const uint16_t*)v + d);
-}
+constexpr int SL_PF = 8;   // steps per prefetch batch (two batches in flight)
 
-// raw pair -> u16 halves with the invalid value 65535 (u8: 255 -> 65535)
-template <typename TC>
-__device__ __forceinline__ uint32_t decode_cpair(uint32_t raw, bool in_range) {
-    if (!in_range) return 0xFFFFFFFFu;
-    if (sizeof(TC) == 2) return raw;
-    const uint32_t x = (raw & 0xFFu) | ((raw & 0xFF00u) << 8);
-    const uint32_t inv = ((x + 0x00010001u) >> 8) & 0x00010001u;   // 1 in halves equal to 255
-    return x | (inv * 0xFF00u);
-}
+// Raw C' words of one lane: K pairs of TC (2, 4 or 8 bytes)
+template <typename TC, int K>
+struct CRaw {
+    static constexpr int kBytes = 2 * K * (int)sizeof(TC);
+    uint32_t w[kBytes > 4 ? 2 : 1];
+};
 
-// delta pair (u16 halves, each < 256) -> 2 bytes
-__device__ __forceinline__ uint32_t pack_bytes(uint32_t v) { return (v & 0xFFu) | ((v >> 8) & 0xFF00u); }
-__device__ __forceinline__ uint32_t unpack_bytes(uint32_t b) { return (b & 0xFFu) | ((b & 0xFF00u) << 8); }
-
-// One recurrence step for a chain (reference src/sgm_optimizer.cpp:44-80).
-// Returns the delta pairs in dl[] and updates prev[]; returns the new minL.
-template <int K>
-__device__ __forceinline__ uint32_t sgm_step(const apg_hip_params& p, uint32_t (&prev)[K], const uint32_t (&c)[K],
-                                             uint32_t minprev, int g, int gprev, bool start, uint32_t (&dl)[K]) {
-    uint32_t L[K];
-    if (start || minprev == 0xFFFFu) {
-#pragma unroll
-        for (int j = 0; j < K; ++j) {
-            L[j] = c[j];
-            dl[j] = 0;
-        }
+// voff2: offset of the second word of an 8-byte lane (the first word's offset
+// again when the lane's last pair lies past D, so nothing past the pixel is read)
+template <typename TC, int K>
+__device__ __forceinline__ CRaw<TC, K> load_c(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t voff2, uint32_t soff) {
+    CRaw<TC, K> v;
+    if constexpr (CRaw<TC, K>::kBytes == 2) {
+        v.w[0] = buf_ld16(r, voff, soff);
+    } else if constexpr (CRaw<TC, K>::kBytes == 4) {
+        v.w[0] = buf_ld32(r, voff, soff);
     } else {
-        const uint32_t P2 = (uint32_t)adaptive_p2(p, g - gprev);
-        const u16x2 mp = splat2(minprev);
-        const u16x2 mp2 = add_sat2(mp, splat2(P2));
-        const u16x2 p1 = splat2((uint32_t)p.P1);
-        const uint32_t left_hi = wave_shr1(prev[K - 1], 0xFFFFFFFFu);
-        const uint32_t right_lo = wave_shl1(prev[0], 0xFFFFFFFFu);
-#pragma unroll
-        for (int j = 0; j < K; ++j) {
-            const uint32_t pj = prev[j];
-            const uint32_t before = j == 0 ? left_hi : prev[j - 1];
-            const uint32_t after = j == K - 1 ? right_lo : prev[j + 1];
-            const u16x2 dm1 = as_v2(__builtin_amdgcn_alignbit(pj, before, 16));
-            const u16x2 dp1 = as_v2(__builtin_amdgcn_alignbit(after, pj, 16));
-            u16x2 t2 = min2(as_v2(pj), min2(add_sat2(dm1, p1), add_sat2(dp1, p1)));
-            t2 = min2(t2, mp2);
-            const u16x2 dd = t2 - mp;
-            dl[j] = as_u32(dd);
-            L[j] = as_u32(add_sat2(as_v2(c[j]), dd));
-        }
+        v.w[0] = buf_ld32(r, voff, soff);
+        v.w[1] = buf_ld32(r, voff2, soff);
     }
-    uint32_t lm = 0xFFFFu;
-#pragma unroll
-    for (int j = 0; j < K; ++j) {
-        prev[j] = L[j];
-        lm = min(lm, min(L[j] & 0xFFFFu, L[j] >> 16));
-    }
-    return wave_min_u32(lm);
+    return v;
 }
 
-constexpr int SF_PF = 8;   // steps per prefetch batch (per chain)
+// pair j of the lane -> u16 halves, invalid (u8 255 / u16 65535) -> 65535
+template <typename TC, int K>
+__device__ __forceinline__ uint32_t decode_c(const CRaw<TC, K>& v, int j) {
+    if constexpr (sizeof(TC) == 2) {
+        return v.w[j];
+    } else {
+        const uint32_t x = __builtin_amdgcn_perm(0u, v.w[0], j == 0 ? 0x0c010c00u : 0x0c030c02u);
+        const u16x2 t = (as_v2(x) + splat2(1)) >> (u16x2)8;   // 1 where the byte is 255
+        return as_u32(t * splat2(0xFF00) + as_v2(x));
+    }
+}
 
-// Scanline of family f, member s: start pixel, step and length of chain A.
-__device__ __forceinline__ void family_line(int f, int s, int w, int h, long& p0, long& step, int& n) {
+// Scanline f (0 rows, 1 columns, 2 diagonal +x+y rays, 3 diagonal +x-y rays),
+// member s: first pixel, pixel step and length of chain A.
+__device__ __forceinline__ void family_line(int f, int s, int w, int h, int& p0, int& step, int& n) {
     int sx, sy, dx, dy;
     if (f == 0) { dx = 1; dy = 0; sx = 0; sy = s; n = w; }
     else if (f == 1) { dx = 0; dy = 1; sx = s; sy = 0; n = h; }
@@ -101,153 +66,197 @@ __device__ __forceinline__ void family_line(int f, int s, int w, int h, long& p0
         const int nx = w - sx, ny = dy > 0 ? h - sy : sy + 1;
         n = min(nx, ny);
     }
-    p0 = (long)sy * w + sx;
-    step = (long)dy * w + dx;
+    p0 = sy * w + sx;
+    step = dy * w + dx;
 }
 
-template <typename TC, int K>
-__global__ __launch_bounds__(64) void sgm_pairs_kernel(apg_hip_params p, int w, int h, int D, int nfam,
+// One chain's state and step (reference src/sgm_optimizer.cpp:21-81).
+template <int K>
+struct Chain {
+    uint32_t prev[K];
+    uint32_t minprev;   // wave-uniform, 65535 when no valid state
+};
+
+// L = C + min(L'(d), L'(d-1)+P1, L'(d+1)+P1, minL'+P2) - minL'; with no valid
+// previous state (all 65535, minL' = 65535) every term saturates and L = C,
+// the delta 0.  p1s is P1 in both halves; p1l / p1r (pairs 0 and K-1) hold
+// 65535 instead in the half whose neighbour lies outside [0, 128K), so the
+// saturating add ignores the neighbour the DPP shift filled with 0.
+// Returns the lane's minimum over its new L values (min of halves); the
+// caller reduces it over the wave into ch.minprev.
+template <int K>
+__device__ __forceinline__ uint32_t chain_step(Chain<K>& ch, const uint32_t (&c)[K], uint32_t P2, uint32_t p1s,
+                                               uint32_t p1l, uint32_t p1r, uint32_t (&dl)[K]) {
+    const uint32_t mp = ch.minprev;
+    const uint32_t mp2 = min(mp + P2, 0xFFFFu);
+    const uint32_t left_hi = (uint32_t)__builtin_amdgcn_mov_dpp((int)ch.prev[K - 1], kDppWaveShr1, 0xF, 0xF, true);
+    const uint32_t right_lo = (uint32_t)__builtin_amdgcn_mov_dpp((int)ch.prev[0], kDppWaveShl1, 0xF, 0xF, true);
+    uint32_t lm = 0xFFFFFFFFu, Ln[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        const uint32_t pj = ch.prev[j];
+        const uint32_t before = j == 0 ? left_hi : ch.prev[j - 1];
+        const uint32_t after = j == K - 1 ? right_lo : ch.prev[j + 1];
+        // (L'(d-1), L'(d)) and (L'(d+1), L'(d+2)) of the pair's first d, plus P1
+        const u16x2 dm1 = add_sat2(as_v2(__builtin_amdgcn_alignbit(pj, before, 16)), as_v2(j == 0 ? p1l : p1s));
+        const u16x2 dp1 = add_sat2(as_v2(__builtin_amdgcn_alignbit(after, pj, 16)), as_v2(j == K - 1 ? p1r : p1s));
+        u16x2 t = min2(min2(dm1, dp1), as_v2(pj));
+        t = min2(t, splat2(mp2));
+        const u16x2 dd = t - splat2(mp);
+        dl[j] = as_u32(dd);
+        Ln[j] = as_u32(add_sat2(as_v2(c[j]), dd));
+        lm = as_u32(min2(as_v2(lm), as_v2(Ln[j])));
+    }
+#pragma unroll
+    for (int j = 0; j < K; ++j) ch.prev[j] = Ln[j];
+    return min(lm & 0xFFFFu, lm >> 16);
+}
+
+// delta pairs (u16 halves < 256) -> bytes of the lane's plane word
+template <int K>
+__device__ __forceinline__ uint32_t pack_deltas(const uint32_t (&dl)[K]) {
+    if constexpr (K == 1) return __builtin_amdgcn_perm(0u, dl[0], 0x0c0c0200u);
+    else return __builtin_amdgcn_perm(dl[1], dl[0], 0x06040200u);
+}
+
+template <typename TC, int K, bool FULL>
+__global__ __launch_bounds__(64) void sgm_lines_kernel(apg_hip_params p, int w, int h, int D, int nfam,
                                                        const uint8_t* __restrict__ gray, const TC* __restrict__ cost,
-                                                       uint8_t* dsum0, uint8_t* dsum1, uint8_t* dsum2, uint8_t* dsum3) {
+                                                       PlaneSet planes) {
     const int lane = threadIdx.x;
     // block -> (family, member): rows, then columns, then the two diagonal ray sets
     int b = blockIdx.x, f = 0;
-    const int cnt[4] = {h, w, w + h - 1, w + h - 1};
-    while (f < nfam - 1 && b >= cnt[f]) { b -= cnt[f]; ++f; }
-    uint8_t* dsum = f == 0 ? dsum0 : f == 1 ? dsum1 : f == 2 ? dsum2 : dsum3;
-    long p0, step;
-    int n;
+    {
+        const int cnt[4] = {h, w, w + h - 1, w + h - 1};
+        while (f < nfam - 1 && b >= cnt[f]) { b -= cnt[f]; ++f; }
+    }
+    int p0, step, n;
     family_line(f, b, w, h, p0, step, n);
-    uint32_t prevA[K], prevB[K];
+    const auto rc = raw_rsrc(cost);
+    const auto rpa = raw_rsrc(planes.p[2 * f]);
+    const auto rpb = raw_rsrc(planes.p[2 * f + 1]);
+    constexpr uint32_t esz = sizeof(TC);
+    // lanes whose pairs lie past D read the pixel's first word and store nothing
+    const bool lin = FULL || 2 * K * lane < D;
+    const uint32_t cvoff = lin ? (uint32_t)(2 * K * lane) * esz : 0u;
+    const uint32_t pvoff = (uint32_t)(2 * K * lane);
+    bool inr[K];
 #pragma unroll
-    for (int j = 0; j < K; ++j) prevA[j] = prevB[j] = 0xFFFFFFFFu;
-    uint32_t minA = 0xFFFFu, minB = 0xFFFFu;
-    int gA = 0, gB = 0;
-    bool inr[K];   // lane's pair j lies inside D
+    for (int j = 0; j < K; ++j) inr[j] = FULL || 2 * (K * lane + j) < D;
+    const uint32_t cvoff2 = (FULL || inr[K - 1]) ? cvoff + 4 : cvoff;
+    const uint32_t P1 = (uint32_t)p.P1;
+    const uint32_t p1s = P1 | (P1 << 16);
+    // lane 0's first d-1 and lane 63's last d+1 lie outside [0, 128K)
+    const uint32_t p1l = lane == 0 ? (0xFFFFu | (P1 << 16)) : p1s;
+    const uint32_t p1r = lane == 63 ? (P1 | 0xFFFF0000u) : p1s;
+    const int pA0 = p0, pB0 = p0 + (n - 1) * step;
+    const uint32_t pstep = (uint32_t)step * (uint32_t)D;
+    Chain<K> A, B;
 #pragma unroll
-    for (int j = 0; j < K; ++j) inr[j] = 2 * (lane * K + j) + 1 < D;
-    // lane byte offset of pair j in a delta plane row: 2*(lane*K + j); lanes past D
-    // write/read pair 0 of the pixel with an unchanged value (see below)
-    auto doff = [&](int j) { return inr[j] ? 2 * (lane * K + j) : 0; };
-    // Raw words of one batch of SF_PF steps for both chains: C' pairs and the
-    // other direction's delta.  The delta word is fetched from the delta plane
-    // when its producing step (n-1-t) finished before the fetch is issued
-    // (before step `done_before`), else from the C' pair just fetched (a cache
-    // hit, ignored).  The next batch is in flight while the current one runs.
-    uint32_t rA[SF_PF][K], rB[SF_PF][K], qA[SF_PF][K], qB[SF_PF][K];
-    auto fetch = [&](int t0, int done_before) {
+    for (int j = 0; j < K; ++j) A.prev[j] = B.prev[j] = 0xFFFFFFFFu;
+    A.minprev = B.minprev = 0xFFFFu;
+    // adaptive P2 of the next 64 steps of each chain, one step per lane; the
+    // gray values of the 64 steps after those are loaded one chunk ahead
+    uint32_t p2A = 0, p2B = 0, lastA = 0, lastB = 0, ngA, ngB;
+    auto load_gray = [&](int t0) {
+        const int tl = min(t0 + lane, n - 1);
+        ngA = gray[pA0 + tl * step];
+        ngB = gray[pB0 - tl * step];
+    };
+    load_gray(0);
+    auto refresh_p2 = [&](int t0) {
+        const uint32_t ga = ngA, gb = ngB;
+        load_gray(t0 + 64);
+        const uint32_t gpa = (uint32_t)__builtin_amdgcn_update_dpp((int)lastA, (int)ga, kDppWaveShr1, 0xF, 0xF, false);
+        const uint32_t gpb = (uint32_t)__builtin_amdgcn_update_dpp((int)lastB, (int)gb, kDppWaveShr1, 0xF, 0xF, false);
+        p2A = (uint32_t)adaptive_p2(p, (int)ga - (int)gpa);
+        p2B = (uint32_t)adaptive_p2(p, (int)gb - (int)gpb);
+        lastA = (uint32_t)__builtin_amdgcn_readlane((int)ga, 63);
+        lastB = (uint32_t)__builtin_amdgcn_readlane((int)gb, 63);
+    };
+    CRaw<TC, K> XA[SL_PF], XB[SL_PF], YA[SL_PF], YB[SL_PF];
+    auto load = [&](CRaw<TC, K> (&ra)[SL_PF], CRaw<TC, K> (&rb)[SL_PF], int t0) {
 #pragma unroll
-        for (int i = 0; i < SF_PF; ++i) {
+        for (int i = 0; i < SL_PF; ++i) {
             const int t = min(t0 + i, n - 1);
-            const long pa = p0 + step * t, pb = p0 + step * (n - 1 - t);
-            const TC* va = cost + (size_t)pa * D;
-            const TC* vb = cost + (size_t)pb * D;
-            const bool od = 2 * t > n - 1 && n - 1 - t < done_before;
-#pragma unroll
-            for (int j = 0; j < K; ++j) {
-                rA[i][j] = raw_cpair<TC>(va, lane * K + j, D);
-                rB[i][j] = raw_cpair<TC>(vb, lane * K + j, D);
-                const uint8_t* ea = od ? dsum + (size_t)pa * D + doff(j) : (const uint8_t*)va;
-                const uint8_t* eb = od ? dsum + (size_t)pb * D + doff(j) : (const uint8_t*)vb;
-                qA[i][j] = *(const uint16_t*)ea;
-                qB[i][j] = *(const uint16_t*)eb;
-            }
+            ra[i] = load_c<TC, K>(rc, cvoff, cvoff2, (uint32_t)(pA0 + t * step) * D * esz);
+            rb[i] = load_c<TC, K>(rc, cvoff, cvoff2, (uint32_t)(pB0 - t * step) * D * esz);
         }
     };
-    fetch(0, -SF_PF);
-    uint32_t gchA = 0, gchB = 0;
-    for (int t0 = 0; t0 < n; t0 += SF_PF) {
-        if ((t0 & 63) == 0) {
-            // gray of the next 64 pixels of both chains, one per lane
-            const int tl = min(t0 + lane, n - 1);
-            gchA = gray[p0 + step * tl];
-            gchB = gray[p0 + step * (n - 1 - tl)];
-        }
-        uint32_t cA[SF_PF][K], cB[SF_PF][K], xA[SF_PF][K], xB[SF_PF][K];
+    uint32_t soA = (uint32_t)pA0 * (uint32_t)D, soB = (uint32_t)pB0 * (uint32_t)D;   // plane offsets
+    auto run = [&](const CRaw<TC, K> (&ra)[SL_PF], const CRaw<TC, K> (&rb)[SL_PF], int t0) {
 #pragma unroll
-        for (int i = 0; i < SF_PF; ++i)
-#pragma unroll
-            for (int j = 0; j < K; ++j) {
-                cA[i][j] = rA[i][j];
-                cB[i][j] = rB[i][j];
-                xA[i][j] = qA[i][j];
-                xB[i][j] = qB[i][j];
-            }
-        // deltas of this batch were prefetched iff their producer preceded t0 - SF_PF
-        const int pf_done = t0 - SF_PF;
-        // a batch that needs some delta not prefetched re-reads all its deltas at use time
-        const int tl_last = min(t0 + SF_PF - 1, n - 1);
-        bool slow = false;
-        for (int t = t0; t <= tl_last; ++t) slow |= (2 * t > n - 1) && !(n - 1 - t < pf_done);
-        if (t0 + SF_PF < n) fetch(t0 + SF_PF, t0);
-#pragma unroll
-        for (int i = 0; i < SF_PF; ++i) {
+        for (int i = 0; i < SL_PF; ++i) {
             const int t = t0 + i;
             if (t >= n) break;
-            const int ga = __builtin_amdgcn_readlane((int)gchA, t & 63);
-            const int gb = __builtin_amdgcn_readlane((int)gchB, t & 63);
-            uint32_t ca[K], cb[K], dA[K], dB[K];
+            if (i == 0 && (t & 63) == 0) refresh_p2(t);
+            const uint32_t P2a = (uint32_t)__builtin_amdgcn_readlane((int)p2A, t & 63);
+            const uint32_t P2b = (uint32_t)__builtin_amdgcn_readlane((int)p2B, t & 63);
+            uint32_t ca[K], cb[K], da[K], db[K];
 #pragma unroll
             for (int j = 0; j < K; ++j) {
-                ca[j] = decode_cpair<TC>(cA[i][j], inr[j]);
-                cb[j] = decode_cpair<TC>(cB[i][j], inr[j]);
+                ca[j] = decode_c<TC, K>(ra[i], j);
+                cb[j] = decode_c<TC, K>(rb[i], j);
+                if (!FULL && !inr[j]) ca[j] = cb[j] = 0xFFFFFFFFu;
             }
-            minA = sgm_step<K>(p, prevA, ca, minA, ga, gA, t == 0, dA);
-            minB = sgm_step<K>(p, prevB, cb, minB, gb, gB, t == 0, dB);
-            gA = ga;
-            gB = gb;
-            const long pa = p0 + step * t, pb = p0 + step * (n - 1 - t);
-            uint8_t* da = dsum + (size_t)pa * D;
-            uint8_t* db = dsum + (size_t)pb * D;
-            // Before the chains meet each stores its own delta; at the meeting
-            // pixel (odd n) both store the pair sum; afterwards each adds the
-            // delta the other direction stored here at step n-1-t.
-            const bool second = 2 * t > n - 1, middle = 2 * t == n - 1;
-#pragma unroll
-            for (int j = 0; j < K; ++j) {
-                uint32_t oA = unpack_bytes(xA[i][j]), oB = unpack_bytes(xB[i][j]);
-                if (slow && second) {
-                    oA = unpack_bytes(*(const uint16_t*)(da + doff(j)));
-                    oB = unpack_bytes(*(const uint16_t*)(db + doff(j)));
-                }
-                const uint32_t addA = second ? oA : (middle ? dB[j] : 0u);
-                const uint32_t addB = second ? oB : (middle ? dA[j] : 0u);
-                if (inr[j]) {
-                    *(uint16_t*)(da + doff(j)) = (uint16_t)pack_bytes(dA[j] + addA);
-                    *(uint16_t*)(db + doff(j)) = (uint16_t)pack_bytes(dB[j] + addB);
+            uint32_t ma = chain_step<K>(A, ca, P2a, p1s, p1l, p1r, da);
+            uint32_t mb = chain_step<K>(B, cb, P2b, p1s, p1l, p1r, db);
+            wave_min2_u32(ma, mb);
+            A.minprev = ma;
+            B.minprev = mb;
+            const uint32_t wa = pack_deltas<K>(da), wb = pack_deltas<K>(db);
+            if (lin) {
+                if (K == 1 || (!FULL && !inr[K - 1])) {
+                    // K == 2 and only pair 0 inside D: the lane's word is half a word
+                    buf_st16(rpa, wa, pvoff, soA);
+                    buf_st16(rpb, wb, pvoff, soB);
+                } else {
+                    buf_st32(rpa, wa, pvoff, soA);
+                    buf_st32(rpb, wb, pvoff, soB);
                 }
             }
+            soA += pstep;
+            soB -= pstep;
         }
+    };
+    load(XA, XB, 0);
+    load(YA, YB, SL_PF);
+    for (int t0 = 0; t0 < n; t0 += 2 * SL_PF) {
+        run(XA, XB, t0);
+        load(XA, XB, t0 + 2 * SL_PF);
+        run(YA, YB, t0 + SL_PF);
+        load(YA, YB, t0 + 3 * SL_PF);
     }
 }
 
 template <typename TC>
-static hipError_t launch_pairs_t(hipStream_t s, const apg_hip_params& p, int w, int h, int D, const uint8_t* gray,
-                                 const TC* cost, uint8_t* const dsum[4]) {
+static hipError_t launch_lines_t(hipStream_t s, const apg_hip_params& p, int w, int h, int D, const uint8_t* gray,
+                                 const TC* cost, const PlaneSet& planes) {
     const int K = (D + 127) / 128;
     const int nfam = p.paths == 8 ? 4 : 2;
     const int nblk = h + w + (nfam == 4 ? 2 * (w + h - 1) : 0);
-#define APG_PAIRS(KK)                                                                                          \
-    case KK:                                                                                                   \
-        hipLaunchKernelGGL((sgm_pairs_kernel<TC, KK>), dim3(nblk), dim3(64), 0, s, p, w, h, D, nfam, gray, cost, \
-                           dsum[0], dsum[1], dsum[2], dsum[3]);                                                \
-        break;
-    switch (K) {
-        APG_PAIRS(1)
-        APG_PAIRS(2)
-        APG_PAIRS(3)
-        APG_PAIRS(4)
-        default: return hipErrorInvalidValue;
+    const bool full = D == 128 * K;
+#define APG_LINES(KK, FF)                                                                                      \
+    hipLaunchKernelGGL((sgm_lines_kernel<TC, KK, FF>), dim3(nblk), dim3(64), 0, s, p, w, h, D, nfam, gray, cost, \
+                       planes)
+    if (K == 1) {
+        if (full) APG_LINES(1, true);
+        else APG_LINES(1, false);
+    } else if (K == 2) {
+        if (full) APG_LINES(2, true);
+        else APG_LINES(2, false);
+    } else {
+        return hipErrorInvalidValue;
     }
-#undef APG_PAIRS
+#undef APG_LINES
     return hipGetLastError();
 }
 
-hipError_t launch_sgm_pairs(hipStream_t s, const apg_hip_params& p, int w, int h, int D, const uint8_t* gray,
-                            const void* cost, bool cost_u8, uint8_t* const dsum[4]) {
-    if (D % 2) return hipErrorInvalidValue;
-    return cost_u8 ? launch_pairs_t<uint8_t>(s, p, w, h, D, gray, (const uint8_t*)cost, dsum)
-                   : launch_pairs_t<uint16_t>(s, p, w, h, D, gray, (const uint16_t*)cost, dsum);
+hipError_t launch_sgm_lines(hipStream_t s, const apg_hip_params& p, int w, int h, int D, const uint8_t* gray,
+                            const void* cost, bool cost_u8, const PlaneSet& planes) {
+    if (D % 2 || D > 256 || (size_t)w * h * D * (cost_u8 ? 1 : 2) >= (1ull << 31)) return hipErrorInvalidValue;
+    return cost_u8 ? launch_lines_t<uint8_t>(s, p, w, h, D, gray, (const uint8_t*)cost, planes)
+                   : launch_lines_t<uint16_t>(s, p, w, h, D, gray, (const uint16_t*)cost, planes);
 }
 
 }  // namespace apgk

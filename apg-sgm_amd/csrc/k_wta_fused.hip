@@ -1,5 +1,5 @@
 // gfx950 kernel of the fused schedule: path sums rebuilt from C' and the
-// per-family delta planes, winner-take-all + second best + legacy subpixel
+// per-direction delta planes (S = npath * C' + sum of the deltas), winner-take-all + second best + legacy subpixel
 // (reference src/sgm_optimizer.cpp:436-555), confidence (reference
 // src/confidence_estimator.cpp:7-63) and the left-right check (reference
 // src/postprocess.cpp:10-40), one lane per pixel.
@@ -16,11 +16,10 @@ constexpr int WF_STRIDE = WF_DC / 2 + 1;   // u32 words per staged pixel row (od
 
 __device__ __forceinline__ float clampf_(float v, float lo, float hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
-template <typename TC>
-__global__ __launch_bounds__(64) void wta_fused_kernel(apg_hip_params p, int w, int h, int D, int nfam,
-                                                       const TC* __restrict__ cost, const uint8_t* __restrict__ ds0,
-                                                       const uint8_t* __restrict__ ds1, const uint8_t* __restrict__ ds2,
-                                                       const uint8_t* __restrict__ ds3, const int16_t* __restrict__ dmin,
+template <typename TC, int NP>
+__global__ __launch_bounds__(64) void wta_fused_kernel(apg_hip_params p, int w, int h, int D,
+                                                       const TC* __restrict__ cost, PlaneSet planes,
+                                                       const int16_t* __restrict__ dmin,
                                                        const int16_t* __restrict__ dmax, const float* __restrict__ dr,
                                                        const uint8_t* __restrict__ gx, const uint8_t* __restrict__ gy,
                                                        float* __restrict__ disp, float* __restrict__ conf_out,
@@ -37,7 +36,7 @@ __global__ __launch_bounds__(64) void wta_fused_kernel(apg_hip_params p, int w, 
     const bool act = lane < npb;
     const int d0 = p.min_disparity;
     const int lo = act ? dmin[pix] : 0, hi = act ? dmax[pix] : 0;
-    const uint32_t npath = nfam == 4 ? 8u : 4u;
+    constexpr uint32_t npath = NP;
     uint32_t bv = 0xFFFFFFFFu, pm = 0xFFFFFFFFu, pm_lag = 0xFFFFFFFFu, secL = 0xFFFFFFFFu, secR = 0xFFFFFFFFu;
     uint32_t raw_prev = 0xFFFFu, v_bm1 = 0xFFFFu, v_bp1 = 0xFFFFu;
     int bd = -1;
@@ -50,17 +49,16 @@ __global__ __launch_bounds__(64) void wta_fused_kernel(apg_hip_params p, int w, 
         const int jw = lane;
         const bool jin = jw < nd / 2;
         const int jo = jin ? 2 * jw : 0;
-        const uint8_t* planes[4] = {ds0, ds1, ds2, ds3};
-        constexpr int WF_G = 8;
+        constexpr int WF_G = NP == 8 ? 4 : 8;
         for (int i0 = 0; i0 < npb; i0 += WF_G) {
-            uint32_t rc[WF_G], rd[WF_G][4];
+            uint32_t rc[WF_G], rd[WF_G][NP];
 #pragma unroll
             for (int g = 0; g < WF_G; ++g) {
                 const size_t q = (p_base + min(i0 + g, npb - 1)) * (size_t)D + dc + jo;
                 rc[g] = sizeof(TC) == 1 ? (uint32_t) * (const uint16_t*)((const uint8_t*)cost + q)
                                         : *(const uint32_t*)((const uint16_t*)cost + q);
 #pragma unroll
-                for (int f = 0; f < 4; ++f) rd[g][f] = f < nfam ? (uint32_t) * (const uint16_t*)(planes[f] + q) : 0u;
+                for (int f = 0; f < NP; ++f) rd[g][f] = (uint32_t) * (const uint16_t*)(planes.p[f] + q);
             }
 #pragma unroll
             for (int g = 0; g < WF_G; ++g) {
@@ -72,7 +70,7 @@ __global__ __launch_bounds__(64) void wta_fused_kernel(apg_hip_params p, int w, 
                 else { c0 = rc[g] & 0xFFFF; c1 = rc[g] >> 16; }
                 uint32_t a0 = npath * c0, a1 = npath * c1;
 #pragma unroll
-                for (int f = 0; f < 4; ++f) {
+                for (int f = 0; f < NP; ++f) {
                     a0 += rd[g][f] & 0xFF;
                     a1 += (rd[g][f] >> 8) & 0xFF;
                 }
@@ -178,22 +176,24 @@ __global__ __launch_bounds__(64) void wta_fused_kernel(apg_hip_params p, int w, 
 }
 
 hipError_t launch_wta_fused(hipStream_t s, const apg_hip_params& p, int w, int h, const void* cost, bool cost_u8,
-                            uint8_t* const dsum[4], const int16_t* dmin, const int16_t* dmax, const float* dr,
+                            const PlaneSet& planes, const int16_t* dmin, const int16_t* dmax, const float* dr,
                             const uint8_t* gx, const uint8_t* gy, float* disp, float* conf, uint8_t* reason,
                             uint8_t* reliable, float* disp_wta, float* best, float* second, uint32_t* sum32) {
     const int D = p.max_disparity - p.min_disparity;
     if (D % 2) return hipErrorInvalidValue;
-    const int nfam = p.paths == 8 ? 4 : 2;
     const size_t npx = (size_t)w * h;
     dim3 grid((unsigned)((npx + WF_PB - 1) / WF_PB));
-    if (cost_u8)
-        hipLaunchKernelGGL(wta_fused_kernel<uint8_t>, grid, dim3(64), 0, s, p, w, h, D, nfam, (const uint8_t*)cost,
-                           dsum[0], dsum[1], dsum[2], dsum[3], dmin, dmax, dr, gx, gy, disp, conf, reason, reliable,
-                           disp_wta, best, second, sum32);
-    else
-        hipLaunchKernelGGL(wta_fused_kernel<uint16_t>, grid, dim3(64), 0, s, p, w, h, D, nfam, (const uint16_t*)cost,
-                           dsum[0], dsum[1], dsum[2], dsum[3], dmin, dmax, dr, gx, gy, disp, conf, reason, reliable,
-                           disp_wta, best, second, sum32);
+#define APG_WTAF(TC, NP)                                                                                          \
+    hipLaunchKernelGGL((wta_fused_kernel<TC, NP>), grid, dim3(64), 0, s, p, w, h, D, (const TC*)cost, planes, dmin, \
+                       dmax, dr, gx, gy, disp, conf, reason, reliable, disp_wta, best, second, sum32)
+    if (p.paths == 8) {
+        if (cost_u8) APG_WTAF(uint8_t, 8);
+        else APG_WTAF(uint16_t, 8);
+    } else {
+        if (cost_u8) APG_WTAF(uint8_t, 4);
+        else APG_WTAF(uint16_t, 4);
+    }
+#undef APG_WTAF
     return hipGetLastError();
 }
 

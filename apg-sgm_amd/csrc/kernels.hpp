@@ -66,8 +66,10 @@ hipError_t launch_median(hipStream_t s, const apg_hip_params& p, int w, int h, c
 // Largest quantised cost any state can take (bound from the LUT maxima);
 // <= 254 allows u8 storage of T and C' with 255 as the invalid value.
 int cost_upper_bound(const CostLut& lut, int cost_max);
+// column_ranges: every pixel's range depends on its column only (dense range init)
 hipError_t launch_cross_v_t(hipStream_t s, const apg_hip_params& p, int w, int h, const int16_t* dmin,
-                            const int16_t* dmax, const uint32_t* arms, const void* tmp, void* cost, bool u8);
+                            const int16_t* dmax, const uint32_t* arms, const void* tmp, void* cost, bool u8,
+                            bool column_ranges);
 hipError_t launch_fill_u32(hipStream_t s, size_t n, uint32_t v, uint32_t* p);
 hipError_t launch_cost_crossh(hipStream_t s, const apg_hip_params& p, const CostLut* d_lut, int w, int h,
                               const uint8_t* gl, const uint8_t* gxl, const uint8_t* gyl, const uint8_t* gr,
@@ -79,10 +81,14 @@ hipError_t launch_rwta_finalize(hipStream_t s, const apg_hip_params& p, const Co
                                 const uint8_t* gxr, const uint8_t* gyr, const uint32_t* c32l, const uint32_t* c32r,
                                 const uint64_t* c64l, const uint64_t* c64r, const int16_t* dmin, const int16_t* dmax,
                                 const uint32_t* rkeys, float* dr);
-hipError_t launch_sgm_pairs(hipStream_t s, const apg_hip_params& p, int w, int h, int D, const uint8_t* gray,
-                            const void* cost, bool cost_u8, uint8_t* const dsum[4]);
+// One u8 delta plane per SGM direction (2 per scanline family: forward, backward)
+struct PlaneSet {
+    uint8_t* p[8];
+};
+hipError_t launch_sgm_lines(hipStream_t s, const apg_hip_params& p, int w, int h, int D, const uint8_t* gray,
+                            const void* cost, bool cost_u8, const PlaneSet& planes);
 hipError_t launch_wta_fused(hipStream_t s, const apg_hip_params& p, int w, int h, const void* cost, bool cost_u8,
-                            uint8_t* const dsum[4], const int16_t* dmin, const int16_t* dmax, const float* dr,
+                            const PlaneSet& planes, const int16_t* dmin, const int16_t* dmax, const float* dr,
                             const uint8_t* gx, const uint8_t* gy, float* disp, float* conf, uint8_t* reason,
                             uint8_t* reliable, float* disp_wta, float* best, float* second, uint32_t* sum32);
 
