@@ -186,14 +186,262 @@ __global__ __launch_bounds__(256) void cost_crossh_kernel(
     }
 }
 
+// Sliding form of the same pass for u8 T and ranges that depend on the
+// column only (the dense range init [min_disparity, min(max_disparity, x+1))).
+// One wavefront walks a row segment of K2_SEG output columns plus the arm
+// halo, 128 disparities at a time (lane l: d-indices 2l, 2l+1).  Per column it
+// computes the two costs, adds them to u16-pair prefix sums kept in an LDS
+// ring of 2*Lmax+2 columns (window sums < 125*255 < 2^16), and Lmax columns
+// behind emits T of one output column.  With column ranges, C(xx, d) is valid
+// iff d <= xx < W + d, so the valid count of the window [x-L, x+R] is
+// |[max(x-L, d), min(x+R, W-1+d)]| and only sums need prefixes; for most
+// columns it is L+R+1 for every lane and the divisor is uniform.
+// floor(s / n) = umulhi(2s, ceil(2^31 / n)) (exact for s * n < 2^31).
+// The loop is software-pipelined: the records of column c+1 are read while
+// column c computes, and an output column's window sums are read right after
+// the prefix write and consumed one column later.
+constexpr int K2_SEG = 256;
+
+struct K2Shm {
+    size_t lut, mag, col, chi, outr, rr, key, ring, total;
+};
+
+__host__ __device__ inline K2Shm k2_layout(int D, int Lmax, int seg) {
+    const int ncol = seg + 2 * Lmax;
+    K2Shm s;
+    size_t o = 0;
+    s.lut = o; o += (65 + 256 + 511) * 4;
+    s.mag = o; o += (2 * kMaxArm + 2) * 4;
+    o = (o + 15) & ~(size_t)15;
+    s.col = o; o += (size_t)(ncol + 1) * 16;   // + the record read ahead of the last column
+    s.chi = o; o += (size_t)(ncol + 1) * 4;
+    o = (o + 15) & ~(size_t)15;
+    s.outr = o; o += (size_t)(seg + 1) * 16;
+    s.rr = o; o += (size_t)(ncol + D) * 16;
+    s.key = o; o += (size_t)(seg + D - 1) * 4;
+    o = (o + 15) & ~(size_t)15;
+    s.ring = o; o += (size_t)(2 * Lmax + 2) * 256;
+    s.total = o;
+    return s;
+}
+
+constexpr int kFarLeft = -(1 << 24);   // x of a column outside the image: x - d is never in [0, W)
+
+template <bool SYM, bool FULL>
+__global__ __launch_bounds__(64) void cost_crossh2_kernel(
+    const CostLut* __restrict__ lut, int w, int h, int d0, int D, int cost_max, int Lmax, int seg,
+    const uint8_t* __restrict__ gl, const uint8_t* __restrict__ gxl, const uint8_t* __restrict__ gyl,
+    const uint8_t* __restrict__ gr, const uint8_t* __restrict__ gxr, const uint8_t* __restrict__ gyr,
+    const uint32_t* __restrict__ c32l, const uint32_t* __restrict__ c32r, const uint64_t* __restrict__ c64l,
+    const uint64_t* __restrict__ c64r, const uint32_t* __restrict__ arms, uint8_t* __restrict__ out,
+    uint32_t* __restrict__ rkeys) {
+    extern __shared__ __align__(16) unsigned char shm[];
+    const K2Shm L = k2_layout(D, Lmax, seg);
+    float* s_lc = (float*)(shm + L.lut);
+    float* s_la = s_lc + 65;
+    float* s_lg = s_la + 256;
+    uint32_t* s_mag = (uint32_t*)(shm + L.mag);
+    uint4* s_col = (uint4*)(shm + L.col);    // census lo, g|gx<<8|gy<<16, x (kFarLeft outside), ring byte offset of P(c+1)
+    uint32_t* s_chi = (uint32_t*)(shm + L.chi);   // census hi (64-bit census)
+    uint4* s_out = (uint4*)(shm + L.outr);   // ring byte offsets P(x+R+1) | P(x-L) << 16, (x-L) | (x+R) << 16,
+                                             // ceil(2^31/(L+R+1)), byte offset of T(x, 0)
+    uint4* s_rr = (uint4*)(shm + L.rr);      // right pixel: census lo, g, gx|gy<<8, census hi
+    uint32_t* s_key = (uint32_t*)(shm + L.key);
+    unsigned char* s_ring = shm + L.ring;    // [2*Lmax+2][64 lanes] u16-pair prefix sums
+
+    const int lane = threadIdx.x;
+    const int y = blockIdx.y;
+    const int x0 = blockIdx.x * seg;
+    const int ncol = seg + 2 * Lmax, rs = 2 * Lmax + 2;
+    const int xb = x0 - Lmax;              // column 0
+    const int xs = xb - (d0 + D - 1);      // right pixel of s_rr[0]
+    const int kbase = x0 - (d0 + D - 1);   // right pixel of s_key[0]
+    const int nown = min(seg, w - x0);
+    const size_t row = (size_t)y * w;
+
+    for (int i = lane; i < 65; i += 64) s_lc[i] = lut->census[i];
+    for (int i = lane; i < 256; i += 64) s_la[i] = lut->ad[i];
+    for (int i = lane; i < 511; i += 64) s_lg[i] = lut->grad[i];
+    for (int n = lane; n < 2 * kMaxArm + 2; n += 64) s_mag[n] = n ? (0x80000000u + (uint32_t)n - 1u) / (uint32_t)n : 0u;
+    for (int c = lane; c <= ncol; c += 64) {
+        const int x = xb + c;
+        uint4 r = make_uint4(0u, 0u, (uint32_t)kFarLeft, (uint32_t)(((c + 1) % rs) * 256));
+        uint32_t hi = 0u;
+        if (c < ncol && x >= 0 && x < w) {
+            const size_t q = row + x;
+            if (SYM) r.x = c32l[q];
+            else { const uint64_t v = c64l[q]; r.x = (uint32_t)v; hi = (uint32_t)(v >> 32); }
+            r.y = (uint32_t)gl[q] | ((uint32_t)gxl[q] << 8) | ((uint32_t)gyl[q] << 16);
+            r.z = (uint32_t)x;
+        }
+        s_col[c] = r;
+        if (!SYM) s_chi[c] = hi;
+    }
+    for (int o = lane; o <= seg; o += 64) {
+        const int x = x0 + o;
+        uint4 r = make_uint4(0u, 0u, 0u, 0u);
+        if (o < seg && x < w) {
+            const uint32_t a = arms[row + x];
+            const int La = a & 0xFF, Ra = (a >> 8) & 0xFF;
+            const int co = Lmax + o;   // column index of x; P(j) sums columns < j
+            r.x = (uint32_t)(((co + Ra + 1) % rs) * 256) | ((uint32_t)(((co - La) % rs) * 256) << 16);
+            r.y = (uint32_t)(x - La) | ((uint32_t)(x + Ra) << 16);
+            const uint32_t n = (uint32_t)(La + Ra + 1);
+            r.z = (0x80000000u + n - 1u) / n;
+            r.w = (uint32_t)((row + x) * D);
+        }
+        s_out[o] = r;
+    }
+    for (int i = lane; i < ncol + D; i += 64) {
+        const int xr = xs + i;
+        uint4 r = make_uint4(0u, 0u, 0u, 0u);
+        if (i < ncol + D - 1 && xr >= 0 && xr < w) {
+            const size_t q = row + xr;
+            if (SYM) r.x = c32r[q];
+            else { const uint64_t v = c64r[q]; r.x = (uint32_t)v; r.w = (uint32_t)(v >> 32); }
+            r.y = gr[q];
+            r.z = (uint32_t)gxr[q] | ((uint32_t)gyr[q] << 8);
+        }
+        s_rr[i] = r;
+    }
+    for (int i = lane; i < seg + D - 1; i += 64) s_key[i] = 0xFFFFFFFFu;
+    __syncthreads();
+
+    const float scale = lut->scale;
+    const auto rout = raw_rsrc(out);
+    const uint32_t lofs = 4u * (uint32_t)lane;
+    const uint32_t uw = (uint32_t)w;
+    for (int dc = 0; dc < D; dc += 128) {
+        const int di0 = dc + 2 * lane;   // the lane's pair (di0, di0 + 1); D even
+        const int dA = d0 + di0;
+        const bool act = FULL || di0 < D;
+        const int dtop = d0 + min(dc + 127, D - 1);   // largest d of the chunk
+        const bool nonneg = d0 >= 0;
+        // record of the right pixel x - d for (c, di) is c + D - 1 - di; lanes past D
+        // read records 0 and 1 (their states are masked)
+        const int rbase = act ? D - 1 - di0 : 1;
+        *(uint32_t*)(s_ring + lofs) = 0u;   // P(0)
+        uint32_t acc = 0u;
+        uint4 cr_n = s_col[0], r0_n = s_rr[rbase], r1_n = s_rr[rbase - 1], or_n = s_out[0];
+        uint32_t ch_n = SYM ? 0u : s_chi[0];
+        uint32_t pp1 = 0u, pp0 = 0u;
+        uint4 por = make_uint4(0u, 0u, 0u, 0u);
+        int pxo = 0;
+        bool pend = false;
+
+        auto flush = [&]() {   // T of the column whose window sums were read last step
+            const uint32_t sw = as_u32(as_v2(pp1) - as_v2(pp0));
+            const int xlo = (int)(por.y & 0xFFFFu), xhi = (int)(por.y >> 16);
+            uint32_t q0, q1;
+            if (nonneg && dtop <= xlo) {
+                // every lane's window is whole: count L+R+1, T valid (d <= x-L <= x)
+                q0 = __umulhi(2u * (sw & 0xFFFFu), por.z);
+                q1 = __umulhi(2u * (sw >> 16), por.z);
+            } else {
+                uint32_t qq[2];
+#pragma unroll
+                for (int e = 0; e < 2; ++e) {
+                    const int d = dA + e;
+                    const int cnt = min(xhi, w - 1 + d) - max(xlo, d) + 1;
+                    const uint32_t se = e ? (sw >> 16) : (sw & 0xFFFFu);
+                    const bool v = d <= pxo && cnt > 0;   // d in [min, min(max, x+1)), window not empty
+                    qq[e] = v ? __umulhi(2u * se, s_mag[cnt > 0 ? cnt : 0]) : 0xFFu;
+                }
+                q0 = qq[0];
+                q1 = qq[1];
+            }
+            if (act) buf_st16(rout, q0 | (q1 << 8), (uint32_t)di0, por.w);
+        };
+
+        auto step = [&](int c, bool keys, bool outp) {
+            const uint4 cr = cr_n, r0 = r0_n, r1 = r1_n;
+            const uint32_t chl = ch_n;
+            // records of column c + 1 (s_col[ncol] and s_rr[ncol + D - 1] are padding)
+            cr_n = s_col[c + 1];
+            if (!SYM) ch_n = s_chi[c + 1];
+            r0_n = s_rr[rbase + c + 1];
+            r1_n = s_rr[rbase + c];
+            const uint32_t g0 = cr.y & 0xFFu, gxy = cr.y >> 8;
+            const int pop0 = SYM ? __popc(cr.x ^ r0.x) : __popc(cr.x ^ r0.x) + __popc(chl ^ r0.w);
+            const int pop1 = SYM ? __popc(cr.x ^ r1.x) : __popc(cr.x ^ r1.x) + __popc(chl ^ r1.w);
+            const uint32_t ad0 = __builtin_amdgcn_sad_u8(r0.y, g0, 0u), ad1 = __builtin_amdgcn_sad_u8(r1.y, g0, 0u);
+            const uint32_t gg0 = __builtin_amdgcn_sad_u8(r0.z, gxy, 0u), gg1 = __builtin_amdgcn_sad_u8(r1.z, gxy, 0u);
+            // same operation order as state_cost (reference src/cost_computer.cpp:176-199)
+            const float f0 = (s_lc[pop0] + s_la[ad0]) + s_lg[gg0];
+            const float f1 = (s_lc[pop1] + s_la[ad1]) + s_lg[gg1];
+            if (pend) flush();
+            const uint32_t cst0 = (uint32_t)clampi((int)(f0 * scale + 0.5f), 0, cost_max);
+            const uint32_t cst1 = (uint32_t)clampi((int)(f1 * scale + 0.5f), 0, cost_max);
+            const int xr0 = (int)cr.z - dA;   // right pixel of d; d + 1 has xr0 - 1
+            const bool v0 = act && (uint32_t)xr0 < uw, v1 = act && (uint32_t)(xr0 - 1) < uw;
+            if (keys) {
+                const int ki = xr0 - kbase;
+                const int k0 = FULL ? ki : max(ki, 1);
+                atomicMin(&s_key[k0], v0 ? (cst0 << 16) | (uint32_t)di0 : 0xFFFFFFFFu);
+                atomicMin(&s_key[k0 - 1], v1 ? (cst1 << 16) | (uint32_t)(di0 + 1) : 0xFFFFFFFFu);
+            }
+            acc = as_u32(as_v2(acc) + as_v2((v0 ? cst0 : 0u) | ((v1 ? cst1 : 0u) << 16)));
+            *(uint32_t*)(s_ring + cr.w + lofs) = acc;   // P(c + 1)
+            pend = outp;
+            if (outp) {
+                por = or_n;
+                or_n = s_out[c - 2 * Lmax + 1];
+                pxo = x0 + c - 2 * Lmax;
+                pp1 = *(const uint32_t*)(s_ring + (por.x & 0xFFFFu) + lofs);
+                pp0 = *(const uint32_t*)(s_ring + (por.x >> 16) + lofs);
+            }
+        };
+
+        // phases of c: keys for own columns [Lmax, Lmax + nown), output (Lmax
+        // behind) for [2Lmax, 2Lmax + nown); columns past 2Lmax + nown lie beyond
+        // the image and feed nothing
+        const int kb = Lmax, ke = Lmax + nown, ob = 2 * Lmax, oe = 2 * Lmax + nown;
+        int c = 0;
+        while (c < oe) {
+            const bool inK = c >= kb && c < ke, inO = c >= ob && c < oe;
+            int nx = oe;
+            if (c < kb) nx = min(nx, kb);
+            if (c < ke) nx = min(nx, ke);
+            if (c < ob) nx = min(nx, ob);
+            if (inK && inO) for (; c < nx; ++c) step(c, true, true);
+            else if (inK) for (; c < nx; ++c) step(c, true, false);
+            else if (inO) for (; c < nx; ++c) step(c, false, true);
+            else for (; c < nx; ++c) step(c, false, false);
+        }
+        if (pend) flush();
+    }
+    __syncthreads();
+    for (int i = lane; i < seg + D - 1; i += 64) {
+        const int xr = kbase + i;
+        const uint32_t k = s_key[i];
+        if (xr >= 0 && xr < w && k != 0xFFFFFFFFu) atomicMin(&rkeys[row + xr], k);
+    }
+}
+
 hipError_t launch_cost_crossh(hipStream_t s, const apg_hip_params& p, const CostLut* d_lut, int w, int h,
                               const uint8_t* gl, const uint8_t* gxl, const uint8_t* gyl, const uint8_t* gr,
                               const uint8_t* gxr, const uint8_t* gyr, const uint32_t* c32l, const uint32_t* c32r,
                               const uint64_t* c64l, const uint64_t* c64r, const int16_t* dmin, const int16_t* dmax,
-                              const uint32_t* arms, void* out, bool out_u8, uint32_t* rkeys) {
+                              const uint32_t* arms, void* out, bool out_u8, uint32_t* rkeys, bool column_ranges) {
     const int D = p.max_disparity - p.min_disparity;
     const bool cross = p.agg_enable != 0;
     const int Lmax = cross ? (p.max_arm_length > 1 ? p.max_arm_length : 1) : 0;
+    if (cross && out_u8 && column_ranges && D % 2 == 0 && (size_t)w * h * D < (1ull << 31)) {
+        const int seg = w < K2_SEG ? w : K2_SEG;
+        const K2Shm L2 = k2_layout(D, Lmax, seg);
+        const bool full = D % 128 == 0;
+        auto k = p.census_type == 1 ? (full ? cost_crossh2_kernel<true, true> : cost_crossh2_kernel<true, false>)
+                                    : (full ? cost_crossh2_kernel<false, true> : cost_crossh2_kernel<false, false>);
+        if (L2.total > 64 * 1024) {
+            hipError_t e =
+                hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)L2.total);
+            if (e != hipSuccess) return e;
+        }
+        hipLaunchKernelGGL(k, dim3((w + seg - 1) / seg, h), dim3(64), L2.total, s, d_lut, w, h, p.min_disparity, D,
+                           p.cost_max, Lmax, seg, gl, gxl, gyl, gr, gxr, gyr, c32l, c32r, c64l, c64r, arms,
+                           (uint8_t*)out, rkeys);
+        return hipGetLastError();
+    }
     const FusedShm L = fused_layout(D, Lmax);
     dim3 grid((w + F_TX - 1) / F_TX, h);
 #define APG_LAUNCH_F(S, X, T)                                                                                      \

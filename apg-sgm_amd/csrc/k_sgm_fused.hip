@@ -132,6 +132,9 @@ __global__ __launch_bounds__(64) void sgm_lines_kernel(apg_hip_params p, int w, 
     }
     int p0, step, n;
     family_line(f, b, w, h, p0, step, n);
+    // the longest scanlines bound the kernel's duration: their waves win
+    // instruction arbitration against the shorter ones sharing a SIMD
+    if (n >= w && n >= h) __builtin_amdgcn_s_setprio(2);
     const auto rc = raw_rsrc(cost);
     const auto rpa = raw_rsrc(planes.p[2 * f]);
     const auto rpb = raw_rsrc(planes.p[2 * f + 1]);

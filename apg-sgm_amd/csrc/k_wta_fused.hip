@@ -10,14 +10,20 @@
 
 namespace apgk {
 
-constexpr int WF_PB = 64;    // pixels per block, one per lane
-constexpr int WF_DC = 128;   // disparities per staged chunk
-constexpr int WF_STRIDE = WF_DC / 2 + 1;   // u32 words per staged pixel row (odd: conflict-free)
+constexpr int WF_PB = 64;   // pixels per block, one per lane
+constexpr int WF_G = 4;     // pixels whose raw words are in flight together while staging
 
 __device__ __forceinline__ float clampf_(float v, float lo, float hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
+// S of the block's 64 pixels is staged in LDS as u16 (row of a pixel: D
+// values, 65535 = invalid, stride `rw` words, odd so that the lanes' rows
+// fall in different banks).  C' is invalid outside the pixel's range, so S
+// needs no separate range mask.  Then each lane scans its own row twice:
+//   1. best = min over d of (S << 8 | d): the earliest minimum (D <= 256);
+//   2. after S(best-1), S(best+1) are read and S at best-1..best+1 is set to
+//      65535, second = min of the row (packed u16 minima).
 template <typename TC, int NP>
-__global__ __launch_bounds__(64) void wta_fused_kernel(apg_hip_params p, int w, int h, int D,
+__global__ __launch_bounds__(64) void wta_fused_kernel(apg_hip_params p, int w, int h, int D, int rw,
                                                        const TC* __restrict__ cost, PlaneSet planes,
                                                        const int16_t* __restrict__ dmin,
                                                        const int16_t* __restrict__ dmax, const float* __restrict__ dr,
@@ -26,8 +32,7 @@ __global__ __launch_bounds__(64) void wta_fused_kernel(apg_hip_params p, int w, 
                                                        uint8_t* __restrict__ reason, uint8_t* __restrict__ reliable,
                                                        float* __restrict__ disp_wta, float* __restrict__ best_out,
                                                        float* __restrict__ second_out, uint32_t* __restrict__ sum32) {
-    __shared__ uint32_t st[WF_PB * WF_STRIDE];
-    constexpr uint32_t CINV = sizeof(TC) == 1 ? 0xFFu : 0xFFFFu;
+    extern __shared__ uint32_t st[];   // [64][rw] words
     const int lane = threadIdx.x;
     const size_t npx = (size_t)w * h;
     const size_t p_base = (size_t)blockIdx.x * WF_PB;
@@ -35,97 +40,93 @@ __global__ __launch_bounds__(64) void wta_fused_kernel(apg_hip_params p, int w, 
     const size_t pix = p_base + lane;
     const bool act = lane < npb;
     const int d0 = p.min_disparity;
-    const int lo = act ? dmin[pix] : 0, hi = act ? dmax[pix] : 0;
-    constexpr uint32_t npath = NP;
-    uint32_t bv = 0xFFFFFFFFu, pm = 0xFFFFFFFFu, pm_lag = 0xFFFFFFFFu, secL = 0xFFFFFFFFu, secR = 0xFFFFFFFFu;
-    uint32_t raw_prev = 0xFFFFu, v_bm1 = 0xFFFFu, v_bp1 = 0xFFFFu;
-    int bd = -1;
-    for (int dc = 0; dc < D; dc += WF_DC) {
-        const int nd = min(WF_DC, D - dc);   // even
-        // stage S for the block's pixels: element (i, j) of the chunk at st[i*STRIDE + j/2] (u16 halves)
-        // one pixel's chunk per wave iteration: 64 lanes x 2 disparities,
-        // contiguous; WF_G pixels of raw words are loaded before any is decoded
-        // (lanes past the chunk read its first pair and store nothing)
-        const int jw = lane;
-        const bool jin = jw < nd / 2;
-        const int jo = jin ? 2 * jw : 0;
-        constexpr int WF_G = NP == 8 ? 4 : 8;
+    const int nw = D / 2;   // words per row (D even)
+    const auto rc = raw_rsrc(cost);
+    // staging: one pixel per wave iteration, lane = pair index (two passes for D > 128)
+    for (int j0 = 0; j0 < nw; j0 += 64) {
+        const int jw = j0 + lane;
+        const bool jin = jw < nw;
+        const uint32_t cvo = (uint32_t)(jin ? jw : 0) * 2u * (uint32_t)sizeof(TC);
+        const uint32_t pvo = (uint32_t)(jin ? jw : 0) * 2u;
         for (int i0 = 0; i0 < npb; i0 += WF_G) {
-            uint32_t rc[WF_G], rd[WF_G][NP];
+            uint32_t rcw[WF_G], rd[WF_G][NP];
 #pragma unroll
             for (int g = 0; g < WF_G; ++g) {
-                const size_t q = (p_base + min(i0 + g, npb - 1)) * (size_t)D + dc + jo;
-                rc[g] = sizeof(TC) == 1 ? (uint32_t) * (const uint16_t*)((const uint8_t*)cost + q)
-                                        : *(const uint32_t*)((const uint16_t*)cost + q);
+                const uint32_t px = (uint32_t)(p_base + min(i0 + g, npb - 1));
+                rcw[g] = sizeof(TC) == 1 ? buf_ld16(rc, cvo, px * (uint32_t)D) : buf_ld32(rc, cvo, px * 2u * (uint32_t)D);
 #pragma unroll
-                for (int f = 0; f < NP; ++f) rd[g][f] = (uint32_t) * (const uint16_t*)(planes.p[f] + q);
+                for (int f = 0; f < NP; ++f) rd[g][f] = buf_ld16(raw_rsrc(planes.p[f]), pvo, px * (uint32_t)D);
             }
 #pragma unroll
             for (int g = 0; g < WF_G; ++g) {
                 const int i = i0 + g;
                 if (i >= npb || !jin) continue;
-                const size_t q = (p_base + i) * (size_t)D + dc + 2 * jw;
-                uint32_t c0, c1;
-                if (sizeof(TC) == 1) { c0 = rc[g] & 0xFF; c1 = (rc[g] >> 8) & 0xFF; }
-                else { c0 = rc[g] & 0xFFFF; c1 = rc[g] >> 16; }
-                uint32_t a0 = npath * c0, a1 = npath * c1;
-#pragma unroll
-                for (int f = 0; f < NP; ++f) {
-                    a0 += rd[g][f] & 0xFF;
-                    a1 += (rd[g][f] >> 8) & 0xFF;
+                uint32_t c16;
+                if (sizeof(TC) == 1) {
+                    const uint32_t x = bytes_to_halves(rcw[g]);
+                    c16 = as_u32(as_v2(x) - (as_v2((x + 0x00010001u) & 0xFF00FF00u)));   // 255 -> 65535
+                } else {
+                    c16 = rcw[g];
                 }
-                const uint32_t s0 = c0 == CINV ? 0xFFFFu : a0, s1 = c1 == CINV ? 0xFFFFu : a1;
-                st[i * WF_STRIDE + jw] = s0 | (s1 << 16);
+                u16x2 ds = as_v2(bytes_to_halves(rd[g][0]));
+#pragma unroll
+                for (int f = 1; f < NP; ++f) ds = ds + as_v2(bytes_to_halves(rd[g][f]));
+                // S = npath * C' + sum of deltas (< 65535 for valid C', checked by the
+                // host); halves with invalid C' (65535) become 65535
+                const uint32_t sv = as_u32(as_v2(c16) * splat2(NP) + ds);
+                const uint32_t inv = as_u32((as_v2(c16) + splat2(1)) == splat2(0) ? splat2(0xFFFF) : splat2(0));
+                const uint32_t S = sv | inv;
+                st[i * rw + jw] = S;
                 if (sum32) {
-                    sum32[q] = s0 == 0xFFFFu ? 0xFFFFFFFFu : s0;
-                    sum32[q + 1] = s1 == 0xFFFFu ? 0xFFFFFFFFu : s1;
+                    const size_t q = (p_base + i) * (size_t)D + 2 * jw;
+                    sum32[q] = (S & 0xFFFFu) == 0xFFFFu ? 0xFFFFFFFFu : (S & 0xFFFFu);
+                    sum32[q + 1] = (S >> 16) == 0xFFFFu ? 0xFFFFFFFFu : (S >> 16);
                 }
             }
         }
-        __syncthreads();
-        if (act) {
-            for (int jw = 0; jw < nd / 2; ++jw) {
-                const uint32_t two = st[lane * WF_STRIDE + jw];
-#pragma unroll
-                for (int e = 0; e < 2; ++e) {
-                    const int di = dc + 2 * jw + e;
-                    const int d = d0 + di;
-                    const uint32_t raw = e ? (two >> 16) : (two & 0xFFFFu);
-                    if (bd >= 0 && d == bd + 1) v_bp1 = raw;
-                    const bool valid = d >= lo && d < hi && raw != 0xFFFFu;
-                    const uint32_t c = valid ? raw : 0xFFFFFFFFu;
-                    const uint32_t pm_new = min(pm, c);
-                    if (valid && c < bv) {
-                        secL = pm_lag;
-                        secR = 0xFFFFFFFFu;
-                        bv = c;
-                        bd = d;
-                        v_bm1 = raw_prev;
-                        v_bp1 = 0xFFFFu;
-                    } else if (valid && bd >= 0 && d >= bd + 2) {
-                        secR = min(secR, c);
-                    }
-                    pm_lag = pm;
-                    pm = pm_new;
-                    raw_prev = raw;
-                }
-            }
-        }
-        __syncthreads();
     }
+    __syncthreads();
     if (!act) return;
+    const int lo = dmin[pix], hi = dmax[pix];
+    uint32_t* srow = st + lane * rw;
+    // pass 1: earliest minimum as the key S << 8 | d-index
+    uint32_t best = 0xFFFFFFFFu;
+    for (int j = 0; j < nw; ++j) {
+        const uint32_t v = srow[j];
+        const uint32_t idx = (uint32_t)(2 * j) | ((uint32_t)(2 * j + 1) << 8);
+        const uint32_t kl = __builtin_amdgcn_perm(v, idx, 0x0c050400u);
+        const uint32_t kh = __builtin_amdgcn_perm(v, idx, 0x0c070601u);
+        best = min(best, min(kl, kh));   // v_min3_u32
+    }
+    const uint32_t bv = best >> 8;
+    const int bi = (int)(best & 0xFFu);
+    // as in the reference, a negative winning disparity counts as no match
+    // (best_d < 0 test, reference src/sgm_optimizer.cpp:491-496)
+    const bool found = bv != 0xFFFFu && d0 + bi >= 0;
+    const uint16_t* row16 = (const uint16_t*)srow;
+    uint32_t v_m1 = 0xFFFFu, v_p1 = 0xFFFFu;
+    if (found) {
+        v_m1 = bi > 0 ? row16[bi - 1] : 0xFFFFu;
+        v_p1 = bi + 1 < D ? row16[bi + 1] : 0xFFFFu;
+        uint16_t* w16 = (uint16_t*)srow;
+        w16[bi] = 0xFFFFu;
+        if (bi > 0) w16[bi - 1] = 0xFFFFu;
+        if (bi + 1 < D) w16[bi + 1] = 0xFFFFu;
+    }
+    // pass 2: second best outside best-1..best+1
+    uint32_t m2 = 0xFFFFFFFFu;
+    for (int j = 0; j < nw; ++j) m2 = as_u32(__builtin_elementwise_min(as_v2(m2), as_v2(srow[j])));
+    const uint32_t sec = min(m2 & 0xFFFFu, m2 >> 16);
     float dv = -1.f, b = INFINITY, sc = INFINITY;
-    if (hi > lo && bd >= 0) {
-        const uint32_t sec = min(secL, secR);
+    if (found) {
+        const int bd = d0 + bi;
         float dval = (float)bd;
-        if (p.post_subpixel && bd - 1 >= lo && bd + 1 < hi) {
-            const int di = bd - d0;
-            if (di > 0 && di + 1 < D && v_bm1 != 0xFFFFu && v_bp1 != 0xFFFFu)
-                dval += legacy_delta((float)v_bm1, (float)bv, (float)v_bp1);
-        }
+        if (p.post_subpixel && bd - 1 >= lo && bd + 1 < hi && bi > 0 && bi + 1 < D && v_m1 != 0xFFFFu &&
+            v_p1 != 0xFFFFu)
+            dval += legacy_delta((float)v_m1, (float)bv, (float)v_p1);
         dv = dval;
         b = (float)bv;
-        sc = sec != 0xFFFFFFFFu ? (float)sec : INFINITY;
+        sc = sec != 0xFFFFu ? (float)sec : INFINITY;
     }
     if (disp_wta) disp_wta[pix] = dv;
     if (best_out) best_out[pix] = b;
@@ -180,12 +181,14 @@ hipError_t launch_wta_fused(hipStream_t s, const apg_hip_params& p, int w, int h
                             const uint8_t* gx, const uint8_t* gy, float* disp, float* conf, uint8_t* reason,
                             uint8_t* reliable, float* disp_wta, float* best, float* second, uint32_t* sum32) {
     const int D = p.max_disparity - p.min_disparity;
-    if (D % 2) return hipErrorInvalidValue;
+    if (D % 2 || D > 256) return hipErrorInvalidValue;
+    const int rw = (D / 2) | 1;   // odd row stride in words
+    const size_t shm = (size_t)WF_PB * rw * 4;
     const size_t npx = (size_t)w * h;
     dim3 grid((unsigned)((npx + WF_PB - 1) / WF_PB));
-#define APG_WTAF(TC, NP)                                                                                          \
-    hipLaunchKernelGGL((wta_fused_kernel<TC, NP>), grid, dim3(64), 0, s, p, w, h, D, (const TC*)cost, planes, dmin, \
-                       dmax, dr, gx, gy, disp, conf, reason, reliable, disp_wta, best, second, sum32)
+#define APG_WTAF(TC, NP)                                                                                            \
+    hipLaunchKernelGGL((wta_fused_kernel<TC, NP>), grid, dim3(64), shm, s, p, w, h, D, rw, (const TC*)cost, planes, \
+                       dmin, dmax, dr, gx, gy, disp, conf, reason, reliable, disp_wta, best, second, sum32)
     if (p.paths == 8) {
         if (cost_u8) APG_WTAF(uint8_t, 8);
         else APG_WTAF(uint16_t, 8);

@@ -187,13 +187,10 @@ hipError_t launch_lr_check(hipStream_t s, const apg_hip_params& p, int w, int h,
     return hipGetLastError();
 }
 
-// PostProcessor::fill_holes, Legacy route (reference src/postprocess.cpp:42-168):
-// reads `in` (the LR-checked disparity), writes `out`.
-__global__ void fill_kernel(int w, int h, int max_gap, const float* __restrict__ in, const uint8_t* __restrict__ reason,
-                            const uint8_t* __restrict__ gray, const float* __restrict__ prior, float* __restrict__ out) {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
-    if (x >= w) return;
-    const float* row = in + (size_t)y * w;
+__device__ __forceinline__ void fill_pixel(int w, int h, int max_gap, int x, int y, const float* row,
+                                           const float* __restrict__ in, const uint8_t* __restrict__ reason,
+                                           const uint8_t* __restrict__ gray, const float* __restrict__ prior,
+                                           float* __restrict__ out) {
     const size_t q = (size_t)y * w + x;
     float o = row[x];
     if (o < 0.f) {
@@ -234,6 +231,25 @@ __global__ void fill_kernel(int w, int h, int max_gap, const float* __restrict__
     out[q] = o;
 }
 
+// PostProcessor::fill_holes, Legacy route (reference src/postprocess.cpp:42-168):
+// reads `in` (the LR-checked disparity), writes `out`.
+// One block per row (ROW_LDS: the row is first copied to LDS, so the gap
+// scans of hole pixels read LDS), otherwise one thread per pixel.
+template <bool ROW_LDS>
+__global__ void fill_kernel(int w, int h, int max_gap, const float* __restrict__ in, const uint8_t* __restrict__ reason,
+                            const uint8_t* __restrict__ gray, const float* __restrict__ prior, float* __restrict__ out) {
+    extern __shared__ float srow[];
+    const int y = blockIdx.y;
+    const float* row = in + (size_t)y * w;
+    if (ROW_LDS) {
+        for (int i = threadIdx.x; i < w; i += blockDim.x) srow[i] = row[i];
+        __syncthreads();
+        row = srow;
+    }
+    for (int x = ROW_LDS ? (int)threadIdx.x : (int)(blockIdx.x * blockDim.x + threadIdx.x); x < w;
+         x += ROW_LDS ? (int)blockDim.x : w)
+        fill_pixel(w, h, max_gap, x, y, row, in, reason, gray, prior, out);
+}
 // PostProcessor::median (reference src/postprocess.cpp:255-309): the element
 // at position size/2 of the sorted valid window.
 constexpr int kMaxMedianR = 3;
@@ -260,14 +276,63 @@ __global__ void median_kernel(int w, int h, int r, const float* __restrict__ in,
     out[q] = win[n / 2];
 }
 
+// Radius-1 median in registers: the 9 window values (invalid or outside the
+// image as +inf) are sorted by a fixed compare-exchange network; the valid
+// ones then lead the sorted order, so the result is sorted[n/2] as in the
+// reference.
+__device__ __forceinline__ void cswap(float& a, float& b) {
+    const float lo = fminf(a, b), hi = fmaxf(a, b);
+    a = lo;
+    b = hi;
+}
+
+__global__ __launch_bounds__(256) void median3_kernel(int w, int h, const float* __restrict__ in,
+                                                      float* __restrict__ out) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= w) return;
+    float v[9];
+    int n = 0;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        const int xx = x + k % 3 - 1, yy = y + k / 3 - 1;
+        float d = -1.f;
+        if (xx >= 0 && yy >= 0 && xx < w && yy < h) d = in[(size_t)yy * w + xx];
+        const bool ok = d >= 0.f;
+        n += ok ? 1 : 0;
+        v[k] = ok ? d : INFINITY;
+    }
+    const size_t q = (size_t)y * w + x;
+    if (n == 0) {
+        out[q] = in[q];
+        return;
+    }
+    // odd-even transposition sort: 9 rounds sort 9 inputs
+#pragma unroll
+    for (int r = 0; r < 9; ++r)
+#pragma unroll
+        for (int k = r & 1; k + 1 < 9; k += 2) cswap(v[k], v[k + 1]);
+    float m = v[0];
+#pragma unroll
+    for (int k = 1; k < 9; ++k) m = (n / 2 == k) ? v[k] : m;
+    out[q] = m;
+}
+
 hipError_t launch_fill(hipStream_t s, const apg_hip_params& p, int w, int h, const float* disp, const uint8_t* reason,
                        const uint8_t* gray, const float* prior, float* out) {
-    hipLaunchKernelGGL(fill_kernel, dim3((w + 127) / 128, h), dim3(128), 0, s, w, h, p.post_max_fill_gap, disp, reason,
-                       gray, prior, out);
+    if ((size_t)w * sizeof(float) <= 32 * 1024)
+        hipLaunchKernelGGL(fill_kernel<true>, dim3(1, h), dim3(256), (size_t)w * sizeof(float), s, w, h,
+                           p.post_max_fill_gap, disp, reason, gray, prior, out);
+    else
+        hipLaunchKernelGGL(fill_kernel<false>, dim3((w + 127) / 128, h), dim3(128), 0, s, w, h, p.post_max_fill_gap,
+                           disp, reason, gray, prior, out);
     return hipGetLastError();
 }
 
 hipError_t launch_median(hipStream_t s, const apg_hip_params& p, int w, int h, const float* disp, float* out) {
+    if (p.post_median_radius == 1) {
+        hipLaunchKernelGGL(median3_kernel, dim3((w + 255) / 256, h), dim3(256), 0, s, w, h, disp, out);
+        return hipGetLastError();
+    }
     if (p.post_median_radius > 0) {
         hipLaunchKernelGGL(median_kernel, dim3((w + 127) / 128, h), dim3(128), 0, s, w, h, p.post_median_radius, disp,
                            out);

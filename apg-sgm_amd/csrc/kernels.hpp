@@ -75,7 +75,8 @@ hipError_t launch_cost_crossh(hipStream_t s, const apg_hip_params& p, const Cost
                               const uint8_t* gl, const uint8_t* gxl, const uint8_t* gyl, const uint8_t* gr,
                               const uint8_t* gxr, const uint8_t* gyr, const uint32_t* c32l, const uint32_t* c32r,
                               const uint64_t* c64l, const uint64_t* c64r, const int16_t* dmin, const int16_t* dmax,
-                              const uint32_t* arms, void* out, bool out_u8, uint32_t* rkeys);
+                              const uint32_t* arms, void* out, bool out_u8, uint32_t* rkeys,
+                              bool column_ranges);
 hipError_t launch_rwta_finalize(hipStream_t s, const apg_hip_params& p, const CostLut* d_lut, int w, int h,
                                 const uint8_t* gl, const uint8_t* gxl, const uint8_t* gyl, const uint8_t* gr,
                                 const uint8_t* gxr, const uint8_t* gyr, const uint32_t* c32l, const uint32_t* c32r,

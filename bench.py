@@ -64,11 +64,23 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--schedule", type=int, default=0, help="0 fused (default), 1 stage by stage")
+    ap.add_argument("--copy-calib", type=int, default=0, metavar="MB",
+                    help="first run one device-to-device copy of MB MiB (a kernel of known traffic, "
+                         "to scale the FETCH_SIZE / WRITE_SIZE counters of a profiled run)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs one GPU per process; --gpus {args.gpus} is not supported "
               "(pairs are independent; multi-GPU would be host-side dispatch only)", file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_calib > 0:
+        import torch
+        n = args.copy_calib << 20
+        src = torch.ones(n, dtype=torch.uint8, device="cuda")
+        dst = torch.empty(n, dtype=torch.uint8, device="cuda")
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     import apg_amd as A
     p = A.params(1, DMAX)
