@@ -132,9 +132,6 @@ __global__ __launch_bounds__(64) void sgm_lines_kernel(apg_hip_params p, int w, 
     }
     int p0, step, n;
     family_line(f, b, w, h, p0, step, n);
-    // the longest scanlines bound the kernel's duration: their waves win
-    // instruction arbitration against the shorter ones sharing a SIMD
-    if (n >= w && n >= h) __builtin_amdgcn_s_setprio(2);
     const auto rc = raw_rsrc(cost);
     const auto rpa = raw_rsrc(planes.p[2 * f]);
     const auto rpb = raw_rsrc(planes.p[2 * f + 1]);
@@ -158,24 +155,19 @@ __global__ __launch_bounds__(64) void sgm_lines_kernel(apg_hip_params p, int w, 
 #pragma unroll
     for (int j = 0; j < K; ++j) A.prev[j] = B.prev[j] = 0xFFFFFFFFu;
     A.minprev = B.minprev = 0xFFFFu;
-    // adaptive P2 of the next 64 steps of each chain, one step per lane; the
-    // gray values of the 64 steps after those are loaded one chunk ahead
-    uint32_t p2A = 0, p2B = 0, lastA = 0, lastB = 0, ngA, ngB;
-    auto load_gray = [&](int t0) {
-        const int tl = min(t0 + lane, n - 1);
-        ngA = gray[pA0 + tl * step];
-        ngB = gray[pB0 - tl * step];
-    };
-    load_gray(0);
+    // the scanline's gray values, staged in LDS once; adaptive P2 of the next
+    // 64 steps of each chain is computed one step per lane from them
+    extern __shared__ uint8_t s_gray[];   // [n]
+    for (int t = lane; t < n; t += 64) s_gray[t] = gray[p0 + t * step];
+    __syncthreads();
+    uint32_t p2A = 0, p2B = 0;
     auto refresh_p2 = [&](int t0) {
-        const uint32_t ga = ngA, gb = ngB;
-        load_gray(t0 + 64);
-        const uint32_t gpa = (uint32_t)__builtin_amdgcn_update_dpp((int)lastA, (int)ga, kDppWaveShr1, 0xF, 0xF, false);
-        const uint32_t gpb = (uint32_t)__builtin_amdgcn_update_dpp((int)lastB, (int)gb, kDppWaveShr1, 0xF, 0xF, false);
-        p2A = (uint32_t)adaptive_p2(p, (int)ga - (int)gpa);
-        p2B = (uint32_t)adaptive_p2(p, (int)gb - (int)gpb);
-        lastA = (uint32_t)__builtin_amdgcn_readlane((int)ga, 63);
-        lastB = (uint32_t)__builtin_amdgcn_readlane((int)gb, 63);
+        // chain A at step t: pixel t after t-1; chain B: pixel n-1-t after n-t
+        const int tl = min(t0 + lane, n - 1);
+        const int ga = s_gray[tl], gpa = s_gray[max(tl - 1, 0)];
+        const int gb = s_gray[n - 1 - tl], gpb = s_gray[min(n - tl, n - 1)];
+        p2A = (uint32_t)adaptive_p2(p, ga - gpa);
+        p2B = (uint32_t)adaptive_p2(p, gb - gpb);
     };
     CRaw<TC, K> XA[SL_PF], XB[SL_PF], YA[SL_PF], YB[SL_PF];
     auto load = [&](CRaw<TC, K> (&ra)[SL_PF], CRaw<TC, K> (&rb)[SL_PF], int t0) {
@@ -232,6 +224,187 @@ __global__ __launch_bounds__(64) void sgm_lines_kernel(apg_hip_params p, int w, 
     }
 }
 
+// Batched-transfer form for u8 C' with D % 16 == 0, D <= 256.  Each chain
+// moves SB steps of C' (SB pixel rows) from HBM into LDS with one 16-byte
+// load per lane, reads its own 2K bytes per step from LDS one step ahead, and
+// stages its deltas in LDS, written back SB steps at a time with one 16-byte
+// store per lane.  That is 4 vector-memory instructions per SB steps of the
+// wave instead of 4 per step, so the loads of the next batch stay in flight
+// however the wait counts are placed.
+constexpr int SB = 8;   // steps per staged batch
+
+// One scanline chain job (or both chains of the line when DUAL), all state
+// local: waves that run several jobs reuse the LDS staging buffers.
+template <int K, bool FULL, bool DUAL>
+__device__ __forceinline__ void sgm_job(const apg_hip_params& p, int w, int h, int D, int f, int b, int chain,
+                                        const uint8_t* __restrict__ gray, const uint8_t* __restrict__ cost,
+                                        const PlaneSet& planes, uint8_t* sh) {
+    constexpr int RB = 128 * K;   // staged row bytes (>= D)
+    uint8_t* s_c = sh;                       // [chain][buffer][SB][RB] C' rows
+    uint8_t* s_d = sh + 4 * SB * RB;         // [chain][buffer][SB][RB] delta rows
+    uint8_t* s_gray = sh + 8 * SB * RB;      // [n]
+    const int lane = threadIdx.x;
+    int p0, step, n;
+    family_line(f, b, w, h, p0, step, n);
+    const auto rc = raw_rsrc(cost);
+    // chain A walks from the line's first pixel; a single-chain wave of the
+    // backward direction is chain A of the reversed line
+    const int pF = p0, pL = p0 + (n - 1) * step;
+    const int pA0 = chain ? pL : pF, pB0 = chain ? pF : pL;
+    if (chain) step = -step;
+    const auto rpa = raw_rsrc(planes.p[2 * f + chain]);
+    const auto rpb = raw_rsrc(planes.p[2 * f + 1]);
+    // batch transfer: lane -> (row j, 16-byte chunk c); NLD instructions cover SB rows
+    const int lpx = D / 16, ppl = 64 / lpx;
+    constexpr int NLD = K == 1 ? 1 : 2;
+    const int jl = lane / lpx, cl = lane % lpx;
+    bool inr[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) inr[j] = FULL || 2 * (K * lane + j) < D;
+    const uint32_t P1 = (uint32_t)p.P1;
+    const uint32_t p1s = P1 | (P1 << 16);
+    const uint32_t p1l = lane == 0 ? (0xFFFFu | (P1 << 16)) : p1s;
+    const uint32_t p1r = lane == 63 ? (P1 | 0xFFFF0000u) : p1s;
+    Chain<K> A, B;
+#pragma unroll
+    for (int j = 0; j < K; ++j) A.prev[j] = B.prev[j] = 0xFFFFFFFFu;
+    A.minprev = B.minprev = 0xFFFFu;
+    for (int t = lane; t < n; t += 64) s_gray[t] = gray[pA0 + t * step];
+    uint32_t p2A = 0, p2B = 0;
+    auto refresh_p2 = [&](int t0) {
+        const int tl = min(t0 + lane, n - 1);
+        const int ga = s_gray[tl], gpa = s_gray[max(tl - 1, 0)];
+        const int gb = s_gray[n - 1 - tl], gpb = s_gray[min(n - tl, n - 1)];
+        p2A = (uint32_t)adaptive_p2(p, ga - gpa);
+        p2B = (uint32_t)adaptive_p2(p, gb - gpb);
+    };
+    // byte offset of (row j, chunk c) of batch k for chain A / B
+    auto voffA = [&](int k, int j) { return (uint32_t)(pA0 + min(k * SB + j, n - 1) * step) * (uint32_t)D + cl * 16u; };
+    auto voffB = [&](int k, int j) { return (uint32_t)(pB0 - min(k * SB + j, n - 1) * step) * (uint32_t)D + cl * 16u; };
+    u32x4 gA[NLD], gB[NLD];
+    auto gload = [&](int k) {
+#pragma unroll
+        for (int r = 0; r < NLD; ++r) {
+            const int j = min(r * ppl + jl, SB - 1);
+            gA[r] = buf_ld128(rc, voffA(k, j), 0u);
+            if (DUAL) gB[r] = buf_ld128(rc, voffB(k, j), 0u);
+        }
+    };
+    auto lstage = [&](int k) {   // batch k's rows -> LDS buffer k & 1
+        const int bf = k & 1;
+#pragma unroll
+        for (int r = 0; r < NLD; ++r) {
+            const int j = r * ppl + jl;
+            if (j < SB) {
+                *(u32x4*)(s_c + ((0 * 2 + bf) * SB + j) * RB + cl * 16) = gA[r];
+                if (DUAL) *(u32x4*)(s_c + ((1 * 2 + bf) * SB + j) * RB + cl * 16) = gB[r];
+            }
+        }
+    };
+    auto flush = [&](int k) {    // deltas of batch k (LDS buffer k & 1) -> planes
+        const int bf = k & 1;
+#pragma unroll
+        for (int r = 0; r < NLD; ++r) {
+            const int j = r * ppl + jl;
+            if (j < SB && k * SB + j < n) {
+                const u32x4 va = *(const u32x4*)(s_d + ((0 * 2 + bf) * SB + j) * RB + cl * 16);
+                __builtin_amdgcn_raw_buffer_store_b128(va, rpa, (int)voffA(k, j), 0, 0);
+                if (DUAL) {
+                    const u32x4 vb = *(const u32x4*)(s_d + ((1 * 2 + bf) * SB + j) * RB + cl * 16);
+                    __builtin_amdgcn_raw_buffer_store_b128(vb, rpb, (int)voffB(k, j), 0, 0);
+                }
+            }
+        }
+    };
+    auto cword = [&](int chain, int k, int i) -> uint32_t {   // the lane's C' bytes of step i of batch k
+        const uint8_t* q = s_c + ((chain * 2 + (k & 1)) * SB + i) * RB + 2 * K * lane;
+        return K == 1 ? (uint32_t)*(const uint16_t*)q : *(const uint32_t*)q;
+    };
+    const int nb = (n + SB - 1) / SB;
+    gload(0);
+    lstage(0);
+    if (nb > 1) gload(1);
+    uint32_t nA = cword(0, 0, 0), nB = DUAL ? cword(1, 0, 0) : 0u;
+    for (int k = 0; k < nb; ++k) {
+        // staged rows first: the stores and loads issued after them are newer,
+        // so waiting for the rows never waits for a store
+        if (k + 1 < nb) lstage(k + 1);
+        if (k > 0) flush(k - 1);
+        if (k + 2 < nb) gload(k + 2);
+        const int bf = k & 1;
+#pragma unroll
+        for (int i = 0; i < SB; ++i) {
+            const int t = k * SB + i;
+            if (t >= n) break;
+            if (i == 0 && (t & 63) == 0) refresh_p2(t);
+            const uint32_t wa = nA, wb = nB;
+            // next step's words (the next batch's rows are already staged)
+            if (i + 1 < SB) {
+                nA = cword(0, k, i + 1);
+                if (DUAL) nB = cword(1, k, i + 1);
+            } else {
+                nA = cword(0, k + 1, 0);
+                if (DUAL) nB = cword(1, k + 1, 0);
+            }
+            const uint32_t P2a = (uint32_t)__builtin_amdgcn_readlane((int)p2A, t & 63);
+            uint32_t ca[K], cb[K], da[K], db[K];
+#pragma unroll
+            for (int j = 0; j < K; ++j) {
+                CRaw<uint8_t, K> xa, xb;
+                xa.w[0] = wa;
+                xb.w[0] = wb;
+                ca[j] = decode_c<uint8_t, K>(xa, j);
+                cb[j] = DUAL ? decode_c<uint8_t, K>(xb, j) : 0u;
+                if (!FULL && !inr[j]) ca[j] = cb[j] = 0xFFFFFFFFu;
+            }
+            uint32_t ma = chain_step<K>(A, ca, P2a, p1s, p1l, p1r, da);
+            if (DUAL) {
+                const uint32_t P2b = (uint32_t)__builtin_amdgcn_readlane((int)p2B, t & 63);
+                uint32_t mb = chain_step<K>(B, cb, P2b, p1s, p1l, p1r, db);
+                wave_min2_u32(ma, mb);
+                B.minprev = mb;
+            } else {
+                ma = wave_min_u32(ma);
+            }
+            A.minprev = ma;
+            uint8_t* qa = s_d + ((0 * 2 + bf) * SB + i) * RB + 2 * K * lane;
+            uint8_t* qb = s_d + ((1 * 2 + bf) * SB + i) * RB + 2 * K * lane;
+            if (K == 1) {
+                *(uint16_t*)qa = (uint16_t)pack_deltas<K>(da);
+                if (DUAL) *(uint16_t*)qb = (uint16_t)pack_deltas<K>(db);
+            } else {
+                *(uint32_t*)qa = pack_deltas<K>(da);
+                if (DUAL) *(uint32_t*)qb = pack_deltas<K>(db);
+            }
+        }
+    }
+    flush(nb - 1);
+}
+
+// DUAL: one wave runs both chains of its scanline (two independent
+// recurrences interleaved).  Otherwise a wave runs one chain (forward or
+// backward) of one scanline: twice the waves, each with half the work, which
+// measured faster at 1280x720 D=128 (more waves share each SIMD's issue
+// slots while the longest scanlines bound the duration).
+template <int K, bool FULL, bool DUAL>
+__global__ __launch_bounds__(64) void sgm_lines4_kernel(apg_hip_params p, int w, int h, int D, int nfam,
+                                                        const uint8_t* __restrict__ gray,
+                                                        const uint8_t* __restrict__ cost, PlaneSet planes) {
+    extern __shared__ __align__(16) uint8_t sh[];
+    const int bid = blockIdx.x;
+    if (DUAL) {
+        int b = bid, f = 0;
+        const int cnt[4] = {h, w, w + h - 1, w + h - 1};
+        while (f < nfam - 1 && b >= cnt[f]) { b -= cnt[f]; ++f; }
+        sgm_job<K, FULL, true>(p, w, h, D, f, b, 0, gray, cost, planes, sh);
+    } else {
+        int b = bid >> 1, f = 0;
+        const int cnt[4] = {h, w, w + h - 1, w + h - 1};
+        while (f < nfam - 1 && b >= cnt[f]) { b -= cnt[f]; ++f; }
+        sgm_job<K, FULL, false>(p, w, h, D, f, b, bid & 1, gray, cost, planes, sh);
+    }
+}
+
 template <typename TC>
 static hipError_t launch_lines_t(hipStream_t s, const apg_hip_params& p, int w, int h, int D, const uint8_t* gray,
                                  const TC* cost, const PlaneSet& planes) {
@@ -239,8 +412,25 @@ static hipError_t launch_lines_t(hipStream_t s, const apg_hip_params& p, int w, 
     const int nfam = p.paths == 8 ? 4 : 2;
     const int nblk = h + w + (nfam == 4 ? 2 * (w + h - 1) : 0);
     const bool full = D == 128 * K;
+    const size_t shm = (size_t)(w > h ? w : h);   // one scanline of gray values
+    if (sizeof(TC) == 1 && D % 16 == 0 && K <= 2) {
+        const size_t shm4 = (size_t)8 * SB * 128 * K + shm;
+        const int nblk4 = 2 * nblk;
+#define APG_LINES4(KK, FF)                                                                                        \
+    hipLaunchKernelGGL((sgm_lines4_kernel<KK, FF, false>), dim3(nblk4), dim3(64), shm4, s, p, w, h, D, nfam,        \
+                       gray, (const uint8_t*)cost, planes)
+        if (K == 1) {
+            if (full) APG_LINES4(1, true);
+            else APG_LINES4(1, false);
+        } else {
+            if (full) APG_LINES4(2, true);
+            else APG_LINES4(2, false);
+        }
+#undef APG_LINES4
+        return hipGetLastError();
+    }
 #define APG_LINES(KK, FF)                                                                                      \
-    hipLaunchKernelGGL((sgm_lines_kernel<TC, KK, FF>), dim3(nblk), dim3(64), 0, s, p, w, h, D, nfam, gray, cost, \
+    hipLaunchKernelGGL((sgm_lines_kernel<TC, KK, FF>), dim3(nblk), dim3(64), shm, s, p, w, h, D, nfam, gray, cost, \
                        planes)
     if (K == 1) {
         if (full) APG_LINES(1, true);

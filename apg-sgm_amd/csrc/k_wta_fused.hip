@@ -22,7 +22,28 @@ __device__ __forceinline__ float clampf_(float v, float lo, float hi) { return v
 //   1. best = min over d of (S << 8 | d): the earliest minimum (D <= 256);
 //   2. after S(best-1), S(best+1) are read and S at best-1..best+1 is set to
 //      65535, second = min of the row (packed u16 minima).
+// S of one (d, d+1) pair from C' and the planes' deltas, all as u16 halves
+// (u8 C' undecoded: 255 = invalid): npath * C' + sum of the deltas (< 65535
+// for valid C', checked by the host), 65535 where C' is invalid.
 template <typename TC, int NP>
+__device__ __forceinline__ uint32_t s_pair(uint32_t ch, const uint32_t (&ph)[NP]) {
+    const uint32_t c16 = sizeof(TC) == 1 ? as_u32(as_v2(ch) - (as_v2((ch + 0x00010001u) & 0xFF00FF00u)))   // 255 -> 65535
+                                         : ch;
+    u16x2 ds = as_v2(ph[0]);
+#pragma unroll
+    for (int f = 1; f < NP; ++f) ds = ds + as_v2(ph[f]);
+    const uint32_t sv = as_u32(as_v2(c16) * splat2(NP) + ds);
+    const uint32_t inv = as_u32((as_v2(c16) + splat2(1)) == splat2(0) ? splat2(0xFFFF) : splat2(0));
+    return sv | inv;
+}
+
+// WIDE (u8 C', D % 16 == 0, no sum32 output): the block's 64 pixels are one
+// contiguous run of 64*D elements in C' and in every plane, and S is
+// elementwise over it, so staging reads 16-byte chunks (WF_WG per lane in
+// flight, all planes) and splits a chunk's (pixel, d) only for the LDS store.
+constexpr int WF_WG = 4;
+
+template <typename TC, int NP, bool WIDE>
 __global__ __launch_bounds__(64) void wta_fused_kernel(apg_hip_params p, int w, int h, int D, int rw,
                                                        const TC* __restrict__ cost, PlaneSet planes,
                                                        const int16_t* __restrict__ dmin,
@@ -42,7 +63,41 @@ __global__ __launch_bounds__(64) void wta_fused_kernel(apg_hip_params p, int w, 
     const int d0 = p.min_disparity;
     const int nw = D / 2;   // words per row (D even)
     const auto rc = raw_rsrc(cost);
+    if (WIDE) {
+        const uint32_t ebase = (uint32_t)(p_base * (size_t)D);
+        const int nchunk = npb * D / 16;
+        for (int k0 = 0; k0 < nchunk; k0 += 64 * WF_WG) {
+            u32x4 cw[WF_WG], pw[WF_WG][NP];
+#pragma unroll
+            for (int g = 0; g < WF_WG; ++g) {
+                const uint32_t off = (uint32_t)min(k0 + g * 64 + lane, nchunk - 1) * 16u;
+                cw[g] = buf_ld128(rc, off, ebase);
+#pragma unroll
+                for (int f = 0; f < NP; ++f) pw[g][f] = buf_ld128(raw_rsrc(planes.p[f]), off, ebase);
+            }
+#pragma unroll
+            for (int g = 0; g < WF_WG; ++g) {
+                const int ch = k0 + g * 64 + lane;
+                if (ch >= nchunk) continue;
+                const int e = ch * 16;
+                const int i = e / D;
+                uint32_t* dst = st + i * rw + (e - i * D) / 2;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+#pragma unroll
+                    for (int hh = 0; hh < 2; ++hh) {
+                        const uint32_t sel = hh ? 0x0c030c02u : 0x0c010c00u;   // bytes -> u16 halves
+                        uint32_t ph[NP];
+#pragma unroll
+                        for (int f = 0; f < NP; ++f) ph[f] = __builtin_amdgcn_perm(0u, pw[g][f][q], sel);
+                        dst[q * 2 + hh] = s_pair<TC, NP>(__builtin_amdgcn_perm(0u, cw[g][q], sel), ph);
+                    }
+                }
+            }
+        }
+    }
     // staging: one pixel per wave iteration, lane = pair index (two passes for D > 128)
+    if (!WIDE)
     for (int j0 = 0; j0 < nw; j0 += 64) {
         const int jw = j0 + lane;
         const bool jin = jw < nw;
@@ -61,21 +116,10 @@ __global__ __launch_bounds__(64) void wta_fused_kernel(apg_hip_params p, int w, 
             for (int g = 0; g < WF_G; ++g) {
                 const int i = i0 + g;
                 if (i >= npb || !jin) continue;
-                uint32_t c16;
-                if (sizeof(TC) == 1) {
-                    const uint32_t x = bytes_to_halves(rcw[g]);
-                    c16 = as_u32(as_v2(x) - (as_v2((x + 0x00010001u) & 0xFF00FF00u)));   // 255 -> 65535
-                } else {
-                    c16 = rcw[g];
-                }
-                u16x2 ds = as_v2(bytes_to_halves(rd[g][0]));
+                uint32_t ph[NP];
 #pragma unroll
-                for (int f = 1; f < NP; ++f) ds = ds + as_v2(bytes_to_halves(rd[g][f]));
-                // S = npath * C' + sum of deltas (< 65535 for valid C', checked by the
-                // host); halves with invalid C' (65535) become 65535
-                const uint32_t sv = as_u32(as_v2(c16) * splat2(NP) + ds);
-                const uint32_t inv = as_u32((as_v2(c16) + splat2(1)) == splat2(0) ? splat2(0xFFFF) : splat2(0));
-                const uint32_t S = sv | inv;
+                for (int f = 0; f < NP; ++f) ph[f] = bytes_to_halves(rd[g][f]);
+                const uint32_t S = s_pair<TC, NP>(sizeof(TC) == 1 ? bytes_to_halves(rcw[g]) : rcw[g], ph);
                 st[i * rw + jw] = S;
                 if (sum32) {
                     const size_t q = (p_base + i) * (size_t)D + 2 * jw;
@@ -186,15 +230,18 @@ hipError_t launch_wta_fused(hipStream_t s, const apg_hip_params& p, int w, int h
     const size_t shm = (size_t)WF_PB * rw * 4;
     const size_t npx = (size_t)w * h;
     dim3 grid((unsigned)((npx + WF_PB - 1) / WF_PB));
-#define APG_WTAF(TC, NP)                                                                                            \
-    hipLaunchKernelGGL((wta_fused_kernel<TC, NP>), grid, dim3(64), shm, s, p, w, h, D, rw, (const TC*)cost, planes, \
-                       dmin, dmax, dr, gx, gy, disp, conf, reason, reliable, disp_wta, best, second, sum32)
+    const bool wide = cost_u8 && D % 16 == 0 && !sum32;
+#define APG_WTAF(TC, NP, WD)                                                                                   \
+    hipLaunchKernelGGL((wta_fused_kernel<TC, NP, WD>), grid, dim3(64), shm, s, p, w, h, D, rw, (const TC*)cost, \
+                       planes, dmin, dmax, dr, gx, gy, disp, conf, reason, reliable, disp_wta, best, second, sum32)
     if (p.paths == 8) {
-        if (cost_u8) APG_WTAF(uint8_t, 8);
-        else APG_WTAF(uint16_t, 8);
+        if (wide) APG_WTAF(uint8_t, 8, true);
+        else if (cost_u8) APG_WTAF(uint8_t, 8, false);
+        else APG_WTAF(uint16_t, 8, false);
     } else {
-        if (cost_u8) APG_WTAF(uint8_t, 4);
-        else APG_WTAF(uint16_t, 4);
+        if (wide) APG_WTAF(uint8_t, 4, true);
+        else if (cost_u8) APG_WTAF(uint8_t, 4, false);
+        else APG_WTAF(uint16_t, 4, false);
     }
 #undef APG_WTAF
     return hipGetLastError();
