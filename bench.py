@@ -9,8 +9,8 @@ the library's stream.  Prints ONE JSON line.
 
 The CPU baseline (reported, not the target) is the reference implementation
 compiled from its own sources (oracle/_ref) when present, else the repo's C
-oracle, run once on the same pair with all host cores.  The GPU disparity of
-the bench pair is compared bit-exactly with that CPU run.
+oracle, on the same pair with all host cores: one parity run (bit-exact
+against the GPU disparity), then about 10 s of repeated runs, median.
 """
 import argparse
 import ctypes as C
@@ -38,23 +38,59 @@ BYTES_PER_STATE = {"pipeline": 12.0, "sgm": 4.0, "cross": 4.0, "cost": 2.0, "wta
 
 
 def cpu_baseline(left, right, p, gpu_disp):
+    """The reference pipeline (oracle/_ref, compiled from the reference's
+    sources) on the bench pair, OpenMP over all host threads: one run for
+    parity with the GPU disparity, then repeated runs for about 10 s, median
+    per-pair time.  The repo's C oracle stands in when the reference build is
+    absent."""
     import pyoracle as O
     c = O.cfg_from(p.as_dict())
-    cores = os.cpu_count() or 1
+    states = left.shape[0] * left.shape[1] * (p.max_disparity - p.min_disparity)
     t0 = time.perf_counter()
     if O.ref_available():
         out = O.ref_pipeline(c, left, right)
-        kind = "reference"
+        kind, threads = "reference", O.ref_max_threads()
     else:
-        O.set_threads(cores)
+        threads = os.cpu_count() or 1
+        O.set_threads(threads)
         out = O.pipeline(c, left, right)
         kind = "port"
-    dt = time.perf_counter() - t0
-    states = left.shape[0] * left.shape[1] * (p.max_disparity - p.min_disparity)
+    t1 = time.perf_counter() - t0
     parity = bool(np.array_equal(out["disparity"], gpu_disp))
-    return {"value": states / dt / 1e6, "unit": "Mdisp/s", "cores": cores, "kind": kind,
-            "sample": f"1 Balanced pipeline on the bench pair {left.shape[1]}x{left.shape[0]} D={DMAX}, "
-                      f"{dt:.2f} s wall, OpenMP over {cores} host threads"}, parity
+    reps = int(min(60, max(3, round(10.0 / max(t1, 1e-3)))))
+    if kind == "reference":
+        ms = O.ref_time_pipeline(c, left, right, threads=0, warmup=0, reps=reps)
+    else:
+        ts = []
+        for _ in range(reps):
+            t = time.perf_counter()
+            O.pipeline(c, left, right)
+            ts.append((time.perf_counter() - t) * 1e3)
+        ms = float(sorted(ts)[len(ts) // 2])
+    return {"value": round(states / (ms * 1e-3) / 1e6, 1), "unit": "Mdisp/s", "cores": threads, "kind": kind,
+            "sample": f"Balanced pipeline on the bench pair {left.shape[1]}x{left.shape[0]} D={DMAX}: median of "
+                      f"{reps} runs ({ms:.1f} ms per pair) after one parity run, OpenMP over {threads} host threads"
+            }, parity
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_balanced_1280x720_d128.md")
+STAGE_KERNEL = {"cost": "cost_crossh2_kernel", "cross": "cross_v3_kernel", "sgm": "sgm_lines4_kernel",
+                "wta": "wta_fused_kernel"}
+
+
+def measured_traffic(stage):
+    """HBM bytes per launch of the stage's kernel from the committed counter
+    summary (FETCH_SIZE + WRITE_SIZE runs, scaled by the copy-kernel
+    calibration), or None when the summary has no line for it."""
+    try:
+        lines = open(TRAFFIC_FILE).read().splitlines()
+    except OSError:
+        return None
+    for ln in lines:
+        f = ln.split()
+        if len(f) == 2 and f[0] == STAGE_KERNEL.get(stage) and f[1].isdigit():
+            return int(f[1])
+    return None
 
 
 def main():
@@ -107,8 +143,8 @@ def main():
     dom = max(("cost", "cross", "sgm", "wta"), key=lambda k: stages[k + "_ms"])
     dom_ms = stages[dom + "_ms"]
     achieved = states * BYTES_PER_STATE[dom] / (dom_ms * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "kernel": f"{dom} stage", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+    roofline = {"bound": "hbm", "kernel": STAGE_KERNEL[dom], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom),
                 "pipeline_frac": round(states * BYTES_PER_STATE["pipeline"] / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
     out = {"metric": "Mdisp/s (W*H*D per sec), Balanced 1280x720 D=128", "value": round(value, 1),
            "unit": "Mdisp/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,

@@ -194,7 +194,8 @@ typedef struct {
 
 /* Schedule of apg_hip_pipeline on this context: 0 (default) runs the fused
  * schedule whenever it is eligible (no raw-volume intermediates requested,
- * even D, 2*max(P2_base, P1+1) <= 254, at most one Cross iteration) and the
+ * D even and <= 256, max(P2_base, P1+1) <= 254, npath*(cost bound +
+ * P2max) < 65535, at most one Cross iteration, volume < 2 GiB) and the
  * stage-by-stage schedule otherwise; 1 always runs stage by stage.  Both
  * produce identical results. */
 APG_HIP_API void apg_hip_set_schedule(apg_hip_ctx ctx, int schedule);
