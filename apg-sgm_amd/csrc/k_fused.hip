@@ -206,18 +206,18 @@ struct K2Shm {
     size_t lut, mag, col, chi, outr, rr, key, ring, total;
 };
 
-__host__ __device__ inline K2Shm k2_layout(int D, int Lmax, int seg) {
+__host__ __device__ inline K2Shm k2_layout(int D, int Lmax, int seg, bool sym) {
     const int ncol = seg + 2 * Lmax;
     K2Shm s;
     size_t o = 0;
     s.lut = o; o += (65 + 256 + 511) * 4;
     s.mag = o; o += (2 * kMaxArm + 2) * 4;
     o = (o + 15) & ~(size_t)15;
-    s.col = o; o += (size_t)(ncol + 1) * 16;   // + the record read ahead of the last column
-    s.chi = o; o += (size_t)(ncol + 1) * 4;
+    s.col = o; o += (size_t)(ncol + 2) * 16;   // + the records read ahead of the last column
+    s.chi = o; o += sym ? 0 : (size_t)(ncol + 2) * 4;
     o = (o + 15) & ~(size_t)15;
     s.outr = o; o += (size_t)(seg + 1) * 16;
-    s.rr = o; o += (size_t)(ncol + D) * 16;
+    s.rr = o; o += (size_t)(ncol + D + 1) * 16;
     s.key = o; o += (size_t)(seg + D - 1) * 4;
     o = (o + 15) & ~(size_t)15;
     s.ring = o; o += (size_t)(2 * Lmax + 2) * 256;
@@ -236,7 +236,7 @@ __global__ __launch_bounds__(64) void cost_crossh2_kernel(
     const uint64_t* __restrict__ c64r, const uint32_t* __restrict__ arms, uint8_t* __restrict__ out,
     uint32_t* __restrict__ rkeys) {
     extern __shared__ __align__(16) unsigned char shm[];
-    const K2Shm L = k2_layout(D, Lmax, seg);
+    const K2Shm L = k2_layout(D, Lmax, seg, SYM);
     float* s_lc = (float*)(shm + L.lut);
     float* s_la = s_lc + 65;
     float* s_lg = s_la + 256;
@@ -263,7 +263,7 @@ __global__ __launch_bounds__(64) void cost_crossh2_kernel(
     for (int i = lane; i < 256; i += 64) s_la[i] = lut->ad[i];
     for (int i = lane; i < 511; i += 64) s_lg[i] = lut->grad[i];
     for (int n = lane; n < 2 * kMaxArm + 2; n += 64) s_mag[n] = n ? (0x80000000u + (uint32_t)n - 1u) / (uint32_t)n : 0u;
-    for (int c = lane; c <= ncol; c += 64) {
+    for (int c = lane; c <= ncol + 1; c += 64) {
         const int x = xb + c;
         uint4 r = make_uint4(0u, 0u, (uint32_t)kFarLeft, (uint32_t)(((c + 1) % rs) * 256));
         uint32_t hi = 0u;
@@ -292,7 +292,7 @@ __global__ __launch_bounds__(64) void cost_crossh2_kernel(
         }
         s_out[o] = r;
     }
-    for (int i = lane; i < ncol + D; i += 64) {
+    for (int i = lane; i < ncol + D + 1; i += 64) {
         const int xr = xs + i;
         uint4 r = make_uint4(0u, 0u, 0u, 0u);
         if (i < ncol + D - 1 && xr >= 0 && xr < w) {
@@ -322,8 +322,27 @@ __global__ __launch_bounds__(64) void cost_crossh2_kernel(
         const int rbase = act ? D - 1 - di0 : 1;
         *(uint32_t*)(s_ring + lofs) = 0u;   // P(0)
         uint32_t acc = 0u;
+        // pipeline registers: records of the next two columns, LUT terms of the next one
         uint4 cr_n = s_col[0], r0_n = s_rr[rbase], r1_n = s_rr[rbase - 1], or_n = s_out[0];
         uint32_t ch_n = SYM ? 0u : s_chi[0];
+        float tc0, ta0, tg0, tc1, ta1, tg1;
+        auto lut_terms = [&](const uint4& cr, uint32_t chl, const uint4& r0, const uint4& r1) {
+            const uint32_t g0 = cr.y & 0xFFu, gxy = cr.y >> 8;
+            const int pop0 = SYM ? __popc(cr.x ^ r0.x) : __popc(cr.x ^ r0.x) + __popc(chl ^ r0.w);
+            const int pop1 = SYM ? __popc(cr.x ^ r1.x) : __popc(cr.x ^ r1.x) + __popc(chl ^ r1.w);
+            tc0 = s_lc[pop0];
+            ta0 = s_la[__builtin_amdgcn_sad_u8(r0.y, g0, 0u)];
+            tg0 = s_lg[__builtin_amdgcn_sad_u8(r0.z, gxy, 0u)];
+            tc1 = s_lc[pop1];
+            ta1 = s_la[__builtin_amdgcn_sad_u8(r1.y, g0, 0u)];
+            tg1 = s_lg[__builtin_amdgcn_sad_u8(r1.z, gxy, 0u)];
+        };
+        lut_terms(cr_n, ch_n, r0_n, r1_n);   // column 0
+        uint4 cr_c = cr_n;                   // column 0's record (its x and ring slot)
+        cr_n = s_col[1];
+        if (!SYM) ch_n = s_chi[1];
+        r0_n = s_rr[rbase + 1];
+        r1_n = s_rr[rbase];
         uint32_t pp1 = 0u, pp0 = 0u;
         uint4 por = make_uint4(0u, 0u, 0u, 0u);
         int pxo = 0;
@@ -354,21 +373,17 @@ __global__ __launch_bounds__(64) void cost_crossh2_kernel(
         };
 
         auto step = [&](int c, bool keys, bool outp) {
-            const uint4 cr = cr_n, r0 = r0_n, r1 = r1_n;
-            const uint32_t chl = ch_n;
-            // records of column c + 1 (s_col[ncol] and s_rr[ncol + D - 1] are padding)
-            cr_n = s_col[c + 1];
-            if (!SYM) ch_n = s_chi[c + 1];
-            r0_n = s_rr[rbase + c + 1];
-            r1_n = s_rr[rbase + c];
-            const uint32_t g0 = cr.y & 0xFFu, gxy = cr.y >> 8;
-            const int pop0 = SYM ? __popc(cr.x ^ r0.x) : __popc(cr.x ^ r0.x) + __popc(chl ^ r0.w);
-            const int pop1 = SYM ? __popc(cr.x ^ r1.x) : __popc(cr.x ^ r1.x) + __popc(chl ^ r1.w);
-            const uint32_t ad0 = __builtin_amdgcn_sad_u8(r0.y, g0, 0u), ad1 = __builtin_amdgcn_sad_u8(r1.y, g0, 0u);
-            const uint32_t gg0 = __builtin_amdgcn_sad_u8(r0.z, gxy, 0u), gg1 = __builtin_amdgcn_sad_u8(r1.z, gxy, 0u);
+            const uint4 cr = cr_c;
             // same operation order as state_cost (reference src/cost_computer.cpp:176-199)
-            const float f0 = (s_lc[pop0] + s_la[ad0]) + s_lg[gg0];
-            const float f1 = (s_lc[pop1] + s_la[ad1]) + s_lg[gg1];
+            const float f0 = (tc0 + ta0) + tg0;
+            const float f1 = (tc1 + ta1) + tg1;
+            // LUT terms of column c + 1 and records of column c + 2 (padded past the end)
+            lut_terms(cr_n, ch_n, r0_n, r1_n);
+            cr_c = cr_n;
+            cr_n = s_col[c + 2];
+            if (!SYM) ch_n = s_chi[c + 2];
+            r0_n = s_rr[rbase + c + 2];
+            r1_n = s_rr[rbase + c + 1];
             if (pend) flush();
             const uint32_t cst0 = (uint32_t)clampi((int)(f0 * scale + 0.5f), 0, cost_max);
             const uint32_t cst1 = (uint32_t)clampi((int)(f1 * scale + 0.5f), 0, cost_max);
@@ -428,7 +443,7 @@ hipError_t launch_cost_crossh(hipStream_t s, const apg_hip_params& p, const Cost
     const int Lmax = cross ? (p.max_arm_length > 1 ? p.max_arm_length : 1) : 0;
     if (cross && out_u8 && column_ranges && D % 2 == 0 && (size_t)w * h * D < (1ull << 31)) {
         const int seg = w < K2_SEG ? w : K2_SEG;
-        const K2Shm L2 = k2_layout(D, Lmax, seg);
+        const K2Shm L2 = k2_layout(D, Lmax, seg, p.census_type == 1);
         const bool full = D % 128 == 0;
         auto k = p.census_type == 1 ? (full ? cost_crossh2_kernel<true, true> : cost_crossh2_kernel<true, false>)
                                     : (full ? cost_crossh2_kernel<false, true> : cost_crossh2_kernel<false, false>);
