@@ -55,6 +55,33 @@ class Intermediates(C.Structure):
                 ("d_second", C.c_void_p)]
 
 
+class Tuning(C.Structure):
+    """apg_hip_tuning: launch shapes of the fused schedule (0 = built-in)."""
+    _fields_ = [("cost_segment", C.c_int), ("sgm_dual", C.c_int), ("cross_v_waves", C.c_int)]
+
+    def as_tuple(self):
+        return (self.cost_segment, self.sgm_dual, self.cross_v_waves)
+
+
+class Pair(C.Structure):
+    """apg_hip_pair: one stereo pair of a batch (device pointers)."""
+    _fields_ = [("params", Params), ("d_left", C.c_void_p), ("d_right", C.c_void_p),
+                ("width", C.c_int), ("height", C.c_int), ("stride", C.c_int), ("channels", C.c_int),
+                ("d_disparity", C.c_void_p), ("d_confidence", C.c_void_p),
+                ("d_invalid_reason", C.c_void_p), ("d_reliable", C.c_void_p)]
+
+
+class Request(C.Structure):
+    """apg_hip_request: one serving request (host pointers)."""
+    _fields_ = [("params", Params), ("left", C.c_void_p), ("right", C.c_void_p),
+                ("width", C.c_int), ("height", C.c_int), ("stride", C.c_int), ("channels", C.c_int),
+                ("disparity", C.c_void_p), ("confidence", C.c_void_p),
+                ("invalid_reason", C.c_void_p), ("reliable", C.c_void_p)]
+
+
+BATCH_GRAPH = 1
+
+
 class ApgError(RuntimeError):
     def __init__(self, code, msg=""):
         super().__init__(f"{STATUS.get(code, code)} ({code}): {msg}")
@@ -108,6 +135,27 @@ def load():
         "apg_hip_set_schedule": (None, [vp, i]),
         "apg_hip_bench_pipeline": (i, [vp, P, vp, vp, i, i, i, i, vp, vp, vp, vp, i, i, C.POINTER(C.c_double),
                                        C.POINTER(Timing)]),
+        "apg_hip_pipeline_workspace_bytes": (sz, [P, i, i]),
+        "apg_hip_set_tuning": (i, [vp, C.POINTER(Tuning)]),
+        "apg_hip_get_tuning": (i, [vp, C.POINTER(Tuning)]),
+        "apg_hip_batch_create": (i, [i, i, C.POINTER(vp)]),
+        "apg_hip_batch_destroy": (None, [vp]),
+        "apg_hip_batch_stream": (vp, [vp]),
+        "apg_hip_batch_synchronize": (i, [vp]),
+        "apg_hip_batch_workspace_bytes": (sz, [vp]),
+        "apg_hip_batch_last_error": (C.c_char_p, [vp]),
+        "apg_hip_batch_run": (i, [vp, C.POINTER(Pair), i, i]),
+        "apg_hip_batch_autotune": (i, [vp, C.POINTER(Pair), i, C.c_char_p]),
+        "apg_hip_batch_tuning_for": (i, [vp, P, i, i, C.POINTER(Tuning)]),
+        "apg_hip_server_create": (i, [i, i, sz, C.POINTER(vp)]),
+        "apg_hip_server_destroy": (None, [vp]),
+        "apg_hip_server_last_error": (C.c_char_p, [vp]),
+        "apg_hip_server_submit": (i, [vp, C.POINTER(Request), C.c_uint64]),
+        "apg_hip_server_poll": (i, [vp, C.POINTER(C.c_uint64), C.POINTER(i), i]),
+        "apg_hip_server_pending": (i, [vp]),
+        "apg_hip_server_resident_bytes": (sz, [vp]),
+        "apg_hip_server_replay": (i, [vp, C.POINTER(Request), C.POINTER(C.c_double), i,
+                                      C.POINTER(C.c_double)]),
         "apg_synth_pair": (i, [i, i, i, i, C.c_uint64, vp, vp, vp]),
         "apg_hip_version_string": (C.c_char_p, []),
     }
@@ -274,3 +322,185 @@ def run_pipeline(ctx: Context, p: Params, left: np.ndarray, right: np.ndarray, i
     for a in [dl, dr, out_d, out_c, out_r, out_m] + list(arrs.values()):
         a.free()
     return res
+
+
+def set_tuning(ctx: Context, tuning=None):
+    """Launch shapes for later pipeline calls on ctx: a Tuning, a
+    (cost_segment, sgm_dual, cross_v_waves) tuple, or None for the built-in."""
+    if tuning is None:
+        ctx._check(ctx.lib.apg_hip_set_tuning(ctx.h, None), "set_tuning")
+        return
+    t = tuning if isinstance(tuning, Tuning) else Tuning(*tuning)
+    ctx._check(ctx.lib.apg_hip_set_tuning(ctx.h, C.byref(t)), "set_tuning")
+
+
+def workspace_bytes(p: Params, w: int, h: int) -> int:
+    """Device bytes of the pipeline workspace for a (w, h) pair (host-only)."""
+    return int(load().apg_hip_pipeline_workspace_bytes(C.byref(p), w, h))
+
+
+class DevicePair:
+    """Device inputs and outputs of one pair (allocated on ctx)."""
+
+    def __init__(self, ctx: Context, p: Params, left: np.ndarray, right: np.ndarray):
+        self.params = p
+        self.h, self.w = left.shape[:2]
+        self.ch = 1 if left.ndim == 2 else left.shape[2]
+        self.left, self.right = ctx.to_device(left), ctx.to_device(right)
+        self.disp = ctx.array((self.h, self.w), np.float32)
+        self.conf = ctx.array((self.h, self.w), np.float32)
+        self.reason = ctx.array((self.h, self.w), np.uint8)
+        self.rel = ctx.array((self.h, self.w), np.uint8)
+
+    def as_pair(self) -> Pair:
+        return Pair(self.params, self.left.ptr, self.right.ptr, self.w, self.h, self.w * self.ch, self.ch,
+                    self.disp.ptr, self.conf.ptr, self.reason.ptr, self.rel.ptr)
+
+    def clear_outputs(self, byte=0xFF):
+        for a in (self.disp, self.conf, self.reason, self.rel):
+            a.fill(byte)
+
+    def outputs(self):
+        return (self.disp.download(), self.conf.download(), self.reason.download(), self.rel.download())
+
+
+class Batch:
+    """apg_hip_batch: many pairs over `slots` concurrent pipelines, optionally
+    replayed as one hipGraph."""
+
+    def __init__(self, device=0, slots=4):
+        self.lib = load()
+        h = C.c_void_p()
+        rc = self.lib.apg_hip_batch_create(device, slots, C.byref(h))
+        if rc:
+            raise ApgError(rc, "apg_hip_batch_create")
+        self.h = h.value
+
+    def _check(self, rc, what):
+        if rc:
+            msg = self.lib.apg_hip_batch_last_error(self.h)
+            raise ApgError(rc, f"{what}: {msg.decode() if msg else ''}")
+
+    @staticmethod
+    def pair_array(pairs):
+        arr = (Pair * len(pairs))()
+        for i, q in enumerate(pairs):
+            arr[i] = q.as_pair() if isinstance(q, DevicePair) else q
+        return arr
+
+    def run(self, pairs, graph=False):
+        arr = pairs if isinstance(pairs, C.Array) else self.pair_array(pairs)
+        self._check(self.lib.apg_hip_batch_run(self.h, arr, len(arr), BATCH_GRAPH if graph else 0), "batch_run")
+
+    def autotune(self, pairs, cache_path=None):
+        arr = pairs if isinstance(pairs, C.Array) else self.pair_array(pairs)
+        path = cache_path.encode() if cache_path else None
+        self._check(self.lib.apg_hip_batch_autotune(self.h, arr, len(arr), path), "batch_autotune")
+
+    def tuning_for(self, p: Params, w: int, h: int) -> Tuning:
+        t = Tuning()
+        self._check(self.lib.apg_hip_batch_tuning_for(self.h, C.byref(p), w, h, C.byref(t)), "tuning_for")
+        return t
+
+    def sync(self):
+        self._check(self.lib.apg_hip_batch_synchronize(self.h), "batch_synchronize")
+
+    def workspace_bytes(self) -> int:
+        return int(self.lib.apg_hip_batch_workspace_bytes(self.h))
+
+    def close(self):
+        if self.h:
+            self.lib.apg_hip_batch_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class HostRequest:
+    """Host buffers of one serving request; keeps them alive with the Request."""
+
+    def __init__(self, p: Params, left: np.ndarray, right: np.ndarray):
+        self.left = np.ascontiguousarray(left)
+        self.right = np.ascontiguousarray(right)
+        h, w = self.left.shape[:2]
+        ch = 1 if self.left.ndim == 2 else self.left.shape[2]
+        self.disparity = np.full((h, w), np.nan, np.float32)
+        self.confidence = np.full((h, w), np.nan, np.float32)
+        self.invalid_reason = np.full((h, w), 0xFF, np.uint8)
+        self.reliable = np.full((h, w), 0xFF, np.uint8)
+        self.req = Request(p, self.left.ctypes.data, self.right.ctypes.data, w, h, w * ch, ch,
+                           self.disparity.ctypes.data, self.confidence.ctypes.data,
+                           self.invalid_reason.ctypes.data, self.reliable.ctypes.data)
+
+    def outputs(self):
+        return (self.disparity, self.confidence, self.invalid_reason, self.reliable)
+
+
+class Server:
+    """apg_hip_server: continuous batching of host requests under an HBM budget."""
+
+    def __init__(self, device=0, slots=4, budget_bytes=0):
+        self.lib = load()
+        h = C.c_void_p()
+        rc = self.lib.apg_hip_server_create(device, slots, budget_bytes, C.byref(h))
+        if rc:
+            raise ApgError(rc, "apg_hip_server_create")
+        self.h = h.value
+
+    def _check(self, rc, what):
+        if rc:
+            msg = self.lib.apg_hip_server_last_error(self.h)
+            raise ApgError(rc, f"{what}: {msg.decode() if msg else ''}")
+
+    def submit(self, r: HostRequest, tag: int):
+        self._check(self.lib.apg_hip_server_submit(self.h, C.byref(r.req), tag), "server_submit")
+
+    def poll(self, max_done=64):
+        tags = (C.c_uint64 * max_done)()
+        sts = (C.c_int * max_done)()
+        k = self.lib.apg_hip_server_poll(self.h, tags, sts, max_done)
+        if k < 0:
+            self._check(8, "server_poll")
+        return [(int(tags[i]), int(sts[i])) for i in range(k)]
+
+    def pending(self) -> int:
+        return int(self.lib.apg_hip_server_pending(self.h))
+
+    def resident_bytes(self) -> int:
+        return int(self.lib.apg_hip_server_resident_bytes(self.h))
+
+    def replay(self, reqs, arrival_s):
+        """Open-loop replay of (HostRequest, arrival second) -> latencies (s)."""
+        n = len(reqs)
+        arr = (Request * n)(*[r.req for r in reqs])
+        arrival = (C.c_double * n)(*[float(a) for a in arrival_s])
+        lat = (C.c_double * n)()
+        self._check(self.lib.apg_hip_server_replay(self.h, arr, arrival, n, lat), "server_replay")
+        return np.ctypeslib.as_array(lat).copy()
+
+    def close(self):
+        if self.h:
+            self.lib.apg_hip_server_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -25,6 +25,21 @@ enum BufId {
 
 enum StageId { S_AUX, S_PRIOR, S_COST, S_RWTA, S_CROSS, S_SGM, S_WTA, S_CONF, S_POST, S_COUNT };
 
+// The parameters build_cost_lut reads.
+struct LutKey {
+    float lc, la, lg, eta, mu;
+    int use_ad, use_grad, cost_max;
+    static LutKey of(const apg_hip_params& p) {
+        LutKey k;
+        memset(&k, 0, sizeof k);
+        k.lc = p.lambda_census; k.la = p.lambda_ad; k.lg = p.lambda_grad; k.eta = p.eta_ad; k.mu = p.mu_grad;
+        k.use_ad = p.use_ad; k.use_grad = p.use_grad; k.cost_max = p.cost_max;
+        return k;
+    }
+    bool operator==(const LutKey& o) const { return memcmp(this, &o, sizeof o) == 0; }
+};
+constexpr int kLutSlots = 8;
+
 }  // namespace
 
 struct apg_hip_ctx_s {
@@ -35,10 +50,15 @@ struct apg_hip_ctx_s {
     bool timing = false;
     hipEvent_t ev[S_COUNT + 1] = {};
     apg_hip_timing last{};
-    CostLut lut_host{};
-    bool lut_valid = false;
-    apg_hip_params lut_params{};
+    // cost LUTs of the last kLutSlots distinct LUT parameter sets: device
+    // copies side by side in B_LUT, host copies kept for the cost bound
+    CostLut lut_host[kLutSlots];
+    LutKey lut_key[kLutSlots];
+    bool lut_used[kLutSlots] = {};
+    int lut_next = 0, lut_cur = -1;
     int schedule = 0;   // 0 automatic (fused when eligible), 1 stage-by-stage
+    KTune tune;
+    bool capturing = false;   // stream capture in progress: no allocation, no host copy
 };
 
 namespace {
@@ -64,6 +84,7 @@ int hip_fail(apg_hip_ctx ctx, hipError_t e, const char* where) {
 hipError_t ensure(apg_hip_ctx ctx, BufId id, size_t bytes) {
     DevBuf& b = ctx->bufs[id];
     if (b.bytes >= bytes && b.ptr) return hipSuccess;
+    if (ctx->capturing) return hipErrorStreamCaptureUnsupported;
     if (b.ptr) {
         hipError_t e = hipStreamSynchronize(ctx->stream);
         if (e != hipSuccess) return e;
@@ -79,16 +100,34 @@ hipError_t ensure(apg_hip_ctx ctx, BufId id, size_t bytes) {
 template <typename T>
 T* buf(apg_hip_ctx ctx, BufId id) { return (T*)ctx->bufs[id].ptr; }
 
+// Selects the device LUT of p's cost parameters, building and uploading it
+// into the least recently built slot when it is not cached.  The upload is
+// stream-ordered, so kernels still reading the slot's previous table finish
+// first.
 int ensure_lut(apg_hip_ctx ctx, const apg_hip_params* p) {
-    if (ctx->lut_valid && memcmp(&ctx->lut_params, p, sizeof(*p)) == 0) return 0;
-    build_cost_lut(*p, &ctx->lut_host);
-    HIPCHK(ensure(ctx, B_LUT, sizeof(CostLut)), "lut alloc");
-    HIPCHK(hipMemcpyAsync(ctx->bufs[B_LUT].ptr, &ctx->lut_host, sizeof(CostLut), hipMemcpyHostToDevice, ctx->stream),
+    const LutKey key = LutKey::of(*p);
+    for (int i = 0; i < kLutSlots; ++i)
+        if (ctx->lut_used[i] && ctx->lut_key[i] == key) {
+            ctx->lut_cur = i;
+            return 0;
+        }
+    if (ctx->capturing) return fail(ctx, 8, "cost LUT not cached before stream capture");
+    HIPCHK(ensure(ctx, B_LUT, kLutSlots * sizeof(CostLut)), "lut alloc");
+    const int i = ctx->lut_next;
+    ctx->lut_next = (i + 1) % kLutSlots;
+    // an evicted slot's host copy may still be the source of a queued upload
+    if (ctx->lut_used[i]) HIPCHK(hipStreamSynchronize(ctx->stream), "lut sync");
+    build_cost_lut(*p, &ctx->lut_host[i]);
+    HIPCHK(hipMemcpyAsync(buf<CostLut>(ctx, B_LUT) + i, &ctx->lut_host[i], sizeof(CostLut), hipMemcpyHostToDevice,
+                          ctx->stream),
            "lut upload");
-    ctx->lut_valid = true;
-    ctx->lut_params = *p;
+    ctx->lut_key[i] = key;
+    ctx->lut_used[i] = true;
+    ctx->lut_cur = i;
     return 0;
 }
+
+const CostLut* cur_lut(apg_hip_ctx ctx) { return buf<CostLut>(ctx, B_LUT) + ctx->lut_cur; }
 
 void mark(apg_hip_ctx ctx, int stage) {
     if (ctx->timing) (void)hipEventRecord(ctx->ev[stage], ctx->stream);
@@ -108,6 +147,49 @@ int finish_timing(apg_hip_ctx ctx) {
         t.post_ms = ms[S_POST]; t.total_ms = tot;
     }
     return 0;
+}
+
+// Workspace of one apg_hip_pipeline call: which schedule runs and the bytes
+// of each buffer it needs.
+struct WsPlan {
+    bool fused = false, u8 = false;
+    size_t bytes[B_COUNT] = {};
+};
+
+WsPlan plan_workspace(const apg_hip_params& p, int w, int h, const CostLut& lut, int schedule,
+                      const apg_hip_intermediates* inter) {
+    WsPlan pl;
+    const size_t n = (size_t)w * h;
+    const int D = p.max_disparity - p.min_disparity;
+    const size_t nv = n * (size_t)D;
+    const size_t cen = p.census_type == 1 ? 4 : 8;
+    for (BufId b : {B_GRAY_L, B_GX_L, B_GY_L, B_GRAY_R, B_GX_R, B_GY_R}) pl.bytes[b] = n;
+    pl.bytes[B_CEN_L] = pl.bytes[B_CEN_R] = n * cen;
+    pl.bytes[B_DMIN] = pl.bytes[B_DMAX] = n * 2;
+    pl.bytes[B_ARMS] = pl.bytes[B_DR] = pl.bytes[B_SCRATCH] = n * 4;
+    pl.bytes[B_LUT] = kLutSlots * sizeof(CostLut);
+    // Fused schedule: the raw cost volume never reaches HBM, C' is stored as
+    // u8 when every reachable cost is <= 254, and the SGM keeps one u8 delta
+    // plane per direction (needs P2max <= 254) instead of u16 path sums.
+    const int p2max = p.P2_base > p.P1 + 1 ? p.P2_base : p.P1 + 1;
+    const bool want_raw = inter && (inter->d_cost || inter->d_cost_agg);
+    const int cbound = cost_upper_bound(lut, p.cost_max);
+    pl.u8 = cbound <= 254;
+    // the WTA stage rebuilds S in u16: npath * (C' + P2max) must stay below 65535
+    const int npaths = p.paths == 8 ? 8 : 4;
+    pl.fused = schedule != 1 && !want_raw && D % 2 == 0 && D <= 256 && p2max <= 254 &&
+               npaths * (cbound + p2max) < 65535 && (!p.agg_enable || p.agg_iterations <= 1) &&
+               nv * (pl.u8 ? 1 : 2) < (1ull << 31);
+    if (pl.fused) {
+        pl.bytes[B_COST] = nv * (pl.u8 ? 1 : 2);
+        if (p.agg_enable) pl.bytes[B_TMP] = nv * (pl.u8 ? 1 : 2);
+        pl.bytes[B_RKEYS] = n * 4;
+        for (int r = 0; r < npaths; ++r) pl.bytes[B_DS0 + r] = nv;
+    } else {
+        pl.bytes[B_COST] = pl.bytes[B_TMP] = pl.bytes[B_SUM] = nv * 2;
+        pl.bytes[B_BEST] = pl.bytes[B_SECOND] = n * 4;
+    }
+    return pl;
 }
 
 int check_ctx(apg_hip_ctx ctx, const apg_hip_params* p) {
@@ -222,7 +304,8 @@ void apg_hip_release_workspace(apg_hip_ctx ctx) {
         b.ptr = nullptr;
         b.bytes = 0;
     }
-    ctx->lut_valid = false;
+    for (bool& u : ctx->lut_used) u = false;
+    ctx->lut_cur = -1;
 }
 
 void apg_hip_enable_stage_timing(apg_hip_ctx ctx, int on) {
@@ -297,7 +380,7 @@ int apg_hip_cost_volume(apg_hip_ctx ctx, const apg_hip_params* p, int w, int h, 
                         const int16_t* dmin, const int16_t* dmax, uint16_t* d_cost) {
     if (int s = check_ctx(ctx, p)) return s;
     if (int s = ensure_lut(ctx, p)) return s;
-    HIPCHK(launch_cost(ctx->stream, *p, buf<CostLut>(ctx, B_LUT), w, h, gl, gxl, gyl, gr, gxr, gyr, c32l, c32r, c64l,
+    HIPCHK(launch_cost(ctx->stream, *p, cur_lut(ctx), w, h, gl, gxl, gyl, gr, gxr, gyr, c32l, c32r, c64l,
                        c64r, dmin, dmax, d_cost),
            "cost");
     return 0;
@@ -410,25 +493,11 @@ int apg_hip_pipeline(apg_hip_ctx ctx, const apg_hip_params* p, const uint8_t* d_
     const size_t n = (size_t)w * h;
     const int D = p->max_disparity - p->min_disparity;
     const size_t nv = n * (size_t)D;
-    const size_t cen = p->census_type == 1 ? 4 : 8;
-    HIPCHK(ensure(ctx, B_GRAY_L, n), "alloc");
-    HIPCHK(ensure(ctx, B_GX_L, n), "alloc");
-    HIPCHK(ensure(ctx, B_GY_L, n), "alloc");
-    HIPCHK(ensure(ctx, B_GRAY_R, n), "alloc");
-    HIPCHK(ensure(ctx, B_GX_R, n), "alloc");
-    HIPCHK(ensure(ctx, B_GY_R, n), "alloc");
-    HIPCHK(ensure(ctx, B_CEN_L, n * cen), "alloc");
-    HIPCHK(ensure(ctx, B_CEN_R, n * cen), "alloc");
-    HIPCHK(ensure(ctx, B_DMIN, n * 2), "alloc");
-    HIPCHK(ensure(ctx, B_DMAX, n * 2), "alloc");
-    HIPCHK(ensure(ctx, B_ARMS, n * 4), "alloc");
-    HIPCHK(ensure(ctx, B_COST, nv * 2), "alloc");
-    HIPCHK(ensure(ctx, B_TMP, nv * 2), "alloc");
-    HIPCHK(ensure(ctx, B_SUM, nv * 2), "alloc");
-    HIPCHK(ensure(ctx, B_DR, n * 4), "alloc");
-    HIPCHK(ensure(ctx, B_BEST, n * 4), "alloc");
-    HIPCHK(ensure(ctx, B_SECOND, n * 4), "alloc");
-    HIPCHK(ensure(ctx, B_SCRATCH, n * 4), "alloc");
+    const WsPlan plan = plan_workspace(*p, w, h, ctx->lut_host[ctx->lut_cur], ctx->schedule, inter);
+    for (int b = 0; b < B_COUNT; ++b)
+        if (b != B_LUT && plan.bytes[b]) HIPCHK(ensure(ctx, (BufId)b, plan.bytes[b]), "alloc");
+    const bool fused = plan.fused, u8 = plan.u8;
+    const int npaths = p->paths == 8 ? 8 : 4;
     uint8_t *gl = buf<uint8_t>(ctx, B_GRAY_L), *gxl = buf<uint8_t>(ctx, B_GX_L), *gyl = buf<uint8_t>(ctx, B_GY_L);
     uint8_t *gr = buf<uint8_t>(ctx, B_GRAY_R), *gxr = buf<uint8_t>(ctx, B_GX_R), *gyr = buf<uint8_t>(ctx, B_GY_R);
     void *cl = ctx->bufs[B_CEN_L].ptr, *cr = ctx->bufs[B_CEN_R].ptr;
@@ -449,46 +518,31 @@ int apg_hip_pipeline(apg_hip_ctx ctx, const apg_hip_params* p, const uint8_t* d_
     mark(ctx, S_PRIOR);
     HIPCHK(launch_range_init(st, *p, w, h, dmin, dmax), "range");
 
-    // Fused schedule: the raw cost volume never reaches HBM, C' is stored as
-    // u8 when every reachable cost is <= 254, and the SGM keeps one u8 delta
-    // plane per direction (needs P2max <= 254) instead of u16 path sums.
-    const int p2max = p->P2_base > p->P1 + 1 ? p->P2_base : p->P1 + 1;
-    const bool want_raw = inter && (inter->d_cost || inter->d_cost_agg);
-    const int cbound = cost_upper_bound(ctx->lut_host, p->cost_max);
-    const bool u8 = cbound <= 254;
-    // the WTA stage rebuilds S in u16: npath * (C' + P2max) must stay below 65535
-    const int npaths = p->paths == 8 ? 8 : 4;
-    const bool fused = ctx->schedule != 1 && !want_raw && D % 2 == 0 && D <= 256 && p2max <= 254 &&
-                       npaths * (cbound + p2max) < 65535 && (!p->agg_enable || p->agg_iterations <= 1) &&
-                       nv * (u8 ? 1 : 2) < (1ull << 31);
     if (fused) {
-        HIPCHK(ensure(ctx, B_RKEYS, n * 4), "alloc");
         PlaneSet planes{};
-        for (int r = 0; r < npaths; ++r) {
-            HIPCHK(ensure(ctx, (BufId)(B_DS0 + r), nv), "alloc");
-            planes.p[r] = buf<uint8_t>(ctx, (BufId)(B_DS0 + r));
-        }
+        for (int r = 0; r < npaths; ++r) planes.p[r] = buf<uint8_t>(ctx, (BufId)(B_DS0 + r));
         uint32_t* rkeys = buf<uint32_t>(ctx, B_RKEYS);
         void* cp = ctx->bufs[B_COST].ptr;   // C' (u8 or u16)
         void* tt = ctx->bufs[B_TMP].ptr;    // T  (u8 or u16)
         if (p->agg_enable) HIPCHK(launch_cross_arms(st, *p, gl, w, h, arms), "arms");
         HIPCHK(launch_fill_u32(st, n, 0xFFFFFFFFu, rkeys), "keys");
         mark(ctx, S_COST);
-        HIPCHK(launch_cost_crossh(st, *p, buf<CostLut>(ctx, B_LUT), w, h, gl, gxl, gyl, gr, gxr, gyr,
+        HIPCHK(launch_cost_crossh(st, *p, cur_lut(ctx), w, h, gl, gxl, gyl, gr, gxr, gyr,
                                   sym ? (const uint32_t*)cl : nullptr, sym ? (const uint32_t*)cr : nullptr,
                                   sym ? nullptr : (const uint64_t*)cl, sym ? nullptr : (const uint64_t*)cr, dmin,
-                                  dmax, arms, p->agg_enable ? tt : cp, u8, rkeys, true),
+                                  dmax, arms, p->agg_enable ? tt : cp, u8, rkeys, true, ctx->tune),
                "cost_crossh");
         mark(ctx, S_RWTA);
-        HIPCHK(launch_rwta_finalize(st, *p, buf<CostLut>(ctx, B_LUT), w, h, gl, gxl, gyl, gr, gxr, gyr,
+        HIPCHK(launch_rwta_finalize(st, *p, cur_lut(ctx), w, h, gl, gxl, gyl, gr, gxr, gyr,
                                     sym ? (const uint32_t*)cl : nullptr, sym ? (const uint32_t*)cr : nullptr,
                                     sym ? nullptr : (const uint64_t*)cl, sym ? nullptr : (const uint64_t*)cr, dmin,
                                     dmax, rkeys, dr),
                "rwta_finalize");
         mark(ctx, S_CROSS);
-        if (p->agg_enable) HIPCHK(launch_cross_v_t(st, *p, w, h, dmin, dmax, arms, tt, cp, u8, true), "cross_v");
+        if (p->agg_enable) HIPCHK(launch_cross_v_t(st, *p, w, h, dmin, dmax, arms, tt, cp, u8, true, ctx->tune),
+                                    "cross_v");
         mark(ctx, S_SGM);
-        HIPCHK(launch_sgm_lines(st, *p, w, h, D, gl, cp, u8, planes), "sgm_lines");
+        HIPCHK(launch_sgm_lines(st, *p, w, h, D, gl, cp, u8, planes, ctx->tune), "sgm_lines");
         mark(ctx, S_WTA);
         HIPCHK(launch_wta_fused(st, *p, w, h, cp, u8, planes, dmin, dmax, dr, gxl, gyl, d_disparity, d_confidence, d_reason,
                                 d_reliable, inter ? inter->d_disp_wta : nullptr, inter ? inter->d_best : nullptr,
@@ -505,7 +559,7 @@ int apg_hip_pipeline(apg_hip_ctx ctx, const apg_hip_params* p, const uint8_t* d_
     }
 
     mark(ctx, S_COST);
-    HIPCHK(launch_cost(st, *p, buf<CostLut>(ctx, B_LUT), w, h, gl, gxl, gyl, gr, gxr, gyr,
+    HIPCHK(launch_cost(st, *p, cur_lut(ctx), w, h, gl, gxl, gyl, gr, gxr, gyr,
                        sym ? (const uint32_t*)cl : nullptr, sym ? (const uint32_t*)cr : nullptr,
                        sym ? nullptr : (const uint64_t*)cl, sym ? nullptr : (const uint64_t*)cr, dmin, dmax, cost),
            "cost");
@@ -551,6 +605,65 @@ int apg_hip_pipeline(apg_hip_ctx ctx, const apg_hip_params* p, const uint8_t* d_
 void apg_hip_set_schedule(apg_hip_ctx ctx, int schedule) {
     if (ctx) ctx->schedule = schedule;
 }
+
+size_t apg_hip_pipeline_workspace_bytes(const apg_hip_params* p, int w, int h) {
+    if (!p || w <= 0 || h <= 0 || apg_hip_validate_params(p)) return 0;
+    CostLut lut;
+    build_cost_lut(*p, &lut);
+    const WsPlan pl = plan_workspace(*p, w, h, lut, 0, nullptr);
+    size_t t = 0;
+    for (size_t b : pl.bytes)
+        if (b) t += b < 256 ? 256 : b;
+    return t;
+}
+
+int apg_hip_set_tuning(apg_hip_ctx ctx, const apg_hip_tuning* t) {
+    if (!ctx) return 8;
+    if (!t) {
+        ctx->tune = KTune();
+        return 0;
+    }
+    if (t->cost_segment != 0 && (t->cost_segment < 32 || t->cost_segment > 1024))
+        return fail(ctx, 4, "cost_segment must be 0 or 32..1024");
+    if (t->sgm_dual != 0 && t->sgm_dual != 1) return fail(ctx, 4, "sgm_dual must be 0 or 1");
+    if (t->cross_v_waves != 0 && (t->cross_v_waves < 256 || t->cross_v_waves > 65536))
+        return fail(ctx, 4, "cross_v_waves must be 0 or 256..65536");
+    ctx->tune.cost_segment = t->cost_segment;
+    ctx->tune.sgm_dual = t->sgm_dual;
+    ctx->tune.cross_v_waves = t->cross_v_waves;
+    return 0;
+}
+
+int apg_hip_get_tuning(apg_hip_ctx ctx, apg_hip_tuning* out) {
+    if (!ctx || !out) return 8;
+    out->cost_segment = ctx->tune.cost_segment;
+    out->sgm_dual = ctx->tune.sgm_dual;
+    out->cross_v_waves = ctx->tune.cross_v_waves;
+    return 0;
+}
+
+}  // extern "C"
+
+// Library-internal (hidden): stream-capture guard used by the batch executor.
+void apgk_ctx_set_capturing(apg_hip_ctx ctx, bool on) {
+    if (ctx) ctx->capturing = on;
+}
+
+size_t apgk_ctx_growth_bytes(apg_hip_ctx ctx, const apg_hip_params* p, int w, int h) {
+    if (!ctx || !p || w <= 0 || h <= 0 || apg_hip_validate_params(p)) return 0;
+    CostLut lut;
+    build_cost_lut(*p, &lut);
+    const WsPlan pl = plan_workspace(*p, w, h, lut, ctx->schedule, nullptr);
+    size_t g = 0;
+    for (int b = 0; b < B_COUNT; ++b) {
+        const size_t want = pl.bytes[b] ? (pl.bytes[b] < 256 ? 256 : pl.bytes[b]) : 0;
+        const size_t have = ctx->bufs[b].ptr ? ctx->bufs[b].bytes : 0;
+        if (want > have) g += want - have;
+    }
+    return g;
+}
+
+extern "C" {
 
 int apg_hip_bench_pipeline(apg_hip_ctx ctx, const apg_hip_params* p, const uint8_t* d_left, const uint8_t* d_right,
                            int w, int h, int stride, int channels, float* d_disp, float* d_conf, uint8_t* d_reason,

@@ -348,13 +348,14 @@ __global__ __launch_bounds__(64) void cross_v3_kernel(int w, int h, int d0, int 
 
 hipError_t launch_cross_v_t(hipStream_t s, const apg_hip_params& p, int w, int h, const int16_t* dmin,
                             const int16_t* dmax, const uint32_t* arms, const void* tmp, void* cost, bool u8,
-                            bool column_ranges) {
+                            bool column_ranges, const KTune& tn) {
     {
         const int D = p.max_disparity - p.min_disparity;
         const int Lmax = p.max_arm_length > 1 ? p.max_arm_length : 1;
         if (u8 && column_ranges && p.min_disparity >= 0 && D % 2 == 0 && (size_t)w * h * D < (1ull << 31)) {
             const int dch = (D + 127) / 128;
-            int nseg = (4096 + w * dch - 1) / (w * dch);
+            const int target = tn.cross_v_waves > 0 ? tn.cross_v_waves : 4096;
+            int nseg = (target + w * dch - 1) / (w * dch);
             nseg = max(1, min(nseg, (h + 63) / 64));
             const int seg = (h + nseg - 1) / nseg;
             nseg = (h + seg - 1) / seg;

@@ -437,12 +437,14 @@ hipError_t launch_cost_crossh(hipStream_t s, const apg_hip_params& p, const Cost
                               const uint8_t* gl, const uint8_t* gxl, const uint8_t* gyl, const uint8_t* gr,
                               const uint8_t* gxr, const uint8_t* gyr, const uint32_t* c32l, const uint32_t* c32r,
                               const uint64_t* c64l, const uint64_t* c64r, const int16_t* dmin, const int16_t* dmax,
-                              const uint32_t* arms, void* out, bool out_u8, uint32_t* rkeys, bool column_ranges) {
+                              const uint32_t* arms, void* out, bool out_u8, uint32_t* rkeys, bool column_ranges,
+                              const KTune& tn) {
     const int D = p.max_disparity - p.min_disparity;
     const bool cross = p.agg_enable != 0;
     const int Lmax = cross ? (p.max_arm_length > 1 ? p.max_arm_length : 1) : 0;
     if (cross && out_u8 && column_ranges && D % 2 == 0 && (size_t)w * h * D < (1ull << 31)) {
-        const int seg = w < K2_SEG ? w : K2_SEG;
+        int seg = tn.cost_segment > 0 ? tn.cost_segment : K2_SEG;
+        seg = w < seg ? w : seg;
         const K2Shm L2 = k2_layout(D, Lmax, seg, p.census_type == 1);
         const bool full = D % 128 == 0;
         auto k = p.census_type == 1 ? (full ? cost_crossh2_kernel<true, true> : cost_crossh2_kernel<true, false>)

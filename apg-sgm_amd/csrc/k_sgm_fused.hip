@@ -407,7 +407,7 @@ __global__ __launch_bounds__(64) void sgm_lines4_kernel(apg_hip_params p, int w,
 
 template <typename TC>
 static hipError_t launch_lines_t(hipStream_t s, const apg_hip_params& p, int w, int h, int D, const uint8_t* gray,
-                                 const TC* cost, const PlaneSet& planes) {
+                                 const TC* cost, const PlaneSet& planes, const KTune& tn) {
     const int K = (D + 127) / 128;
     const int nfam = p.paths == 8 ? 4 : 2;
     const int nblk = h + w + (nfam == 4 ? 2 * (w + h - 1) : 0);
@@ -417,8 +417,12 @@ static hipError_t launch_lines_t(hipStream_t s, const apg_hip_params& p, int w, 
         const size_t shm4 = (size_t)8 * SB * 128 * K + shm;
         const int nblk4 = 2 * nblk;
 #define APG_LINES4(KK, FF)                                                                                        \
-    hipLaunchKernelGGL((sgm_lines4_kernel<KK, FF, false>), dim3(nblk4), dim3(64), shm4, s, p, w, h, D, nfam,        \
-                       gray, (const uint8_t*)cost, planes)
+    if (tn.sgm_dual)                                                                                              \
+        hipLaunchKernelGGL((sgm_lines4_kernel<KK, FF, true>), dim3(nblk), dim3(64), shm4, s, p, w, h, D, nfam,     \
+                           gray, (const uint8_t*)cost, planes);                                                   \
+    else                                                                                                          \
+        hipLaunchKernelGGL((sgm_lines4_kernel<KK, FF, false>), dim3(nblk4), dim3(64), shm4, s, p, w, h, D, nfam,   \
+                           gray, (const uint8_t*)cost, planes)
         if (K == 1) {
             if (full) APG_LINES4(1, true);
             else APG_LINES4(1, false);
@@ -446,10 +450,10 @@ static hipError_t launch_lines_t(hipStream_t s, const apg_hip_params& p, int w, 
 }
 
 hipError_t launch_sgm_lines(hipStream_t s, const apg_hip_params& p, int w, int h, int D, const uint8_t* gray,
-                            const void* cost, bool cost_u8, const PlaneSet& planes) {
+                            const void* cost, bool cost_u8, const PlaneSet& planes, const KTune& tn) {
     if (D % 2 || D > 256 || (size_t)w * h * D * (cost_u8 ? 1 : 2) >= (1ull << 31)) return hipErrorInvalidValue;
-    return cost_u8 ? launch_lines_t<uint8_t>(s, p, w, h, D, gray, (const uint8_t*)cost, planes)
-                   : launch_lines_t<uint16_t>(s, p, w, h, D, gray, (const uint16_t*)cost, planes);
+    return cost_u8 ? launch_lines_t<uint8_t>(s, p, w, h, D, gray, (const uint8_t*)cost, planes, tn)
+                   : launch_lines_t<uint16_t>(s, p, w, h, D, gray, (const uint16_t*)cost, planes, tn);
 }
 
 }  // namespace apgk

@@ -8,6 +8,13 @@
 
 #include "../../include/apg_sgm_hip.h"
 
+// Library-internal: marks a context whose stream is being captured into a
+// hipGraph, so the pipeline refuses workspace growth and LUT uploads.
+void apgk_ctx_set_capturing(apg_hip_ctx ctx, bool on);
+// Library-internal: device bytes a pipeline call on (p, w, h) would add to
+// the context's workspace.
+size_t apgk_ctx_growth_bytes(apg_hip_ctx ctx, const apg_hip_params* p, int w, int h);
+
 namespace apgk {
 
 // Cost LUTs built on the host with the same libm expf as the reference
@@ -18,6 +25,15 @@ struct CostLut {
     float grad[511];
     float scale;
     int pad[1];
+};
+
+// Launch-shape choices of the fused schedule (0 = default): output columns per
+// wave of the cost + Cross-H pass, one or both directions of a scanline per
+// SGM wave, target wave count of the Cross-V pass.
+struct KTune {
+    int cost_segment = 0;
+    int sgm_dual = 0;
+    int cross_v_waves = 0;
 };
 
 void build_cost_lut(const apg_hip_params& p, CostLut* out);
@@ -69,14 +85,14 @@ int cost_upper_bound(const CostLut& lut, int cost_max);
 // column_ranges: every pixel's range depends on its column only (dense range init)
 hipError_t launch_cross_v_t(hipStream_t s, const apg_hip_params& p, int w, int h, const int16_t* dmin,
                             const int16_t* dmax, const uint32_t* arms, const void* tmp, void* cost, bool u8,
-                            bool column_ranges);
+                            bool column_ranges, const KTune& tn = KTune());
 hipError_t launch_fill_u32(hipStream_t s, size_t n, uint32_t v, uint32_t* p);
 hipError_t launch_cost_crossh(hipStream_t s, const apg_hip_params& p, const CostLut* d_lut, int w, int h,
                               const uint8_t* gl, const uint8_t* gxl, const uint8_t* gyl, const uint8_t* gr,
                               const uint8_t* gxr, const uint8_t* gyr, const uint32_t* c32l, const uint32_t* c32r,
                               const uint64_t* c64l, const uint64_t* c64r, const int16_t* dmin, const int16_t* dmax,
                               const uint32_t* arms, void* out, bool out_u8, uint32_t* rkeys,
-                              bool column_ranges);
+                              bool column_ranges, const KTune& tn = KTune());
 hipError_t launch_rwta_finalize(hipStream_t s, const apg_hip_params& p, const CostLut* d_lut, int w, int h,
                                 const uint8_t* gl, const uint8_t* gxl, const uint8_t* gyl, const uint8_t* gr,
                                 const uint8_t* gxr, const uint8_t* gyr, const uint32_t* c32l, const uint32_t* c32r,
@@ -87,7 +103,7 @@ struct PlaneSet {
     uint8_t* p[8];
 };
 hipError_t launch_sgm_lines(hipStream_t s, const apg_hip_params& p, int w, int h, int D, const uint8_t* gray,
-                            const void* cost, bool cost_u8, const PlaneSet& planes);
+                            const void* cost, bool cost_u8, const PlaneSet& planes, const KTune& tn = KTune());
 hipError_t launch_wta_fused(hipStream_t s, const apg_hip_params& p, int w, int h, const void* cost, bool cost_u8,
                             const PlaneSet& planes, const int16_t* dmin, const int16_t* dmax, const float* dr,
                             const uint8_t* gx, const uint8_t* gy, float* disp, float* conf, uint8_t* reason,

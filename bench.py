@@ -93,6 +93,197 @@ def measured_traffic(stage):
     return None
 
 
+# ------------------------------------------------------------ mixed forms ----
+
+SEED_BATCH = 20260202
+SEED_SERVE = 20260303
+TUNE_CACHE = os.path.join(ROOT, "apg-sgm_amd", "apg_amd", "launch_shapes_gfx950.json")
+
+
+def splitmix64(state):
+    """Integer-only seeded stream (splitmix64): yields 64-bit values."""
+    m = (1 << 64) - 1
+    while True:
+        state = (state + 0x9E3779B97F4A7C15) & m
+        z = state
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & m
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & m
+        yield z ^ (z >> 31)
+
+
+def batch_specs(n=64, seed=SEED_BATCH):
+    """Uneven batch: W uniform in [320, 1920] step 16, H in [240, 1080] step 8,
+    D from {64, 128, 192, 256} restricted to D < W/2; Balanced; pair i is
+    generated with seed + i by the noise + 4 rectangles generator."""
+    rng = splitmix64(seed)
+    specs = []
+    for i in range(n):
+        w = 320 + 16 * (next(rng) % 101)
+        h = 240 + 8 * (next(rng) % 106)
+        ds = [d for d in (64, 128, 192, 256) if d < w / 2]
+        specs.append((w, h, ds[next(rng) % len(ds)], seed + i))
+    return specs
+
+
+def bench_batched(args):
+    """Batched form against the plain per-pair path on the same 64 device-
+    resident pairs: plain = apg_hip_pipeline per pair on one stream with a
+    synchronize after each pair; batched = apg_hip_batch_run over `slots`
+    streams replayed as one hipGraph, launch shapes from the autotuner."""
+    import apg_amd as A
+    specs = batch_specs(args.pairs)
+    states = sum(w * h * d for w, h, d, _ in specs)
+    ctx = A.Context(0)
+    pairs = []
+    for w, h, d, sd in specs:
+        left, right, _ = A.synth_pair(0, w, h, d, sd)
+        pairs.append(A.DevicePair(ctx, A.params(1, d), left, right))
+    # plain per-pair path
+    for q in pairs[: args.warmup]:
+        ctx._check(ctx.lib.apg_hip_pipeline(ctx.h, C.byref(q.params), q.left.ptr, q.right.ptr, q.w, q.h, q.w, 1,
+                                            q.disp.ptr, q.conf.ptr, q.reason.ptr, q.rel.ptr, None), "pipeline")
+    ctx.sync()
+    plain_ms = []
+    for _ in range(args.reps):
+        t0 = time.perf_counter()
+        for q in pairs:
+            ctx._check(ctx.lib.apg_hip_pipeline(ctx.h, C.byref(q.params), q.left.ptr, q.right.ptr, q.w, q.h, q.w, 1,
+                                                q.disp.ptr, q.conf.ptr, q.reason.ptr, q.rel.ptr, None), "pipeline")
+            ctx.sync()
+        plain_ms.append((time.perf_counter() - t0) * 1e3)
+    want = [q.outputs() for q in pairs]
+    ctx.lib.apg_hip_release_workspace(ctx.h)
+    # batched form
+    b = A.Batch(0, args.slots)
+    arr = b.pair_array(pairs)
+    t0 = time.perf_counter()
+    b.autotune(arr, args.tune_cache)
+    tune_s = time.perf_counter() - t0
+    for q in pairs:
+        q.clear_outputs()
+    b.run(arr, graph=True)      # direct run + capture
+    b.sync()
+    b.run(arr, graph=True)      # first replay
+    b.sync()
+    batch_ms = []
+    for _ in range(args.reps):
+        t0 = time.perf_counter()
+        b.run(arr, graph=True)
+        b.sync()
+        batch_ms.append((time.perf_counter() - t0) * 1e3)
+    direct_ms = []
+    for _ in range(max(1, args.reps // 2)):
+        t0 = time.perf_counter()
+        b.run(arr)
+        b.sync()
+        direct_ms.append((time.perf_counter() - t0) * 1e3)
+    same = all(all(np.array_equal(x, y, equal_nan=True) for x, y in zip(q.outputs(), w_))
+               for q, w_ in zip(pairs, want))
+    tuned = {}
+    for q in pairs:
+        t = b.tuning_for(q.params, q.w, q.h).as_tuple()
+        tuned[str(t)] = tuned.get(str(t), 0) + 1
+    ws = b.workspace_bytes()
+    b.close()
+    ctx.close()
+    med = lambda v: float(sorted(v)[len(v) // 2])  # noqa: E731
+    p_ms, g_ms, d_ms = med(plain_ms), med(batch_ms), med(direct_ms)
+    value = states / (g_ms * 1e-3) / 1e6
+    plain = states / (p_ms * 1e-3) / 1e6
+    return {"metric": f"Mdisp/s, batched form: {len(specs)} uneven pairs (sum W*H*D per batch / batch wall time)",
+            "value": round(value, 1), "unit": "Mdisp/s", "n_gpus": 1, "steps": args.reps, "higher_is_better": True,
+            "dtype": "u8/u16", "data": "synthetic",
+            "config": {"workload": f"{len(specs)} pairs, W in [320,1920]/16, H in [240,1080]/8, D in "
+                                   "{64,128,192,256} < W/2, Balanced, noise + 4 rectangles", "seed": SEED_BATCH,
+                       "states_per_batch": states, "slots": args.slots},
+            "ms_per_batch": round(g_ms, 3), "ms_per_batch_direct": round(d_ms, 3),
+            "plain_per_pair": {"value": round(plain, 1), "ms_per_batch": round(p_ms, 3),
+                               "how": "apg_hip_pipeline per pair, one stream, synchronize after each pair"},
+            "speedup_vs_plain": round(value / plain, 3), "bit_identical_to_plain": same,
+            "autotune_s": round(tune_s, 2), "launch_shapes": tuned, "workspace_bytes": ws}
+
+
+def serve_specs(n, rate, seed):
+    """Poisson arrivals (exponential gaps from the integer stream) at `rate`
+    per second; class 60% 640x480 D64 / 30% 1280x720 D128 / 10% 1920x1080
+    D256; mode 70% Balanced / 30% Fast."""
+    rng = splitmix64(seed)
+    t, out = 0.0, []
+    for _ in range(n):
+        u = (next(rng) >> 11) * (1.0 / (1 << 53))
+        t += -np.log1p(-u) / rate
+        c = next(rng) % 10
+        cls = 0 if c < 6 else (1 if c < 9 else 2)
+        mode = 1 if next(rng) % 10 < 7 else 0
+        out.append((t, cls, mode, next(rng) % 4))
+    return out
+
+
+SERVE_CLASSES = [(640, 480, 64), (1280, 720, 128), (1920, 1080, 256)]
+
+
+def bench_serving(args):
+    """Serving form: open-loop Poisson traces replayed in real time through
+    apg_hip_server_replay (C++ loop: admission under the HBM budget, slots
+    on their own streams, upload / pipeline / download per request).  The
+    plain per-pair path is the same server with one slot (no overlap)."""
+    import apg_amd as A
+    pool = {}
+    for ci, (w, h, d) in enumerate(SERVE_CLASSES):
+        for v in range(4):
+            pool[(ci, v)] = A.synth_pair(0, w, h, d, SEED_SERVE + 10 * ci + v)[:2]
+    reqs_cache = {}
+
+    def request(ci, mode, v):
+        # one request object (and one set of output buffers) per input and mode
+        key = (ci, mode, v)
+        if key not in reqs_cache:
+            w, h, d = SERVE_CLASSES[ci]
+            reqs_cache[key] = A.HostRequest(A.params(mode, d), *pool[(ci, v)])
+        return reqs_cache[key]
+
+    def run(server, rate, n):
+        tr = serve_specs(n, rate, SEED_SERVE + int(rate))
+        reqs = [request(ci, mode, v) for _, ci, mode, v in tr]
+        arrival = [t for t, _, _, _ in tr]
+        t0 = time.perf_counter()
+        lat = server.replay(reqs, arrival)
+        wall = time.perf_counter() - t0
+        st = sum(SERVE_CLASSES[ci][0] * SERVE_CLASSES[ci][1] * SERVE_CLASSES[ci][2] for _, ci, _, _ in tr)
+        return {"rate": rate, "requests": n, "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
+                "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3), "mdisp_s": round(st / wall / 1e6, 1),
+                "req_s": round(n / wall, 1)}
+
+    warm = [(ci, m, v) for ci in range(3) for m in (0, 1) for v in range(4)]
+    out_rates = []
+    srv = A.Server(0, args.slots, args.budget_gb << 30)
+    srv.replay([request(*k) for k in warm], [0.0] * len(warm))
+    for rate in args.rates:
+        n = int(max(100, min(args.max_requests, rate * args.seconds)))
+        out_rates.append(run(srv, rate, n))
+    resident = srv.resident_bytes()
+    srv.close()
+    one = A.Server(0, 1, args.budget_gb << 30)
+    one.replay([request(*k) for k in warm], [0.0] * len(warm))
+    sat_n = int(min(args.max_requests, 1000))
+    plain = run(one, 1e6, sat_n)
+    one.close()
+    srv = A.Server(0, args.slots, args.budget_gb << 30)
+    srv.replay([request(*k) for k in warm], [0.0] * len(warm))
+    sat = run(srv, 1e6, sat_n)
+    srv.close()
+    best = max(r["mdisp_s"] for r in out_rates + [sat])
+    return {"metric": "Mdisp/s, serving form: Poisson mix 60% 640x480 D64 / 30% 1280x720 D128 / 10% 1920x1080 D256, "
+                      "70% Balanced / 30% Fast", "value": sat["mdisp_s"], "unit": "Mdisp/s", "n_gpus": 1,
+            "higher_is_better": True, "dtype": "u8/u16", "data": "synthetic",
+            "config": {"workload": "serving mix, host inputs and outputs (upload + pipeline + download per "
+                                   "request)", "slots": args.slots, "hbm_budget_gb": args.budget_gb,
+                       "seed": SEED_SERVE},
+            "saturated": sat, "plain_per_pair_saturated": plain,
+            "speedup_vs_plain": round(sat["mdisp_s"] / plain["mdisp_s"], 3), "rates": out_rates,
+            "best_mdisp_s": best, "resident_bytes": resident}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -103,6 +294,19 @@ def main():
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MB",
                     help="first run one device-to-device copy of MB MiB (a kernel of known traffic, "
                          "to scale the FETCH_SIZE / WRITE_SIZE counters of a profiled run)")
+    ap.add_argument("--form", choices=("pair", "batched", "serving"), default="pair",
+                    help="pair: the single-pair headline (default); batched / serving: the mixed-size forms")
+    ap.add_argument("--slots", type=int, default=4, help="concurrent pipelines of the batched / serving forms")
+    ap.add_argument("--pairs", type=int, default=64, help="batched form: pairs per batch")
+    ap.add_argument("--reps", type=int, default=5, help="batched form: timed batches")
+    ap.add_argument("--tune-cache", default=TUNE_CACHE, help="batched form: launch-shape cache (JSON)")
+    ap.add_argument("--rates", type=float, nargs="+", default=[50, 200, 500, 1000, 2000],
+                    help="serving form: arrival rates (requests/s)")
+    ap.add_argument("--seconds", type=float, default=2.0, help="serving form: trace length per rate")
+    ap.add_argument("--max-requests", type=int, default=2000)
+    ap.add_argument("--budget-gb", type=int, default=64, help="serving form: HBM admission budget")
+    ap.add_argument("--shapes", default=None, metavar="SEG,DUAL,CVW",
+                    help="pair form: launch shapes (cost_segment, sgm_dual, cross_v_waves; 0 = built-in)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs one GPU per process; --gpus {args.gpus} is not supported "
@@ -118,12 +322,19 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
+    if args.form != "pair":
+        out = bench_batched(args) if args.form == "batched" else bench_serving(args)
+        print(json.dumps(out))
+        return
+
     import apg_amd as A
     p = A.params(1, DMAX)
     left, right, _ = A.synth_pair(1, W, H, DMAX, SEED)
     states = W * H * DMAX
     ctx = A.Context(0)
     A.set_schedule(ctx, args.schedule)
+    if args.shapes:
+        A.set_tuning(ctx, tuple(int(v) for v in args.shapes.split(",")))
     dl, dr = ctx.to_device(left), ctx.to_device(right)
     od = ctx.array((H, W), np.float32)
     oc = ctx.array((H, W), np.float32)

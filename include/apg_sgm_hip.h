@@ -224,6 +224,124 @@ APG_HIP_API int apg_hip_bench_pipeline(apg_hip_ctx ctx, const apg_hip_params* p,
                                        int warmup, int steps, double* ms_total,
                                        apg_hip_timing* stage_avg);
 
+/* Device bytes the workspace of one apg_hip_pipeline call on a (w, h) pair
+ * with parameters p needs (fused schedule when eligible); 0 for invalid
+ * parameters.  Used for HBM admission.  No counterpart in the reference
+ * (StereoWorkspace::allocated_bytes, include/apg_sgm/workspace.hpp:10-75,
+ * reports it after the fact). */
+APG_HIP_API size_t apg_hip_pipeline_workspace_bytes(const apg_hip_params* p, int w, int h);
+
+/* ---- launch shapes ----------------------------------------------------- */
+
+/* Launch shapes of the fused schedule; 0 keeps the built-in choice.  They
+ * change how work is split across waves, never the arithmetic: outputs are
+ * identical for every setting.  No counterpart in the reference (its CPU
+ * stages have no launch shapes). */
+typedef struct {
+    int cost_segment;    /* output columns per wave of the cost + Cross-H pass: 32..1024 (256) */
+    int sgm_dual;        /* 1: one wave runs both directions of a scanline (0) */
+    int cross_v_waves;   /* target wave count of the Cross-V pass: 256..65536 (4096) */
+} apg_hip_tuning;
+
+/* NULL restores the built-in choice.  Applies to later pipeline calls. */
+APG_HIP_API int apg_hip_set_tuning(apg_hip_ctx ctx, const apg_hip_tuning* t);
+APG_HIP_API int apg_hip_get_tuning(apg_hip_ctx ctx, apg_hip_tuning* out);
+
+/* ---- batched form ------------------------------------------------------ */
+
+/* One stereo pair of a batch: device inputs and outputs, as for
+ * apg_hip_pipeline.  No counterpart in the reference, which processes one
+ * pair per StereoMatcher::compute call (include/apg_sgm/pipeline.hpp). */
+typedef struct {
+    apg_hip_params params;
+    const uint8_t* d_left;
+    const uint8_t* d_right;
+    int width, height, stride, channels;
+    float* d_disparity;
+    float* d_confidence;
+    uint8_t* d_invalid_reason;
+    uint8_t* d_reliable;
+} apg_hip_pair;
+
+typedef struct apg_hip_batch_s* apg_hip_batch;
+
+/* A batch executor on `device` with `slots` concurrent pipelines (1..16;
+ * 0 = 4), each with its own stream and workspace, joined to one origin
+ * stream. */
+APG_HIP_API int apg_hip_batch_create(int device, int slots, apg_hip_batch* out);
+APG_HIP_API void apg_hip_batch_destroy(apg_hip_batch b);
+APG_HIP_API void* apg_hip_batch_stream(apg_hip_batch b);   /* origin hipStream_t */
+APG_HIP_API int apg_hip_batch_synchronize(apg_hip_batch b);
+APG_HIP_API size_t apg_hip_batch_workspace_bytes(apg_hip_batch b);
+APG_HIP_API const char* apg_hip_batch_last_error(apg_hip_batch b);
+
+#define APG_HIP_BATCH_GRAPH 1   /* replay the batch as one captured hipGraph */
+
+/* Runs every pair, stream-ordered after earlier work on the origin stream;
+ * later work on it waits for all pairs.  Pairs are spread over the slots by
+ * longest-processing-time order of W*H*D, each with the launch shapes its
+ * size bucket was tuned to.  With APG_HIP_BATCH_GRAPH the whole batch is one
+ * hipGraphLaunch whenever the same pair list ran before on this batch: a new
+ * list runs directly once (sizing workspaces and LUTs), is then captured,
+ * and replays from the next call on.  Results are identical to
+ * apg_hip_pipeline on each pair alone. */
+APG_HIP_API int apg_hip_batch_run(apg_hip_batch b, const apg_hip_pair* pairs, int n, int flags);
+
+/* Autotuner: for each size bucket (W/256, H/128, D, paths, Cross on/off,
+ * census) of `pairs` not yet tuned, times the candidate launch shapes of
+ * each stage on the bucket's first pair (its outputs are overwritten with
+ * the same values) and keeps the fastest.  With cache_path, known buckets
+ * are read from that JSON file first and every bucket is written back. */
+APG_HIP_API int apg_hip_batch_autotune(apg_hip_batch b, const apg_hip_pair* pairs, int n,
+                                       const char* cache_path);
+/* Launch shapes the batch uses for a pair of this size and mode. */
+APG_HIP_API int apg_hip_batch_tuning_for(apg_hip_batch b, const apg_hip_params* p, int w, int h,
+                                         apg_hip_tuning* out);
+
+/* ---- serving form ------------------------------------------------------ */
+
+/* One request in host memory: inputs are copied in and results copied out
+ * by the server.  Field meaning as apg_hip_pair. */
+typedef struct {
+    apg_hip_params params;
+    const uint8_t* left;
+    const uint8_t* right;
+    int width, height, stride, channels;
+    float* disparity;            /* w*h, required */
+    float* confidence;           /* w*h or NULL */
+    uint8_t* invalid_reason;     /* w*h or NULL */
+    uint8_t* reliable;           /* w*h or NULL */
+} apg_hip_request;
+
+typedef struct apg_hip_server_s* apg_hip_server;
+
+/* Continuous-batching server: `slots` pipelines (1..16; 0 = 4) on their own
+ * streams, each with a device-resident workspace that is kept between
+ * requests.  A request is admitted into a free slot when the device bytes
+ * of all slot workspaces stay within hbm_budget_bytes (0 = 64 GiB). */
+APG_HIP_API int apg_hip_server_create(int device, int slots, size_t hbm_budget_bytes, apg_hip_server* out);
+APG_HIP_API void apg_hip_server_destroy(apg_hip_server s);
+APG_HIP_API const char* apg_hip_server_last_error(apg_hip_server s);
+
+/* Queues a request (FIFO).  ResourceLimitExceeded when the pair alone does
+ * not fit the budget.  Buffers must stay valid until its tag is returned. */
+APG_HIP_API int apg_hip_server_submit(apg_hip_server s, const apg_hip_request* r, uint64_t tag);
+/* Non-blocking: retires finished requests (results are then in the host
+ * buffers) and admits queued ones into free slots.  Writes up to max_done
+ * finished tags and statuses; returns their count, or -1 on error. */
+APG_HIP_API int apg_hip_server_poll(apg_hip_server s, uint64_t* done_tags, int* done_status, int max_done);
+/* Requests queued or in flight. */
+APG_HIP_API int apg_hip_server_pending(apg_hip_server s);
+/* Device bytes held by the slot workspaces and staging buffers. */
+APG_HIP_API size_t apg_hip_server_resident_bytes(apg_hip_server s);
+
+/* Open-loop replay: request i is submitted when arrival_s[i] seconds have
+ * passed since the call began, and the loop polls until all are done.
+ * latency_s[i] = completion observed by the host - arrival.  Returns 0 or
+ * the first failing request's status. */
+APG_HIP_API int apg_hip_server_replay(apg_hip_server s, const apg_hip_request* reqs, const double* arrival_s, int n,
+                                      double* latency_s);
+
 /* ---- synthetic inputs ------------------------------------------------ */
 
 /* Seeded synthetic stereo pair (host memory).  kind 0: noise texture, flat

@@ -1,0 +1,202 @@
+"""Batched and serving forms (apg_hip_batch_*, apg_hip_server_*, C ABI).
+
+Every pair's outputs (disparity, confidence, invalid_reason, reliable mask)
+must be bit-identical to apg_hip_pipeline on that pair alone, whatever slot
+it lands on, whatever runs beside it, whatever launch shapes it uses, and on
+every replay of a captured graph.  The alone results are themselves pinned
+to the oracle by test_gpu_pipeline.py; two small pairs of the mix are
+checked against the oracle here as well.
+"""
+import json
+
+import numpy as np
+import pytest
+
+import apg_amd as A
+import pyoracle as O
+
+# (kind, w, h, D, mode): uneven sizes, D from 16 to 256 (K = 1 and 2 SGM
+# lanes, full and partial), Balanced and Fast, 8 paths, 64-bit census
+MIX = [
+    (1, 320, 240, 64, 1, {}),
+    (0, 160, 120, 32, 0, {}),
+    (2, 400, 200, 128, 1, {}),
+    (1, 256, 160, 48, 1, {}),
+    (0, 96, 64, 16, 0, {}),
+    (1, 560, 96, 256, 1, {}),
+    (2, 420, 150, 192, 1, {}),
+    (0, 128, 100, 40, 1, {"paths": 8}),
+    (1, 208, 136, 64, 1, {"census_type": 0}),
+    (2, 300, 180, 96, 0, {}),
+    (0, 640, 48, 128, 1, {}),
+    (1, 72, 300, 24, 1, {}),
+]
+
+
+def _params(mode, D, over):
+    return A.params(mode, D, **over)
+
+
+def _inputs(i, kind, w, h, D):
+    left, right, _ = A.synth_pair(kind, w, h, D, 777 + 31 * i)
+    return left, right
+
+
+@pytest.fixture(scope="module")
+def alone(ctx):
+    """apg_hip_pipeline on each pair alone (default launch shapes)."""
+    A.set_schedule(ctx, 0)
+    A.set_tuning(ctx, None)
+    res = []
+    for i, (kind, w, h, D, mode, over) in enumerate(MIX):
+        left, right = _inputs(i, kind, w, h, D)
+        g = A.run_pipeline(ctx, _params(mode, D, over), left, right)
+        res.append((g.disparity, g.confidence, g.invalid_reason, g.reliable))
+    return res
+
+
+@pytest.fixture(scope="module")
+def device_pairs(ctx):
+    pairs = []
+    for i, (kind, w, h, D, mode, over) in enumerate(MIX):
+        left, right = _inputs(i, kind, w, h, D)
+        pairs.append(A.DevicePair(ctx, _params(mode, D, over), left, right))
+    return pairs
+
+
+def _same(got, want, what):
+    names = ("disparity", "confidence", "invalid_reason", "reliable")
+    for n, a, b in zip(names, got, want):
+        assert a.shape == b.shape, (what, n)
+        if not np.array_equal(a, b, equal_nan=True):
+            bad = np.argwhere(~((a == b) | (np.isnan(a) & np.isnan(b))) if a.dtype.kind == "f" else a != b)
+            raise AssertionError(f"{what}: {n} differs at {len(bad)} pixels, first {tuple(bad[0])}")
+
+
+@pytest.mark.gpu
+def test_alone_results_of_small_pairs_match_oracle(alone):
+    for i in (1, 4):
+        kind, w, h, D, mode, over = MIX[i]
+        print(f"oracle check pair {i}: {w}x{h} D={D} mode={mode}", flush=True)
+        left, right = _inputs(i, kind, w, h, D)
+        o = O.pipeline(O.cfg_from(_params(mode, D, over).as_dict()), left, right)
+        _same(alone[i], (o["disparity"], o["confidence"], o["invalid_reason"], o["reliable"]), f"pair {i}")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("slots", [1, 3, 4])
+def test_batch_direct_matches_alone(alone, device_pairs, slots):
+    print(f"batch direct slots={slots}", flush=True)
+    with A.Batch(0, slots) as b:
+        for q in device_pairs:
+            q.clear_outputs()
+        b.run(device_pairs)
+        b.sync()
+        for i, q in enumerate(device_pairs):
+            _same(q.outputs(), alone[i], f"slots={slots} pair {i}")
+
+
+@pytest.mark.gpu
+def test_batch_graph_replays_match_alone(ctx, alone, device_pairs):
+    # the first graph call runs directly and captures; each later call is one
+    # graph launch.  Outputs are cleared and unrelated eager work runs on
+    # another stream between replays, and every replay is checked.
+    print("batch graph capture + 3 replays", flush=True)
+    other = A.DevicePair(ctx, A.params(1, 32), *_inputs(99, 0, 96, 64, 32))
+    with A.Batch(0, 4) as b:
+        arr = b.pair_array(device_pairs)
+        for rep in range(4):
+            for q in device_pairs:
+                q.clear_outputs()
+            b.run(arr, graph=True)
+            b.sync()
+            for i, q in enumerate(device_pairs):
+                _same(q.outputs(), alone[i], f"graph call {rep} pair {i}")
+            b2 = b.pair_array([other])
+            with A.Batch(0, 1) as eager:
+                eager.run(b2)
+                eager.sync()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("shape", [(64, 0, 1024), (512, 1, 16384), (32, 1, 256), (1024, 0, 65536), (128, 1, 2048)])
+def test_launch_shapes_do_not_change_results(ctx, alone, shape):
+    print(f"launch shapes {shape}", flush=True)
+    try:
+        A.set_tuning(ctx, shape)
+        for i in (0, 2, 5, 6, 7, 8):
+            kind, w, h, D, mode, over = MIX[i]
+            left, right = _inputs(i, kind, w, h, D)
+            g = A.run_pipeline(ctx, _params(mode, D, over), left, right)
+            _same((g.disparity, g.confidence, g.invalid_reason, g.reliable), alone[i], f"shape {shape} pair {i}")
+    finally:
+        A.set_tuning(ctx, None)
+
+
+@pytest.mark.gpu
+def test_autotune_cache_round_trip(tmp_path, alone, device_pairs):
+    path = str(tmp_path / "shapes.json")
+    sub = [device_pairs[i] for i in (0, 2, 5)]
+    print("autotune 3 buckets", flush=True)
+    with A.Batch(0, 2) as b:
+        b.autotune(sub, path)
+        tuned = [b.tuning_for(q.params, q.w, q.h).as_tuple() for q in sub]
+        doc = json.load(open(path))
+        assert doc["arch"].startswith("gfx")
+        assert len(doc["entries"]) == 3
+        for q in device_pairs:
+            q.clear_outputs()
+        b.run(device_pairs)
+        b.sync()
+        for i, q in enumerate(device_pairs):
+            _same(q.outputs(), alone[i], f"tuned batch pair {i}")
+    with A.Batch(0, 2) as b2:
+        # every bucket is in the cache: nothing is re-timed, the shapes are read back
+        b2.autotune(sub, path)
+        assert [b2.tuning_for(q.params, q.w, q.h).as_tuple() for q in sub] == tuned
+
+
+def _requests():
+    reqs = []
+    for i, (kind, w, h, D, mode, over) in enumerate(MIX):
+        left, right = _inputs(i, kind, w, h, D)
+        reqs.append(A.HostRequest(_params(mode, D, over), left, right))
+    return reqs
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("spacing_ms", [0.0, 0.3])
+def test_server_results_same_alone_and_in_mix(alone, spacing_ms):
+    # each request twice, interleaved, so equal pairs run beside different neighbours
+    reqs = _requests() + _requests()[::-1]
+    n = len(MIX)
+    arrival = [k * spacing_ms * 1e-3 for k in range(len(reqs))]
+    print(f"server replay {len(reqs)} requests spacing {spacing_ms} ms", flush=True)
+    with A.Server(0, 4) as s:
+        lat = s.replay(reqs, arrival)
+        assert s.pending() == 0
+        assert 0 < s.resident_bytes() <= 64 << 30
+    assert np.all(lat > 0)
+    for k, r in enumerate(reqs):
+        i = k if k < n else 2 * n - 1 - k
+        _same(r.outputs(), alone[i], f"request {k} (pair {i})")
+
+
+@pytest.mark.gpu
+def test_server_budget_admission(alone):
+    reqs = _requests()
+    need = max(A.workspace_bytes(r.req.params, r.req.width, r.req.height) for r in reqs)
+    # room for about one large workspace: admission must wait and release idle
+    # slots, and every result must still be the same
+    budget = int(need * 1.6)
+    print(f"server budget {budget} bytes", flush=True)
+    with A.Server(0, 4, budget) as s:
+        lat = s.replay(reqs, [0.0] * len(reqs))
+        assert s.resident_bytes() <= budget
+        big = A.HostRequest(A.params(1, 256), *_inputs(5, 1, 1600, 900, 256))
+        with pytest.raises(A.ApgError) as e:
+            s.submit(big, 1)
+        assert e.value.code == 6
+    assert np.all(lat > 0)
+    for i, r in enumerate(reqs):
+        _same(r.outputs(), alone[i], f"budget request {i}")
