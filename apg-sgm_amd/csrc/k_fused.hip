@@ -188,7 +188,7 @@ __global__ __launch_bounds__(256) void cost_crossh_kernel(
 
 // Sliding form of the same pass for u8 T and ranges that depend on the
 // column only (the dense range init [min_disparity, min(max_disparity, x+1))).
-// One wavefront walks a row segment of K2_SEG output columns plus the arm
+// One wavefront walks a row segment of `seg` output columns plus the arm
 // halo, 128 disparities at a time (lane l: d-indices 2l, 2l+1).  Per column it
 // computes the two costs, adds them to u16-pair prefix sums kept in an LDS
 // ring of 2*Lmax+2 columns (window sums < 125*255 < 2^16), and Lmax columns
@@ -200,7 +200,7 @@ __global__ __launch_bounds__(256) void cost_crossh_kernel(
 // The loop is software-pipelined: the records of column c+1 are read while
 // column c computes, and an output column's window sums are read right after
 // the prefix write and consumed one column later.
-constexpr int K2_SEG = 256;
+constexpr int K2_SEG = 64;   // measured best at 1280x720 D=128 (more waves; the halo costs less)
 
 struct K2Shm {
     size_t lut, mag, col, chi, outr, rr, key, ring, total;
@@ -382,8 +382,9 @@ __global__ __launch_bounds__(64) void cost_crossh2_kernel(
             cr_c = cr_n;
             cr_n = s_col[c + 2];
             if (!SYM) ch_n = s_chi[c + 2];
+            // d + 1 at column c + 2 pairs with the right pixel of d at column c + 1
+            r1_n = r0_n;
             r0_n = s_rr[rbase + c + 2];
-            r1_n = s_rr[rbase + c + 1];
             if (pend) flush();
             const uint32_t cst0 = (uint32_t)clampi((int)(f0 * scale + 0.5f), 0, cost_max);
             const uint32_t cst1 = (uint32_t)clampi((int)(f1 * scale + 0.5f), 0, cost_max);

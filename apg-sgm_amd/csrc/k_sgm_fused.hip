@@ -11,6 +11,8 @@
 // j < K, as packed u16 with 65535 as path infinity.  Loads and stores use
 // buffer instructions with scalar per-step offsets; the C' words of the next
 // SL_PF steps of both chains are in flight while the current ones are used.
+#include <type_traits>
+
 #include "device_common.hpp"
 #include "kernels.hpp"
 
@@ -41,13 +43,15 @@ __device__ __forceinline__ CRaw<TC, K> load_c(__amdgpu_buffer_rsrc_t r, uint32_t
     return v;
 }
 
-// pair j of the lane -> u16 halves, invalid (u8 255 / u16 65535) -> 65535
-template <typename TC, int K>
+// pair j of the lane -> u16 halves, invalid (u8 255 / u16 65535) -> 65535.
+// ALLV: the caller knows every state of the pixel is valid (no u8 255).
+template <typename TC, int K, bool ALLV = false>
 __device__ __forceinline__ uint32_t decode_c(const CRaw<TC, K>& v, int j) {
     if constexpr (sizeof(TC) == 2) {
         return v.w[j];
     } else {
         const uint32_t x = __builtin_amdgcn_perm(0u, v.w[0], j == 0 ? 0x0c010c00u : 0x0c030c02u);
+        if constexpr (ALLV) return x;
         const u16x2 t = (as_v2(x) + splat2(1)) >> (u16x2)8;   // 1 where the byte is 255
         return as_u32(t * splat2(0xFF00) + as_v2(x));
     }
@@ -252,6 +256,19 @@ __device__ __forceinline__ void sgm_job(const apg_hip_params& p, int w, int h, i
     const int pF = p0, pL = p0 + (n - 1) * step;
     const int pA0 = chain ? pL : pF, pB0 = chain ? pF : pL;
     if (chain) step = -step;
+    // columns whose every disparity is valid (0 <= x - d < w for all d): no
+    // invalid C' byte there, so a batch inside them skips the sentinel decode
+    const int xlo = p.min_disparity + D - 1, xhi = w - 1 + p.min_disparity;
+    const int dxA = f == 1 ? 0 : (chain ? -1 : 1);   // x step of chain A: rows and rays move +x forward
+    const int xA0 = pA0 % w, xB0 = pB0 % w;
+    auto batch_valid = [&](int k) {
+        const int t0 = k * SB, t1 = min(k * SB + SB, n) - 1;
+        const int a0 = xA0 + dxA * t0, a1 = xA0 + dxA * t1;
+        const int b0 = xB0 - dxA * t0, b1 = xB0 - dxA * t1;
+        bool v = min(a0, a1) >= xlo && max(a0, a1) <= xhi;
+        if (DUAL) v = v && min(b0, b1) >= xlo && max(b0, b1) <= xhi;
+        return v;
+    };
     const auto rpa = raw_rsrc(planes.p[2 * f + chain]);
     const auto rpb = raw_rsrc(planes.p[2 * f + 1]);
     // batch transfer: lane -> (row j, 16-byte chunk c); NLD instructions cover SB rows
@@ -332,6 +349,8 @@ __device__ __forceinline__ void sgm_job(const apg_hip_params& p, int w, int h, i
         if (k > 0) flush(k - 1);
         if (k + 2 < nb) gload(k + 2);
         const int bf = k & 1;
+        auto run_batch = [&](auto allv_tag) {
+        constexpr bool ALLV = decltype(allv_tag)::value;
 #pragma unroll
         for (int i = 0; i < SB; ++i) {
             const int t = k * SB + i;
@@ -353,8 +372,8 @@ __device__ __forceinline__ void sgm_job(const apg_hip_params& p, int w, int h, i
                 CRaw<uint8_t, K> xa, xb;
                 xa.w[0] = wa;
                 xb.w[0] = wb;
-                ca[j] = decode_c<uint8_t, K>(xa, j);
-                cb[j] = DUAL ? decode_c<uint8_t, K>(xb, j) : 0u;
+                ca[j] = decode_c<uint8_t, K, ALLV>(xa, j);
+                cb[j] = DUAL ? decode_c<uint8_t, K, ALLV>(xb, j) : 0u;
                 if (!FULL && !inr[j]) ca[j] = cb[j] = 0xFFFFFFFFu;
             }
             uint32_t ma = chain_step<K>(A, ca, P2a, p1s, p1l, p1r, da);
@@ -377,6 +396,9 @@ __device__ __forceinline__ void sgm_job(const apg_hip_params& p, int w, int h, i
                 if (DUAL) *(uint32_t*)qb = pack_deltas<K>(db);
             }
         }
+        };
+        if (batch_valid(k)) run_batch(std::true_type{});
+        else run_batch(std::false_type{});
     }
     flush(nb - 1);
 }

@@ -238,7 +238,7 @@ APG_HIP_API size_t apg_hip_pipeline_workspace_bytes(const apg_hip_params* p, int
  * identical for every setting.  No counterpart in the reference (its CPU
  * stages have no launch shapes). */
 typedef struct {
-    int cost_segment;    /* output columns per wave of the cost + Cross-H pass: 32..1024 (256) */
+    int cost_segment;    /* output columns per wave of the cost + Cross-H pass: 32..1024 (64) */
     int sgm_dual;        /* 1: one wave runs both directions of a scanline (0) */
     int cross_v_waves;   /* target wave count of the Cross-V pass: 256..65536 (4096) */
 } apg_hip_tuning;

@@ -31,6 +31,12 @@ CASES = [
     (2, 140, 70, 40, 1, {"P2_base": 200}),
     (0, 1, 9, 4, 1, {}),
     (1, 70, 1, 8, 0, {}),
+    # batched-transfer SGM (u8 C', D % 16 == 0) across the all-valid column
+    # test: negative and positive min disparity, rays of 8 paths, K = 2
+    (1, 200, 90, 64, 1, {"min_disparity": -16}),
+    (2, 230, 70, 48, 1, {"min_disparity": 20}),
+    (0, 180, 120, 64, 1, {"paths": 8}),
+    (1, 420, 90, 192, 1, {"paths": 8}),
 ]
 
 FUSED_KEYS = ["sgm", "disp_right", "disp_wta", "best", "second",
