@@ -530,7 +530,8 @@ int apg_hip_pipeline(apg_hip_ctx ctx, const apg_hip_params* p, const uint8_t* d_
         HIPCHK(launch_cost_crossh(st, *p, cur_lut(ctx), w, h, gl, gxl, gyl, gr, gxr, gyr,
                                   sym ? (const uint32_t*)cl : nullptr, sym ? (const uint32_t*)cr : nullptr,
                                   sym ? nullptr : (const uint64_t*)cl, sym ? nullptr : (const uint64_t*)cr, dmin,
-                                  dmax, arms, p->agg_enable ? tt : cp, u8, rkeys, true, ctx->tune),
+                                  dmax, arms, p->agg_enable ? tt : cp, u8, rkeys, true,
+                                  cost_clamp_free(ctx->lut_host[ctx->lut_cur], p->cost_max), ctx->tune),
                "cost_crossh");
         mark(ctx, S_RWTA);
         HIPCHK(launch_rwta_finalize(st, *p, cur_lut(ctx), w, h, gl, gxl, gyl, gr, gxr, gyr,

@@ -37,6 +37,26 @@ int cost_upper_bound(const CostLut& lut, int cost_max) {
     return q < 0 ? 0 : (q > cost_max ? cost_max : q);
 }
 
+bool cost_clamp_free(const CostLut& lut, int cost_max) {
+    float mc = 0.f, ma = 0.f, mg = 0.f;
+    for (float v : lut.census) {
+        if (!(v >= 0.f)) return false;
+        mc = v > mc ? v : mc;
+    }
+    for (float v : lut.ad) {
+        if (!(v >= 0.f)) return false;
+        ma = v > ma ? v : ma;
+    }
+    for (float v : lut.grad) {
+        if (!(v >= 0.f)) return false;
+        mg = v > mg ? v : mg;
+    }
+    if (!(lut.scale >= 0.f)) return false;
+    const float c = (mc + ma) + mg;
+    const float q = c * lut.scale + 0.5f;
+    return q < 2147483520.f && (int)q <= cost_max;
+}
+
 // ------------------------------------------------------------------ aux ----
 constexpr int AUX_TX = 64, AUX_TY = 4;
 

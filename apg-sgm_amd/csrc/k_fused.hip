@@ -4,6 +4,8 @@
 //                        volume never reaches HBM);
 //   rwta_finalize_kernel: right-view disparity with the legacy subpixel step
 //                        from the per-pixel winning keys.
+#include <type_traits>
+
 #include "device_common.hpp"
 #include "kernels.hpp"
 
@@ -227,7 +229,7 @@ __host__ __device__ inline K2Shm k2_layout(int D, int Lmax, int seg, bool sym) {
 
 constexpr int kFarLeft = -(1 << 24);   // x of a column outside the image: x - d is never in [0, W)
 
-template <bool SYM, bool FULL>
+template <bool SYM, bool FULL, bool NOCLAMP>
 __global__ __launch_bounds__(64) void cost_crossh2_kernel(
     const CostLut* __restrict__ lut, int w, int h, int d0, int D, int cost_max, int Lmax, int seg,
     const uint8_t* __restrict__ gl, const uint8_t* __restrict__ gxl, const uint8_t* __restrict__ gyl,
@@ -372,7 +374,12 @@ __global__ __launch_bounds__(64) void cost_crossh2_kernel(
             if (act) buf_st16(rout, q0 | (q1 << 8), (uint32_t)di0, por.w);
         };
 
-        auto step = [&](int c, bool keys, bool outp) {
+        // keys: own column (right-view WTA candidates); outp: an output column's
+        // window sums are read; allv: every lane's right pixels x - d, x - d - 1
+        // lie in the image (no per-lane validity test)
+        auto step = [&](int c, auto keys_t, auto outp_t, auto allv_t) {
+            constexpr bool keys = decltype(keys_t)::value, outp = decltype(outp_t)::value;
+            constexpr bool allv = decltype(allv_t)::value;
             const uint4 cr = cr_c;
             // same operation order as state_cost (reference src/cost_computer.cpp:176-199)
             const float f0 = (tc0 + ta0) + tg0;
@@ -386,10 +393,15 @@ __global__ __launch_bounds__(64) void cost_crossh2_kernel(
             r1_n = r0_n;
             r0_n = s_rr[rbase + c + 2];
             if (pend) flush();
-            const uint32_t cst0 = (uint32_t)clampi((int)(f0 * scale + 0.5f), 0, cost_max);
-            const uint32_t cst1 = (uint32_t)clampi((int)(f1 * scale + 0.5f), 0, cost_max);
+            // NOCLAMP: the host showed every LUT term >= 0 and the largest
+            // quantised sum <= cost_max, so the clamp of the reference is the identity
+            const uint32_t cst0 = NOCLAMP ? (uint32_t)(int)(f0 * scale + 0.5f)
+                                          : (uint32_t)clampi((int)(f0 * scale + 0.5f), 0, cost_max);
+            const uint32_t cst1 = NOCLAMP ? (uint32_t)(int)(f1 * scale + 0.5f)
+                                          : (uint32_t)clampi((int)(f1 * scale + 0.5f), 0, cost_max);
             const int xr0 = (int)cr.z - dA;   // right pixel of d; d + 1 has xr0 - 1
-            const bool v0 = act && (uint32_t)xr0 < uw, v1 = act && (uint32_t)(xr0 - 1) < uw;
+            const bool v0 = allv ? act : act && (uint32_t)xr0 < uw;
+            const bool v1 = allv ? act : act && (uint32_t)(xr0 - 1) < uw;
             if (keys) {
                 const int ki = xr0 - kbase;
                 const int k0 = FULL ? ki : max(ki, 1);
@@ -411,18 +423,43 @@ __global__ __launch_bounds__(64) void cost_crossh2_kernel(
         // phases of c: keys for own columns [Lmax, Lmax + nown), output (Lmax
         // behind) for [2Lmax, 2Lmax + nown); columns past 2Lmax + nown lie beyond
         // the image and feed nothing
+        // the image columns where every d of the chunk has its right pixel
+        // inside the image, [vx0, vx1], in c: [cvb, cve); the loops over them
+        // run two columns per iteration so the pipeline registers alternate
+        // instead of being copied
         const int kb = Lmax, ke = Lmax + nown, ob = 2 * Lmax, oe = 2 * Lmax + nown;
+        const int vx0 = max(dtop, 0), vx1 = min(w - 1 + d0 + dc, w - 1);
+        const int cvb = min(oe, max(0, vx0 - xb)), cve = max(cvb, min(oe, vx1 - xb + 1));
+        using TT = std::true_type;
+        using FF = std::false_type;
         int c = 0;
+        auto run = [&](int nx, auto k_t, auto o_t, auto v_t) {
+            if constexpr (decltype(v_t)::value)
+                for (; c + 1 < nx; c += 2) {
+                    step(c, k_t, o_t, v_t);
+                    step(c + 1, k_t, o_t, v_t);
+                }
+            for (; c < nx; ++c) step(c, k_t, o_t, v_t);
+        };
         while (c < oe) {
-            const bool inK = c >= kb && c < ke, inO = c >= ob && c < oe;
+            const bool inK = c >= kb && c < ke, inO = c >= ob, inV = c >= cvb && c < cve;
             int nx = oe;
             if (c < kb) nx = min(nx, kb);
             if (c < ke) nx = min(nx, ke);
             if (c < ob) nx = min(nx, ob);
-            if (inK && inO) for (; c < nx; ++c) step(c, true, true);
-            else if (inK) for (; c < nx; ++c) step(c, true, false);
-            else if (inO) for (; c < nx; ++c) step(c, false, true);
-            else for (; c < nx; ++c) step(c, false, false);
+            if (c < cvb) nx = min(nx, cvb);
+            if (c < cve) nx = min(nx, cve);
+            if (inV) {
+                if (inK && inO) run(nx, TT{}, TT{}, TT{});
+                else if (inK) run(nx, TT{}, FF{}, TT{});
+                else if (inO) run(nx, FF{}, TT{}, TT{});
+                else run(nx, FF{}, FF{}, TT{});
+            } else {
+                if (inK && inO) run(nx, TT{}, TT{}, FF{});
+                else if (inK) run(nx, TT{}, FF{}, FF{});
+                else if (inO) run(nx, FF{}, TT{}, FF{});
+                else run(nx, FF{}, FF{}, FF{});
+            }
         }
         if (pend) flush();
     }
@@ -439,7 +476,7 @@ hipError_t launch_cost_crossh(hipStream_t s, const apg_hip_params& p, const Cost
                               const uint8_t* gxr, const uint8_t* gyr, const uint32_t* c32l, const uint32_t* c32r,
                               const uint64_t* c64l, const uint64_t* c64r, const int16_t* dmin, const int16_t* dmax,
                               const uint32_t* arms, void* out, bool out_u8, uint32_t* rkeys, bool column_ranges,
-                              const KTune& tn) {
+                              bool clamp_free, const KTune& tn) {
     const int D = p.max_disparity - p.min_disparity;
     const bool cross = p.agg_enable != 0;
     const int Lmax = cross ? (p.max_arm_length > 1 ? p.max_arm_length : 1) : 0;
@@ -448,8 +485,15 @@ hipError_t launch_cost_crossh(hipStream_t s, const apg_hip_params& p, const Cost
         seg = w < seg ? w : seg;
         const K2Shm L2 = k2_layout(D, Lmax, seg, p.census_type == 1);
         const bool full = D % 128 == 0;
-        auto k = p.census_type == 1 ? (full ? cost_crossh2_kernel<true, true> : cost_crossh2_kernel<true, false>)
-                                    : (full ? cost_crossh2_kernel<false, true> : cost_crossh2_kernel<false, false>);
+        decltype(&cost_crossh2_kernel<true, true, true>) k;
+        if (clamp_free)
+            k = p.census_type == 1
+                    ? (full ? cost_crossh2_kernel<true, true, true> : cost_crossh2_kernel<true, false, true>)
+                    : (full ? cost_crossh2_kernel<false, true, true> : cost_crossh2_kernel<false, false, true>);
+        else
+            k = p.census_type == 1
+                    ? (full ? cost_crossh2_kernel<true, true, false> : cost_crossh2_kernel<true, false, false>)
+                    : (full ? cost_crossh2_kernel<false, true, false> : cost_crossh2_kernel<false, false, false>);
         if (L2.total > 64 * 1024) {
             hipError_t e =
                 hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)L2.total);

@@ -82,6 +82,9 @@ hipError_t launch_median(hipStream_t s, const apg_hip_params& p, int w, int h, c
 // Largest quantised cost any state can take (bound from the LUT maxima);
 // <= 254 allows u8 storage of T and C' with 255 as the invalid value.
 int cost_upper_bound(const CostLut& lut, int cost_max);
+// Every LUT term >= 0 and the largest quantised sum <= cost_max: the clamp of
+// the quantised cost to [0, cost_max] never changes a value.
+bool cost_clamp_free(const CostLut& lut, int cost_max);
 // column_ranges: every pixel's range depends on its column only (dense range init)
 hipError_t launch_cross_v_t(hipStream_t s, const apg_hip_params& p, int w, int h, const int16_t* dmin,
                             const int16_t* dmax, const uint32_t* arms, const void* tmp, void* cost, bool u8,
@@ -92,7 +95,8 @@ hipError_t launch_cost_crossh(hipStream_t s, const apg_hip_params& p, const Cost
                               const uint8_t* gxr, const uint8_t* gyr, const uint32_t* c32l, const uint32_t* c32r,
                               const uint64_t* c64l, const uint64_t* c64r, const int16_t* dmin, const int16_t* dmax,
                               const uint32_t* arms, void* out, bool out_u8, uint32_t* rkeys,
-                              bool column_ranges, const KTune& tn = KTune());
+                              bool column_ranges, bool clamp_free = false,
+                              const KTune& tn = KTune());
 hipError_t launch_rwta_finalize(hipStream_t s, const apg_hip_params& p, const CostLut* d_lut, int w, int h,
                                 const uint8_t* gl, const uint8_t* gxl, const uint8_t* gyl, const uint8_t* gr,
                                 const uint8_t* gxr, const uint8_t* gyr, const uint32_t* c32l, const uint32_t* c32r,

@@ -37,6 +37,8 @@ CASES = [
     (2, 230, 70, 48, 1, {"min_disparity": 20}),
     (0, 180, 120, 64, 1, {"paths": 8}),
     (1, 420, 90, 192, 1, {"paths": 8}),
+    # a negative AD weight: costs below 0 and above cost_max are clamped
+    (2, 150, 90, 64, 1, {"eta_ad": -0.3, "cost_max": 200}),
 ]
 
 FUSED_KEYS = ["sgm", "disp_right", "disp_wta", "best", "second",
