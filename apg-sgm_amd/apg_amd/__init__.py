@@ -37,6 +37,10 @@ class Params(C.Structure):
         ("conf_min_texture", C.c_float), ("conf_reliable_threshold", C.c_float),
         ("post_subpixel", C.c_int), ("post_lr_check", C.c_int), ("post_lr_max_diff", C.c_int),
         ("post_median_radius", C.c_int), ("post_max_fill_gap", C.c_int),
+        ("prior_enable", C.c_int), ("prior_uniqueness_ratio", C.c_float), ("prior_texture_threshold", C.c_int),
+        ("prior_lr_max_diff", C.c_int), ("prior_search_radius", C.c_int), ("prior_min_supports", C.c_int),
+        ("prior_max_supports", C.c_int),
+        ("refine_enable", C.c_int), ("refine_iterations", C.c_int), ("refine_random_radius", C.c_int),
     ]
 
     def as_dict(self):

@@ -221,12 +221,17 @@ void apg_hip_params_from_mode(int mode, int max_disparity, apg_hip_params* c) {
     c->conf_min_texture = 2.f; c->conf_reliable_threshold = 0.25f;
     c->post_subpixel = 1; c->post_lr_check = 1; c->post_lr_max_diff = 1;
     c->post_median_radius = 1; c->post_max_fill_gap = 32;
+    c->prior_enable = 0; c->prior_uniqueness_ratio = 0.4f; c->prior_texture_threshold = 4;
+    c->prior_lr_max_diff = 1; c->prior_search_radius = 16; c->prior_min_supports = 32;
+    c->prior_max_supports = 4000;
+    c->refine_enable = 0; c->refine_iterations = 3; c->refine_random_radius = 4;
     if (mode == 0) {
         c->use_ad = 0; c->use_grad = 0; c->agg_enable = 0; c->paths = 4; c->adaptive_penalties = 0;
     } else if (mode == 1) {
         c->agg_iterations = 1; c->paths = 4; c->adaptive_penalties = 1;
     } else {
         c->paths = 8; c->adaptive_penalties = 1;
+        c->prior_enable = 1; c->refine_enable = 1; c->refine_iterations = 3;
     }
 }
 

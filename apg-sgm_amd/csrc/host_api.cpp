@@ -51,6 +51,16 @@ apg_hip_params to_device_params(const PipelineConfig& c) {
     p.post_lr_max_diff = c.post.lr_max_diff;
     p.post_median_radius = c.post.median_radius;
     p.post_max_fill_gap = c.post.max_fill_gap;
+    p.prior_enable = c.prior.enable ? 1 : 0;
+    p.prior_uniqueness_ratio = c.prior.uniqueness_ratio;
+    p.prior_texture_threshold = c.prior.texture_threshold;
+    p.prior_lr_max_diff = c.prior.lr_max_diff;
+    p.prior_search_radius = c.prior.search_radius;
+    p.prior_min_supports = c.prior.min_supports;
+    p.prior_max_supports = c.prior.max_supports;
+    p.refine_enable = c.refine.enable ? 1 : 0;
+    p.refine_iterations = c.refine.iterations;
+    p.refine_random_radius = c.refine.random_radius;
     return p;
 }
 

@@ -48,6 +48,13 @@ typedef struct {
     int conf_lr_max_diff;
     float conf_min_texture, conf_reliable_threshold;
     int post_subpixel, post_lr_check, post_lr_max_diff, post_median_radius, post_max_fill_gap;
+    /* PriorParams (reference include/apg_sgm/config.hpp:37-45): sparse census
+     * supports -> per-pixel search ranges (HighQuality) */
+    int prior_enable;
+    float prior_uniqueness_ratio;
+    int prior_texture_threshold, prior_lr_max_diff, prior_search_radius, prior_min_supports, prior_max_supports;
+    /* RefineParams (:54-58): randomised raster refinement (HighQuality) */
+    int refine_enable, refine_iterations, refine_random_radius;
 } apg_hip_params;
 
 typedef struct apg_hip_ctx_s* apg_hip_ctx;

@@ -49,13 +49,18 @@ void orc_config_from_mode(int mode, int max_disparity, orc_cfg* c) {
     c->conf_min_texture = 2.f; c->conf_reliable_threshold = 0.25f;
     c->post_subpixel = 1; c->post_lr_check = 1; c->post_lr_max_diff = 1;
     c->post_median_radius = 1; c->post_max_fill_gap = 32;
+    c->prior_enable = 0; c->prior_uniqueness_ratio = 0.4f; c->prior_texture_threshold = 4;
+    c->prior_lr_max_diff = 1; c->prior_search_radius = 16; c->prior_min_supports = 32;
+    c->prior_max_supports = 4000;
+    c->refine_enable = 0; c->refine_iterations = 3; c->refine_random_radius = 4;
     if (mode == 0) {            /* Fast */
         c->use_ad = 0; c->use_grad = 0; c->agg_enable = 0;
         c->paths = 4; c->adaptive_penalties = 0;
     } else if (mode == 1) {     /* Balanced */
         c->agg_iterations = 1; c->paths = 4; c->adaptive_penalties = 1;
-    } else {                    /* HighQuality (prior/refine are host stages outside this oracle) */
+    } else {                    /* HighQuality */
         c->paths = 8; c->adaptive_penalties = 1;
+        c->prior_enable = 1; c->refine_enable = 1; c->refine_iterations = 3;
     }
 }
 
@@ -734,7 +739,11 @@ int orc_pipeline(const orc_cfg* cfg, const uint8_t* left, const uint8_t* right, 
         c64l = malloc(n * 8); c64r = malloc(n * 8);
         orc_census64(gl, w, h, c64l); orc_census64(gr, w, h, c64r);
     }
-    orc_range_init(cfg, w, h, dmin, dmax);
+    float* d_prior = malloc(n * 4);
+    orc_prior_estimate(cfg, w, h, gxl, gyl, c32l, c32r, c64l, c64r, dmin, dmax, d_prior);
+    if (inter && inter->dmin) memcpy(inter->dmin, dmin, n * 2);
+    if (inter && inter->dmax) memcpy(inter->dmax, dmax, n * 2);
+    if (inter && inter->d_prior) memcpy(inter->d_prior, d_prior, n * 4);
     orc_cost_volume(cfg, w, h, gl, gxl, gyl, gr, gxr, gyr, c32l, c32r, c64l, c64r, dmin, dmax, cost);
     if (inter && inter->cost) memcpy(inter->cost, cost, nv * 2);
     orc_right_wta(cfg, w, h, cost, dr);
@@ -748,11 +757,12 @@ int orc_pipeline(const orc_cfg* cfg, const uint8_t* left, const uint8_t* right, 
     if (inter && inter->best) memcpy(inter->best, best, n * 4);
     if (inter && inter->second) memcpy(inter->second, second, n * 4);
     orc_confidence(cfg, w, h, disparity, dr, gxl, gyl, best, second, confidence, reliable, reason);
+    orc_refine(cfg, w, h, gl, gxl, gyl, gr, gxr, gyr, c32l, c32r, c64l, c64r, dmin, dmax, reliable, disparity);
     orc_lr_check(cfg, w, h, disparity, dr, reason);
-    orc_fill_holes(cfg, w, h, disparity, reason, gl, NULL);
+    orc_fill_holes(cfg, w, h, disparity, reason, gl, d_prior);
     orc_median(cfg, w, h, disparity);
     free(gl); free(gxl); free(gyl); free(gr); free(gxr); free(gyr);
     free(c32l); free(c32r); free(c64l); free(c64r);
-    free(dmin); free(dmax); free(cost); free(acc); free(dr); free(best); free(second);
+    free(dmin); free(dmax); free(cost); free(acc); free(dr); free(best); free(second); free(d_prior);
     return 0;
 }

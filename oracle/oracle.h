@@ -57,6 +57,12 @@ typedef struct {
     float conf_min_texture, conf_reliable_threshold;
     /* PostParams */
     int post_subpixel, post_lr_check, post_lr_max_diff, post_median_radius, post_max_fill_gap;
+    /* PriorParams (include/apg_sgm/config.hpp:37-45) */
+    int prior_enable;
+    float prior_uniqueness_ratio;
+    int prior_texture_threshold, prior_lr_max_diff, prior_search_radius, prior_min_supports, prior_max_supports;
+    /* RefineParams (include/apg_sgm/config.hpp:54-58) */
+    int refine_enable, refine_iterations, refine_random_radius;
 } orc_cfg;
 
 /* PipelineConfig::from_mode (src/config.cpp:5-45) */
@@ -133,8 +139,41 @@ void orc_fill_holes(const orc_cfg* cfg, int w, int h, float* disp, const uint8_t
 /* valid-only median (src/postprocess.cpp:255-309) */
 void orc_median(const orc_cfg* cfg, int w, int h, float* disp);
 
-/* Dense pipeline (src/pipeline.cpp:435-605, dense branch :559-590) for
- * prior-off configurations. Any intermediate pointer may be NULL.
+/* ---- HighQuality stages (oracle_hq.cpp) ---------------------------- */
+
+typedef struct {
+    int x, y;
+    float disparity, confidence;
+} orc_support;
+
+/* PriorEstimator::extract_supports (src/prior_estimator.cpp:13-166); writes at
+ * most `cap` supports, returns how many the reference keeps */
+int orc_prior_supports(const orc_cfg* cfg, int w, int h, const uint8_t* gx_l, const uint8_t* gy_l,
+                       const uint32_t* c32l, const uint32_t* c32r, const uint64_t* c64l, const uint64_t* c64r,
+                       orc_support* out, int cap);
+/* PriorEstimator::interpolate_prior (src/prior_estimator.cpp:168-375) */
+void orc_prior_interpolate(const orc_support* sup, int nsup, int w, int h, float* d_prior, float* conf,
+                           float* spread, float* pmin, float* pmax);
+/* PriorEstimator::apply_search_range (src/prior_estimator.cpp:377-412) */
+void orc_prior_apply_range(const orc_cfg* cfg, int w, int h, const float* d_prior, const float* conf,
+                           const float* spread, const float* pmin, const float* pmax, int16_t* dmin,
+                           int16_t* dmax);
+/* PriorEstimator::estimate (src/prior_estimator.cpp:414-435); returns the
+ * support count */
+int orc_prior_estimate(const orc_cfg* cfg, int w, int h, const uint8_t* gx_l, const uint8_t* gy_l,
+                       const uint32_t* c32l, const uint32_t* c32r, const uint64_t* c64l, const uint64_t* c64r,
+                       int16_t* dmin, int16_t* dmax, float* d_prior);
+/* Refiner::refine, scalar route (src/refiner.cpp:14-98), in place on disp */
+void orc_refine(const orc_cfg* cfg, int w, int h, const uint8_t* gl, const uint8_t* gxl, const uint8_t* gyl,
+                const uint8_t* gr, const uint8_t* gxr, const uint8_t* gyr, const uint32_t* c32l,
+                const uint32_t* c32r, const uint64_t* c64l, const uint64_t* c64r, const int16_t* dmin,
+                const int16_t* dmax, const uint8_t* reliable, float* disp);
+
+/* Pipeline (src/pipeline.cpp:435-605): aux, prior (or the dense range
+ * init), cost, right WTA, Cross, SGM, WTA, confidence, refine, LR check,
+ * fill (with the prior), median.  The dense branch with the per-pixel ranges
+ * is also what the packed branch computes (the reference's own dense ==
+ * packed tests, test_sanity.cpp:83-476).  Any intermediate pointer may be NULL.
  * Returns 0 on success or the reference's StatusCode on validation failure. */
 typedef struct {
     uint16_t* cost;          /* C  (W*H*D), raw cost volume */
@@ -144,6 +183,9 @@ typedef struct {
     float* disp_wta;         /* W*H, WTA output before post (d_before_refine) */
     float* best;             /* W*H */
     float* second;           /* W*H */
+    int16_t* dmin;           /* W*H search range (prior-narrowed in HighQuality) */
+    int16_t* dmax;           /* W*H */
+    float* d_prior;          /* W*H, -1 where there is no prior */
 } orc_intermediates;
 
 int orc_pipeline(const orc_cfg* cfg,

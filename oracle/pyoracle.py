@@ -32,6 +32,10 @@ class Cfg(C.Structure):
         ("conf_min_texture", C.c_float), ("conf_reliable_threshold", C.c_float),
         ("post_subpixel", C.c_int), ("post_lr_check", C.c_int), ("post_lr_max_diff", C.c_int),
         ("post_median_radius", C.c_int), ("post_max_fill_gap", C.c_int),
+        ("prior_enable", C.c_int), ("prior_uniqueness_ratio", C.c_float), ("prior_texture_threshold", C.c_int),
+        ("prior_lr_max_diff", C.c_int), ("prior_search_radius", C.c_int), ("prior_min_supports", C.c_int),
+        ("prior_max_supports", C.c_int),
+        ("refine_enable", C.c_int), ("refine_iterations", C.c_int), ("refine_random_radius", C.c_int),
     ]
 
     def as_dict(self):
@@ -40,7 +44,12 @@ class Cfg(C.Structure):
 
 class Inter(C.Structure):
     _fields_ = [("cost", C.c_void_p), ("cost_agg", C.c_void_p), ("sgm", C.c_void_p), ("disp_right", C.c_void_p),
-                ("disp_wta", C.c_void_p), ("best", C.c_void_p), ("second", C.c_void_p)]
+                ("disp_wta", C.c_void_p), ("best", C.c_void_p), ("second", C.c_void_p),
+                ("dmin", C.c_void_p), ("dmax", C.c_void_p), ("d_prior", C.c_void_p)]
+
+
+class Support(C.Structure):
+    _fields_ = [("x", C.c_int), ("y", C.c_int), ("disparity", C.c_float), ("confidence", C.c_float)]
 
 
 _orc = None
@@ -90,9 +99,11 @@ def cfg(mode=1, max_disparity=128, **over) -> Cfg:
 
 
 def cfg_from(d: dict) -> Cfg:
-    c = Cfg()
+    """Cfg from a dict; fields it lacks keep the preset of its mode."""
+    c = cfg(d.get("mode", 1), d.get("max_disparity", 128))
     for k, _ in Cfg._fields_:
-        setattr(c, k, d[k])
+        if k in d:
+            setattr(c, k, d[k])
     return c
 
 
@@ -217,8 +228,10 @@ def pipeline(c: Cfg, left, right, intermediates=False):
         out.update({"cost": np.empty((h, w, D), np.uint16), "cost_agg": np.empty((h, w, D), np.uint16),
                     "sgm": np.empty((h, w, D), np.uint32), "disp_right": np.empty((h, w), np.float32),
                     "disp_wta": np.empty((h, w), np.float32), "best": np.empty((h, w), np.float32),
-                    "second": np.empty((h, w), np.float32)})
-        inter = Inter(*[out[k].ctypes.data for k in ("cost", "cost_agg", "sgm", "disp_right", "disp_wta", "best", "second")])
+                    "second": np.empty((h, w), np.float32), "dmin": np.empty((h, w), np.int16),
+                    "dmax": np.empty((h, w), np.int16), "d_prior": np.empty((h, w), np.float32)})
+        inter = Inter(*[out[k].ctypes.data for k in ("cost", "cost_agg", "sgm", "disp_right", "disp_wta", "best",
+                                                     "second", "dmin", "dmax", "d_prior")])
     rc = oracle().orc_pipeline(C.byref(c), _p(left), _p(right), w, h, w * ch, ch, _p(out["disparity"]),
                                _p(out["confidence"]), _p(out["invalid_reason"]), _p(out["reliable"]),
                                C.byref(inter) if inter is not None else None)
@@ -248,7 +261,7 @@ def ref_stages(c: Cfg, left, right):
     return o
 
 
-def ref_pipeline(c: Cfg, left, right, backend=1):
+def ref_pipeline(c: Cfg, left, right, backend=0):
     left = np.ascontiguousarray(left); right = np.ascontiguousarray(right)
     h, w = left.shape[:2]
     ch = 1 if left.ndim == 2 else left.shape[2]
@@ -282,3 +295,26 @@ def ref_time_pipeline(c: Cfg, left, right, threads=0, warmup=1, reps=1, backend=
 
 def ref_max_threads():
     return int(ref().ref_max_threads())
+
+
+def prior_supports(c: Cfg, gx, gy, census_l, census_r):
+    """extract_supports -> list of (x, y, disparity, confidence)."""
+    h, w = gx.shape
+    cap = max(1, c.prior_max_supports)
+    buf = (Support * cap)()
+    sym = census_l.dtype == np.uint32
+    n = oracle().orc_prior_supports(C.byref(c), w, h, _p(gx), _p(gy), _p(census_l) if sym else None,
+                                    _p(census_r) if sym else None, None if sym else _p(census_l),
+                                    None if sym else _p(census_r), buf, cap)
+    return [(buf[i].x, buf[i].y, buf[i].disparity, buf[i].confidence) for i in range(min(n, cap))]
+
+
+def prior_estimate(c: Cfg, gx, gy, census_l, census_r):
+    """estimate -> (dmin, dmax, d_prior, support count)."""
+    h, w = gx.shape
+    dmin = np.empty((h, w), np.int16); dmax = np.empty((h, w), np.int16); dp = np.empty((h, w), np.float32)
+    sym = census_l.dtype == np.uint32
+    n = oracle().orc_prior_estimate(C.byref(c), w, h, _p(gx), _p(gy), _p(census_l) if sym else None,
+                                    _p(census_r) if sym else None, None if sym else _p(census_l),
+                                    None if sym else _p(census_r), _p(dmin), _p(dmax), _p(dp))
+    return dmin, dmax, dp, int(n)

@@ -57,6 +57,16 @@ static PipelineConfig make_cfg(const orc_cfg* c, int backend) {
     cfg.post.lr_max_diff = c->post_lr_max_diff;
     cfg.post.median_radius = c->post_median_radius;
     cfg.post.max_fill_gap = c->post_max_fill_gap;
+    cfg.prior.enable = c->prior_enable != 0;
+    cfg.prior.uniqueness_ratio = c->prior_uniqueness_ratio;
+    cfg.prior.texture_threshold = c->prior_texture_threshold;
+    cfg.prior.lr_max_diff = c->prior_lr_max_diff;
+    cfg.prior.search_radius = c->prior_search_radius;
+    cfg.prior.min_supports = c->prior_min_supports;
+    cfg.prior.max_supports = c->prior_max_supports;
+    cfg.refine.enable = c->refine_enable != 0;
+    cfg.refine.iterations = c->refine_iterations;
+    cfg.refine.random_radius = c->refine_random_radius;
     return cfg;
 }
 

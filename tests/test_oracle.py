@@ -129,8 +129,9 @@ def test_golden_small(name):
     cfg = O.cfg_from(json.loads(bytes(z[f"{name}__cfg"]).decode()))
     o = O.pipeline(cfg, z[f"{name}__left"], z[f"{name}__right"], intermediates=True)
     for k in ("cost", "cost_agg", "sgm", "disp_wta", "best", "second", "disparity", "confidence",
-              "invalid_reason", "reliable", "disp_right"):
+              "invalid_reason", "reliable", "disp_right", "dmin", "dmax"):
         np.testing.assert_array_equal(o[k], z[f"{name}__{k}"], err_msg=k)
+    np.testing.assert_array_equal(o["disp_wta"], z[f"{name}__d_before_refine"], err_msg="d_before_refine")
 
 
 def _hashed():
@@ -145,6 +146,7 @@ def test_golden_hashes(name):
     left, right, _ = A.synth_pair(gen["kind"], gen["w"], gen["h"], gen["max_disp"], gen["seed"])
     assert hex(O.fnv1a64(left)) == g["input_fnv"]["left"] and hex(O.fnv1a64(right)) == g["input_fnv"]["right"]
     o = O.pipeline(O.cfg_from(g["cfg"]), left, right, intermediates=True)
+    o["d_before_refine"] = o["disp_wta"]
     got = {k: hex(O.fnv1a64(o[k])) for k in g["fnv"]}
     assert got == g["fnv"]
 
@@ -262,6 +264,10 @@ def test_sgm_first_pixel_and_restart():
     (1, 1, 4, 1, {}), (2, 3, 4, 1, {}), (9, 7, 16, 1, {}), (5, 40, 8, 0, {}), (40, 5, 64, 1, {}),
     (30, 20, 8, 1, {"min_disparity": 3}), (25, 18, 10, 1, {"min_disparity": -6}),
     (50, 30, 20, 1, {"paths": 8, "agg_iterations": 3}), (33, 21, 15, 1, {"census_type": 0, "paths": 8}),
+    # HighQuality: prior ranges (enough supports, too few, the case-B budget), refiner
+    (64, 48, 16, 2, {}), (96, 64, 24, 2, {"census_type": 0}),
+    (120, 80, 32, 2, {"prior_max_supports": 10, "prior_min_supports": 5}),
+    (40, 30, 8, 2, {"prior_min_supports": 100000}),
 ])
 def test_oracle_matches_reference_build(w, h, D, mode, over):
     """Edge sizes (1x1, thin, D > width, shifted / negative ranges) against
@@ -275,5 +281,6 @@ def test_oracle_matches_reference_build(w, h, D, mode, over):
     o = O.pipeline(c, left, right, intermediates=True)
     for k in ("cost", "cost_agg", "sgm", "disp_wta", "best", "second"):
         np.testing.assert_array_equal(o[k], rs[k], err_msg=k)
-    for k in ("disparity", "confidence", "invalid_reason", "reliable", "disp_right"):
+    for k in ("disparity", "confidence", "invalid_reason", "reliable", "disp_right", "dmin", "dmax"):
         np.testing.assert_array_equal(o[k], rp[k], err_msg=k)
+    np.testing.assert_array_equal(o["disp_wta"], rp["d_before_refine"], err_msg="d_before_refine")
