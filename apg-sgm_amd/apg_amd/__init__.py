@@ -56,7 +56,7 @@ class Timing(C.Structure):
 class Intermediates(C.Structure):
     _fields_ = [("d_cost", C.c_void_p), ("d_cost_agg", C.c_void_p), ("d_sum32", C.c_void_p),
                 ("d_disp_right", C.c_void_p), ("d_disp_wta", C.c_void_p), ("d_best", C.c_void_p),
-                ("d_second", C.c_void_p)]
+                ("d_second", C.c_void_p), ("d_dmin", C.c_void_p), ("d_dmax", C.c_void_p), ("d_prior", C.c_void_p)]
 
 
 class Tuning(C.Structure):
@@ -140,6 +140,8 @@ def load():
         "apg_hip_bench_pipeline": (i, [vp, P, vp, vp, i, i, i, i, vp, vp, vp, vp, i, i, C.POINTER(C.c_double),
                                        C.POINTER(Timing)]),
         "apg_hip_pipeline_workspace_bytes": (sz, [P, i, i]),
+        "apg_hip_prior_estimate": (i, [vp, P, i, i] + [vp] * 9 + [C.POINTER(i)]),
+        "apg_hip_refine": (i, [vp, P, i, i] + [vp] * 14),
         "apg_hip_set_tuning": (i, [vp, C.POINTER(Tuning)]),
         "apg_hip_get_tuning": (i, [vp, C.POINTER(Tuning)]),
         "apg_hip_batch_create": (i, [i, i, C.POINTER(vp)]),
@@ -284,11 +286,15 @@ class PipelineOutputs:
     disp_wta: np.ndarray | None = None
     best: np.ndarray | None = None
     second: np.ndarray | None = None
+    dmin: np.ndarray | None = None
+    dmax: np.ndarray | None = None
+    d_prior: np.ndarray | None = None
 
 
-ALL_INTERMEDIATES = ("cost", "cost_agg", "sgm", "disp_right", "disp_wta", "best", "second")
+ALL_INTERMEDIATES = ("cost", "cost_agg", "sgm", "disp_right", "disp_wta", "best", "second", "dmin", "dmax",
+                     "d_prior")
 # requesting the raw (cost) or post-Cross (cost_agg) volume selects the stage-by-stage schedule
-FUSED_INTERMEDIATES = ("sgm", "disp_right", "disp_wta", "best", "second")
+FUSED_INTERMEDIATES = ("sgm", "disp_right", "disp_wta", "best", "second", "dmin", "dmax", "d_prior")
 
 
 def set_schedule(ctx: "Context", schedule: int):
@@ -314,7 +320,8 @@ def run_pipeline(ctx: Context, p: Params, left: np.ndarray, right: np.ndarray, i
     if want:
         shapes = {"cost": ((h, w, D), np.uint16), "cost_agg": ((h, w, D), np.uint16), "sgm": ((h, w, D), np.uint32),
                   "disp_right": ((h, w), np.float32), "disp_wta": ((h, w), np.float32), "best": ((h, w), np.float32),
-                  "second": ((h, w), np.float32)}
+                  "second": ((h, w), np.float32), "dmin": ((h, w), np.int16), "dmax": ((h, w), np.int16),
+                  "d_prior": ((h, w), np.float32)}
         arrs = {k: ctx.array(*shapes[k]) for k in want}
         inter = Intermediates(*[arrs[k].ptr if k in arrs else None for k in ALL_INTERMEDIATES])
     rc = ctx.lib.apg_hip_pipeline(ctx.h, C.byref(p), dl.ptr, dr.ptr, w, h, w * ch, ch, out_d.ptr, out_c.ptr,

@@ -280,7 +280,9 @@ int apg_hip_batch_run(apg_hip_batch b, const apg_hip_pair* pairs, int n, int fla
     if (n < 0 || (n > 0 && !pairs)) return bfail(b, 4, "invalid pair list");
     if (n == 0) return 0;
     if (int s = bhip(b, hipSetDevice(b->device), "hipSetDevice")) return s;
-    if (!(flags & APG_HIP_BATCH_GRAPH)) {
+    bool host_steps = false;   // the prior and the refiner synchronise: no capture
+    for (int i = 0; i < n; ++i) host_steps = host_steps || pairs[i].params.prior_enable || pairs[i].params.refine_enable;
+    if (!(flags & APG_HIP_BATCH_GRAPH) || host_steps) {
         // a direct run may grow slot workspaces, which a cached graph points into
         drop_graph(b);
         return enqueue(b, pairs, n);

@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "device_common.hpp"
+#include "hq_host.hpp"
 #include "kernels.hpp"
 
 using namespace apgk;
@@ -20,7 +21,8 @@ struct DevBuf {
 enum BufId {
     B_SRC_L, B_SRC_R, B_GRAY_L, B_GX_L, B_GY_L, B_GRAY_R, B_GX_R, B_GY_R, B_CEN_L, B_CEN_R,
     B_DMIN, B_DMAX, B_ARMS, B_COST, B_TMP, B_SUM, B_DR, B_BEST, B_SECOND, B_SCRATCH, B_LUT,
-    B_RKEYS, B_DS0, B_DS1, B_DS2, B_DS3, B_DS4, B_DS5, B_DS6, B_DS7, B_COUNT
+    B_RKEYS, B_DS0, B_DS1, B_DS2, B_DS3, B_DS4, B_DS5, B_DS6, B_DS7, B_PRIOR, B_CAND_D, B_CAND_U, B_GRID,
+    B_COUNT
 };
 
 enum StageId { S_AUX, S_PRIOR, S_COST, S_RWTA, S_CROSS, S_SGM, S_WTA, S_CONF, S_POST, S_COUNT };
@@ -59,6 +61,7 @@ struct apg_hip_ctx_s {
     int schedule = 0;   // 0 automatic (fused when eligible), 1 stage-by-stage
     KTune tune;
     bool capturing = false;   // stream capture in progress: no allocation, no host copy
+    std::vector<float> prior_grid_host;   // source of the last prior grid upload
 };
 
 namespace {
@@ -177,7 +180,19 @@ WsPlan plan_workspace(const apg_hip_params& p, int w, int h, const CostLut& lut,
     pl.u8 = cbound <= 254;
     // the WTA stage rebuilds S in u16: npath * (C' + P2max) must stay below 65535
     const int npaths = p.paths == 8 ? 8 : 4;
-    pl.fused = schedule != 1 && !want_raw && D % 2 == 0 && D <= 256 && p2max <= 254 &&
+    if (p.prior_enable) {
+        const size_t ns = (size_t)prior_samples_x(w) * prior_samples_y(h);
+        const size_t nc = (size_t)((w + kPriorCell - 1) / kPriorCell) * ((h + kPriorCell - 1) / kPriorCell);
+        pl.bytes[B_PRIOR] = n * 4;
+        pl.bytes[B_CAND_D] = ns * 2;
+        pl.bytes[B_CAND_U] = ns * 4;
+        pl.bytes[B_GRID] = nc * 5 * 4;
+    }
+    // the fused schedule assumes the dense range init and runs confidence and
+    // the LR check in one kernel, so the prior and the refiner (which sits
+    // between them) take the stage-by-stage schedule
+    pl.fused = schedule != 1 && !want_raw && !p.prior_enable && !p.refine_enable && D % 2 == 0 && D <= 256 &&
+               p2max <= 254 &&
                npaths * (cbound + p2max) < 65535 && (!p.agg_enable || p.agg_iterations <= 1) &&
                nv * (pl.u8 ? 1 : 2) < (1ull << 31);
     if (pl.fused) {
@@ -480,7 +495,151 @@ int apg_hip_median(apg_hip_ctx ctx, const apg_hip_params* p, int w, int h, const
     return 0;
 }
 
+}  // extern "C"
+
+// defined with the pipeline below
+int run_prior(apg_hip_ctx ctx, const apg_hip_params* p, int w, int h, const uint8_t* gxl, const uint8_t* gyl,
+              const void* cl, const void* cr, int16_t* dmin, int16_t* dmax, float* d_prior, int* n_supports);
+int run_refine(apg_hip_ctx ctx, const apg_hip_params* p, int w, int h, const uint8_t* gl, const uint8_t* gxl,
+               const uint8_t* gyl, const uint8_t* gr, const uint8_t* gxr, const uint8_t* gyr, const void* cl,
+               const void* cr, const int16_t* dmin, const int16_t* dmax, const uint8_t* reliable, float* disp);
+
+extern "C" {
+
+int apg_hip_prior_estimate(apg_hip_ctx ctx, const apg_hip_params* p, int w, int h, const uint8_t* d_gx_l,
+                           const uint8_t* d_gy_l, const uint32_t* d_c32l, const uint32_t* d_c32r,
+                           const uint64_t* d_c64l, const uint64_t* d_c64r, int16_t* d_dmin, int16_t* d_dmax,
+                           float* d_prior, int* n_supports) {
+    if (int s = check_ctx(ctx, p)) return s;
+    if (w <= 0 || h <= 0 || !d_dmin || !d_dmax || !d_prior) return fail(ctx, 1, "empty input");
+    const bool sym = p->census_type == 1;
+    if (p->prior_enable && (sym ? (!d_c32l || !d_c32r) : (!d_c64l || !d_c64r)))
+        return fail(ctx, 4, "census buffers");
+    HIPCHK(launch_range_init(ctx->stream, *p, w, h, d_dmin, d_dmax), "range");
+    if (int s = run_prior(ctx, p, w, h, d_gx_l, d_gy_l, sym ? (const void*)d_c32l : (const void*)d_c64l,
+                          sym ? (const void*)d_c32r : (const void*)d_c64r, d_dmin, d_dmax, d_prior, n_supports))
+        return s;
+    HIPCHK(hipStreamSynchronize(ctx->stream), "prior sync");
+    return 0;
+}
+
+int apg_hip_refine(apg_hip_ctx ctx, const apg_hip_params* p, int w, int h, const uint8_t* d_gray_l,
+                   const uint8_t* d_gx_l, const uint8_t* d_gy_l, const uint8_t* d_gray_r, const uint8_t* d_gx_r,
+                   const uint8_t* d_gy_r, const uint32_t* d_c32l, const uint32_t* d_c32r, const uint64_t* d_c64l,
+                   const uint64_t* d_c64r, const int16_t* d_dmin, const int16_t* d_dmax, const uint8_t* d_reliable,
+                   float* d_disparity) {
+    if (int s = check_ctx(ctx, p)) return s;
+    if (w <= 0 || h <= 0) return fail(ctx, 1, "empty input");
+    if (!p->refine_enable) return 0;
+    const bool sym = p->census_type == 1;
+    if (sym ? (!d_c32l || !d_c32r) : (!d_c64l || !d_c64r)) return fail(ctx, 4, "census buffers");
+    return run_refine(ctx, p, w, h, d_gray_l, d_gx_l, d_gy_l, d_gray_r, d_gx_r, d_gy_r,
+                      sym ? (const void*)d_c32l : (const void*)d_c64l, sym ? (const void*)d_c32r : (const void*)d_c64r,
+                      d_dmin, d_dmax, d_reliable, d_disparity);
+}
+
 // -------------------------------------------------------------- pipeline ----
+
+}  // extern "C"
+
+namespace {
+template <typename T>
+hipError_t download_vec(hipStream_t st, std::vector<T>& v, const void* d, size_t count) {
+    v.resize(count);
+    return count ? hipMemcpyAsync(v.data(), d, count * sizeof(T), hipMemcpyDeviceToHost, st) : hipSuccess;
+}
+
+}  // namespace
+
+// PriorEstimator::estimate (reference src/prior_estimator.cpp:414-435) after
+// the dense range init: device candidates, host support selection and cell
+// grid, device per-pixel prior and ranges.  d_prior is -1 where there is no
+// prior.  Synchronises the context stream once.
+int run_prior(apg_hip_ctx ctx, const apg_hip_params* p, int w, int h, const uint8_t* gxl, const uint8_t* gyl,
+              const void* cl, const void* cr, int16_t* dmin, int16_t* dmax, float* d_prior, int* n_supports) {
+    hipStream_t st = ctx->stream;
+    const size_t n = (size_t)w * h;
+    if (n_supports) *n_supports = 0;
+    HIPCHK(launch_fill_u32(st, n, 0xBF800000u /* -1.0f */, (uint32_t*)d_prior), "prior init");
+    const int sw = prior_samples_x(w), sh = prior_samples_y(h);
+    if (!p->prior_enable || sw <= 0 || sh <= 0) return 0;
+    if (ctx->capturing) return fail(ctx, 8, "the HighQuality prior synchronises and cannot be captured");
+    const bool sym = p->census_type == 1;
+    const size_t ncell = (size_t)((w + kPriorCell - 1) / kPriorCell) * ((h + kPriorCell - 1) / kPriorCell);
+    HIPCHK(ensure(ctx, B_CAND_D, (size_t)sw * sh * 2), "prior alloc");
+    HIPCHK(ensure(ctx, B_CAND_U, (size_t)sw * sh * 4), "prior alloc");
+    HIPCHK(ensure(ctx, B_GRID, ncell * 5 * 4), "prior alloc");
+    int16_t* cd = buf<int16_t>(ctx, B_CAND_D);
+    float* cu = buf<float>(ctx, B_CAND_U);
+    HIPCHK(launch_prior_candidates(st, *p, w, h, gxl, gyl, sym ? (const uint32_t*)cl : nullptr,
+                                   sym ? (const uint32_t*)cr : nullptr, sym ? nullptr : (const uint64_t*)cl,
+                                   sym ? nullptr : (const uint64_t*)cr, cd, cu),
+           "prior candidates");
+    std::vector<int16_t> hd;
+    std::vector<float> hu;
+    const size_t ns = (size_t)sw * sh;
+    HIPCHK(download_vec(st, hd, cd, ns), "prior download");
+    HIPCHK(download_vec(st, hu, cu, ns), "prior download");
+    HIPCHK(hipStreamSynchronize(st), "prior sync");
+    PriorGrid g;
+    const int nsup = prior_build_grid(*p, w, h, hd.data(), hu.data(), g);
+    if (n_supports) *n_supports = nsup;
+    if (nsup < p->prior_min_supports || nsup == 0) return 0;
+    const size_t nc = (size_t)g.gw * g.gh;
+    std::vector<float>& planes = ctx->prior_grid_host;
+    planes.resize(nc * 5);
+    std::copy(g.d.begin(), g.d.end(), planes.begin());
+    std::copy(g.c.begin(), g.c.end(), planes.begin() + nc);
+    std::copy(g.s.begin(), g.s.end(), planes.begin() + 2 * nc);
+    std::copy(g.lo.begin(), g.lo.end(), planes.begin() + 3 * nc);
+    std::copy(g.hi.begin(), g.hi.end(), planes.begin() + 4 * nc);
+    float* dg = buf<float>(ctx, B_GRID);
+    HIPCHK(hipMemcpyAsync(dg, planes.data(), nc * 5 * sizeof(float), hipMemcpyHostToDevice, st), "prior upload");
+    HIPCHK(launch_prior_ranges(st, *p, w, h, g.gw, g.gh, dg, dmin, dmax, d_prior), "prior ranges");
+    return 0;
+}
+
+// Refiner::refine (reference src/refiner.cpp:90-98) on the host between the
+// confidence and the LR check; synchronises the context stream.
+int run_refine(apg_hip_ctx ctx, const apg_hip_params* p, int w, int h, const uint8_t* gl, const uint8_t* gxl,
+               const uint8_t* gyl, const uint8_t* gr, const uint8_t* gxr, const uint8_t* gyr, const void* cl,
+               const void* cr, const int16_t* dmin, const int16_t* dmax, const uint8_t* reliable, float* disp) {
+    if (ctx->capturing) return fail(ctx, 8, "the HighQuality refiner synchronises and cannot be captured");
+    hipStream_t st = ctx->stream;
+    const size_t n = (size_t)w * h;
+    const bool sym = p->census_type == 1;
+    std::vector<uint8_t> hgl, hgxl, hgyl, hgr, hgxr, hgyr, hrel;
+    std::vector<uint32_t> c32l, c32r;
+    std::vector<uint64_t> c64l, c64r;
+    std::vector<int16_t> hmin, hmax;
+    std::vector<float> hd;
+    HIPCHK(download_vec(st, hgl, gl, n), "refine download");
+    HIPCHK(download_vec(st, hgxl, gxl, n), "refine download");
+    HIPCHK(download_vec(st, hgyl, gyl, n), "refine download");
+    HIPCHK(download_vec(st, hgr, gr, n), "refine download");
+    HIPCHK(download_vec(st, hgxr, gxr, n), "refine download");
+    HIPCHK(download_vec(st, hgyr, gyr, n), "refine download");
+    HIPCHK(download_vec(st, hrel, reliable, n), "refine download");
+    HIPCHK(download_vec(st, hmin, dmin, n), "refine download");
+    HIPCHK(download_vec(st, hmax, dmax, n), "refine download");
+    HIPCHK(download_vec(st, hd, disp, n), "refine download");
+    if (sym) {
+        HIPCHK(download_vec(st, c32l, cl, n), "refine download");
+        HIPCHK(download_vec(st, c32r, cr, n), "refine download");
+    } else {
+        HIPCHK(download_vec(st, c64l, cl, n), "refine download");
+        HIPCHK(download_vec(st, c64r, cr, n), "refine download");
+    }
+    HIPCHK(hipStreamSynchronize(st), "refine sync");
+    refine_host(*p, w, h, hgl.data(), hgxl.data(), hgyl.data(), hgr.data(), hgxr.data(), hgyr.data(),
+                sym ? c32l.data() : nullptr, sym ? c32r.data() : nullptr, sym ? nullptr : c64l.data(),
+                sym ? nullptr : c64r.data(), hmin.data(), hmax.data(), hrel.data(), hd.data());
+    HIPCHK(hipMemcpyAsync(disp, hd.data(), n * 4, hipMemcpyHostToDevice, st), "refine upload");
+    HIPCHK(hipStreamSynchronize(st), "refine sync");
+    return 0;
+}
+
+extern "C" {
 
 // Dense branch of StereoMatcher::compute_impl (reference src/pipeline.cpp:435-605)
 // on the device: aux -> range -> cost -> right WTA (raw volume) -> Cross ->
@@ -522,6 +681,16 @@ int apg_hip_pipeline(apg_hip_ctx ctx, const apg_hip_params* p, const uint8_t* d_
            "aux R");
     mark(ctx, S_PRIOR);
     HIPCHK(launch_range_init(st, *p, w, h, dmin, dmax), "range");
+    float* d_prior = p->prior_enable ? buf<float>(ctx, B_PRIOR) : nullptr;
+    if (d_prior) {
+        if (int s = run_prior(ctx, p, w, h, gxl, gyl, cl, cr, dmin, dmax, d_prior, nullptr)) return s;
+    }
+    if (inter && inter->d_dmin) HIPCHK(hipMemcpyAsync(inter->d_dmin, dmin, n * 2, hipMemcpyDeviceToDevice, st), "copy dmin");
+    if (inter && inter->d_dmax) HIPCHK(hipMemcpyAsync(inter->d_dmax, dmax, n * 2, hipMemcpyDeviceToDevice, st), "copy dmax");
+    if (inter && inter->d_prior) {
+        if (d_prior) HIPCHK(hipMemcpyAsync(inter->d_prior, d_prior, n * 4, hipMemcpyDeviceToDevice, st), "copy prior");
+        else HIPCHK(launch_fill_u32(st, n, 0xBF800000u, (uint32_t*)inter->d_prior), "fill prior");
+    }
 
     if (fused) {
         PlaneSet planes{};
@@ -600,9 +769,13 @@ int apg_hip_pipeline(apg_hip_ctx ctx, const apg_hip_params* p, const uint8_t* d_
     mark(ctx, S_CONF);
     HIPCHK(launch_confidence(st, *p, w, h, d_disparity, dr, gxl, gyl, best, second, d_confidence, d_reliable, d_reason),
            "confidence");
+    if (p->refine_enable) {
+        if (int s = run_refine(ctx, p, w, h, gl, gxl, gyl, gr, gxr, gyr, cl, cr, dmin, dmax, d_reliable, d_disparity))
+            return s;
+    }
     mark(ctx, S_POST);
     HIPCHK(launch_lr_check(st, *p, w, h, d_disparity, dr, d_reason), "lr");
-    HIPCHK(launch_fill(st, *p, w, h, d_disparity, d_reason, gl, nullptr, scratch), "fill");
+    HIPCHK(launch_fill(st, *p, w, h, d_disparity, d_reason, gl, d_prior, scratch), "fill");
     HIPCHK(launch_median(st, *p, w, h, scratch, d_disparity), "median");
     mark(ctx, S_COUNT);
     return finish_timing(ctx);

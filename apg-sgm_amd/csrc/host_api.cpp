@@ -223,14 +223,38 @@ uint64_t CostComputer::census9x7(const Image8& gray, int x, int y) {
 
 // ---------------------------------------------------------- PriorEstimator --
 void PriorEstimator::estimate(const PipelineConfig& cfg, PipelineBuffers& buf) const {
-    if (cfg.prior.enable) throw std::runtime_error("apg: the HighQuality prior is not part of this device build");
     const int w = buf.left_gray.width(), h = buf.left_gray.height();
     buf.range.allocate(w, h, cfg.min_disparity, cfg.max_disparity);
-    for (int y = 0; y < h; ++y)
-        for (int x = 0; x < w; ++x) buf.range.set_pixel(x, y, cfg.min_disparity, std::min(cfg.max_disparity, x + 1));
     buf.d_prior = Image32f(w, h, -1.f);
     buf.supports.clear();
     buf.support_count = 0;
+    if (!cfg.prior.enable) {
+        for (int y = 0; y < h; ++y)
+            for (int x = 0; x < w; ++x)
+                buf.range.set_pixel(x, y, cfg.min_disparity, std::min(cfg.max_disparity, x + 1));
+        return;
+    }
+    // the prior on the device; the support list itself stays there (its
+    // count is reported)
+    apg_hip_ctx ctx = stage_ctx();
+    const apg_hip_params p = to_device_params(cfg);
+    const size_t n = npx(w, h);
+    const bool sym = cfg.cost.census == CensusType::SymmetricCensus9x7;
+    Dev gx(ctx, buf.left_gx.data(), n), gy(ctx, buf.left_gy.data(), n);
+    Dev cl(ctx, sym ? (const void*)buf.census_left.data() : (const void*)buf.census_left64.data(), n * (sym ? 4 : 8));
+    Dev cr(ctx, sym ? (const void*)buf.census_right.data() : (const void*)buf.census_right64.data(), n * (sym ? 4 : 8));
+    Dev dmin(ctx, n * 2), dmax(ctx, n * 2), dp(ctx, n * 4);
+    int nsup = 0;
+    check(ctx,
+          apg_hip_prior_estimate(ctx, &p, w, h, gx.as<uint8_t>(), gy.as<uint8_t>(), sym ? cl.as<uint32_t>() : nullptr,
+                                 sym ? cr.as<uint32_t>() : nullptr, sym ? nullptr : cl.as<uint64_t>(),
+                                 sym ? nullptr : cr.as<uint64_t>(), dmin.as<int16_t>(), dmax.as<int16_t>(),
+                                 dp.as<float>(), &nsup),
+          "prior");
+    dmin.get(buf.range.dmin.data(), n * 2);
+    dmax.get(buf.range.dmax.data(), n * 2);
+    dp.get(buf.d_prior.data(), n * 4);
+    buf.support_count = static_cast<size_t>(nsup);
 }
 
 // ---------------------------------------------------------- CostAggregator --
@@ -494,9 +518,8 @@ Status StereoMatcher::compute_impl(const ImageView8& left, const ImageView8& rig
     if (!s.ok()) return s;
     s = validate_config(cfg_);
     if (!s.ok()) return s;
-    if (cfg_.prior.enable || cfg_.hierarchical.enable || cfg_.refine.enable)
-        return Status(StatusCode::UnsupportedBackend,
-                      "prior / hierarchical / refine stages (HighQuality preset) are not part of this device build");
+    if (cfg_.hierarchical.enable)
+        return Status(StatusCode::UnsupportedBackend, "the hierarchical range estimator is not part of this device build");
     const apg_hip_params p = to_device_params(cfg_);
     const int vs = apg_hip_validate_params(&p);
     if (vs) return Status(static_cast<StatusCode>(vs), "configuration outside the device path's supported range");

@@ -1,0 +1,44 @@
+// Host-side parts of the HighQuality stages: the support selection and grid
+// statistics of the prior (a few thousand supports on a 16-pixel grid: small,
+// sort-heavy, order-sensitive) and the refiner (a sequential raster sweep
+// whose random jitter comes from one std::mt19937 stream).  Everything per
+// pixel or per disparity state runs on the device (k_prior.hip).
+#pragma once
+
+#include <stdint.h>
+
+#include <vector>
+
+#include "../../include/apg_sgm_hip.h"
+
+namespace apgk {
+
+constexpr int kPriorCell = 16;   // prior grid cell (reference src/prior_estimator.cpp:22)
+constexpr int16_t kNoCandidate = INT16_MIN;
+
+// Samples of the prior: even x in [2, w-2), even y in [2, h-2).
+inline int prior_samples_x(int w) { return w >= 3 ? (w - 3) / 2 : 0; }
+inline int prior_samples_y(int h) { return h >= 3 ? (h - 3) / 2 : 0; }
+
+// Per-cell prior grid after hole filling: disparity (-1 = none),
+// confidence, spread, lowest and highest support disparity.
+struct PriorGrid {
+    int gw = 0, gh = 0;
+    std::vector<float> d, c, s, lo, hi;
+};
+
+// PriorEstimator::extract_supports' selection and interpolate_prior's grid
+// stages (reference src/prior_estimator.cpp:29-166, 181-310) from the
+// device's per-sample candidates (cand_d = kNoCandidate where the sample is
+// not a candidate).  Returns the number of supports kept.
+int prior_build_grid(const apg_hip_params& p, int w, int h, const int16_t* cand_d, const float* cand_u,
+                     PriorGrid& grid);
+
+// Refiner::refine (reference src/refiner.cpp:14-98) in place on disp.
+// census64 is used when census32 is null.
+void refine_host(const apg_hip_params& p, int w, int h, const uint8_t* gl, const uint8_t* gxl, const uint8_t* gyl,
+                 const uint8_t* gr, const uint8_t* gxr, const uint8_t* gyr, const uint32_t* c32l,
+                 const uint32_t* c32r, const uint64_t* c64l, const uint64_t* c64r, const int16_t* dmin,
+                 const int16_t* dmax, const uint8_t* reliable, float* disp);
+
+}  // namespace apgk

@@ -113,4 +113,13 @@ hipError_t launch_wta_fused(hipStream_t s, const apg_hip_params& p, int w, int h
                             const uint8_t* gx, const uint8_t* gy, float* disp, float* conf, uint8_t* reason,
                             uint8_t* reliable, float* disp_wta, float* best, float* second, uint32_t* sum32);
 
+// HighQuality prior (k_prior.hip): per-sample census candidates, then the
+// per-pixel prior and search range from the host-built cell grid
+// (5 planes of gw*gh floats: disparity, confidence, spread, lowest, highest).
+hipError_t launch_prior_candidates(hipStream_t s, const apg_hip_params& p, int w, int h, const uint8_t* gx,
+                                   const uint8_t* gy, const uint32_t* c32l, const uint32_t* c32r,
+                                   const uint64_t* c64l, const uint64_t* c64r, int16_t* cand_d, float* cand_u);
+hipError_t launch_prior_ranges(hipStream_t s, const apg_hip_params& p, int w, int h, int gw, int gh,
+                               const float* grid, int16_t* dmin, int16_t* dmax, float* d_prior);
+
 }  // namespace apgk

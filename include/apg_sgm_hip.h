@@ -184,6 +184,32 @@ APG_HIP_API int apg_hip_fill_holes(apg_hip_ctx ctx, const apg_hip_params* p, int
 APG_HIP_API int apg_hip_median(apg_hip_ctx ctx, const apg_hip_params* p, int w, int h,
                                const float* d_disp, float* d_out);
 
+/* PriorEstimator::estimate (reference src/prior_estimator.cpp:414-435):
+ * the dense range init, then, when p->prior_enable and enough supports are
+ * found, the prior-narrowed per-pixel ranges.  The census-only candidates of
+ * extract_supports and the per-pixel interpolation / range of
+ * interpolate_prior + apply_search_range run on the device; the support
+ * selection and the cell-grid statistics run on the host in between, so the
+ * call synchronises the context stream.  d_prior: w*h floats, -1 where there
+ * is no prior.  *n_supports (may be NULL): supports kept. */
+APG_HIP_API int apg_hip_prior_estimate(apg_hip_ctx ctx, const apg_hip_params* p, int w, int h,
+                                       const uint8_t* d_gx_l, const uint8_t* d_gy_l,
+                                       const uint32_t* d_census32_l, const uint32_t* d_census32_r,
+                                       const uint64_t* d_census64_l, const uint64_t* d_census64_r,
+                                       int16_t* d_dmin, int16_t* d_dmax, float* d_prior, int* n_supports);
+
+/* Refiner::refine (reference src/refiner.cpp:14-98), in place on
+ * d_disparity: a sequential raster sweep driven by one std::mt19937 stream,
+ * run on the host (inputs are copied down and the result back up; the call
+ * synchronises).  No-op unless p->refine_enable. */
+APG_HIP_API int apg_hip_refine(apg_hip_ctx ctx, const apg_hip_params* p, int w, int h,
+                               const uint8_t* d_gray_l, const uint8_t* d_gx_l, const uint8_t* d_gy_l,
+                               const uint8_t* d_gray_r, const uint8_t* d_gx_r, const uint8_t* d_gy_r,
+                               const uint32_t* d_census32_l, const uint32_t* d_census32_r,
+                               const uint64_t* d_census64_l, const uint64_t* d_census64_r,
+                               const int16_t* d_dmin, const int16_t* d_dmax, const uint8_t* d_reliable,
+                               float* d_disparity);
+
 /* ---- whole pipeline -------------------------------------------------- */
 
 /* Optional device intermediates of the pipeline (any may be NULL). Setting
@@ -197,6 +223,9 @@ typedef struct {
     float* d_disp_wta;       /* w*h, WTA output before post-processing */
     float* d_best;           /* w*h */
     float* d_second;         /* w*h */
+    int16_t* d_dmin;         /* w*h search range (prior-narrowed in HighQuality) */
+    int16_t* d_dmax;         /* w*h */
+    float* d_prior;          /* w*h prior disparity, -1 where there is none */
 } apg_hip_intermediates;
 
 /* Schedule of apg_hip_pipeline on this context: 0 (default) runs the fused
@@ -207,9 +236,16 @@ typedef struct {
  * produce identical results. */
 APG_HIP_API void apg_hip_set_schedule(apg_hip_ctx ctx, int schedule);
 
-/* StereoMatcher::compute_impl, dense branch (reference src/pipeline.cpp:435-605)
- * on device inputs and outputs.  Intermediate buffers come from the context
- * workspace.  d_left / d_right: strided u8 images. */
+/* StereoMatcher::compute_impl (reference src/pipeline.cpp:435-605) on device
+ * inputs and outputs: aux, range (prior when p->prior_enable), cost, right
+ * WTA, Cross, SGM, WTA, confidence, refiner (p->refine_enable), LR check,
+ * fill (with the prior), median.  The HighQuality preset's packed volume is
+ * computed as the dense volume over the prior-narrowed per-pixel ranges,
+ * which the reference's own dense == packed tests show to be identical
+ * (tests/test_sanity.cpp:83-476).  With the prior or the refiner the call
+ * synchronises the context stream (their host steps) and cannot be
+ * captured.  Intermediate buffers come from the context workspace.
+ * d_left / d_right: strided u8 images. */
 APG_HIP_API int apg_hip_pipeline(apg_hip_ctx ctx, const apg_hip_params* p,
                                  const uint8_t* d_left, const uint8_t* d_right,
                                  int w, int h, int stride, int channels,
@@ -290,7 +326,8 @@ APG_HIP_API const char* apg_hip_batch_last_error(apg_hip_batch b);
  * size bucket was tuned to.  With APG_HIP_BATCH_GRAPH the whole batch is one
  * hipGraphLaunch whenever the same pair list ran before on this batch: a new
  * list runs directly once (sizing workspaces and LUTs), is then captured,
- * and replays from the next call on.  Results are identical to
+ * and replays from the next call on.  A list with a prior or refiner stage
+ * (HighQuality) always runs directly.  Results are identical to
  * apg_hip_pipeline on each pair alone. */
 APG_HIP_API int apg_hip_batch_run(apg_hip_batch b, const apg_hip_pair* pairs, int n, int flags);
 

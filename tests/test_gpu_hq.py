@@ -1,0 +1,92 @@
+"""HighQuality pipeline on the GPU (prior-narrowed ranges, 8 paths, refiner)
+against the oracle and against the golden vectors the reference build wrote:
+ranges, prior, every volume, d_before_refine and every output bit-exact."""
+import ctypes as C
+import json
+import os
+
+import numpy as np
+import pytest
+
+import apg_amd as A
+import pyoracle as O
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+# (kind, w, h, D, overrides): both census types, the case-B budget, a prior
+# that finds too few supports, D > 128 (K = 2 lanes), odd sizes
+CASES = [
+    (2, 160, 120, 48, {}),
+    (0, 200, 150, 64, {}),
+    (1, 256, 128, 96, {"census_type": 0}),
+    (2, 120, 90, 32, {"prior_max_supports": 10, "prior_min_supports": 5}),
+    (0, 96, 64, 24, {"prior_min_supports": 10 ** 6}),
+    (1, 333, 101, 160, {}),
+]
+
+KEYS = ["cost", "cost_agg", "disp_right", "disp_wta", "best", "second", "dmin", "dmax", "d_prior",
+        "disparity", "confidence", "invalid_reason", "reliable"]
+
+
+def _cmp(got, want, what):
+    assert got.shape == want.shape, what
+    if not np.array_equal(got, want, equal_nan=True):
+        bad = np.argwhere(got != want)
+        i = tuple(bad[0])
+        raise AssertionError(f"{what}: {len(bad)} mismatches, first at {i}: gpu={got[i]} oracle={want[i]}")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind,w,h,D,over", CASES)
+def test_hq_pipeline_matches_oracle(ctx, kind, w, h, D, over):
+    print(f"HQ kind={kind} {w}x{h} D={D} {over}", flush=True)
+    p = A.params(2, D, **over)
+    left, right, _ = A.synth_pair(kind, w, h, D, 4321 + w)
+    o = O.pipeline(O.cfg_from(p.as_dict()), left, right, intermediates=True)
+    g = A.run_pipeline(ctx, p, left, right, intermediates=True)
+    for k in KEYS:
+        _cmp(getattr(g, k), o[k], k)
+    # the u32 sums of the oracle and the GPU's u16 sums widened on the way out
+    _cmp(g.sgm, o["sgm"], "sgm")
+
+
+def _golden_small():
+    z = np.load(os.path.join(GOLD, "golden_small.npz"))
+    return z, sorted({k.split("__")[0] for k in z.files if k.startswith("hq_")})
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", _golden_small()[1])
+def test_hq_golden_vectors(ctx, name):
+    # straight against the reference build's outputs (packed-volume pipeline)
+    z, _ = _golden_small()
+    p = A.Params()
+    for k, v in json.loads(bytes(z[f"{name}__cfg"]).decode()).items():
+        setattr(p, k, v)
+    print(f"HQ golden {name}", flush=True)
+    g = A.run_pipeline(ctx, p, z[f"{name}__left"], z[f"{name}__right"], intermediates=True)
+    for k in ("cost", "cost_agg", "best", "second", "disparity", "confidence", "invalid_reason", "reliable",
+              "disp_right", "dmin", "dmax"):
+        _cmp(getattr(g, k), z[f"{name}__{k}"], k)
+    _cmp(g.disp_wta, z[f"{name}__d_before_refine"], "d_before_refine")
+
+
+@pytest.mark.gpu
+def test_prior_entry_point_matches_oracle(ctx):
+    p = A.params(2, 64)
+    left, right, _ = A.synth_pair(2, 240, 160, 64, 77)
+    c = O.cfg_from(p.as_dict())
+    _, gxl, gyl, cl = O.aux(c, left)
+    _, _, _, cr = O.aux(c, right)
+    want_min, want_max, want_dp, want_n = O.prior_estimate(c, gxl, gyl, cl, cr)
+    h, w = left.shape
+    dgx, dgy, dcl, dcr = (ctx.to_device(a) for a in (gxl, gyl, cl, cr))
+    dmin, dmax, dp = ctx.array((h, w), np.int16), ctx.array((h, w), np.int16), ctx.array((h, w), np.float32)
+    n = C.c_int(-1)
+    print("prior entry point 240x160 D=64", flush=True)
+    ctx._check(ctx.lib.apg_hip_prior_estimate(ctx.h, C.byref(p), w, h, dgx.ptr, dgy.ptr, dcl.ptr, dcr.ptr, None,
+                                              None, dmin.ptr, dmax.ptr, dp.ptr, C.byref(n)), "prior")
+    assert n.value == want_n
+    _cmp(dmin.download(), want_min, "dmin")
+    _cmp(dmax.download(), want_max, "dmax")
+    _cmp(dp.download(), want_dp, "d_prior")
