@@ -195,6 +195,181 @@ void CostComputer::compute_volume(const PipelineConfig& cfg, PipelineBuffers& bu
     out.get(buf.cost.data(), buf.cost.bytes());
 }
 
+// ------------------------------------------------------------ packed forms --
+namespace {
+
+// The dense span [d0, d1) that covers every pixel's range of a layout, and
+// the layout's ranges as a SearchRange over it.
+struct DenseSpan {
+    int d0 = 0, d1 = 0;
+    SearchRange range;
+    bool empty() const { return d1 <= d0; }
+};
+
+DenseSpan span_of(const PackedVolumeLayout& L) {
+    DenseSpan s;
+    const size_t n = npx(L.width, L.height);
+    bool any = false;
+    for (size_t q = 0; q < n; ++q)
+        if (L.disp_width(q) > 0) {
+            s.d0 = any ? std::min(s.d0, (int)L.dmin[q]) : L.dmin[q];
+            s.d1 = any ? std::max(s.d1, L.dmax(q)) : L.dmax(q);
+            any = true;
+        }
+    s.range.allocate(L.width, L.height, s.d0, s.d1);
+    for (size_t q = 0; q < n; ++q) {
+        s.range.dmin.data()[q] = L.dmin[q];
+        s.range.dmax.data()[q] = (int16_t)L.dmax(q);
+    }
+    return s;
+}
+
+// packed -> dense over the span (states outside a pixel's range: `fill`),
+// with an optional value map (u16 -> u32 sentinel widening)
+template <typename TS, typename TD, typename Map>
+void unpack(const PackedCostVolume<TS>& v, const DenseSpan& s, CostVolumeT<TD>& out, TD fill, Map map) {
+    const PackedVolumeLayout& L = *v.layout();
+    out.allocate(L.width, L.height, s.d0, s.d1, fill);
+    const size_t n = npx(L.width, L.height);
+    const int D = s.d1 - s.d0;
+    for (size_t q = 0; q < n; ++q) {
+        const TS* src = v.data() + L.offsets[q];
+        TD* dst = out.data() + q * D + (L.dmin[q] - s.d0);
+        for (int k = 0, e = L.disp_width(q); k < e; ++k) dst[k] = map(src[k]);
+    }
+}
+
+template <typename TS, typename TD, typename Map>
+void pack(const CostVolumeT<TS>& in, PackedCostVolume<TD>& v, Map map) {
+    const PackedVolumeLayout& L = *v.layout();
+    const size_t n = npx(L.width, L.height);
+    const int D = in.D();
+    for (size_t q = 0; q < n; ++q) {
+        const TS* src = in.data() + q * D + (L.dmin[q] - in.d0());
+        TD* dst = v.data() + L.offsets[q];
+        for (int k = 0, e = L.disp_width(q); k < e; ++k) dst[k] = map(src[k]);
+    }
+}
+
+template <typename T>
+T same(T v) { return v; }
+uint32_t widen(uint16_t v) { return v == kInvalidCost ? kInvalidCost32 : v; }
+uint16_t narrow(uint32_t v) { return v == kInvalidCost32 ? kInvalidCost : static_cast<uint16_t>(v); }
+
+PipelineConfig over_span(const PipelineConfig& cfg, const DenseSpan& s) {
+    PipelineConfig c = cfg;
+    c.min_disparity = s.d0;
+    c.max_disparity = s.d1;
+    return c;
+}
+
+}  // namespace
+
+void CostComputer::compute_volume_packed(const PipelineConfig& cfg, const PipelineBuffers& buf,
+                                         PackedCostVolume16& packed_cost) const {
+    if (!packed_cost.layout()) packed_cost.allocate_for_overwrite(buf.range);
+    const DenseSpan s = span_of(*packed_cost.layout());
+    if (s.empty()) return;
+    // the dense kernel over the span with the layout's ranges
+    PipelineBuffers b;
+    b.left_gray = buf.left_gray;
+    b.left_gx = buf.left_gx;
+    b.left_gy = buf.left_gy;
+    b.right_gray = buf.right_gray;
+    b.right_gx = buf.right_gx;
+    b.right_gy = buf.right_gy;
+    b.census_left = buf.census_left;
+    b.census_right = buf.census_right;
+    b.census_left64 = buf.census_left64;
+    b.census_right64 = buf.census_right64;
+    b.range = s.range;
+    compute_volume(over_span(cfg, s), b);
+    pack(b.cost, packed_cost, same<uint16_t>);
+}
+
+void CostAggregator::aggregate_packed(const PipelineConfig& cfg, const Image8& left_gray,
+                                      PackedCostVolume16& packed_cost) const {
+    if (!cfg.aggregation.enable || !packed_cost.layout()) return;
+    const DenseSpan s = span_of(*packed_cost.layout());
+    if (s.empty()) return;
+    PipelineBuffers b;
+    b.left_gray = left_gray;
+    b.range = s.range;
+    unpack(packed_cost, s, b.cost, kInvalidCost, same<uint16_t>);
+    aggregate(over_span(cfg, s), b);
+    pack(b.cost, packed_cost, same<uint16_t>);
+}
+
+void SgmOptimizer::optimize_packed(const PipelineConfig& cfg, const Image8& gray, const PackedCostVolume16& packed_cost,
+                                   PackedCostVolume16& packed_sgm16) const {
+    packed_sgm16.allocate_for_overwrite(packed_cost.layout());
+    if (!packed_cost.layout()) return;
+    const DenseSpan s = span_of(*packed_cost.layout());
+    if (s.empty()) return;
+    PipelineBuffers b;
+    b.left_gray = gray;
+    unpack(packed_cost, s, b.cost, kInvalidCost, same<uint16_t>);
+    optimize(over_span(cfg, s), b);
+    pack(b.cost32, packed_sgm16, narrow);
+}
+
+void SgmOptimizer::optimize_packed(const PipelineConfig& cfg, const Image8& gray, const PackedCostVolume16& packed_cost,
+                                   PackedCostVolume32& packed_cost32) const {
+    packed_cost32.allocate_for_overwrite(packed_cost.layout());
+    if (!packed_cost.layout()) return;
+    const DenseSpan s = span_of(*packed_cost.layout());
+    if (s.empty()) return;
+    PipelineBuffers b;
+    b.left_gray = gray;
+    unpack(packed_cost, s, b.cost, kInvalidCost, same<uint16_t>);
+    optimize(over_span(cfg, s), b);
+    pack(b.cost32, packed_cost32, same<uint32_t>);
+}
+
+void SgmOptimizer::winner_take_all_packed(const PipelineConfig& cfg, const PackedCostVolume32& vol, Image32f& disp,
+                                          Image32f* best, Image32f* second) const {
+    if (!vol.layout()) return;
+    const DenseSpan s = span_of(*vol.layout());
+    CostVolume32 dense;
+    unpack(vol, s, dense, kInvalidCost32, same<uint32_t>);
+    winner_take_all(over_span(cfg, s), dense, s.range, disp, best, second);
+}
+
+void SgmOptimizer::winner_take_all_packed(const PipelineConfig& cfg, const PackedCostVolume16& vol, Image32f& disp,
+                                          Image32f* best, Image32f* second) const {
+    if (!vol.layout()) return;
+    const DenseSpan s = span_of(*vol.layout());
+    CostVolume dense;
+    unpack(vol, s, dense, kInvalidCost, same<uint16_t>);
+    winner_take_all(over_span(cfg, s), dense, s.range, disp, best, second);
+}
+
+void SgmOptimizer::aggregate_path_packed(const PipelineConfig& cfg, const Image8& gray, const PackedCostVolume16& base,
+                                         PackedCostVolume16& acc, int dx, int dy) const {
+    if (!base.layout() || !acc.layout()) return;
+    const DenseSpan s = span_of(*base.layout());
+    if (s.empty()) return;
+    CostVolume dense;
+    CostVolume32 dacc;
+    unpack(base, s, dense, kInvalidCost, same<uint16_t>);
+    unpack(acc, s, dacc, 0u, widen);
+    aggregate_path(over_span(cfg, s), gray, dense, dacc, dx, dy);
+    pack(dacc, acc, narrow);
+}
+
+void SgmOptimizer::aggregate_path_packed(const PipelineConfig& cfg, const Image8& gray, const PackedCostVolume16& base,
+                                         PackedCostVolume32& acc, int dx, int dy) const {
+    if (!base.layout() || !acc.layout()) return;
+    const DenseSpan s = span_of(*base.layout());
+    if (s.empty()) return;
+    CostVolume dense;
+    CostVolume32 dacc;
+    unpack(base, s, dense, kInvalidCost, same<uint16_t>);
+    unpack(acc, s, dacc, 0u, same<uint32_t>);
+    aggregate_path(over_span(cfg, s), gray, dense, dacc, dx, dy);
+    pack(dacc, acc, same<uint32_t>);
+}
+
 // Direct definitions of the descriptors (reference src/cost_computer.cpp:33-61).
 uint32_t CostComputer::symmetric_census9x7(const Image8& gray, int x, int y) {
     uint32_t bits = 0;

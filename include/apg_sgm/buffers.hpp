@@ -7,38 +7,11 @@
 #include <vector>
 
 #include "apg_sgm/image.hpp"
+#include "apg_sgm/packed_volume.hpp"
+#include "apg_sgm/search_range.hpp"
 #include "apg_sgm/types.hpp"
 
 namespace apg {
-
-struct SearchRange {
-    Image16s dmin;
-    Image16s dmax;
-    int global_min = 0;
-    int global_max = 0;
-
-    int D() const { return global_max - global_min; }
-    void allocate(int w, int h, int gmin, int gmax) {
-        global_min = gmin;
-        global_max = gmax;
-        dmin = Image16s(w, h, static_cast<int16_t>(gmin));
-        dmax = Image16s(w, h, static_cast<int16_t>(gmax));
-    }
-    bool valid(int x, int y) const { return dmax.at(x, y) > dmin.at(x, y); }
-    void set_pixel(int x, int y, int lo, int hi) {
-        lo = std::max(lo, global_min);
-        hi = std::min(hi, global_max);
-        if (hi < lo) hi = lo;
-        dmin.at(x, y) = static_cast<int16_t>(lo);
-        dmax.at(x, y) = static_cast<int16_t>(hi);
-    }
-    void release() {
-        dmin.release();
-        dmax.release();
-        global_min = global_max = 0;
-    }
-    size_t bytes() const { return dmin.bytes() + dmax.bytes(); }
-};
 
 template <typename T>
 class CostVolumeT {
@@ -80,8 +53,7 @@ private:
 using CostVolume = CostVolumeT<uint16_t>;
 using CostVolume32 = CostVolumeT<uint32_t>;
 
-// Dense-path subset of the reference's PipelineBuffers (the packed
-// HighQuality volumes are not part of this build's host interface).
+// The reference's PipelineBuffers (reference include/apg_sgm/buffers.hpp:66-160).
 struct PipelineBuffers {
     Image8 left, right;
     Image8 left_gray, right_gray;
@@ -94,6 +66,9 @@ struct PipelineBuffers {
     Image32f d_prior;
     CostVolume cost;
     CostVolume32 cost32;
+    PackedCostVolume16 packed_cost;
+    PackedCostVolume16 packed_sgm16;
+    PackedCostVolume32 packed_cost32;
     Image32f disparity;
     Image32f d_before_refine;
     Image32f d_after_refine;
@@ -111,6 +86,7 @@ struct PipelineBuffers {
         t += census_left64.capacity() * 8 + census_right64.capacity() * 8;
         t += range.bytes() + supports.capacity() * sizeof(SupportMatch) + d_prior.bytes();
         t += cost.capacity_bytes() + cost32.capacity_bytes();
+        t += packed_cost.allocated_bytes() + packed_sgm16.allocated_bytes() + packed_cost32.allocated_bytes();
         t += disparity.bytes() + d_before_refine.bytes() + d_after_refine.bytes();
         t += disparity_right.bytes() + confidence.bytes() + invalid_reason.bytes() + reliable_mask.bytes();
         return t;

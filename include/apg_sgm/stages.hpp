@@ -8,7 +8,10 @@
 // They keep the reference's host-buffer contract (PipelineBuffers in, volumes
 // out on the host), so every call uploads its inputs, runs the device
 // kernels on the calling thread's default device context and downloads its
-// outputs.  They are the parity boundary; throughput is measured at the
+// outputs.  The packed (ragged) methods run the dense kernels over the
+// layout's per-pixel ranges and pack the result, which the reference's own
+// dense == packed tests show to be the same values (reference
+// tests/test_sanity.cpp:83-476).  They are the parity boundary; throughput is measured at the
 // session / C-ABI level where volumes stay in HBM.  Failures throw
 // std::runtime_error / std::bad_alloc like the reference's stages.
 #pragma once
@@ -31,6 +34,10 @@ public:
                      PipelineBuffers& buf) const;
     void compute_aux(const Image8& left, const Image8& right, const PipelineConfig& cfg, PipelineBuffers& buf) const;
     void compute_volume(const PipelineConfig& cfg, PipelineBuffers& buf) const;
+    // ragged volume over packed_cost's layout (allocated from buf.range when
+    // it has none); reference src/cost_computer.cpp:205-268
+    void compute_volume_packed(const PipelineConfig& cfg, const PipelineBuffers& buf,
+                               PackedCostVolume16& packed_cost) const;
 
     static uint32_t symmetric_census9x7(const Image8& gray, int x, int y);
     static uint64_t census9x7(const Image8& gray, int x, int y);
@@ -47,6 +54,8 @@ public:
 class CostAggregator {
 public:
     void aggregate(const PipelineConfig& cfg, PipelineBuffers& buf) const;
+    // reference src/cost_aggregator.cpp:151-403
+    void aggregate_packed(const PipelineConfig& cfg, const Image8& left_gray, PackedCostVolume16& packed_cost) const;
 };
 
 class SgmOptimizer {
@@ -58,6 +67,20 @@ public:
                          Image32f& disp_out, Image32f* best_cost = nullptr, Image32f* second_cost = nullptr) const;
     void winner_take_all(const PipelineConfig& cfg, const CostVolume& vol, const SearchRange& range,
                          Image32f& disp_out, Image32f* best_cost = nullptr, Image32f* second_cost = nullptr) const;
+    // ragged forms over the cost volume's layout (reference
+    // src/sgm_optimizer.cpp:83-182, 276-404, 557-710); the outputs take that layout
+    void optimize_packed(const PipelineConfig& cfg, const Image8& gray, const PackedCostVolume16& packed_cost,
+                         PackedCostVolume16& packed_sgm16) const;
+    void optimize_packed(const PipelineConfig& cfg, const Image8& gray, const PackedCostVolume16& packed_cost,
+                         PackedCostVolume32& packed_cost32) const;
+    void winner_take_all_packed(const PipelineConfig& cfg, const PackedCostVolume32& vol, Image32f& disp_out,
+                                Image32f* best_cost = nullptr, Image32f* second_cost = nullptr) const;
+    void winner_take_all_packed(const PipelineConfig& cfg, const PackedCostVolume16& vol, Image32f& disp_out,
+                                Image32f* best_cost = nullptr, Image32f* second_cost = nullptr) const;
+    void aggregate_path_packed(const PipelineConfig& cfg, const Image8& gray, const PackedCostVolume16& base,
+                               PackedCostVolume16& acc, int dx, int dy) const;
+    void aggregate_path_packed(const PipelineConfig& cfg, const Image8& gray, const PackedCostVolume16& base,
+                               PackedCostVolume32& acc, int dx, int dy) const;
 };
 
 class ConfidenceEstimator {

@@ -114,3 +114,47 @@ def test_session_c_abi_matches_oracle_and_fingerprints():
     lib.apg_session_destroy(s)
     lib.apg_session_destroy(opts_s)
     lib.apg_result_destroy(r)
+
+
+def _pack(dense, dmin, dmax, d0):
+    """Dense [h][w][D] over [d0, d0 + D) -> ragged states in raster order."""
+    h, w, D = dense.shape
+    out = []
+    for q in range(h * w):
+        y, x = divmod(q, w)
+        lo, hi = int(dmin[y, x]), int(dmax[y, x])
+        if hi > lo:
+            out.append(dense[y, x, lo - d0:hi - d0])
+    return np.concatenate(out) if out else np.empty(0, dense.dtype)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("w,h,dmax", [(96, 64, 24), (130, 70, 40)])
+def test_packed_stage_classes_match_oracle(tmp_path, w, h, dmax):
+    # HighQuality ranges from the prior; the oracle's dense volumes over the
+    # same ranges, packed, are what the reference's packed methods compute
+    left, right, _ = A.synth_pair(2, w, h, dmax, 55)
+    lp, rp = tmp_path / "l.raw", tmp_path / "r.raw"
+    left.tofile(lp)
+    right.tofile(rp)
+    print(f"packed stage classes {w}x{h} D={dmax}", flush=True)
+    out = subprocess.run([os.path.join(BUILD, "packed_classes_main"), str(lp), str(rp), str(w), str(h), str(dmax),
+                          str(tmp_path)], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    c = O.cfg(2, dmax)
+    o = O.pipeline(c, left, right, intermediates=True)
+    rd = lambda name, dt: np.fromfile(tmp_path / name, dt)  # noqa: E731
+    dmn, dmx = rd("dmin.bin", np.int16).reshape(h, w), rd("dmax.bin", np.int16).reshape(h, w)
+    np.testing.assert_array_equal(dmn, o["dmin"])
+    np.testing.assert_array_equal(dmx, o["dmax"])
+    np.testing.assert_array_equal(rd("cost.bin", np.uint16), _pack(o["cost"], dmn, dmx, 0))
+    np.testing.assert_array_equal(rd("cost_agg.bin", np.uint16), _pack(o["cost_agg"], dmn, dmx, 0))
+    s32 = _pack(o["sgm"], dmn, dmx, 0)
+    np.testing.assert_array_equal(rd("sgm32.bin", np.uint32), s32)
+    np.testing.assert_array_equal(rd("sgm16.bin", np.uint16), np.where(s32 == 0xFFFFFFFF, 0xFFFF, s32).astype(np.uint16))
+    for k, f in (("disp_wta", "disp.bin"), ("disp_wta", "disp16.bin"), ("best", "best.bin"), ("second", "second.bin")):
+        np.testing.assert_array_equal(rd(f, np.float32).reshape(h, w), o[k], err_msg=f)
+    # one left-to-right path over the raw ragged cost into a zeroed u32 accumulator
+    g, gx, gy, cen = O.aux(c, left)
+    acc = O.sgm_path(c, g, o["cost"], np.zeros(o["cost"].shape, np.uint32), 1, 0)
+    np.testing.assert_array_equal(rd("path.bin", np.uint32), _pack(acc, dmn, dmx, 0))
