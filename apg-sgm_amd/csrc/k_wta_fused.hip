@@ -43,7 +43,11 @@ __device__ __forceinline__ uint32_t s_pair(uint32_t ch, const uint32_t (&ph)[NP]
 // flight, all planes) and splits a chunk's (pixel, d) only for the LDS store.
 constexpr int WF_WG = 4;
 
-template <typename TC, int NP, bool WIDE>
+// LP lanes per pixel (1 or 2): with 2, a block holds 32 pixels, each lane
+// scans half of its pixel's row and the two halves are combined by a DPP
+// swap of adjacent lanes; half the LDS per wave, so twice the waves per CU
+// keep loads in flight while others scan.
+template <typename TC, int NP, bool WIDE, int LP>
 __global__ __launch_bounds__(64) void wta_fused_kernel(apg_hip_params p, int w, int h, int D, int rw,
                                                        const TC* __restrict__ cost, PlaneSet planes,
                                                        const int16_t* __restrict__ dmin,
@@ -53,13 +57,15 @@ __global__ __launch_bounds__(64) void wta_fused_kernel(apg_hip_params p, int w, 
                                                        uint8_t* __restrict__ reason, uint8_t* __restrict__ reliable,
                                                        float* __restrict__ disp_wta, float* __restrict__ best_out,
                                                        float* __restrict__ second_out, uint32_t* __restrict__ sum32) {
-    extern __shared__ uint32_t st[];   // [64][rw] words
+    extern __shared__ uint32_t st[];   // [PB][rw] words
+    constexpr int PB = WF_PB / LP;     // pixels per block
     const int lane = threadIdx.x;
+    const int pi = lane / LP, hf = lane % LP;
     const size_t npx = (size_t)w * h;
-    const size_t p_base = (size_t)blockIdx.x * WF_PB;
-    const int npb = (int)min((size_t)WF_PB, npx - p_base);
-    const size_t pix = p_base + lane;
-    const bool act = lane < npb;
+    const size_t p_base = (size_t)blockIdx.x * PB;
+    const int npb = (int)min((size_t)PB, npx - p_base);
+    const size_t pix = p_base + pi;
+    const bool act = pi < npb;
     const int d0 = p.min_disparity;
     const int nw = D / 2;   // words per row (D even)
     const auto rc = raw_rsrc(cost);
@@ -132,16 +138,18 @@ __global__ __launch_bounds__(64) void wta_fused_kernel(apg_hip_params p, int w, 
     __syncthreads();
     if (!act) return;
     const int lo = dmin[pix], hi = dmax[pix];
-    uint32_t* srow = st + lane * rw;
+    uint32_t* srow = st + pi * rw;
+    const int j0 = hf * (nw / LP), j1 = j0 + nw / LP;   // this lane's words of the row
     // pass 1: earliest minimum as the key S << 8 | d-index
     uint32_t best = 0xFFFFFFFFu;
-    for (int j = 0; j < nw; ++j) {
+    for (int j = j0; j < j1; ++j) {
         const uint32_t v = srow[j];
         const uint32_t idx = (uint32_t)(2 * j) | ((uint32_t)(2 * j + 1) << 8);
         const uint32_t kl = __builtin_amdgcn_perm(v, idx, 0x0c050400u);
         const uint32_t kh = __builtin_amdgcn_perm(v, idx, 0x0c070601u);
         best = min(best, min(kl, kh));   // v_min3_u32
     }
+    if (LP == 2) best = min(best, (uint32_t)__builtin_amdgcn_mov_dpp((int)best, 0xB1, 0xF, 0xF, false));
     const uint32_t bv = best >> 8;
     const int bi = (int)(best & 0xFFu);
     // as in the reference, a negative winning disparity counts as no match
@@ -157,9 +165,13 @@ __global__ __launch_bounds__(64) void wta_fused_kernel(apg_hip_params p, int w, 
         if (bi > 0) w16[bi - 1] = 0xFFFFu;
         if (bi + 1 < D) w16[bi + 1] = 0xFFFFu;
     }
-    // pass 2: second best outside best-1..best+1
+    // pass 2: second best outside best-1..best+1 (both lanes of a pixel
+    // poisoned the same words, and LDS operations of a wave are in order)
     uint32_t m2 = 0xFFFFFFFFu;
-    for (int j = 0; j < nw; ++j) m2 = as_u32(__builtin_elementwise_min(as_v2(m2), as_v2(srow[j])));
+    for (int j = j0; j < j1; ++j) m2 = as_u32(__builtin_elementwise_min(as_v2(m2), as_v2(srow[j])));
+    if (LP == 2)
+        m2 = as_u32(__builtin_elementwise_min(as_v2(m2), as_v2((uint32_t)__builtin_amdgcn_mov_dpp((int)m2, 0xB1, 0xF,
+                                                                                                 0xF, false))));
     const uint32_t sec = min(m2 & 0xFFFFu, m2 >> 16);
     float dv = -1.f, b = INFINITY, sc = INFINITY;
     if (found) {
@@ -172,6 +184,7 @@ __global__ __launch_bounds__(64) void wta_fused_kernel(apg_hip_params p, int w, 
         b = (float)bv;
         sc = sec != 0xFFFFu ? (float)sec : INFINITY;
     }
+    if (hf != 0) return;   // the pixel's first lane writes its outputs
     if (disp_wta) disp_wta[pix] = dv;
     if (best_out) best_out[pix] = b;
     if (second_out) second_out[pix] = sc;
@@ -227,13 +240,15 @@ hipError_t launch_wta_fused(hipStream_t s, const apg_hip_params& p, int w, int h
     const int D = p.max_disparity - p.min_disparity;
     if (D % 2 || D > 256) return hipErrorInvalidValue;
     const int rw = (D / 2) | 1;   // odd row stride in words
-    const size_t shm = (size_t)WF_PB * rw * 4;
     const size_t npx = (size_t)w * h;
-    dim3 grid((unsigned)((npx + WF_PB - 1) / WF_PB));
     const bool wide = cost_u8 && D % 16 == 0 && !sum32;
+    const int lp = wide ? 2 : 1;
+    const size_t shm = (size_t)(WF_PB / lp) * rw * 4;
+    dim3 grid((unsigned)((npx + WF_PB / lp - 1) / (WF_PB / lp)));
 #define APG_WTAF(TC, NP, WD)                                                                                   \
-    hipLaunchKernelGGL((wta_fused_kernel<TC, NP, WD>), grid, dim3(64), shm, s, p, w, h, D, rw, (const TC*)cost, \
-                       planes, dmin, dmax, dr, gx, gy, disp, conf, reason, reliable, disp_wta, best, second, sum32)
+    hipLaunchKernelGGL((wta_fused_kernel<TC, NP, WD, (WD ? 2 : 1)>), grid, dim3(64), shm, s, p, w, h, D, rw,         \
+                       (const TC*)cost, planes, dmin, dmax, dr, gx, gy, disp, conf, reason, reliable, disp_wta, best,   \
+                       second, sum32)
     if (p.paths == 8) {
         if (wide) APG_WTAF(uint8_t, 8, true);
         else if (cost_u8) APG_WTAF(uint8_t, 8, false);
