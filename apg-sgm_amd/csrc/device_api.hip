@@ -672,12 +672,14 @@ int apg_hip_pipeline(apg_hip_ctx ctx, const apg_hip_params* p, const uint8_t* d_
     float* scratch = buf<float>(ctx, B_SCRATCH);
     const bool sym = p->census_type == 1;
 
+    // the fused schedule's right-view WTA keys start cleared, by the aux pass
+    uint32_t* rkeys = fused ? buf<uint32_t>(ctx, B_RKEYS) : nullptr;
     mark(ctx, S_AUX);
     HIPCHK(launch_aux(st, *p, d_left, w, h, stride, channels, gl, gxl, gyl, sym ? (uint32_t*)cl : nullptr,
                       sym ? nullptr : (uint64_t*)cl),
            "aux L");
     HIPCHK(launch_aux(st, *p, d_right, w, h, stride, channels, gr, gxr, gyr, sym ? (uint32_t*)cr : nullptr,
-                      sym ? nullptr : (uint64_t*)cr),
+                      sym ? nullptr : (uint64_t*)cr, rkeys),
            "aux R");
     mark(ctx, S_PRIOR);
     HIPCHK(launch_range_init(st, *p, w, h, dmin, dmax), "range");
@@ -695,11 +697,9 @@ int apg_hip_pipeline(apg_hip_ctx ctx, const apg_hip_params* p, const uint8_t* d_
     if (fused) {
         PlaneSet planes{};
         for (int r = 0; r < npaths; ++r) planes.p[r] = buf<uint8_t>(ctx, (BufId)(B_DS0 + r));
-        uint32_t* rkeys = buf<uint32_t>(ctx, B_RKEYS);
         void* cp = ctx->bufs[B_COST].ptr;   // C' (u8 or u16)
         void* tt = ctx->bufs[B_TMP].ptr;    // T  (u8 or u16)
         if (p->agg_enable) HIPCHK(launch_cross_arms(st, *p, gl, w, h, arms), "arms");
-        HIPCHK(launch_fill_u32(st, n, 0xFFFFFFFFu, rkeys), "keys");
         mark(ctx, S_COST);
         HIPCHK(launch_cost_crossh(st, *p, cur_lut(ctx), w, h, gl, gxl, gyl, gr, gxr, gyr,
                                   sym ? (const uint32_t*)cl : nullptr, sym ? (const uint32_t*)cr : nullptr,

@@ -66,7 +66,8 @@ constexpr int AUX_TX = 64, AUX_TY = 4;
 __global__ __launch_bounds__(256) void aux_kernel(const uint8_t* __restrict__ src, int w, int h, int stride, int ch,
                                                  int census_type, uint8_t* __restrict__ gray,
                                                  uint8_t* __restrict__ gx, uint8_t* __restrict__ gy,
-                                                 uint32_t* __restrict__ c32, uint64_t* __restrict__ c64) {
+                                                 uint32_t* __restrict__ c32, uint64_t* __restrict__ c64,
+                                                 uint32_t* __restrict__ keys) {
     __shared__ uint8_t tile[AUX_TY + 6][AUX_TX + 8];
     const int x0 = blockIdx.x * AUX_TX, y0 = blockIdx.y * AUX_TY;
     const int tid = threadIdx.y * AUX_TX + threadIdx.x;
@@ -89,6 +90,8 @@ __global__ __launch_bounds__(256) void aux_kernel(const uint8_t* __restrict__ sr
     const int gyv = (int)tile[cy + 1][cx] - (int)tile[cy - 1][cx];
     gx[pi] = (uint8_t)min(abs(gxv), 255);
     gy[pi] = (uint8_t)min(abs(gyv), 255);
+    // the fused schedule's right-view WTA key, cleared here to save a fill pass
+    if (keys) keys[pi] = 0xFFFFFFFFu;
     if (census_type == 1) {
         uint32_t bits = 0;
         int b = 0;
@@ -118,9 +121,11 @@ __global__ __launch_bounds__(256) void aux_kernel(const uint8_t* __restrict__ sr
 }
 
 hipError_t launch_aux(hipStream_t s, const apg_hip_params& p, const uint8_t* src, int w, int h, int stride,
-                      int ch, uint8_t* gray, uint8_t* gx, uint8_t* gy, uint32_t* c32, uint64_t* c64) {
+                      int ch, uint8_t* gray, uint8_t* gx, uint8_t* gy, uint32_t* c32, uint64_t* c64,
+                      uint32_t* keys) {
     dim3 grid((w + AUX_TX - 1) / AUX_TX, (h + AUX_TY - 1) / AUX_TY), block(AUX_TX, AUX_TY);
-    hipLaunchKernelGGL(aux_kernel, grid, block, 0, s, src, w, h, stride, ch, p.census_type, gray, gx, gy, c32, c64);
+    hipLaunchKernelGGL(aux_kernel, grid, block, 0, s, src, w, h, stride, ch, p.census_type, gray, gx, gy, c32, c64,
+                       keys);
     return hipGetLastError();
 }
 
