@@ -12,6 +12,7 @@
 // buffer instructions with scalar per-step offsets; the C' words of the next
 // SL_PF steps of both chains are in flight while the current ones are used.
 #include <type_traits>
+#include <utility>
 
 #include "device_common.hpp"
 #include "kernels.hpp"
@@ -427,6 +428,190 @@ __global__ __launch_bounds__(64) void sgm_lines4_kernel(apg_hip_params p, int w,
     }
 }
 
+// Half-wave form for u8 C' with D % 16 == 0, D <= 128: one wave runs both
+// chains of its scanline, chain A (forward) in lanes 0-31 and chain B
+// (backward) in lanes 32-63.  Half-lane l holds the disparities 4l..4l+3 as
+// two packed u16 pairs, so the pair (4l+1, 4l+2) is the +1 neighbour of the
+// first pair and the -1 neighbour of the second, and each half reduces its
+// minimum with four in-row rotations and one row swap (v_permlane16_swap):
+// about 31 VALU per step of two chains against about 25 per step of one
+// chain in the 64-lane form.  minL' and the per-step P2 live in VGPRs as
+// u16 pairs (one value per half), so the recurrence needs no readlane.
+// The C' rows of PF batches are in flight per wave (register slot k % PF
+// holds batch k); the loop runs PF batches per trip so every slot index is
+// static.
+template <int N, typename F, int... I>
+__device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, I...>) {
+    (f(std::integral_constant<int, I>{}), ...);
+}
+// f(integral_constant<int, 0>) ... f(integral_constant<int, N-1>)
+template <int N, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+    static_for_impl<N>(f, std::make_integer_sequence<int, N>{});
+}
+
+template <bool FULL, int PF>
+__global__ __launch_bounds__(64) void sgm_half_kernel(apg_hip_params p, int w, int h, int D, int nfam,
+                                                      const uint8_t* __restrict__ gray,
+                                                      const uint8_t* __restrict__ cost, PlaneSet planes, int nmax) {
+    extern __shared__ __align__(16) uint8_t sh[];
+    constexpr int RB = 128;                  // staged row bytes (>= D)
+    uint8_t* s_c = sh;                       // [chain][buffer][SB][RB] C' rows
+    uint8_t* s_d = sh + 4 * SB * RB;         // [chain][buffer][SB][RB] delta rows
+    uint32_t* s_p2 = (uint32_t*)(sh + 8 * SB * RB);   // [chain][nmax] P2 | P2 << 16 per step
+    const int lane = threadIdx.x;
+    const int hf = lane >> 5, hl = lane & 31;
+    int b = blockIdx.x, f = 0;
+    {
+        const int cnt[4] = {h, w, w + h - 1, w + h - 1};
+        while (f < nfam - 1 && b >= cnt[f]) { b -= cnt[f]; ++f; }
+    }
+    int p0, step, n;
+    family_line(f, b, w, h, p0, step, n);
+    const auto rc = raw_rsrc(cost);
+    const int pA0 = p0, pB0 = p0 + (n - 1) * step;
+    const int xlo = p.min_disparity + D - 1, xhi = w - 1 + p.min_disparity;
+    const int dxA = f == 1 ? 0 : 1;
+    const int xA0 = pA0 % w, xB0 = pB0 % w;
+    auto batch_valid = [&](int k) {
+        const int t0 = k * SB, t1 = min(k * SB + SB, n) - 1;
+        const int a0 = xA0 + dxA * t0, a1 = xA0 + dxA * t1;
+        const int b0 = xB0 - dxA * t0, b1 = xB0 - dxA * t1;
+        return min(a0, a1) >= xlo && max(a0, a1) <= xhi && min(b0, b1) >= xlo && max(b0, b1) <= xhi;
+    };
+    const auto rpa = raw_rsrc(planes.p[2 * f]);
+    const auto rpb = raw_rsrc(planes.p[2 * f + 1]);
+    // P2 of every step of both chains: chain A at step t is pixel t after t-1,
+    // chain B pixel n-1-t after n-t (the first step of each compares a pixel
+    // with itself)
+    for (int t = lane; t < n; t += 64) {
+        const int ga = gray[pA0 + t * step], gpa = gray[pA0 + max(t - 1, 0) * step];
+        const int u = n - 1 - t;
+        const int gb = gray[pA0 + u * step], gpb = gray[pA0 + min(u + 1, n - 1) * step];
+        const uint32_t a = (uint32_t)adaptive_p2(p, ga - gpa), bb = (uint32_t)adaptive_p2(p, gb - gpb);
+        s_p2[t] = a | (a << 16);
+        s_p2[nmax + t] = bb | (bb << 16);
+    }
+    // batch transfer: lane -> (row j, 16-byte chunk c); one instruction per chain covers SB rows
+    const int lpx = D / 16, ppl = 64 / lpx;
+    const int jl = lane / lpx, cl = lane % lpx;
+    const bool inr = FULL || 4 * hl < D;
+    const uint32_t P1 = (uint32_t)p.P1;
+    const uint32_t p1s = P1 | (P1 << 16);
+    // half-lane 0's d-1 and half-lane 31's d+1 lie outside [0, 128)
+    const uint32_t p1l = hl == 0 ? (0xFFFFu | (P1 << 16)) : p1s;
+    const uint32_t p1r = hl == 31 ? (P1 | 0xFFFF0000u) : p1s;
+    uint32_t prev0 = 0xFFFFFFFFu, prev1 = 0xFFFFFFFFu, mpp = 0xFFFFFFFFu;
+    auto voffA = [&](int k, int j) { return (uint32_t)(pA0 + min(k * SB + j, n - 1) * step) * (uint32_t)D + cl * 16u; };
+    auto voffB = [&](int k, int j) { return (uint32_t)(pB0 - min(k * SB + j, n - 1) * step) * (uint32_t)D + cl * 16u; };
+    // batch k's rows arrive in register slot k % PF, PF batches ahead of use
+    u32x4 gA[PF], gB[PF];
+    auto gload = [&](int k, auto slot_t) {
+        constexpr int q = decltype(slot_t)::value;
+        const int j = min(jl, SB - 1);
+        gA[q] = buf_ld128(rc, voffA(k, j), 0u);
+        gB[q] = buf_ld128(rc, voffB(k, j), 0u);
+    };
+    auto lstage = [&](int k, auto slot_t) {   // batch k's rows -> LDS buffer k & 1
+        constexpr int q = decltype(slot_t)::value;
+        const int bf = k & 1;
+        if (jl < SB) {
+            *(u32x4*)(s_c + ((0 * 2 + bf) * SB + jl) * RB + cl * 16) = gA[q];
+            *(u32x4*)(s_c + ((1 * 2 + bf) * SB + jl) * RB + cl * 16) = gB[q];
+        }
+    };
+    auto flush = [&](int k) {    // deltas of batch k (LDS buffer k & 1) -> planes
+        const int bf = k & 1;
+        if (jl < SB && k * SB + jl < n) {
+            const u32x4 va = *(const u32x4*)(s_d + ((0 * 2 + bf) * SB + jl) * RB + cl * 16);
+            const u32x4 vb = *(const u32x4*)(s_d + ((1 * 2 + bf) * SB + jl) * RB + cl * 16);
+            __builtin_amdgcn_raw_buffer_store_b128(va, rpa, (int)voffA(k, jl), 0, 0);
+            __builtin_amdgcn_raw_buffer_store_b128(vb, rpb, (int)voffB(k, jl), 0, 0);
+        }
+    };
+    // the lane's C' bytes (4 d of its chain) of step i of batch k, and the P2 pair
+    const uint32_t lofs = (uint32_t)(hf * 2 * SB * RB + 4 * hl);
+    auto cword = [&](int k, int i) -> uint32_t {
+        return *(const uint32_t*)(s_c + lofs + ((k & 1) * SB + i) * RB);
+    };
+    const uint32_t* p2h = s_p2 + hf * nmax;
+    auto p2word = [&](int t) -> uint32_t { return p2h[min(t, n - 1)]; };
+    const int nb = (n + SB - 1) / SB;
+    // loads are issued unconditionally (past the end: the last batch again), so
+    // the wait before staging a batch counts the loads issued after it and
+    // leaves them in flight
+    static_for<PF>([&](auto q_t) { gload(min((int)decltype(q_t)::value, nb - 1), q_t); });
+    lstage(0, std::integral_constant<int, 0>{});
+    __syncthreads();
+    uint32_t nC = cword(0, 0), nP = p2word(0);
+    // INNER: batch k is not the last one (all SB steps exist, batch k + 1 exists)
+    auto batch = [&](int k, auto q_t, auto inner_t) {
+        constexpr int q = decltype(q_t)::value;
+        constexpr bool INNER = decltype(inner_t)::value;
+        // staged rows first: the stores and loads issued after them are newer,
+        // so waiting for the rows never waits for a store
+        if (INNER || k + 1 < nb) lstage(k + 1, std::integral_constant<int, (q + 1) % PF>{});
+        if (k > 0) flush(k - 1);
+        gload(min(k + PF, nb - 1), q_t);
+        const int bf = k & 1;
+        auto run_batch = [&](auto allv_tag) {
+            constexpr bool ALLV = decltype(allv_tag)::value;
+#pragma unroll
+            for (int i = 0; i < SB; ++i) {
+                const int t = k * SB + i;
+                if (!INNER && t >= n) break;
+                const uint32_t wc = nC, p2p = nP;
+                nC = i + 1 < SB ? cword(k, i + 1) : cword(k + 1, 0);
+                nP = p2word(t + 1);
+                CRaw<uint8_t, 2> x;
+                x.w[0] = wc;
+                uint32_t c0 = decode_c<uint8_t, 2, ALLV>(x, 0), c1 = decode_c<uint8_t, 2, ALLV>(x, 1);
+                if (!FULL && !inr) c0 = c1 = 0xFFFFFFFFu;
+                // L = C + min(L'(d), L'(d-1)+P1, L'(d+1)+P1, minL'+P2) - minL'
+                const u16x2 mp2 = add_sat2(as_v2(mpp), as_v2(p2p));
+                const uint32_t left_hi = (uint32_t)__builtin_amdgcn_mov_dpp((int)prev1, kDppWaveShr1, 0xF, 0xF, true);
+                const uint32_t right_lo = (uint32_t)__builtin_amdgcn_mov_dpp((int)prev0, kDppWaveShl1, 0xF, 0xF, true);
+                const u16x2 sl = add_sat2(as_v2(__builtin_amdgcn_alignbit(prev0, left_hi, 16)), as_v2(p1l));
+                const u16x2 sm = add_sat2(as_v2(__builtin_amdgcn_alignbit(prev1, prev0, 16)), as_v2(p1s));
+                const u16x2 sr = add_sat2(as_v2(__builtin_amdgcn_alignbit(right_lo, prev1, 16)), as_v2(p1r));
+                const u16x2 t0 = min2(min2(min2(sl, sm), as_v2(prev0)), mp2);
+                const u16x2 t1 = min2(min2(min2(sm, sr), as_v2(prev1)), mp2);
+                const u16x2 d0 = t0 - as_v2(mpp), d1 = t1 - as_v2(mpp);
+                prev0 = as_u32(add_sat2(as_v2(c0), d0));
+                prev1 = as_u32(add_sat2(as_v2(c1), d1));
+                const uint32_t lm2 = as_u32(min2(as_v2(prev0), as_v2(prev1)));
+                uint32_t m = min(lm2 & 0xFFFFu, lm2 >> 16);
+                // minimum over the half: rotations within each 16-lane row, then
+                // rows 0/1 and 2/3 exchanged
+                m = min(m, dpp<kDppRowRor8>(0xFFFFFFFFu, m));
+                m = min(m, dpp<kDppRowRor4>(0xFFFFFFFFu, m));
+                m = min(m, dpp<kDppRowRor2>(0xFFFFFFFFu, m));
+                m = min(m, dpp<kDppRowRor1>(0xFFFFFFFFu, m));
+                const auto sw = __builtin_amdgcn_permlane16_swap(m, m, false, false);
+                m = min((uint32_t)sw[0], (uint32_t)sw[1]);
+                mpp = m | (m << 16);
+                *(uint32_t*)(s_d + lofs + (bf * SB + i) * RB) =
+                    __builtin_amdgcn_perm(as_u32(d1), as_u32(d0), 0x06040200u);
+            }
+        };
+        if (!INNER || batch_valid(k)) run_batch(std::integral_constant<bool, INNER>{});
+        else run_batch(std::false_type{});
+    };
+    // groups of PF inner batches (batch k in slot k % PF), then the rest
+    int k0 = 0;
+    for (; k0 + PF < nb; k0 += PF)
+        static_for<PF>([&](auto q_t) { batch(k0 + decltype(q_t)::value, q_t, std::true_type{}); });
+    static_for<PF>([&](auto q_t) {
+        const int k = k0 + decltype(q_t)::value;
+        if (k < nb) batch(k, q_t, std::false_type{});
+    });
+    flush(nb - 1);
+}
+
+// batches of C' rows loaded ahead per wave: 3 measured 242 us against 262 us
+// for 2 at 1280x720 D=128 (4 and 6 no better)
+constexpr int kHalfPF = 3;
+
 template <typename TC>
 static hipError_t launch_lines_t(hipStream_t s, const apg_hip_params& p, int w, int h, int D, const uint8_t* gray,
                                  const TC* cost, const PlaneSet& planes, const KTune& tn) {
@@ -435,11 +620,29 @@ static hipError_t launch_lines_t(hipStream_t s, const apg_hip_params& p, int w, 
     const int nblk = h + w + (nfam == 4 ? 2 * (w + h - 1) : 0);
     const bool full = D == 128 * K;
     const size_t shm = (size_t)(w > h ? w : h);   // one scanline of gray values
+    if (sizeof(TC) == 1 && D % 16 == 0 && K == 1 && tn.sgm_dual == 0) {
+        // both chains of a scanline per wave, 32 lanes each; P2 of every step staged
+        const int nmax = w > h ? w : h;
+        const size_t shmh = (size_t)8 * SB * 128 + (size_t)2 * nmax * 4;
+        if (shmh > 64 * 1024) {
+            hipError_t e = hipFuncSetAttribute(full ? (const void*)sgm_half_kernel<true, kHalfPF>
+                                                    : (const void*)sgm_half_kernel<false, kHalfPF>,
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmh);
+            if (e != hipSuccess) return e;
+        }
+        if (full)
+            hipLaunchKernelGGL((sgm_half_kernel<true, kHalfPF>), dim3(nblk), dim3(64), shmh, s, p, w, h, D, nfam, gray,
+                               (const uint8_t*)cost, planes, nmax);
+        else
+            hipLaunchKernelGGL((sgm_half_kernel<false, kHalfPF>), dim3(nblk), dim3(64), shmh, s, p, w, h, D, nfam, gray,
+                               (const uint8_t*)cost, planes, nmax);
+        return hipGetLastError();
+    }
     if (sizeof(TC) == 1 && D % 16 == 0 && K <= 2) {
         const size_t shm4 = (size_t)8 * SB * 128 * K + shm;
         const int nblk4 = 2 * nblk;
 #define APG_LINES4(KK, FF)                                                                                        \
-    if (tn.sgm_dual)                                                                                              \
+    if (tn.sgm_dual == 1)                                                                                         \
         hipLaunchKernelGGL((sgm_lines4_kernel<KK, FF, true>), dim3(nblk), dim3(64), shm4, s, p, w, h, D, nfam,     \
                            gray, (const uint8_t*)cost, planes);                                                   \
     else                                                                                                          \

@@ -282,7 +282,10 @@ APG_HIP_API size_t apg_hip_pipeline_workspace_bytes(const apg_hip_params* p, int
  * stages have no launch shapes). */
 typedef struct {
     int cost_segment;    /* output columns per wave of the cost + Cross-H pass: 32..1024 (64) */
-    int sgm_dual;        /* 1: one wave runs both directions of a scanline (0) */
+    int sgm_dual;        /* SGM wave shape: 0 built-in (D <= 128: both directions of a
+                            scanline per wave, 32 lanes each; else one direction per
+                            wave), 1 both directions over 64 lanes each, 2 one
+                            direction per wave */
     int cross_v_waves;   /* target wave count of the Cross-V pass: 256..65536 (4096) */
 } apg_hip_tuning;
 
