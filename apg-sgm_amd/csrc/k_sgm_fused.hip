@@ -492,6 +492,11 @@ __global__ __launch_bounds__(64) void sgm_half_kernel(apg_hip_params p, int w, i
         s_p2[t] = a | (a << 16);
         s_p2[nmax + t] = bb | (bb << 16);
     }
+    // the P2 words past the last step are read with their batch (never used)
+    for (int t = n + lane; t < ((n + SB - 1) / SB) * SB; t += 64) {
+        s_p2[t] = 0u;
+        s_p2[nmax + t] = 0u;
+    }
     // batch transfer: lane -> (row j, 16-byte chunk c); one instruction per chain covers SB rows
     const int lpx = D / 16, ppl = 64 / lpx;
     const int jl = lane / lpx, cl = lane % lpx;
@@ -529,13 +534,24 @@ __global__ __launch_bounds__(64) void sgm_half_kernel(apg_hip_params p, int w, i
             __builtin_amdgcn_raw_buffer_store_b128(vb, rpb, (int)voffB(k, jl), 0, 0);
         }
     };
-    // the lane's C' bytes (4 d of its chain) of step i of batch k, and the P2 pair
+    // the lane's C' words (4 d of its chain) and P2 pairs of all SB steps of a
+    // batch are read into register set k & 1 one batch ahead, so no LDS
+    // latency lies on the recurrence
     const uint32_t lofs = (uint32_t)(hf * 2 * SB * RB + 4 * hl);
-    auto cword = [&](int k, int i) -> uint32_t {
-        return *(const uint32_t*)(s_c + lofs + ((k & 1) * SB + i) * RB);
-    };
     const uint32_t* p2h = s_p2 + hf * nmax;
-    auto p2word = [&](int t) -> uint32_t { return p2h[min(t, n - 1)]; };
+    uint32_t W[2][SB], P[2][SB];
+    auto wread = [&](int k, auto set_t) {
+        constexpr int e = decltype(set_t)::value;
+#pragma unroll
+        for (int i = 0; i < SB; ++i) W[e][i] = *(const uint32_t*)(s_c + lofs + ((k & 1) * SB + i) * RB);
+        const u32x4* pp = (const u32x4*)(p2h + k * SB);
+        const u32x4 a = pp[0], b = pp[1];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            P[e][i] = a[i];
+            P[e][4 + i] = b[i];
+        }
+    };
     const int nb = (n + SB - 1) / SB;
     // loads are issued unconditionally (past the end: the last batch again), so
     // the wait before staging a batch counts the loads issued after it and
@@ -543,15 +559,20 @@ __global__ __launch_bounds__(64) void sgm_half_kernel(apg_hip_params p, int w, i
     static_for<PF>([&](auto q_t) { gload(min((int)decltype(q_t)::value, nb - 1), q_t); });
     lstage(0, std::integral_constant<int, 0>{});
     __syncthreads();
-    uint32_t nC = cword(0, 0), nP = p2word(0);
+    wread(0, std::integral_constant<int, 0>{});
     // INNER: batch k is not the last one (all SB steps exist, batch k + 1 exists)
     auto batch = [&](int k, auto q_t, auto inner_t) {
         constexpr int q = decltype(q_t)::value;
         constexpr bool INNER = decltype(inner_t)::value;
         // staged rows first: the stores and loads issued after them are newer,
         // so waiting for the rows never waits for a store
-        if (INNER || k + 1 < nb) lstage(k + 1, std::integral_constant<int, (q + 1) % PF>{});
-        if (k > 0) flush(k - 1);
+        if (INNER || k + 1 < nb) {
+            lstage(k + 1, std::integral_constant<int, (q + 1) % PF>{});
+            wread(k + 1, std::integral_constant<int, (q + 1) & 1>{});
+        }
+        // the previous batch's deltas: in an inner batch after half its steps, so
+        // the wait for the LDS reads of the flush finds the word reads done
+        if (!INNER && k > 0) flush(k - 1);
         gload(min(k + PF, nb - 1), q_t);
         const int bf = k & 1;
         auto run_batch = [&](auto allv_tag) {
@@ -560,9 +581,8 @@ __global__ __launch_bounds__(64) void sgm_half_kernel(apg_hip_params p, int w, i
             for (int i = 0; i < SB; ++i) {
                 const int t = k * SB + i;
                 if (!INNER && t >= n) break;
-                const uint32_t wc = nC, p2p = nP;
-                nC = i + 1 < SB ? cword(k, i + 1) : cword(k + 1, 0);
-                nP = p2word(t + 1);
+                if (INNER && i == SB / 2 && k > 0) flush(k - 1);
+                const uint32_t wc = W[q & 1][i], p2p = P[q & 1][i];
                 CRaw<uint8_t, 2> x;
                 x.w[0] = wc;
                 uint32_t c0 = decode_c<uint8_t, 2, ALLV>(x, 0), c1 = decode_c<uint8_t, 2, ALLV>(x, 1);
@@ -608,9 +628,10 @@ __global__ __launch_bounds__(64) void sgm_half_kernel(apg_hip_params p, int w, i
     flush(nb - 1);
 }
 
-// batches of C' rows loaded ahead per wave: 3 measured 242 us against 262 us
-// for 2 at 1280x720 D=128 (4 and 6 no better)
-constexpr int kHalfPF = 3;
+// batches of C' rows loaded ahead per wave: 3 or 4 measured 242 us against
+// 262 us for 2 at 1280x720 D=128 (6 no better); even, so the register set of
+// a batch's words (k & 1) is static within the PF-batch trip
+constexpr int kHalfPF = 4;
 
 template <typename TC>
 static hipError_t launch_lines_t(hipStream_t s, const apg_hip_params& p, int w, int h, int D, const uint8_t* gray,
@@ -622,7 +643,7 @@ static hipError_t launch_lines_t(hipStream_t s, const apg_hip_params& p, int w, 
     const size_t shm = (size_t)(w > h ? w : h);   // one scanline of gray values
     if (sizeof(TC) == 1 && D % 16 == 0 && K == 1 && tn.sgm_dual == 0) {
         // both chains of a scanline per wave, 32 lanes each; P2 of every step staged
-        const int nmax = w > h ? w : h;
+        const int nmax = ((w > h ? w : h) + SB - 1) / SB * SB;   // whole batches of P2 words
         const size_t shmh = (size_t)8 * SB * 128 + (size_t)2 * nmax * 4;
         if (shmh > 64 * 1024) {
             hipError_t e = hipFuncSetAttribute(full ? (const void*)sgm_half_kernel<true, kHalfPF>
