@@ -193,23 +193,32 @@ __global__ __launch_bounds__(256) void cost_crossh_kernel(
 // One wavefront walks a row segment of `seg` output columns plus the arm
 // halo, 128 disparities at a time (lane l: d-indices 2l, 2l+1).  Per column it
 // computes the two costs, adds them to u16-pair prefix sums kept in an LDS
-// ring of 2*Lmax+2 columns (window sums < 125*255 < 2^16), and Lmax columns
-// behind emits T of one output column.  With column ranges, C(xx, d) is valid
-// iff d <= xx < W + d, so the valid count of the window [x-L, x+R] is
-// |[max(x-L, d), min(x+R, W-1+d)]| and only sums need prefixes; for most
-// columns it is L+R+1 for every lane and the divisor is uniform.
+// ring of 2*Lmax+2 columns, and Lmax columns behind emits T of one output
+// column.  The prefix sums hold twice the cost sums (window sums < 125*255,
+// doubled < 2^16, exact modulo 2^16), which is the numerator the divisor's
+// magic takes.  With column ranges, C(xx, d) is valid iff d <= xx < W + d, so
+// the valid count of the window [x-L, x+R] is |[max(x-L, d), min(x+R, W-1+d)]|
+// and only sums need prefixes; once x - Lmax >= the chunk's largest d (and
+// d >= 0) the window is whole for every lane, its count L+R+1 and the
+// divisor uniform (the FAST phase).
 // floor(s / n) = umulhi(2s, ceil(2^31 / n)) (exact for s * n < 2^31).
+// Records are pre-formatted for the cost terms: g repeated in four bytes, so
+// v_sad_u8 of two of them is 4|dg|, the AD LUT's byte offset; 4gx | 4gy << 16,
+// so v_sad_u16 gives the gradient LUT's byte offset.  The right-pixel records
+// are split by index parity: the lanes of one column read consecutive
+// 16-byte records (no LDS bank conflicts).
 // The loop is software-pipelined: the records of column c+1 are read while
 // column c computes, and an output column's window sums are read right after
 // the prefix write and consumed one column later.
 constexpr int K2_SEG = 64;   // measured best at 1280x720 D=128 (more waves; the halo costs less)
 
 struct K2Shm {
-    size_t lut, mag, col, chi, outr, rr, key, ring, total;
+    size_t lut, mag, col, chi, outr, rre, rro, key, ring, total;
 };
 
 __host__ __device__ inline K2Shm k2_layout(int D, int Lmax, int seg, bool sym) {
     const int ncol = seg + 2 * Lmax;
+    const int nrr = (ncol + D + 2) / 2 + 1;   // records per parity array
     K2Shm s;
     size_t o = 0;
     s.lut = o; o += (65 + 256 + 511) * 4;
@@ -219,7 +228,8 @@ __host__ __device__ inline K2Shm k2_layout(int D, int Lmax, int seg, bool sym) {
     s.chi = o; o += sym ? 0 : (size_t)(ncol + 2) * 4;
     o = (o + 15) & ~(size_t)15;
     s.outr = o; o += (size_t)(seg + 1) * 16;
-    s.rr = o; o += (size_t)(ncol + D + 1) * 16;
+    s.rre = o; o += (size_t)nrr * 16;
+    s.rro = o; o += (size_t)nrr * 16;
     s.key = o; o += (size_t)(seg + D - 1) * 4;
     o = (o + 15) & ~(size_t)15;
     s.ring = o; o += (size_t)(2 * Lmax + 2) * 256;
@@ -228,6 +238,10 @@ __host__ __device__ inline K2Shm k2_layout(int D, int Lmax, int seg, bool sym) {
 }
 
 constexpr int kFarLeft = -(1 << 24);   // x of a column outside the image: x - d is never in [0, W)
+
+// (g, gx, gy) -> the two pre-formatted record words
+__device__ __forceinline__ uint32_t rep4(uint32_t g) { return g * 0x01010101u; }
+__device__ __forceinline__ uint32_t grad4(uint32_t gx, uint32_t gy) { return (gx << 2) | (gy << 18); }
 
 template <bool SYM, bool FULL, bool NOCLAMP>
 __global__ __launch_bounds__(64) void cost_crossh2_kernel(
@@ -243,20 +257,19 @@ __global__ __launch_bounds__(64) void cost_crossh2_kernel(
     float* s_la = s_lc + 65;
     float* s_lg = s_la + 256;
     uint32_t* s_mag = (uint32_t*)(shm + L.mag);
-    uint4* s_col = (uint4*)(shm + L.col);    // census lo, g|gx<<8|gy<<16, x (kFarLeft outside), ring byte offset of P(c+1)
+    uint4* s_col = (uint4*)(shm + L.col);    // census lo, rep4(g), grad4(gx, gy), x (kFarLeft outside)
     uint32_t* s_chi = (uint32_t*)(shm + L.chi);   // census hi (64-bit census)
-    uint4* s_out = (uint4*)(shm + L.outr);   // ring byte offsets P(x+R+1) | P(x-L) << 16, (x-L) | (x+R) << 16,
-                                             // ceil(2^31/(L+R+1)), byte offset of T(x, 0)
-    uint4* s_rr = (uint4*)(shm + L.rr);      // right pixel: census lo, g, gx|gy<<8, census hi
+    uint4* s_out = (uint4*)(shm + L.outr);   // ring byte offsets of P(x+R+1) and of P(x-L),
+                                             // ceil(2^31/(L+R+1)), (x-L) | (x+R) << 16
     uint32_t* s_key = (uint32_t*)(shm + L.key);
-    unsigned char* s_ring = shm + L.ring;    // [2*Lmax+2][64 lanes] u16-pair prefix sums
+    unsigned char* s_ring = shm + L.ring;    // [2*Lmax+2][64 lanes] u16-pair doubled prefix sums
 
     const int lane = threadIdx.x;
     const int y = blockIdx.y;
     const int x0 = blockIdx.x * seg;
     const int ncol = seg + 2 * Lmax, rs = 2 * Lmax + 2;
     const int xb = x0 - Lmax;              // column 0
-    const int xs = xb - (d0 + D - 1);      // right pixel of s_rr[0]
+    const int xs = xb - (d0 + D - 1);      // right pixel of record 0
     const int kbase = x0 - (d0 + D - 1);   // right pixel of s_key[0]
     const int nown = min(seg, w - x0);
     const size_t row = (size_t)y * w;
@@ -267,14 +280,15 @@ __global__ __launch_bounds__(64) void cost_crossh2_kernel(
     for (int n = lane; n < 2 * kMaxArm + 2; n += 64) s_mag[n] = n ? (0x80000000u + (uint32_t)n - 1u) / (uint32_t)n : 0u;
     for (int c = lane; c <= ncol + 1; c += 64) {
         const int x = xb + c;
-        uint4 r = make_uint4(0u, 0u, (uint32_t)kFarLeft, (uint32_t)(((c + 1) % rs) * 256));
+        uint4 r = make_uint4(0u, 0u, 0u, (uint32_t)kFarLeft);
         uint32_t hi = 0u;
         if (c < ncol && x >= 0 && x < w) {
             const size_t q = row + x;
             if (SYM) r.x = c32l[q];
             else { const uint64_t v = c64l[q]; r.x = (uint32_t)v; hi = (uint32_t)(v >> 32); }
-            r.y = (uint32_t)gl[q] | ((uint32_t)gxl[q] << 8) | ((uint32_t)gyl[q] << 16);
-            r.z = (uint32_t)x;
+            r.y = rep4(gl[q]);
+            r.z = grad4(gxl[q], gyl[q]);
+            r.w = (uint32_t)x;
         }
         s_col[c] = r;
         if (!SYM) s_chi[c] = hi;
@@ -286,14 +300,15 @@ __global__ __launch_bounds__(64) void cost_crossh2_kernel(
             const uint32_t a = arms[row + x];
             const int La = a & 0xFF, Ra = (a >> 8) & 0xFF;
             const int co = Lmax + o;   // column index of x; P(j) sums columns < j
-            r.x = (uint32_t)(((co + Ra + 1) % rs) * 256) | ((uint32_t)(((co - La) % rs) * 256) << 16);
-            r.y = (uint32_t)(x - La) | ((uint32_t)(x + Ra) << 16);
+            r.x = (uint32_t)(((co + Ra + 1) % rs) * 256);
+            r.y = (uint32_t)(((co - La) % rs) * 256);
             const uint32_t n = (uint32_t)(La + Ra + 1);
             r.z = (0x80000000u + n - 1u) / n;
-            r.w = (uint32_t)((row + x) * D);
+            r.w = (uint32_t)(x - La) | ((uint32_t)(x + Ra) << 16);
         }
         s_out[o] = r;
     }
+    // right pixel xs + i -> parity array i & 1, entry i >> 1
     for (int i = lane; i < ncol + D + 1; i += 64) {
         const int xr = xs + i;
         uint4 r = make_uint4(0u, 0u, 0u, 0u);
@@ -301,10 +316,10 @@ __global__ __launch_bounds__(64) void cost_crossh2_kernel(
             const size_t q = row + xr;
             if (SYM) r.x = c32r[q];
             else { const uint64_t v = c64r[q]; r.x = (uint32_t)v; r.w = (uint32_t)(v >> 32); }
-            r.y = gr[q];
-            r.z = (uint32_t)gxr[q] | ((uint32_t)gyr[q] << 8);
+            r.y = rep4(gr[q]);
+            r.z = grad4(gxr[q], gyr[q]);
         }
-        s_rr[i] = r;
+        ((uint4*)(shm + ((i & 1) ? L.rro : L.rre)))[i >> 1] = r;
     }
     for (int i = lane; i < seg + D - 1; i += 64) s_key[i] = 0xFFFFFFFFu;
     __syncthreads();
@@ -313,52 +328,66 @@ __global__ __launch_bounds__(64) void cost_crossh2_kernel(
     const auto rout = raw_rsrc(out);
     const uint32_t lofs = 4u * (uint32_t)lane;
     const uint32_t uw = (uint32_t)w;
+    const uint32_t la_base = (uint32_t)(uintptr_t)(__attribute__((address_space(3))) float*)s_la;
+    const uint32_t lg_base = (uint32_t)(uintptr_t)(__attribute__((address_space(3))) float*)s_lg;
     for (int dc = 0; dc < D; dc += 128) {
         const int di0 = dc + 2 * lane;   // the lane's pair (di0, di0 + 1); D even
         const int dA = d0 + di0;
         const bool act = FULL || di0 < D;
         const int dtop = d0 + min(dc + 127, D - 1);   // largest d of the chunk
         const bool nonneg = d0 >= 0;
-        // record of the right pixel x - d for (c, di) is c + D - 1 - di; lanes past D
-        // read records 0 and 1 (their states are masked)
-        const int rbase = act ? D - 1 - di0 : 1;
+        // record of the right pixel x - d for (c, di) is c + D - 1 - di = c + 2m + 1
+        // (m = D/2 - 1 - di0/2); lanes past D read records 1 + c (their states are masked)
+        const int m = act ? D / 2 - 1 - di0 / 2 : 0;
+        auto rr_at = [&](int j) -> uint4 {   // record 2m + 1 + j
+            const int u = j + 1;
+            return ((const uint4*)(shm + ((u & 1) ? L.rro : L.rre)))[(u >> 1) + m];
+        };
+        // s_key entry of the right pixel xr = x - dA is xr - kbase
+        uint32_t* key_lane = s_key - dA - kbase;
         *(uint32_t*)(s_ring + lofs) = 0u;   // P(0)
         uint32_t acc = 0u;
+        uint32_t ringw = 256u;              // ring byte offset of P(c + 1)
+        const uint32_t ringe = (uint32_t)rs * 256u;
         // pipeline registers: records of the next two columns, LUT terms of the next one
-        uint4 cr_n = s_col[0], r0_n = s_rr[rbase], r1_n = s_rr[rbase - 1], or_n = s_out[0];
+        uint4 cr_n = s_col[0], r0_n = rr_at(0), r1_n = rr_at(-1), or_n = s_out[0];
         uint32_t ch_n = SYM ? 0u : s_chi[0];
         float tc0, ta0, tg0, tc1, ta1, tg1;
+        // LDS byte addresses of the AD and gradient LUT entries come out of the
+        // SAD instructions themselves (the table base is their accumulator)
+        auto lds_f32 = [](uint32_t a) -> float { return *(const __attribute__((address_space(3))) float*)(uintptr_t)a; };
         auto lut_terms = [&](const uint4& cr, uint32_t chl, const uint4& r0, const uint4& r1) {
-            const uint32_t g0 = cr.y & 0xFFu, gxy = cr.y >> 8;
             const int pop0 = SYM ? __popc(cr.x ^ r0.x) : __popc(cr.x ^ r0.x) + __popc(chl ^ r0.w);
             const int pop1 = SYM ? __popc(cr.x ^ r1.x) : __popc(cr.x ^ r1.x) + __popc(chl ^ r1.w);
             tc0 = s_lc[pop0];
-            ta0 = s_la[__builtin_amdgcn_sad_u8(r0.y, g0, 0u)];
-            tg0 = s_lg[__builtin_amdgcn_sad_u8(r0.z, gxy, 0u)];
+            ta0 = lds_f32(__builtin_amdgcn_sad_u8(r0.y, cr.y, la_base));
+            tg0 = lds_f32(__builtin_amdgcn_sad_u16(r0.z, cr.z, lg_base));
             tc1 = s_lc[pop1];
-            ta1 = s_la[__builtin_amdgcn_sad_u8(r1.y, g0, 0u)];
-            tg1 = s_lg[__builtin_amdgcn_sad_u8(r1.z, gxy, 0u)];
+            ta1 = lds_f32(__builtin_amdgcn_sad_u8(r1.y, cr.y, la_base));
+            tg1 = lds_f32(__builtin_amdgcn_sad_u16(r1.z, cr.z, lg_base));
         };
         lut_terms(cr_n, ch_n, r0_n, r1_n);   // column 0
-        uint4 cr_c = cr_n;                   // column 0's record (its x and ring slot)
+        uint4 cr_c = cr_n;                   // column 0's record (its x)
         cr_n = s_col[1];
         if (!SYM) ch_n = s_chi[1];
-        r0_n = s_rr[rbase + 1];
-        r1_n = s_rr[rbase];
+        r0_n = rr_at(1);
+        r1_n = rr_at(0);
         uint32_t pp1 = 0u, pp0 = 0u;
         uint4 por = make_uint4(0u, 0u, 0u, 0u);
         int pxo = 0;
         bool pend = false;
 
-        auto flush = [&]() {   // T of the column whose window sums were read last step
-            const uint32_t sw = as_u32(as_v2(pp1) - as_v2(pp0));
-            const int xlo = (int)(por.y & 0xFFFFu), xhi = (int)(por.y >> 16);
+        // T of the column whose window sums were read last step; FAST: whole
+        // windows for every lane (count L+R+1, T valid)
+        auto flush = [&](auto fast_t) {
+            constexpr bool FAST = decltype(fast_t)::value;
+            const uint32_t sw = as_u32(as_v2(pp1) - as_v2(pp0));   // doubled window sums
             uint32_t q0, q1;
-            if (nonneg && dtop <= xlo) {
-                // every lane's window is whole: count L+R+1, T valid (d <= x-L <= x)
-                q0 = __umulhi(2u * (sw & 0xFFFFu), por.z);
-                q1 = __umulhi(2u * (sw >> 16), por.z);
+            if (FAST) {
+                q0 = __umulhi(sw & 0xFFFFu, por.z);
+                q1 = __umulhi(sw >> 16, por.z);
             } else {
+                const int xlo = (int)(por.w & 0xFFFFu), xhi = (int)(por.w >> 16);
                 uint32_t qq[2];
 #pragma unroll
                 for (int e = 0; e < 2; ++e) {
@@ -366,18 +395,19 @@ __global__ __launch_bounds__(64) void cost_crossh2_kernel(
                     const int cnt = min(xhi, w - 1 + d) - max(xlo, d) + 1;
                     const uint32_t se = e ? (sw >> 16) : (sw & 0xFFFFu);
                     const bool v = d <= pxo && cnt > 0;   // d in [min, min(max, x+1)), window not empty
-                    qq[e] = v ? __umulhi(2u * se, s_mag[cnt > 0 ? cnt : 0]) : 0xFFu;
+                    qq[e] = v ? __umulhi(se, s_mag[cnt > 0 ? cnt : 0]) : 0xFFu;
                 }
                 q0 = qq[0];
                 q1 = qq[1];
             }
-            if (act) buf_st16(rout, q0 | (q1 << 8), (uint32_t)di0, por.w);
+            if (act) buf_st16(rout, q0 | (q1 << 8), (uint32_t)di0, (uint32_t)((int)row + pxo) * (uint32_t)D);
         };
 
         // keys: own column (right-view WTA candidates); outp: an output column's
         // window sums are read; allv: every lane's right pixels x - d, x - d - 1
-        // lie in the image (no per-lane validity test)
-        auto step = [&](int c, auto keys_t, auto outp_t, auto allv_t) {
+        // lie in the image (no per-lane validity test); fast: the column flushed
+        // in this step has whole windows
+        auto step = [&](int c, auto keys_t, auto outp_t, auto allv_t, auto fast_t) {
             constexpr bool keys = decltype(keys_t)::value, outp = decltype(outp_t)::value;
             constexpr bool allv = decltype(allv_t)::value;
             const uint4 cr = cr_c;
@@ -391,77 +421,90 @@ __global__ __launch_bounds__(64) void cost_crossh2_kernel(
             if (!SYM) ch_n = s_chi[c + 2];
             // d + 1 at column c + 2 pairs with the right pixel of d at column c + 1
             r1_n = r0_n;
-            r0_n = s_rr[rbase + c + 2];
-            if (pend) flush();
+            r0_n = rr_at(c + 2);
+            if (pend) flush(fast_t);
             // NOCLAMP: the host showed every LUT term >= 0 and the largest
             // quantised sum <= cost_max, so the clamp of the reference is the identity
             const uint32_t cst0 = NOCLAMP ? (uint32_t)(int)(f0 * scale + 0.5f)
                                           : (uint32_t)clampi((int)(f0 * scale + 0.5f), 0, cost_max);
             const uint32_t cst1 = NOCLAMP ? (uint32_t)(int)(f1 * scale + 0.5f)
                                           : (uint32_t)clampi((int)(f1 * scale + 0.5f), 0, cost_max);
-            const int xr0 = (int)cr.z - dA;   // right pixel of d; d + 1 has xr0 - 1
+            const int xr0 = (int)cr.w - dA;   // right pixel of d; d + 1 has xr0 - 1
             const bool v0 = allv ? act : act && (uint32_t)xr0 < uw;
             const bool v1 = allv ? act : act && (uint32_t)(xr0 - 1) < uw;
             if (keys) {
-                const int ki = xr0 - kbase;
-                const int k0 = FULL ? ki : max(ki, 1);
-                atomicMin(&s_key[k0], v0 ? (cst0 << 16) | (uint32_t)di0 : 0xFFFFFFFFu);
-                atomicMin(&s_key[k0 - 1], v1 ? (cst1 << 16) | (uint32_t)(di0 + 1) : 0xFFFFFFFFu);
+                uint32_t* k0p = FULL ? key_lane + (int)cr.w : s_key + max(xr0 - kbase, 1);
+                atomicMin(k0p, v0 ? (cst0 << 16) | (uint32_t)di0 : 0xFFFFFFFFu);
+                atomicMin(k0p - 1, v1 ? (cst1 << 16) | (uint32_t)(di0 + 1) : 0xFFFFFFFFu);
             }
-            acc = as_u32(as_v2(acc) + as_v2((v0 ? cst0 : 0u) | ((v1 ? cst1 : 0u) << 16)));
-            *(uint32_t*)(s_ring + cr.w + lofs) = acc;   // P(c + 1)
+            const uint32_t pair = (v0 ? cst0 : 0u) | ((v1 ? cst1 : 0u) << 16);
+            acc = as_u32(as_v2(pair) * splat2(2) + as_v2(acc));
+            *(uint32_t*)(s_ring + ringw + lofs) = acc;   // P(c + 1)
+            ringw = ringw + 256u == ringe ? 0u : ringw + 256u;
             pend = outp;
             if (outp) {
                 por = or_n;
                 or_n = s_out[c - 2 * Lmax + 1];
                 pxo = x0 + c - 2 * Lmax;
-                pp1 = *(const uint32_t*)(s_ring + (por.x & 0xFFFFu) + lofs);
-                pp0 = *(const uint32_t*)(s_ring + (por.x >> 16) + lofs);
+                pp1 = *(const uint32_t*)(s_ring + por.x + lofs);
+                pp0 = *(const uint32_t*)(s_ring + por.y + lofs);
             }
         };
 
         // phases of c: keys for own columns [Lmax, Lmax + nown), output (Lmax
         // behind) for [2Lmax, 2Lmax + nown); columns past 2Lmax + nown lie beyond
-        // the image and feed nothing
-        // the image columns where every d of the chunk has its right pixel
-        // inside the image, [vx0, vx1], in c: [cvb, cve); the loops over them
-        // run two columns per iteration so the pipeline registers alternate
-        // instead of being copied
+        // the image and feed nothing.  The image columns where every d of the
+        // chunk has its right pixel inside the image, [vx0, vx1], in c: [cvb,
+        // cve).  Steps from cf on flush columns x >= dtop + Lmax (whole windows).
+        // The loops over allv columns run two columns per iteration so the
+        // pipeline registers alternate instead of being copied.
         const int kb = Lmax, ke = Lmax + nown, ob = 2 * Lmax, oe = 2 * Lmax + nown;
         const int vx0 = max(dtop, 0), vx1 = min(w - 1 + d0 + dc, w - 1);
         const int cvb = min(oe, max(0, vx0 - xb)), cve = max(cvb, min(oe, vx1 - xb + 1));
+        const int cf = nonneg ? min(oe, max(ob + 1, dtop + Lmax - x0 + 2 * Lmax + 1)) : oe;
         using TT = std::true_type;
         using FF = std::false_type;
         int c = 0;
-        auto run = [&](int nx, auto k_t, auto o_t, auto v_t) {
+        auto run = [&](int nx, auto k_t, auto o_t, auto v_t, auto f_t) {
             if constexpr (decltype(v_t)::value)
                 for (; c + 1 < nx; c += 2) {
-                    step(c, k_t, o_t, v_t);
-                    step(c + 1, k_t, o_t, v_t);
+                    step(c, k_t, o_t, v_t, f_t);
+                    step(c + 1, k_t, o_t, v_t, f_t);
                 }
-            for (; c < nx; ++c) step(c, k_t, o_t, v_t);
+            for (; c < nx; ++c) step(c, k_t, o_t, v_t, f_t);
+        };
+        auto run_o = [&](int nx, auto k_t, auto v_t, bool inF) {
+            if (inF) run(nx, k_t, TT{}, v_t, TT{});
+            else run(nx, k_t, TT{}, v_t, FF{});
         };
         while (c < oe) {
-            const bool inK = c >= kb && c < ke, inO = c >= ob, inV = c >= cvb && c < cve;
+            const bool inK = c >= kb && c < ke, inO = c >= ob, inV = c >= cvb && c < cve, inF = c >= cf;
             int nx = oe;
             if (c < kb) nx = min(nx, kb);
             if (c < ke) nx = min(nx, ke);
             if (c < ob) nx = min(nx, ob);
             if (c < cvb) nx = min(nx, cvb);
             if (c < cve) nx = min(nx, cve);
+            if (c < cf) nx = min(nx, cf);
             if (inV) {
-                if (inK && inO) run(nx, TT{}, TT{}, TT{});
-                else if (inK) run(nx, TT{}, FF{}, TT{});
-                else if (inO) run(nx, FF{}, TT{}, TT{});
-                else run(nx, FF{}, FF{}, TT{});
+                if (inO) {
+                    if (inK) run_o(nx, TT{}, TT{}, inF);
+                    else run_o(nx, FF{}, TT{}, inF);
+                } else {
+                    if (inK) run(nx, TT{}, FF{}, TT{}, FF{});
+                    else run(nx, FF{}, FF{}, TT{}, FF{});
+                }
             } else {
-                if (inK && inO) run(nx, TT{}, TT{}, FF{});
-                else if (inK) run(nx, TT{}, FF{}, FF{});
-                else if (inO) run(nx, FF{}, TT{}, FF{});
-                else run(nx, FF{}, FF{}, FF{});
+                if (inO) {
+                    if (inK) run_o(nx, TT{}, FF{}, inF);
+                    else run_o(nx, FF{}, FF{}, inF);
+                } else {
+                    if (inK) run(nx, TT{}, FF{}, FF{}, FF{});
+                    else run(nx, FF{}, FF{}, FF{}, FF{});
+                }
             }
         }
-        if (pend) flush();
+        if (pend) flush(FF{});
     }
     __syncthreads();
     for (int i = lane; i < seg + D - 1; i += 64) {
