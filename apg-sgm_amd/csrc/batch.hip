@@ -368,7 +368,7 @@ int tune_pair(apg_hip_ctx ctx, const apg_hip_pair& q, apg_hip_tuning* out) {
         if (int s = time_shapes(ctx, q, apg_hip_tuning{0, 0, cv}, reps, &t)) return s;
         if (t.cross_ms < 0.98 * bv) bv = t.cross_ms, r.cross_v_waves = cv;
     }
-    for (int sd : {1, 2}) {
+    for (int sd : {1, 2, 4}) {
         if (int s = time_shapes(ctx, q, apg_hip_tuning{0, sd, 0}, reps, &t)) return s;
         if (t.sgm_ms < 0.98 * bs) bs = t.sgm_ms, r.sgm_dual = sd;
     }

@@ -804,7 +804,7 @@ int apg_hip_set_tuning(apg_hip_ctx ctx, const apg_hip_tuning* t) {
     }
     if (t->cost_segment != 0 && (t->cost_segment < 32 || t->cost_segment > 1024))
         return fail(ctx, 4, "cost_segment must be 0 or 32..1024");
-    if (t->sgm_dual < 0 || t->sgm_dual > 2) return fail(ctx, 4, "sgm_dual must be 0, 1 or 2");
+    if (t->sgm_dual < 0 || t->sgm_dual > 4) return fail(ctx, 4, "sgm_dual must be 0..4");
     if (t->cross_v_waves != 0 && (t->cross_v_waves < 256 || t->cross_v_waves > 65536))
         return fail(ctx, 4, "cross_v_waves must be 0 or 256..65536");
     ctx->tune.cost_segment = t->cost_segment;

@@ -23,7 +23,7 @@ __device__ __forceinline__ u16x2 add_sat2(u16x2 a, u16x2 b) { return __builtin_e
 __device__ __forceinline__ u16x2 min2(u16x2 a, u16x2 b) { return __builtin_elementwise_min(a, b); }
 
 // DPP control words (CDNA3/CDNA4 ISA, DPP_CTRL enumeration)
-constexpr int kDppRowShr1 = 0x111, kDppRowShr2 = 0x112, kDppRowShr4 = 0x114, kDppRowShr8 = 0x118;
+constexpr int kDppRowShl1 = 0x101, kDppRowShr1 = 0x111, kDppRowShr2 = 0x112, kDppRowShr4 = 0x114, kDppRowShr8 = 0x118;
 constexpr int kDppWaveShl1 = 0x130, kDppWaveShr1 = 0x138;
 constexpr int kDppRowBcast15 = 0x142, kDppRowBcast31 = 0x143;
 constexpr int kDppRowRor1 = 0x121, kDppRowRor2 = 0x122, kDppRowRor4 = 0x124, kDppRowRor8 = 0x128;

@@ -506,7 +506,7 @@ __global__ __launch_bounds__(64) void sgm_half_kernel(apg_hip_params p, int w, i
     // half-lane 0's d-1 and half-lane 31's d+1 lie outside [0, 128)
     const uint32_t p1l = hl == 0 ? (0xFFFFu | (P1 << 16)) : p1s;
     const uint32_t p1r = hl == 31 ? (P1 | 0xFFFF0000u) : p1s;
-    uint32_t prev0 = 0xFFFFFFFFu, prev1 = 0xFFFFFFFFu, mpp = 0xFFFFFFFFu;
+    uint32_t prev0 = 0xFFFFFFFFu, prev1 = 0xFFFFFFFFu, mp = 0xFFFFu;   // minL' of the lane's chain
     auto voffA = [&](int k, int j) { return (uint32_t)(pA0 + min(k * SB + j, n - 1) * step) * (uint32_t)D + cl * 16u; };
     auto voffB = [&](int k, int j) { return (uint32_t)(pB0 - min(k * SB + j, n - 1) * step) * (uint32_t)D + cl * 16u; };
     // batch k's rows arrive in register slot k % PF, PF batches ahead of use
@@ -588,15 +588,18 @@ __global__ __launch_bounds__(64) void sgm_half_kernel(apg_hip_params p, int w, i
                 uint32_t c0 = decode_c<uint8_t, 2, ALLV>(x, 0), c1 = decode_c<uint8_t, 2, ALLV>(x, 1);
                 if (!FULL && !inr) c0 = c1 = 0xFFFFFFFFu;
                 // L = C + min(L'(d), L'(d-1)+P1, L'(d+1)+P1, minL'+P2) - minL'
-                const u16x2 mp2 = add_sat2(as_v2(mpp), as_v2(p2p));
+                //   = C + min(u - minL', P2), u = min(L'(d), L'(d+-1)+P1) >= minL'
+                // (path values stay far below 65535 - P2 unless all are 65535,
+                // where u = minL' and the delta is 0), so after the reduction
+                // only three operations remain on the recurrence
                 const uint32_t left_hi = (uint32_t)__builtin_amdgcn_mov_dpp((int)prev1, kDppWaveShr1, 0xF, 0xF, true);
                 const uint32_t right_lo = (uint32_t)__builtin_amdgcn_mov_dpp((int)prev0, kDppWaveShl1, 0xF, 0xF, true);
                 const u16x2 sl = add_sat2(as_v2(__builtin_amdgcn_alignbit(prev0, left_hi, 16)), as_v2(p1l));
                 const u16x2 sm = add_sat2(as_v2(__builtin_amdgcn_alignbit(prev1, prev0, 16)), as_v2(p1s));
                 const u16x2 sr = add_sat2(as_v2(__builtin_amdgcn_alignbit(right_lo, prev1, 16)), as_v2(p1r));
-                const u16x2 t0 = min2(min2(min2(sl, sm), as_v2(prev0)), mp2);
-                const u16x2 t1 = min2(min2(min2(sm, sr), as_v2(prev1)), mp2);
-                const u16x2 d0 = t0 - as_v2(mpp), d1 = t1 - as_v2(mpp);
+                const u16x2 u0 = min2(min2(sl, sm), as_v2(prev0));
+                const u16x2 u1 = min2(min2(sm, sr), as_v2(prev1));
+                const u16x2 d0 = min2(u0 - splat2(mp), as_v2(p2p)), d1 = min2(u1 - splat2(mp), as_v2(p2p));
                 prev0 = as_u32(add_sat2(as_v2(c0), d0));
                 prev1 = as_u32(add_sat2(as_v2(c1), d1));
                 const uint32_t lm2 = as_u32(min2(as_v2(prev0), as_v2(prev1)));
@@ -608,10 +611,212 @@ __global__ __launch_bounds__(64) void sgm_half_kernel(apg_hip_params p, int w, i
                 m = min(m, dpp<kDppRowRor2>(0xFFFFFFFFu, m));
                 m = min(m, dpp<kDppRowRor1>(0xFFFFFFFFu, m));
                 const auto sw = __builtin_amdgcn_permlane16_swap(m, m, false, false);
-                m = min((uint32_t)sw[0], (uint32_t)sw[1]);
-                mpp = m | (m << 16);
+                mp = min((uint32_t)sw[0], (uint32_t)sw[1]);
                 *(uint32_t*)(s_d + lofs + (bf * SB + i) * RB) =
                     __builtin_amdgcn_perm(as_u32(d1), as_u32(d0), 0x06040200u);
+            }
+        };
+        if (!INNER || batch_valid(k)) run_batch(std::integral_constant<bool, INNER>{});
+        else run_batch(std::false_type{});
+    };
+    // groups of PF inner batches (batch k in slot k % PF), then the rest
+    int k0 = 0;
+    for (; k0 + PF < nb; k0 += PF)
+        static_for<PF>([&](auto q_t) { batch(k0 + decltype(q_t)::value, q_t, std::true_type{}); });
+    static_for<PF>([&](auto q_t) {
+        const int k = k0 + decltype(q_t)::value;
+        if (k < nb) batch(k, q_t, std::false_type{});
+    });
+    flush(nb - 1);
+}
+
+// Quarter-wave form for u8 C', 4 paths, D % 16 == 0, D <= 128: one wave runs
+// the forward and backward chains of two scanlines of one family (rows 2i,
+// 2i+1 or columns 2j, 2j+1), 16 lanes per chain (chain q = lane >> 4: q & 1
+// the direction, q >> 1 the scanline).  Half-lane l holds the disparities
+// 8l..8l+7 as four packed u16 pairs; the three inner neighbour pairs are
+// shared by adjacent pairs, and each chain's minimum is one 16-lane row
+// reduced by four rotations.  About 48 VALU per step of four chains, and one
+// wave per SIMD at 1280x720, so the recurrence's latency is hidden by the
+// four pairs' independent work rather than by other waves.  Measured 225 us
+// against 231 us for the half-wave form at 1280x720 D=128, but slower on
+// most other shapes (one wave per SIMD exposes the recurrence's latency when
+// the scanlines are long or few): a launch-shape option, not the default.
+template <bool FULL, int PF>
+__global__ __launch_bounds__(64) void sgm_quad_kernel(apg_hip_params p, int w, int h, int D,
+                                                      const uint8_t* __restrict__ gray,
+                                                      const uint8_t* __restrict__ cost, PlaneSet planes, int nmax) {
+    extern __shared__ __align__(16) uint8_t sh[];
+    constexpr int RB = 128;                  // staged row bytes (>= D)
+    constexpr int NC = 4;                    // chains per wave
+    uint8_t* s_c = sh;                       // [chain][buffer][SB][RB] C' rows
+    uint8_t* s_d = sh + NC * 2 * SB * RB;    // [chain][buffer][SB][RB] delta rows
+    uint32_t* s_p2 = (uint32_t*)(sh + 2 * NC * 2 * SB * RB);   // [chain][nmax] P2 | P2 << 16 per step
+    const int lane = threadIdx.x;
+    const int hq = lane >> 4, hl = lane & 15;
+    // block -> family (rows, then columns) and its scanline pair
+    const int nrow = (h + 1) / 2;
+    const int f = (int)blockIdx.x < nrow ? 0 : 1;
+    const int sp = f == 0 ? (int)blockIdx.x : (int)blockIdx.x - nrow;
+    const int nline = f == 0 ? h : w;
+    // scanlines s0 = 2sp and s1 = 2sp + 1 (s0 again past the last one: the same
+    // values are computed twice and stored twice)
+    int p0a, step, n;
+    family_line(f, 2 * sp, w, h, p0a, step, n);
+    int p0b = p0a;
+    if (2 * sp + 1 < nline) { int st2, n2; family_line(f, 2 * sp + 1, w, h, p0b, st2, n2); }
+    const auto rc = raw_rsrc(cost);
+    // chain q: first pixel and pixel step
+    auto cfirst = [&](int q) { const int pl = (q >> 1) ? p0b : p0a; return (q & 1) ? pl + (n - 1) * step : pl; };
+    auto cstep = [&](int q) { return (q & 1) ? -step : step; };
+    const int xlo = p.min_disparity + D - 1, xhi = w - 1 + p.min_disparity;
+    auto batch_valid = [&](int k) {
+        const int t0 = k * SB, t1 = min(k * SB + SB, n) - 1;
+        if (f == 1) {   // columns: x fixed per scanline
+            const int xa = p0a % w, xb2 = p0b % w;
+            return min(xa, xb2) >= xlo && max(xa, xb2) <= xhi;
+        }
+        // rows: forward chains at x = t, backward at x = w - 1 - t
+        return t0 >= xlo && t1 <= xhi && w - 1 - t1 >= xlo && w - 1 - t0 <= xhi;
+    };
+    // P2 of every step of each chain (forward: pixel t after t-1; backward:
+    // pixel n-1-t after n-t)
+    for (int i = lane; i < 2 * n; i += 64) {
+        const int s = i >= n, t = i - s * n;
+        const int pl = s ? p0b : p0a;
+        const int ga = gray[pl + t * step], gpa = gray[pl + max(t - 1, 0) * step];
+        const int u = n - 1 - t;
+        const int gb = gray[pl + u * step], gpb = gray[pl + min(u + 1, n - 1) * step];
+        const uint32_t a = (uint32_t)adaptive_p2(p, ga - gpa), bb = (uint32_t)adaptive_p2(p, gb - gpb);
+        s_p2[(2 * s) * nmax + t] = a | (a << 16);
+        s_p2[(2 * s + 1) * nmax + t] = bb | (bb << 16);
+    }
+    for (int t = n + lane; t < ((n + SB - 1) / SB) * SB; t += 64)
+#pragma unroll
+        for (int q = 0; q < NC; ++q) s_p2[q * nmax + t] = 0u;
+    // batch transfer: lane -> (row j, 16-byte chunk c); one instruction per chain covers SB rows
+    const int lpx = D / 16;
+    const int jl = lane / lpx, cl = lane % lpx;
+    const bool inr = FULL || 8 * hl < D;
+    const uint32_t P1 = (uint32_t)p.P1;
+    const uint32_t p1s = P1 | (P1 << 16);
+    // half-lane 0's d-1 and half-lane 15's d+1 lie outside [0, 128)
+    const uint32_t p1l = hl == 0 ? (0xFFFFu | (P1 << 16)) : p1s;
+    const uint32_t p1r = hl == 15 ? (P1 | 0xFFFF0000u) : p1s;
+    uint32_t pv[4] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, mp = 0xFFFFu;
+    auto voff = [&](int q, int k, int j) {
+        return (uint32_t)(cfirst(q) + min(k * SB + j, n - 1) * cstep(q)) * (uint32_t)D + cl * 16u;
+    };
+    // batch k's rows arrive in register slot k % PF, PF batches ahead of use
+    u32x4 g[PF][NC];
+    auto gload = [&](int k, auto slot_t) {
+        constexpr int s = decltype(slot_t)::value;
+        const int j = min(jl, SB - 1);
+#pragma unroll
+        for (int q = 0; q < NC; ++q) g[s][q] = buf_ld128(rc, voff(q, k, j), 0u);
+    };
+    auto lstage = [&](int k, auto slot_t) {   // batch k's rows -> LDS buffer k & 1
+        constexpr int s = decltype(slot_t)::value;
+        const int bf = k & 1;
+        if (jl < SB) {
+#pragma unroll
+            for (int q = 0; q < NC; ++q) *(u32x4*)(s_c + ((q * 2 + bf) * SB + jl) * RB + cl * 16) = g[s][q];
+        }
+    };
+    auto flush = [&](int k) {    // deltas of batch k (LDS buffer k & 1) -> planes
+        const int bf = k & 1;
+        if (jl < SB && k * SB + jl < n) {
+#pragma unroll
+            for (int q = 0; q < NC; ++q) {
+                const u32x4 v = *(const u32x4*)(s_d + ((q * 2 + bf) * SB + jl) * RB + cl * 16);
+                __builtin_amdgcn_raw_buffer_store_b128(v, raw_rsrc(planes.p[2 * f + (q & 1)]), (int)voff(q, k, jl), 0,
+                                                       0);
+            }
+        }
+    };
+    // the lane's C' words (8 d of its chain) and P2 pairs of all SB steps of a
+    // batch, read into register set k & 1 one batch ahead
+    const uint32_t lofs = (uint32_t)(hq * 2 * SB * RB + 8 * hl);
+    const uint32_t* p2h = s_p2 + hq * nmax;
+    uint2 W[2][SB];
+    uint32_t P[2][SB];
+    auto wread = [&](int k, auto set_t) {
+        constexpr int e = decltype(set_t)::value;
+#pragma unroll
+        for (int i = 0; i < SB; ++i) W[e][i] = *(const uint2*)(s_c + lofs + ((k & 1) * SB + i) * RB);
+        const u32x4* pp = (const u32x4*)(p2h + k * SB);
+        const u32x4 a = pp[0], b = pp[1];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            P[e][i] = a[i];
+            P[e][4 + i] = b[i];
+        }
+    };
+    const int nb = (n + SB - 1) / SB;
+    // loads are issued unconditionally (past the end: the last batch again), so
+    // the wait before staging a batch counts the loads issued after it
+    static_for<PF>([&](auto q_t) { gload(min((int)decltype(q_t)::value, nb - 1), q_t); });
+    lstage(0, std::integral_constant<int, 0>{});
+    __syncthreads();
+    wread(0, std::integral_constant<int, 0>{});
+    // INNER: batch k is not the last one (all SB steps exist, batch k + 1 exists)
+    auto batch = [&](int k, auto q_t, auto inner_t) {
+        constexpr int q = decltype(q_t)::value;
+        constexpr bool INNER = decltype(inner_t)::value;
+        if (INNER || k + 1 < nb) {
+            lstage(k + 1, std::integral_constant<int, (q + 1) % PF>{});
+            wread(k + 1, std::integral_constant<int, (q + 1) & 1>{});
+        }
+        if (!INNER && k > 0) flush(k - 1);
+        gload(min(k + PF, nb - 1), q_t);
+        const int bf = k & 1;
+        auto run_batch = [&](auto allv_tag) {
+            constexpr bool ALLV = decltype(allv_tag)::value;
+#pragma unroll
+            for (int i = 0; i < SB; ++i) {
+                const int t = k * SB + i;
+                if (!INNER && t >= n) break;
+                if (INNER && i == SB / 2 && k > 0) flush(k - 1);
+                const uint2 wc = W[q & 1][i];
+                const uint32_t p2p = P[q & 1][i];
+                uint32_t c[4];
+                {
+                    CRaw<uint8_t, 2> x0, x1;
+                    x0.w[0] = wc.x;
+                    x1.w[0] = wc.y;
+                    c[0] = decode_c<uint8_t, 2, ALLV>(x0, 0);
+                    c[1] = decode_c<uint8_t, 2, ALLV>(x0, 1);
+                    c[2] = decode_c<uint8_t, 2, ALLV>(x1, 0);
+                    c[3] = decode_c<uint8_t, 2, ALLV>(x1, 1);
+                }
+                if (!FULL && !inr) c[0] = c[1] = c[2] = c[3] = 0xFFFFFFFFu;
+                // L = C + min(u - minL', P2), u = min(L'(d), L'(d+-1)+P1) (as in the half-wave form)
+                const uint32_t left_hi = (uint32_t)__builtin_amdgcn_mov_dpp((int)pv[3], kDppRowShr1, 0xF, 0xF, true);
+                const uint32_t right_lo = (uint32_t)__builtin_amdgcn_mov_dpp((int)pv[0], kDppRowShl1, 0xF, 0xF, true);
+                // s[j]: (L'(d-1), L'(d)) + P1 of pair j's first d, j = 0..4 (s[j+1] is pair j's d+1 term)
+                u16x2 s[5];
+                s[0] = add_sat2(as_v2(__builtin_amdgcn_alignbit(pv[0], left_hi, 16)), as_v2(p1l));
+#pragma unroll
+                for (int j = 1; j < 4; ++j) s[j] = add_sat2(as_v2(__builtin_amdgcn_alignbit(pv[j], pv[j - 1], 16)), as_v2(p1s));
+                s[4] = add_sat2(as_v2(__builtin_amdgcn_alignbit(right_lo, pv[3], 16)), as_v2(p1r));
+                uint32_t dl[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const u16x2 uu = min2(min2(s[j], s[j + 1]), as_v2(pv[j]));
+                    const u16x2 dd = min2(uu - splat2(mp), as_v2(p2p));
+                    dl[j] = as_u32(dd);
+                    pv[j] = as_u32(add_sat2(as_v2(c[j]), dd));
+                }
+                const uint32_t lm2 = as_u32(min2(min2(as_v2(pv[0]), as_v2(pv[1])), min2(as_v2(pv[2]), as_v2(pv[3]))));
+                uint32_t m = min(lm2 & 0xFFFFu, lm2 >> 16);
+                // minimum over the chain's 16-lane row: every lane ends with it
+                m = min(m, dpp<kDppRowRor8>(0xFFFFFFFFu, m));
+                m = min(m, dpp<kDppRowRor4>(0xFFFFFFFFu, m));
+                m = min(m, dpp<kDppRowRor2>(0xFFFFFFFFu, m));
+                mp = min(m, dpp<kDppRowRor1>(0xFFFFFFFFu, m));
+                *(uint2*)(s_d + lofs + (bf * SB + i) * RB) =
+                    make_uint2(__builtin_amdgcn_perm(dl[1], dl[0], 0x06040200u),
+                               __builtin_amdgcn_perm(dl[3], dl[2], 0x06040200u));
             }
         };
         if (!INNER || batch_valid(k)) run_batch(std::integral_constant<bool, INNER>{});
@@ -641,7 +846,26 @@ static hipError_t launch_lines_t(hipStream_t s, const apg_hip_params& p, int w, 
     const int nblk = h + w + (nfam == 4 ? 2 * (w + h - 1) : 0);
     const bool full = D == 128 * K;
     const size_t shm = (size_t)(w > h ? w : h);   // one scanline of gray values
-    if (sizeof(TC) == 1 && D % 16 == 0 && K == 1 && tn.sgm_dual == 0) {
+    if (sizeof(TC) == 1 && D % 16 == 0 && K == 1 && nfam == 2 && tn.sgm_dual == 4) {
+        // both chains of two scanlines per wave, 16 lanes each; P2 of every step staged
+        const int nmax = ((w > h ? w : h) + SB - 1) / SB * SB;
+        const size_t shmq = (size_t)16 * SB * 128 + (size_t)4 * nmax * 4;
+        const int nq = (h + 1) / 2 + (w + 1) / 2;
+        if (shmq > 64 * 1024) {
+            hipError_t e = hipFuncSetAttribute(full ? (const void*)sgm_quad_kernel<true, kHalfPF>
+                                                    : (const void*)sgm_quad_kernel<false, kHalfPF>,
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmq);
+            if (e != hipSuccess) return e;
+        }
+        if (full)
+            hipLaunchKernelGGL((sgm_quad_kernel<true, kHalfPF>), dim3(nq), dim3(64), shmq, s, p, w, h, D, gray,
+                               (const uint8_t*)cost, planes, nmax);
+        else
+            hipLaunchKernelGGL((sgm_quad_kernel<false, kHalfPF>), dim3(nq), dim3(64), shmq, s, p, w, h, D, gray,
+                               (const uint8_t*)cost, planes, nmax);
+        return hipGetLastError();
+    }
+    if (sizeof(TC) == 1 && D % 16 == 0 && K == 1 && (tn.sgm_dual == 0 || tn.sgm_dual == 3)) {
         // both chains of a scanline per wave, 32 lanes each; P2 of every step staged
         const int nmax = ((w > h ? w : h) + SB - 1) / SB * SB;   // whole batches of P2 words
         const size_t shmh = (size_t)8 * SB * 128 + (size_t)2 * nmax * 4;

@@ -32,7 +32,8 @@ struct CostLut {
 // SGM wave, target wave count of the Cross-V pass.
 struct KTune {
     int cost_segment = 0;
-    int sgm_dual = 0;   // 0 built-in, 1 both chains on 64 lanes each, 2 one chain per wave
+    int sgm_dual = 0;   // 0 built-in, 1 both chains on 64 lanes each, 2 one chain per wave, 3 on 32 lanes
+                        // each, 4 both chains of two scanlines on 16 lanes each
     int cross_v_waves = 0;
 };
 

@@ -285,7 +285,9 @@ typedef struct {
     int sgm_dual;        /* SGM wave shape: 0 built-in (D <= 128: both directions of a
                             scanline per wave, 32 lanes each; else one direction per
                             wave), 1 both directions over 64 lanes each, 2 one
-                            direction per wave */
+                            direction per wave, 3 both directions over 32 lanes each,
+                            4 (4 paths, D <= 128) both directions of two scanlines
+                            per wave, 16 lanes each */
     int cross_v_waves;   /* target wave count of the Cross-V pass: 256..65536 (4096) */
 } apg_hip_tuning;
 

@@ -91,6 +91,34 @@ def test_pipeline_fused_bit_exact(ctx, kind, w, h, D, mode, over):
     _check(g2, o, ["disparity", "confidence", "invalid_reason", "reliable"])
 
 
+# every SGM wave shape of the fused schedule (launch-shape option sgm_dual):
+# 64-lane dual and single chains, 32-lane half waves, and (4 paths) 16-lane
+# quarter waves over two scanlines -- odd scanline counts (the quarter form's
+# last wave holds one scanline twice), D < 128, negative min disparity, 8 paths
+SGM_SHAPE_CASES = [
+    (1, 160, 121, 128, {}),
+    (2, 97, 64, 48, {"min_disparity": -8}),
+    (0, 131, 77, 64, {}),
+    (1, 90, 60, 32, {"paths": 8}),
+    (2, 200, 90, 112, {"min_disparity": 10}),
+]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("sgm_dual", [1, 2, 3, 4])
+@pytest.mark.parametrize("kind,w,h,D,over", SGM_SHAPE_CASES)
+def test_pipeline_sgm_wave_shapes(ctx, kind, w, h, D, over, sgm_dual):
+    print(f"sgm_dual={sgm_dual} kind={kind} {w}x{h} D={D} {over}", flush=True)
+    p, left, right, o = _case(kind, w, h, D, 1, over)
+    A.set_schedule(ctx, 0)
+    A.set_tuning(ctx, (0, sgm_dual, 0))
+    try:
+        g = A.run_pipeline(ctx, p, left, right, intermediates=A.FUSED_INTERMEDIATES)
+    finally:
+        A.set_tuning(ctx, None)
+    _check(g, o, FUSED_KEYS)
+
+
 @pytest.mark.gpu
 def test_pipeline_color_input(ctx):
     rng = np.random.default_rng(5)
