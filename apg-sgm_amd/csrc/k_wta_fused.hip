@@ -68,6 +68,11 @@ __global__ __launch_bounds__(64) void wta_fused_kernel(apg_hip_params p, int w, 
     const bool act = pi < npb;
     const int d0 = p.min_disparity;
     const int nw = D / 2;   // words per row (D even)
+    // two lanes per pixel with D % 64 == 0: the row's second half starts one
+    // word late and rows are rw = nw + 2 words apart (rw = 2 mod 32), so the
+    // 32 lanes (pixel, half) of one scan read, and the staging writes, fall
+    // in 32 different banks; logical word j sits at j + (j >= gapw)
+    const int gapw = (LP == 2 && D % 64 == 0) ? nw / 2 : (1 << 30);
     const auto rc = raw_rsrc(cost);
     if (WIDE) {
         const uint32_t ebase = (uint32_t)(p_base * (size_t)D);
@@ -87,7 +92,8 @@ __global__ __launch_bounds__(64) void wta_fused_kernel(apg_hip_params p, int w, 
                 if (ch >= nchunk) continue;
                 const int e = ch * 16;
                 const int i = e / D;
-                uint32_t* dst = st + i * rw + (e - i * D) / 2;
+                const int wo = (e - i * D) / 2;   // a chunk's 8 words lie in one half (nw / 2 % 8 == 0)
+                uint32_t* dst = st + i * rw + wo + (wo >= gapw ? 1 : 0);
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
 #pragma unroll
@@ -140,10 +146,12 @@ __global__ __launch_bounds__(64) void wta_fused_kernel(apg_hip_params p, int w, 
     const int lo = dmin[pix], hi = dmax[pix];
     uint32_t* srow = st + pi * rw;
     const int j0 = hf * (nw / LP), j1 = j0 + nw / LP;   // this lane's words of the row
+    const uint32_t* lrow = srow + (j0 >= gapw ? 1 : 0);   // them, past the gap
+    auto p16 = [&](int u) { return u + ((u >> 1) >= gapw ? 2 : 0); };   // u16 entry u of the row
     // pass 1: earliest minimum as the key S << 8 | d-index
     uint32_t best = 0xFFFFFFFFu;
     for (int j = j0; j < j1; ++j) {
-        const uint32_t v = srow[j];
+        const uint32_t v = lrow[j];
         const uint32_t idx = (uint32_t)(2 * j) | ((uint32_t)(2 * j + 1) << 8);
         const uint32_t kl = __builtin_amdgcn_perm(v, idx, 0x0c050400u);
         const uint32_t kh = __builtin_amdgcn_perm(v, idx, 0x0c070601u);
@@ -158,17 +166,17 @@ __global__ __launch_bounds__(64) void wta_fused_kernel(apg_hip_params p, int w, 
     const uint16_t* row16 = (const uint16_t*)srow;
     uint32_t v_m1 = 0xFFFFu, v_p1 = 0xFFFFu;
     if (found) {
-        v_m1 = bi > 0 ? row16[bi - 1] : 0xFFFFu;
-        v_p1 = bi + 1 < D ? row16[bi + 1] : 0xFFFFu;
+        v_m1 = bi > 0 ? row16[p16(bi - 1)] : 0xFFFFu;
+        v_p1 = bi + 1 < D ? row16[p16(bi + 1)] : 0xFFFFu;
         uint16_t* w16 = (uint16_t*)srow;
-        w16[bi] = 0xFFFFu;
-        if (bi > 0) w16[bi - 1] = 0xFFFFu;
-        if (bi + 1 < D) w16[bi + 1] = 0xFFFFu;
+        w16[p16(bi)] = 0xFFFFu;
+        if (bi > 0) w16[p16(bi - 1)] = 0xFFFFu;
+        if (bi + 1 < D) w16[p16(bi + 1)] = 0xFFFFu;
     }
     // pass 2: second best outside best-1..best+1 (both lanes of a pixel
     // poisoned the same words, and LDS operations of a wave are in order)
     uint32_t m2 = 0xFFFFFFFFu;
-    for (int j = j0; j < j1; ++j) m2 = as_u32(__builtin_elementwise_min(as_v2(m2), as_v2(srow[j])));
+    for (int j = j0; j < j1; ++j) m2 = as_u32(__builtin_elementwise_min(as_v2(m2), as_v2(lrow[j])));
     if (LP == 2)
         m2 = as_u32(__builtin_elementwise_min(as_v2(m2), as_v2((uint32_t)__builtin_amdgcn_mov_dpp((int)m2, 0xB1, 0xF,
                                                                                                  0xF, false))));
@@ -239,10 +247,12 @@ hipError_t launch_wta_fused(hipStream_t s, const apg_hip_params& p, int w, int h
                             uint8_t* reliable, float* disp_wta, float* best, float* second, uint32_t* sum32) {
     const int D = p.max_disparity - p.min_disparity;
     if (D % 2 || D > 256) return hipErrorInvalidValue;
-    const int rw = (D / 2) | 1;   // odd row stride in words
     const size_t npx = (size_t)w * h;
     const bool wide = cost_u8 && D % 16 == 0 && !sum32;
     const int lp = wide ? 2 : 1;
+    // row stride in words: odd for one lane per pixel; with two lanes and
+    // D % 64 == 0, D/2 + 2 (2 mod 32) plus the one-word gap between the halves
+    const int rw = (wide && D % 64 == 0) ? D / 2 + 2 : (D / 2) | 1;
     const size_t shm = (size_t)(WF_PB / lp) * rw * 4;
     dim3 grid((unsigned)((npx + WF_PB / lp - 1) / (WF_PB / lp)));
 #define APG_WTAF(TC, NP, WD)                                                                                   \
