@@ -25,8 +25,15 @@ __device__ __forceinline__ float clampf_(float v, float lo, float hi) { return v
 // S of one (d, d+1) pair from C' and the planes' deltas, all as u16 halves
 // (u8 C' undecoded: 255 = invalid): npath * C' + sum of the deltas (< 65535
 // for valid C', checked by the host), 65535 where C' is invalid.
-template <typename TC, int NP>
+// ALLV: the caller knows no C' entry of the pair is invalid.
+template <typename TC, int NP, bool ALLV = false>
 __device__ __forceinline__ uint32_t s_pair(uint32_t ch, const uint32_t (&ph)[NP]) {
+    if (ALLV) {
+        u16x2 ds = as_v2(ph[0]);
+#pragma unroll
+        for (int f = 1; f < NP; ++f) ds = ds + as_v2(ph[f]);
+        return as_u32(as_v2(ch) * splat2(NP) + ds);
+    }
     const uint32_t c16 = sizeof(TC) == 1 ? as_u32(as_v2(ch) - (as_v2((ch + 0x00010001u) & 0xFF00FF00u)))   // 255 -> 65535
                                          : ch;
     u16x2 ds = as_v2(ph[0]);
@@ -77,7 +84,12 @@ __global__ __launch_bounds__(64) void wta_fused_kernel(apg_hip_params p, int w, 
     if (WIDE) {
         const uint32_t ebase = (uint32_t)(p_base * (size_t)D);
         const int nchunk = npb * D / 16;
-        for (int k0 = 0; k0 < nchunk; k0 += 64 * WF_WG) {
+        // every state of the block valid: its pixels lie in one row with x >= the
+        // largest disparity (column ranges: C'(x, d) is valid iff d <= x)
+        const int xb0 = (int)(p_base % (size_t)w);
+        const bool allv = xb0 + npb <= w && xb0 >= d0 + D - 1 && d0 >= 0;
+        auto stage = [&](int k0, auto allv_t) {
+            constexpr bool AV = decltype(allv_t)::value;
             u32x4 cw[WF_WG], pw[WF_WG][NP];
 #pragma unroll
             for (int g = 0; g < WF_WG; ++g) {
@@ -102,10 +114,14 @@ __global__ __launch_bounds__(64) void wta_fused_kernel(apg_hip_params p, int w, 
                         uint32_t ph[NP];
 #pragma unroll
                         for (int f = 0; f < NP; ++f) ph[f] = __builtin_amdgcn_perm(0u, pw[g][f][q], sel);
-                        dst[q * 2 + hh] = s_pair<TC, NP>(__builtin_amdgcn_perm(0u, cw[g][q], sel), ph);
+                        dst[q * 2 + hh] = s_pair<TC, NP, AV>(__builtin_amdgcn_perm(0u, cw[g][q], sel), ph);
                     }
                 }
             }
+        };
+        for (int k0 = 0; k0 < nchunk; k0 += 64 * WF_WG) {
+            if (allv) stage(k0, std::true_type{});
+            else stage(k0, std::false_type{});
         }
     }
     // staging: one pixel per wave iteration, lane = pair index (two passes for D > 128)

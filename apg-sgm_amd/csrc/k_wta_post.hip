@@ -187,17 +187,15 @@ hipError_t launch_lr_check(hipStream_t s, const apg_hip_params& p, int w, int h,
     return hipGetLastError();
 }
 
-__device__ __forceinline__ void fill_pixel(int w, int h, int max_gap, int x, int y, const float* row,
+// A hole pixel x of row y once the gap scans have stopped at xl (left) and
+// xr (right), as the reference's loops leave them (reference
+// src/postprocess.cpp:60-130).
+__device__ __forceinline__ float fill_hole(int w, int h, int x, int y, int xl, int xr, const float* row,
                                            const float* __restrict__ in, const uint8_t* __restrict__ reason,
-                                           const uint8_t* __restrict__ gray, const float* __restrict__ prior,
-                                           float* __restrict__ out) {
+                                           const uint8_t* __restrict__ gray, const float* __restrict__ prior) {
     const size_t q = (size_t)y * w + x;
     float o = row[x];
-    if (o < 0.f) {
-        int xl = x - 1;
-        while (xl >= 0 && xl >= x - max_gap && row[xl] < 0.f) --xl;
-        int xr = x + 1;
-        while (xr < w && xr <= x + max_gap && row[xr] < 0.f) ++xr;
+    {
         const float dl = (xl >= 0 && row[xl] >= 0.f) ? row[xl] : -1.f;
         const float drr = (xr < w && row[xr] >= 0.f) ? row[xr] : -1.f;
         if (reason[q] == R_OCCLUSION) {
@@ -228,7 +226,78 @@ __device__ __forceinline__ void fill_pixel(int w, int h, int max_gap, int x, int
             else if (drr >= 0.f) o = drr;
         }
     }
+    return o;
+}
+
+__device__ __forceinline__ void fill_pixel(int w, int h, int max_gap, int x, int y, const float* row,
+                                           const float* __restrict__ in, const uint8_t* __restrict__ reason,
+                                           const uint8_t* __restrict__ gray, const float* __restrict__ prior,
+                                           float* __restrict__ out) {
+    const size_t q = (size_t)y * w + x;
+    float o = row[x];
+    if (o < 0.f) {
+        int xl = x - 1;
+        while (xl >= 0 && xl >= x - max_gap && row[xl] < 0.f) --xl;
+        int xr = x + 1;
+        while (xr < w && xr <= x + max_gap && row[xr] < 0.f) ++xr;
+        o = fill_hole(w, h, x, y, xl, xr, row, in, reason, gray, prior);
+    }
     out[q] = o;
+}
+
+// The same fill with the gap scans replaced by bit searches: one block per
+// row, the row in LDS with one 64-bit mask of its valid pixels per 64
+// columns (a wavefront ballot).  The left scan stops at the nearest valid
+// pixel in [x - max_gap - 1, x - 1] if there is one, else at
+// max(x - max_gap - 1, -1); the right scan likewise.
+__global__ __launch_bounds__(256) void fill_mask_kernel(int w, int h, int max_gap, const float* __restrict__ in,
+                                                        const uint8_t* __restrict__ reason,
+                                                        const uint8_t* __restrict__ gray,
+                                                        const float* __restrict__ prior, float* __restrict__ out) {
+    extern __shared__ __align__(8) unsigned char fsh[];
+    const int nseg = (w + 63) / 64;
+    uint64_t* smask = (uint64_t*)fsh;
+    float* srow = (float*)(fsh + (size_t)nseg * 8);
+    const int y = blockIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const float* row = in + (size_t)y * w;
+    for (int s = wave; s < nseg; s += 4) {
+        const int x = s * 64 + lane;
+        const float v = x < w ? row[x] : -1.f;
+        if (x < w) srow[x] = v;
+        const uint64_t m = __ballot(v >= 0.f);
+        if (lane == 0) smask[s] = m;
+    }
+    __syncthreads();
+    for (int x = tid; x < w; x += 256) {
+        float o = srow[x];
+        if (o < 0.f) {
+            const int lo = x - max_gap - 1, hi = x + max_gap + 1;
+            // nearest valid pixel left of x, at or right of lo
+            int L = -1;
+            {
+                int s = x >> 6;
+                uint64_t m = smask[s] & ((1ull << (x & 63)) - 1ull);
+                while (m == 0ull && s > 0 && s * 64 > lo) m = smask[--s];
+                if (m != 0ull) L = s * 64 + 63 - __clzll((long long)m);
+                if (L < lo) L = -1;
+            }
+            // nearest valid pixel right of x, at or left of hi
+            int Rv = -1;
+            {
+                int s = x >> 6;
+                const int b = x & 63;
+                uint64_t m = b == 63 ? 0ull : smask[s] & ~((2ull << b) - 1ull);
+                while (m == 0ull && s + 1 < nseg && (s + 1) * 64 <= hi) m = smask[++s];
+                if (m != 0ull) Rv = s * 64 + __ffsll((long long)m) - 1;
+                if (Rv > hi) Rv = -1;
+            }
+            const int xl = L >= 0 ? L : max(lo, -1);
+            const int xr = Rv >= 0 ? Rv : min(hi, w);
+            o = fill_hole(w, h, x, y, xl, xr, srow, in, reason, gray, prior);
+        }
+        out[(size_t)y * w + x] = o;
+    }
 }
 
 // PostProcessor::fill_holes, Legacy route (reference src/postprocess.cpp:42-168):
@@ -319,7 +388,11 @@ __global__ __launch_bounds__(256) void median3_kernel(int w, int h, const float*
 
 hipError_t launch_fill(hipStream_t s, const apg_hip_params& p, int w, int h, const float* disp, const uint8_t* reason,
                        const uint8_t* gray, const float* prior, float* out) {
-    if ((size_t)w * sizeof(float) <= 32 * 1024)
+    const size_t shm = (size_t)(w + 63) / 64 * 8 + (size_t)w * sizeof(float);
+    if (shm <= 64 * 1024)
+        hipLaunchKernelGGL(fill_mask_kernel, dim3(h), dim3(256), shm, s, w, h, p.post_max_fill_gap, disp, reason,
+                           gray, prior, out);
+    else if ((size_t)w * sizeof(float) <= 32 * 1024)
         hipLaunchKernelGGL(fill_kernel<true>, dim3(1, h), dim3(256), (size_t)w * sizeof(float), s, w, h,
                            p.post_max_fill_gap, disp, reason, gray, prior, out);
     else
