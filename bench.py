@@ -74,7 +74,7 @@ def cpu_baseline(left, right, p, gpu_disp):
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_balanced_1280x720_d128.md")
-STAGE_KERNEL = {"cost": "cost_crossh2_kernel", "cross": "cross_v3_kernel", "sgm": "sgm_lines4_kernel",
+STAGE_KERNEL = {"cost": "cost_crossh2_kernel", "cross": "cross_v3_kernel", "sgm": "sgm_half_kernel",
                 "wta": "wta_fused_kernel"}
 
 
