@@ -675,14 +675,11 @@ int apg_hip_pipeline(apg_hip_ctx ctx, const apg_hip_params* p, const uint8_t* d_
     // the fused schedule's right-view WTA keys start cleared, by the aux pass
     uint32_t* rkeys = fused ? buf<uint32_t>(ctx, B_RKEYS) : nullptr;
     mark(ctx, S_AUX);
-    HIPCHK(launch_aux(st, *p, d_left, w, h, stride, channels, gl, gxl, gyl, sym ? (uint32_t*)cl : nullptr,
-                      sym ? nullptr : (uint64_t*)cl),
-           "aux L");
-    HIPCHK(launch_aux(st, *p, d_right, w, h, stride, channels, gr, gxr, gyr, sym ? (uint32_t*)cr : nullptr,
-                      sym ? nullptr : (uint64_t*)cr, rkeys),
-           "aux R");
+    // both views and the dense range init in one launch
+    const AuxViews av_l{d_left, gl, gxl, gyl, sym ? (uint32_t*)cl : nullptr, sym ? nullptr : (uint64_t*)cl};
+    const AuxViews av_r{d_right, gr, gxr, gyr, sym ? (uint32_t*)cr : nullptr, sym ? nullptr : (uint64_t*)cr};
+    HIPCHK(launch_aux_pair(st, *p, w, h, stride, channels, av_l, av_r, rkeys, dmin, dmax), "aux");
     mark(ctx, S_PRIOR);
-    HIPCHK(launch_range_init(st, *p, w, h, dmin, dmax), "range");
     float* d_prior = p->prior_enable ? buf<float>(ctx, B_PRIOR) : nullptr;
     if (d_prior) {
         if (int s = run_prior(ctx, p, w, h, gxl, gyl, cl, cr, dmin, dmax, d_prior, nullptr)) return s;

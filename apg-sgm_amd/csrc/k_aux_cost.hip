@@ -63,12 +63,48 @@ constexpr int AUX_TX = 64, AUX_TY = 4;
 // build_gray_and_grad + census (reference src/cost_computer.cpp:33-122,
 // include/apg_sgm/image_view.hpp:50-56). The LDS tile is loaded with
 // clamp-to-edge coordinates, which is exactly Image8::sample.
+// One launch can serve both views (blockIdx.z selects the image); the left
+// view may also write the dense range init [min_disparity, min(max_disparity,
+// x+1)) (range_kernel below) and the right view clear the right-WTA keys.
+struct AuxImage {
+    const uint8_t* src;
+    uint8_t *gray, *gx, *gy;
+    uint32_t* c32;
+    uint64_t* c64;
+    uint32_t* keys;
+    int16_t *dmin, *dmax;
+};
+struct AuxPair {
+    AuxImage img[2];
+};
+
+__device__ __forceinline__ void aux_pixel(const AuxImage& im, int w, int h, int stride, int ch, int census_type,
+                                          int gmin, int gmax, uint8_t (&tile)[AUX_TY + 6][AUX_TX + 8]);
+
+__global__ __launch_bounds__(256) void aux_pair_kernel(AuxPair ap, int w, int h, int stride, int ch, int census_type,
+                                                       int gmin, int gmax) {
+    __shared__ uint8_t tile[AUX_TY + 6][AUX_TX + 8];
+    aux_pixel(ap.img[blockIdx.z], w, h, stride, ch, census_type, gmin, gmax, tile);
+}
+
 __global__ __launch_bounds__(256) void aux_kernel(const uint8_t* __restrict__ src, int w, int h, int stride, int ch,
                                                  int census_type, uint8_t* __restrict__ gray,
                                                  uint8_t* __restrict__ gx, uint8_t* __restrict__ gy,
                                                  uint32_t* __restrict__ c32, uint64_t* __restrict__ c64,
                                                  uint32_t* __restrict__ keys) {
     __shared__ uint8_t tile[AUX_TY + 6][AUX_TX + 8];
+    aux_pixel(AuxImage{src, gray, gx, gy, c32, c64, keys, nullptr, nullptr}, w, h, stride, ch, census_type, 0, 0, tile);
+}
+
+__device__ __forceinline__ void aux_pixel(const AuxImage& im, int w, int h, int stride, int ch, int census_type,
+                                          int gmin, int gmax, uint8_t (&tile)[AUX_TY + 6][AUX_TX + 8]) {
+    const uint8_t* __restrict__ src = im.src;
+    uint8_t* __restrict__ gray = im.gray;
+    uint8_t* __restrict__ gx = im.gx;
+    uint8_t* __restrict__ gy = im.gy;
+    uint32_t* __restrict__ c32 = im.c32;
+    uint64_t* __restrict__ c64 = im.c64;
+    uint32_t* __restrict__ keys = im.keys;
     const int x0 = blockIdx.x * AUX_TX, y0 = blockIdx.y * AUX_TY;
     const int tid = threadIdx.y * AUX_TX + threadIdx.x;
     for (int i = tid; i < (AUX_TY + 6) * (AUX_TX + 8); i += 256) {
@@ -92,6 +128,10 @@ __global__ __launch_bounds__(256) void aux_kernel(const uint8_t* __restrict__ sr
     gy[pi] = (uint8_t)min(abs(gyv), 255);
     // the fused schedule's right-view WTA key, cleared here to save a fill pass
     if (keys) keys[pi] = 0xFFFFFFFFu;
+    if (im.dmin) {   // as range_kernel
+        im.dmin[pi] = (int16_t)gmin;
+        im.dmax[pi] = (int16_t)max(min(gmax, x + 1), gmin);
+    }
     if (census_type == 1) {
         uint32_t bits = 0;
         int b = 0;
@@ -118,6 +158,17 @@ __global__ __launch_bounds__(256) void aux_kernel(const uint8_t* __restrict__ sr
             }
         c64[pi] = bits;
     }
+}
+
+hipError_t launch_aux_pair(hipStream_t s, const apg_hip_params& p, int w, int h, int stride, int ch,
+                           const AuxViews& L, const AuxViews& R, uint32_t* rkeys, int16_t* dmin, int16_t* dmax) {
+    AuxPair ap;
+    ap.img[0] = AuxImage{L.src, L.gray, L.gx, L.gy, L.c32, L.c64, nullptr, dmin, dmax};
+    ap.img[1] = AuxImage{R.src, R.gray, R.gx, R.gy, R.c32, R.c64, rkeys, nullptr, nullptr};
+    dim3 grid((w + AUX_TX - 1) / AUX_TX, (h + AUX_TY - 1) / AUX_TY, 2), block(AUX_TX, AUX_TY);
+    hipLaunchKernelGGL(aux_pair_kernel, grid, block, 0, s, ap, w, h, stride, ch, p.census_type, p.min_disparity,
+                       p.max_disparity);
+    return hipGetLastError();
 }
 
 hipError_t launch_aux(hipStream_t s, const apg_hip_params& p, const uint8_t* src, int w, int h, int stride,

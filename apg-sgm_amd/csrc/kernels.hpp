@@ -47,6 +47,17 @@ constexpr int kMaxSgmK = 4;           // packed u16 pairs per lane in the SGM sc
 hipError_t launch_aux(hipStream_t s, const apg_hip_params& p, const uint8_t* src, int w, int h, int stride,
                       int ch, uint8_t* gray, uint8_t* gx, uint8_t* gy, uint32_t* c32, uint64_t* c64,
                       uint32_t* keys = nullptr);
+// one view's inputs and aux outputs
+struct AuxViews {
+    const uint8_t* src;
+    uint8_t *gray, *gx, *gy;
+    uint32_t* c32;
+    uint64_t* c64;
+};
+// both views in one launch; also the dense range init (left) and the cleared
+// right-WTA keys (rkeys may be null)
+hipError_t launch_aux_pair(hipStream_t s, const apg_hip_params& p, int w, int h, int stride, int ch,
+                           const AuxViews& L, const AuxViews& R, uint32_t* rkeys, int16_t* dmin, int16_t* dmax);
 hipError_t launch_range_init(hipStream_t s, const apg_hip_params& p, int w, int h, int16_t* dmin, int16_t* dmax);
 hipError_t launch_cost(hipStream_t s, const apg_hip_params& p, const CostLut* d_lut, int w, int h,
                        const uint8_t* gl, const uint8_t* gxl, const uint8_t* gyl,
