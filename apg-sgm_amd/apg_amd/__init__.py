@@ -56,7 +56,8 @@ class Timing(C.Structure):
 class Intermediates(C.Structure):
     _fields_ = [("d_cost", C.c_void_p), ("d_cost_agg", C.c_void_p), ("d_sum32", C.c_void_p),
                 ("d_disp_right", C.c_void_p), ("d_disp_wta", C.c_void_p), ("d_best", C.c_void_p),
-                ("d_second", C.c_void_p), ("d_dmin", C.c_void_p), ("d_dmax", C.c_void_p), ("d_prior", C.c_void_p)]
+                ("d_second", C.c_void_p), ("d_dmin", C.c_void_p), ("d_dmax", C.c_void_p), ("d_prior", C.c_void_p),
+                ("d_after_refine", C.c_void_p)]
 
 
 class Tuning(C.Structure):
@@ -289,12 +290,14 @@ class PipelineOutputs:
     dmin: np.ndarray | None = None
     dmax: np.ndarray | None = None
     d_prior: np.ndarray | None = None
+    after_refine: np.ndarray | None = None
 
 
 ALL_INTERMEDIATES = ("cost", "cost_agg", "sgm", "disp_right", "disp_wta", "best", "second", "dmin", "dmax",
-                     "d_prior")
+                     "d_prior", "after_refine")
 # requesting the raw (cost) or post-Cross (cost_agg) volume selects the stage-by-stage schedule
-FUSED_INTERMEDIATES = ("sgm", "disp_right", "disp_wta", "best", "second", "dmin", "dmax", "d_prior")
+FUSED_INTERMEDIATES = ("sgm", "disp_right", "disp_wta", "best", "second", "dmin", "dmax", "d_prior",
+                       "after_refine")
 
 
 def set_schedule(ctx: "Context", schedule: int):
@@ -321,7 +324,7 @@ def run_pipeline(ctx: Context, p: Params, left: np.ndarray, right: np.ndarray, i
         shapes = {"cost": ((h, w, D), np.uint16), "cost_agg": ((h, w, D), np.uint16), "sgm": ((h, w, D), np.uint32),
                   "disp_right": ((h, w), np.float32), "disp_wta": ((h, w), np.float32), "best": ((h, w), np.float32),
                   "second": ((h, w), np.float32), "dmin": ((h, w), np.int16), "dmax": ((h, w), np.int16),
-                  "d_prior": ((h, w), np.float32)}
+                  "d_prior": ((h, w), np.float32), "after_refine": ((h, w), np.float32)}
         arrs = {k: ctx.array(*shapes[k]) for k in want}
         inter = Intermediates(*[arrs[k].ptr if k in arrs else None for k in ALL_INTERMEDIATES])
     rc = ctx.lib.apg_hip_pipeline(ctx.h, C.byref(p), dl.ptr, dr.ptr, w, h, w * ch, ch, out_d.ptr, out_c.ptr,

@@ -62,6 +62,7 @@ struct apg_hip_ctx_s {
     KTune tune;
     bool capturing = false;   // stream capture in progress: no allocation, no host copy
     std::vector<float> prior_grid_host;   // source of the last prior grid upload
+    std::vector<apg_hip_support> last_supports;   // the last prior's supports
 };
 
 namespace {
@@ -523,6 +524,14 @@ int apg_hip_prior_estimate(apg_hip_ctx ctx, const apg_hip_params* p, int w, int 
     return 0;
 }
 
+int apg_hip_last_supports(apg_hip_ctx ctx, apg_hip_support* out, int capacity, int* count) {
+    if (!ctx) return 8;
+    const int n = (int)ctx->last_supports.size();
+    if (count) *count = n;
+    if (out && capacity > 0) std::copy_n(ctx->last_supports.begin(), std::min(n, capacity), out);
+    return 0;
+}
+
 int apg_hip_refine(apg_hip_ctx ctx, const apg_hip_params* p, int w, int h, const uint8_t* d_gray_l,
                    const uint8_t* d_gx_l, const uint8_t* d_gy_l, const uint8_t* d_gray_r, const uint8_t* d_gx_r,
                    const uint8_t* d_gy_r, const uint32_t* d_c32l, const uint32_t* d_c32r, const uint64_t* d_c64l,
@@ -560,6 +569,7 @@ int run_prior(apg_hip_ctx ctx, const apg_hip_params* p, int w, int h, const uint
     hipStream_t st = ctx->stream;
     const size_t n = (size_t)w * h;
     if (n_supports) *n_supports = 0;
+    ctx->last_supports.clear();
     HIPCHK(launch_fill_u32(st, n, 0xBF800000u /* -1.0f */, (uint32_t*)d_prior), "prior init");
     const int sw = prior_samples_x(w), sh = prior_samples_y(h);
     if (!p->prior_enable || sw <= 0 || sh <= 0) return 0;
@@ -582,7 +592,7 @@ int run_prior(apg_hip_ctx ctx, const apg_hip_params* p, int w, int h, const uint
     HIPCHK(download_vec(st, hu, cu, ns), "prior download");
     HIPCHK(hipStreamSynchronize(st), "prior sync");
     PriorGrid g;
-    const int nsup = prior_build_grid(*p, w, h, hd.data(), hu.data(), g);
+    const int nsup = prior_build_grid(*p, w, h, hd.data(), hu.data(), g, &ctx->last_supports);
     if (n_supports) *n_supports = nsup;
     if (nsup < p->prior_min_supports || nsup == 0) return 0;
     const size_t nc = (size_t)g.gw * g.gh;
@@ -716,10 +726,14 @@ int apg_hip_pipeline(apg_hip_ctx ctx, const apg_hip_params* p, const uint8_t* d_
         mark(ctx, S_SGM);
         HIPCHK(launch_sgm_lines(st, *p, w, h, D, gl, cp, u8, planes, ctx->tune), "sgm_lines");
         mark(ctx, S_WTA);
+        // without the refiner, the disparity after it is the WTA output
+        float* wta_out = inter ? (inter->d_disp_wta ? inter->d_disp_wta : inter->d_after_refine) : nullptr;
         HIPCHK(launch_wta_fused(st, *p, w, h, cp, u8, planes, dmin, dmax, dr, gxl, gyl, d_disparity, d_confidence, d_reason,
-                                d_reliable, inter ? inter->d_disp_wta : nullptr, inter ? inter->d_best : nullptr,
+                                d_reliable, wta_out, inter ? inter->d_best : nullptr,
                                 inter ? inter->d_second : nullptr, inter ? inter->d_sum32 : nullptr),
                "wta_fused");
+        if (inter && inter->d_after_refine && wta_out != inter->d_after_refine)
+            HIPCHK(hipMemcpyAsync(inter->d_after_refine, wta_out, n * 4, hipMemcpyDeviceToDevice, st), "copy refined");
         if (inter && inter->d_disp_right)
             HIPCHK(hipMemcpyAsync(inter->d_disp_right, dr, n * 4, hipMemcpyDeviceToDevice, st), "copy dr");
         mark(ctx, S_CONF);
@@ -770,6 +784,8 @@ int apg_hip_pipeline(apg_hip_ctx ctx, const apg_hip_params* p, const uint8_t* d_
         if (int s = run_refine(ctx, p, w, h, gl, gxl, gyl, gr, gxr, gyr, cl, cr, dmin, dmax, d_reliable, d_disparity))
             return s;
     }
+    if (inter && inter->d_after_refine)
+        HIPCHK(hipMemcpyAsync(inter->d_after_refine, d_disparity, n * 4, hipMemcpyDeviceToDevice, st), "copy refined");
     mark(ctx, S_POST);
     HIPCHK(launch_lr_check(st, *p, w, h, d_disparity, dr, d_reason), "lr");
     HIPCHK(launch_fill(st, *p, w, h, d_disparity, d_reason, gl, d_prior, scratch), "fill");

@@ -1,8 +1,11 @@
 // C++ host layer: the reference's stage classes, StereoMatcher, workspace and
 // session, implemented on the device C ABI (include/apg_sgm_hip.h).
+#include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <new>
 #include <sstream>
 #include <stdexcept>
@@ -734,17 +737,33 @@ Status StereoMatcher::compute_impl(const ImageView8& left, const ImageView8& rig
         const auto t0 = std::chrono::steady_clock::now();
         std::vector<float> dr_host;
         apg_hip_intermediates inter{};
-        Dev* dr_dev = nullptr;
+        // the legacy buffers take the right disparity, the ranges and the
+        // disparities around the refiner; the stats need the ranges when a
+        // prior narrowed them and the refiner's input and output when it ran
+        const bool want_range = extra || (stats && p.prior_enable);
+        const bool want_refine = extra || (stats && p.refine_enable);
+        std::unique_ptr<Dev> dr_dev, dmin_dev, dmax_dev, db_dev, da_dev;
         if (extra) {
-            dr_dev = new Dev(ctx, n * 4);
+            dr_dev.reset(new Dev(ctx, n * 4));
             inter.d_disp_right = dr_dev->as<float>();
         }
-        rc = apg_hip_pipeline(ctx, &p, base, base + o_r, w, h, stride, ch, reinterpret_cast<float*>(base + o_d),
-                              reinterpret_cast<float*>(base + o_c), base + o_re, base + o_m, extra ? &inter : nullptr);
-        if (rc) {
-            delete dr_dev;
-            return status_from(rc, apg_hip_last_error(ctx));
+        if (want_range) {
+            dmin_dev.reset(new Dev(ctx, n * 2));
+            dmax_dev.reset(new Dev(ctx, n * 2));
+            inter.d_dmin = dmin_dev->as<int16_t>();
+            inter.d_dmax = dmax_dev->as<int16_t>();
         }
+        if (want_refine) {
+            db_dev.reset(new Dev(ctx, n * 4));
+            da_dev.reset(new Dev(ctx, n * 4));
+            inter.d_disp_wta = db_dev->as<float>();
+            inter.d_after_refine = da_dev->as<float>();
+        }
+        const bool any_inter = extra || want_range || want_refine;
+        rc = apg_hip_pipeline(ctx, &p, base, base + o_r, w, h, stride, ch, reinterpret_cast<float*>(base + o_d),
+                              reinterpret_cast<float*>(base + o_c), base + o_re, base + o_m,
+                              any_inter ? &inter : nullptr);
+        if (rc) return status_from(rc, apg_hip_last_error(ctx));
         result.disparity = Image32f(w, h, -1.f);
         result.confidence = Image32f(w, h, 0.f);
         result.invalid_reason = Image8u1(w, h, 0);
@@ -757,7 +776,29 @@ Status StereoMatcher::compute_impl(const ImageView8& left, const ImageView8& rig
             extra->disparity_right = Image32f(w, h, -1.f);
             rc = apg_hip_download(ctx, extra->disparity_right.data(), dr_dev->as<void>(), n * 4);
         }
-        delete dr_dev;
+        SearchRange range;
+        if (!rc && want_range) {
+            range.allocate(w, h, cfg_.min_disparity, cfg_.max_disparity);
+            rc = apg_hip_download(ctx, range.dmin.data(), dmin_dev->as<void>(), n * 2);
+            if (!rc) rc = apg_hip_download(ctx, range.dmax.data(), dmax_dev->as<void>(), n * 2);
+        }
+        Image32f before, after;
+        if (!rc && want_refine) {
+            before = Image32f(w, h, -1.f);
+            after = Image32f(w, h, -1.f);
+            rc = apg_hip_download(ctx, before.data(), db_dev->as<void>(), n * 4);
+            if (!rc) rc = apg_hip_download(ctx, after.data(), da_dev->as<void>(), n * 4);
+        }
+        std::vector<SupportMatch> supports;
+        if (!rc && p.prior_enable) {
+            int ns = 0;
+            apg_hip_last_supports(ctx, nullptr, 0, &ns);
+            std::vector<apg_hip_support> hs(ns);
+            apg_hip_last_supports(ctx, hs.data(), ns, &ns);
+            supports.resize(hs.size());
+            for (size_t i = 0; i < hs.size(); ++i)
+                supports[i] = SupportMatch{hs[i].x, hs[i].y, hs[i].disparity, hs[i].confidence};
+        }
         if (rc) return status_from(rc, apg_hip_last_error(ctx));
         const auto t1 = std::chrono::steady_clock::now();
         if (stats) {
@@ -785,13 +826,52 @@ Status StereoMatcher::compute_impl(const ImageView8& left, const ImageView8& rig
                 lr_fail += r == static_cast<uint8_t>(InvalidReason::Mismatch) ||
                            r == static_cast<uint8_t>(InvalidReason::Occlusion);
             }
+            // range statistics (reference src/pipeline.cpp:640-695); without a
+            // prior the ranges are the dense geometric init itself
+            double geom = 0.0, search = 0.0;
+            for (int x = 0; x < w; ++x) {
+                const int hi = std::min(cfg_.max_disparity, x + 1), lo = cfg_.min_disparity;
+                geom += (hi > lo ? hi - lo : 0) * static_cast<double>(h);
+            }
+            if (want_range) {
+                for (size_t i = 0; i < n; ++i) {
+                    const int lo = range.dmin.data()[i], hi = range.dmax.data()[i];
+                    search += hi > lo ? hi - lo : 0;
+                }
+            } else {
+                search = geom;
+            }
+            size_t refine_changed = 0;
+            if (want_refine)
+                for (size_t i = 0; i < n; ++i) {
+                    const float db = before.data()[i], da = after.data()[i];
+                    refine_changed += db >= 0.f && da >= 0.f && std::abs(da - db) > 1e-4f;
+                }
+            const int global_D = cfg_.max_disparity - cfg_.min_disparity;
+            stats->prior_support_count = supports.size();
+            stats->mean_search_width = n ? search / static_cast<double>(n) : 0.0;
+            stats->mean_geometry_width = n ? geom / static_cast<double>(n) : 0.0;
+            stats->geometry_reduction_ratio = global_D > 0 ? 1.0 - stats->mean_geometry_width / global_D : 0.0;
+            stats->prior_incremental_reduction_ratio =
+                stats->mean_geometry_width > 0.0 ? 1.0 - stats->mean_search_width / stats->mean_geometry_width : 0.0;
+            stats->search_reduction_ratio = global_D > 0 ? 1.0 - stats->mean_search_width / global_D : 0.0;
             stats->valid_pixels = valid;
             stats->reliable_pixels = reliable;
             stats->unreliable_pixels = n - reliable;
+            stats->refine_changed_pixels = refine_changed;
+            stats->lr_fail_pixels = lr_fail;
             stats->valid_disparity_ratio = n ? static_cast<float>(valid) / n : 0.f;
             stats->reliable_ratio = n ? static_cast<float>(reliable) / n : 0.f;
             stats->unreliable_ratio = n ? static_cast<float>(n - reliable) / n : 0.f;
+            stats->refine_changed_ratio = n ? static_cast<float>(refine_changed) / n : 0.f;
             stats->lr_fail_ratio = n ? static_cast<float>(lr_fail) / n : 0.f;
+        }
+        if (extra) {
+            extra->range = std::move(range);
+            extra->supports = std::move(supports);
+            extra->support_count = extra->supports.size();
+            extra->d_before_refine = std::move(before);
+            extra->d_after_refine = std::move(after);
         }
         return Status::Ok();
     } catch (const std::bad_alloc& e) {

@@ -33,7 +33,7 @@ float weighted_median(const std::vector<Pt>& pts, float half) {
 }  // namespace
 
 int prior_build_grid(const apg_hip_params& p, int w, int h, const int16_t* cand_d, const float* cand_u,
-                     PriorGrid& g) {
+                     PriorGrid& g, std::vector<apg_hip_support>* supports) {
     const int sw = prior_samples_x(w);
     g.gw = (w + kPriorCell - 1) / kPriorCell;
     g.gh = (h + kPriorCell - 1) / kPriorCell;
@@ -48,6 +48,7 @@ int prior_build_grid(const apg_hip_params& p, int w, int h, const int16_t* cand_
 
     // candidates of each cell, raster order, at most per_cell (extract_supports :35-106)
     std::vector<std::vector<Pt>> cand(ncell);
+    std::vector<std::vector<std::pair<int, int>>> cxy(supports ? ncell : 0);   // their pixels
     for (int cy = 0; cy < g.gh; ++cy)
         for (int cx = 0; cx < g.gw; ++cx) {
             std::vector<Pt>& dst = cand[cy * g.gw + cx];
@@ -56,7 +57,10 @@ int prior_build_grid(const apg_hip_params& p, int w, int h, const int16_t* cand_
             for (int y = y0 + (y0 & 1); y < y1 && (int)dst.size() < per_cell; y += 2)
                 for (int x = x0 + (x0 & 1); x < x1 && (int)dst.size() < per_cell; x += 2) {
                     const size_t si = (size_t)((y - 2) / 2) * sw + (x - 2) / 2;
-                    if (cand_d[si] != kNoCandidate) dst.push_back(Pt((float)cand_d[si], cand_u[si]));
+                    if (cand_d[si] != kNoCandidate) {
+                        dst.push_back(Pt((float)cand_d[si], cand_u[si]));
+                        if (supports) cxy[cy * g.gw + cx].emplace_back(x, y);
+                    }
                 }
         }
 
@@ -86,6 +90,13 @@ int prior_build_grid(const apg_hip_params& p, int w, int h, const int16_t* cand_
             ++taken[cells[(size_t)((uint64_t)k * m / (uint64_t)budget)]];
             ++n;
         }
+    }
+    if (supports) {   // the selected set: each cell's first taken[] candidates
+        supports->clear();
+        for (int i = 0; i < ncell; ++i)
+            for (int k = 0; k < taken[i]; ++k)
+                supports->push_back(apg_hip_support{cxy[i][k].first, cxy[i][k].second, cand[i][k].first,
+                                                    cand[i][k].second});
     }
     if (n < p.prior_min_supports || n == 0) return n;
 

@@ -32,7 +32,7 @@ struct PriorGrid {
 // device's per-sample candidates (cand_d = kNoCandidate where the sample is
 // not a candidate).  Returns the number of supports kept.
 int prior_build_grid(const apg_hip_params& p, int w, int h, const int16_t* cand_d, const float* cand_u,
-                     PriorGrid& grid);
+                     PriorGrid& grid, std::vector<apg_hip_support>* supports = nullptr);
 
 // Refiner::refine (reference src/refiner.cpp:14-98) in place on disp.
 // census64 is used when census32 is null.

@@ -498,7 +498,7 @@ __global__ __launch_bounds__(64) void sgm_half_kernel(apg_hip_params p, int w, i
         s_p2[nmax + t] = 0u;
     }
     // batch transfer: lane -> (row j, 16-byte chunk c); one instruction per chain covers SB rows
-    const int lpx = D / 16, ppl = 64 / lpx;
+    const int lpx = D / 16;
     const int jl = lane / lpx, cl = lane % lpx;
     const bool inr = FULL || 4 * hl < D;
     const uint32_t P1 = (uint32_t)p.P1;

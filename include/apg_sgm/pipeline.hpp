@@ -60,12 +60,23 @@ struct PipelineStats {
     std::string requested_backend = "Auto";
     std::string resolved_backend = "Dense";
     std::string memory_policy = "ReleasePerFrame";
+    // search ranges: mean width of the frame's ranges and of the dense
+    // geometric init [min, min(max, x+1)), and the reductions they give
+    size_t prior_support_count = 0;
+    double mean_search_width = 0.0;
+    double mean_geometry_width = 0.0;
+    double geometry_reduction_ratio = 0.0;
+    double prior_incremental_reduction_ratio = 0.0;
+    double search_reduction_ratio = 0.0;
     size_t valid_pixels = 0;
     size_t reliable_pixels = 0;
     size_t unreliable_pixels = 0;
+    size_t refine_changed_pixels = 0;
+    size_t lr_fail_pixels = 0;
     float valid_disparity_ratio = 0.f;
     float reliable_ratio = 0.f;
     float unreliable_ratio = 0.f;
+    float refine_changed_ratio = 0.f;
     float lr_fail_ratio = 0.f;
     std::string to_json() const;
 };

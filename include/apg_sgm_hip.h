@@ -226,7 +226,23 @@ typedef struct {
     int16_t* d_dmin;         /* w*h search range (prior-narrowed in HighQuality) */
     int16_t* d_dmax;         /* w*h */
     float* d_prior;          /* w*h prior disparity, -1 where there is none */
+    float* d_after_refine;   /* w*h disparity after the refiner (= WTA output when
+                                the refiner is off), before the LR check */
 } apg_hip_intermediates;
+
+/* One support match of the HighQuality prior (reference
+ * include/apg_sgm/types.hpp SupportMatch): pixel, disparity, uniqueness. */
+typedef struct {
+    int x, y;
+    float disparity, confidence;
+} apg_hip_support;
+
+/* The supports the last prior (apg_hip_pipeline or apg_hip_prior_estimate on
+ * this context) selected, in cell order: copies min(count, capacity) entries
+ * to out (out may be NULL with capacity 0) and stores the total in *count.
+ * No counterpart in the reference's C API; it backs PipelineBuffers::supports
+ * (reference include/apg_sgm/buffers.hpp) for the benchmark's metrics. */
+APG_HIP_API int apg_hip_last_supports(apg_hip_ctx ctx, apg_hip_support* out, int capacity, int* count);
 
 /* Schedule of apg_hip_pipeline on this context: 0 (default) runs the fused
  * schedule whenever it is eligible (no raw-volume intermediates requested,

@@ -758,6 +758,7 @@ int orc_pipeline(const orc_cfg* cfg, const uint8_t* left, const uint8_t* right, 
     if (inter && inter->second) memcpy(inter->second, second, n * 4);
     orc_confidence(cfg, w, h, disparity, dr, gxl, gyl, best, second, confidence, reliable, reason);
     orc_refine(cfg, w, h, gl, gxl, gyl, gr, gxr, gyr, c32l, c32r, c64l, c64r, dmin, dmax, reliable, disparity);
+    if (inter && inter->after_refine) memcpy(inter->after_refine, disparity, n * 4);
     orc_lr_check(cfg, w, h, disparity, dr, reason);
     orc_fill_holes(cfg, w, h, disparity, reason, gl, d_prior);
     orc_median(cfg, w, h, disparity);

@@ -186,6 +186,8 @@ typedef struct {
     int16_t* dmin;           /* W*H search range (prior-narrowed in HighQuality) */
     int16_t* dmax;           /* W*H */
     float* d_prior;          /* W*H, -1 where there is no prior */
+    float* after_refine;     /* W*H, disparity after the refiner (d_after_refine,
+                                src/pipeline.cpp:597-599) */
 } orc_intermediates;
 
 int orc_pipeline(const orc_cfg* cfg,

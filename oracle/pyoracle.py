@@ -45,7 +45,8 @@ class Cfg(C.Structure):
 class Inter(C.Structure):
     _fields_ = [("cost", C.c_void_p), ("cost_agg", C.c_void_p), ("sgm", C.c_void_p), ("disp_right", C.c_void_p),
                 ("disp_wta", C.c_void_p), ("best", C.c_void_p), ("second", C.c_void_p),
-                ("dmin", C.c_void_p), ("dmax", C.c_void_p), ("d_prior", C.c_void_p)]
+                ("dmin", C.c_void_p), ("dmax", C.c_void_p), ("d_prior", C.c_void_p),
+                ("after_refine", C.c_void_p)]
 
 
 class Support(C.Structure):
@@ -229,9 +230,10 @@ def pipeline(c: Cfg, left, right, intermediates=False):
                     "sgm": np.empty((h, w, D), np.uint32), "disp_right": np.empty((h, w), np.float32),
                     "disp_wta": np.empty((h, w), np.float32), "best": np.empty((h, w), np.float32),
                     "second": np.empty((h, w), np.float32), "dmin": np.empty((h, w), np.int16),
-                    "dmax": np.empty((h, w), np.int16), "d_prior": np.empty((h, w), np.float32)})
+                    "dmax": np.empty((h, w), np.int16), "d_prior": np.empty((h, w), np.float32),
+                    "after_refine": np.empty((h, w), np.float32)})
         inter = Inter(*[out[k].ctypes.data for k in ("cost", "cost_agg", "sgm", "disp_right", "disp_wta", "best",
-                                                     "second", "dmin", "dmax", "d_prior")])
+                                                     "second", "dmin", "dmax", "d_prior", "after_refine")])
     rc = oracle().orc_pipeline(C.byref(c), _p(left), _p(right), w, h, w * ch, ch, _p(out["disparity"]),
                                _p(out["confidence"]), _p(out["invalid_reason"]), _p(out["reliable"]),
                                C.byref(inter) if inter is not None else None)

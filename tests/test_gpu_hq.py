@@ -24,7 +24,7 @@ CASES = [
     (1, 333, 101, 160, {}),
 ]
 
-KEYS = ["cost", "cost_agg", "disp_right", "disp_wta", "best", "second", "dmin", "dmax", "d_prior",
+KEYS = ["cost", "cost_agg", "disp_right", "disp_wta", "best", "second", "dmin", "dmax", "d_prior", "after_refine",
         "disparity", "confidence", "invalid_reason", "reliable"]
 
 

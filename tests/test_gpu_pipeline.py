@@ -41,7 +41,7 @@ CASES = [
     (2, 150, 90, 64, 1, {"eta_ad": -0.3, "cost_max": 200}),
 ]
 
-FUSED_KEYS = ["sgm", "disp_right", "disp_wta", "best", "second",
+FUSED_KEYS = ["sgm", "disp_right", "disp_wta", "best", "second", "after_refine",
               "disparity", "confidence", "invalid_reason", "reliable"]
 PLAIN_KEYS = ["cost", "cost_agg"] + FUSED_KEYS
 
