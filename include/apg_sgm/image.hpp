@@ -4,6 +4,7 @@
 #pragma once
 
 #include <algorithm>
+#include <string>
 #include <vector>
 
 #include "apg_sgm/types.hpp"
@@ -74,5 +75,15 @@ private:
 using Image32f = Plane<float>;
 using Image16s = Plane<int16_t>;
 using Image8u1 = Plane<uint8_t>;
+
+// File I/O with the reference's formats (reference src/image.cpp:45-205):
+// binary 8-bit PGM (P5) and PPM (P6) in, P5 out; PFM (Pf / PF, first channel,
+// bottom-up rows, byte order from the scale's sign) in, little-endian Pf out;
+// the preview maps d * 255 / dmax to 8 bits (invalid -> 0).
+bool load_image(const std::string& path, Image8& out);
+bool save_pgm(const std::string& path, const Image8& img);
+bool load_pfm(const std::string& path, Image32f& out);
+bool save_pfm(const std::string& path, const Image32f& img);
+bool save_disparity_preview(const std::string& path, const Image32f& disp, float dmax);
 
 }  // namespace apg
