@@ -179,15 +179,18 @@ __global__ __launch_bounds__(64) void wta_fused_kernel(apg_hip_params p, int w, 
     // as in the reference, a negative winning disparity counts as no match
     // (best_d < 0 test, reference src/sgm_optimizer.cpp:491-496)
     const bool found = bv != 0xFFFFu && d0 + bi >= 0;
-    const uint16_t* row16 = (const uint16_t*)srow;
+    // the row is only accessed as 32-bit words, so the compiler keeps these
+    // reads, the poisoning and the second scan in program order (16-bit
+    // accesses to it would not alias the 32-bit scan for the compiler)
+    auto get16 = [&](int u) { const int q = p16(u); return (srow[q >> 1] >> (16 * (q & 1))) & 0xFFFFu; };
+    auto poison = [&](int u) { const int q = p16(u); srow[q >> 1] |= 0xFFFFu << (16 * (q & 1)); };
     uint32_t v_m1 = 0xFFFFu, v_p1 = 0xFFFFu;
     if (found) {
-        v_m1 = bi > 0 ? row16[p16(bi - 1)] : 0xFFFFu;
-        v_p1 = bi + 1 < D ? row16[p16(bi + 1)] : 0xFFFFu;
-        uint16_t* w16 = (uint16_t*)srow;
-        w16[p16(bi)] = 0xFFFFu;
-        if (bi > 0) w16[p16(bi - 1)] = 0xFFFFu;
-        if (bi + 1 < D) w16[p16(bi + 1)] = 0xFFFFu;
+        v_m1 = bi > 0 ? get16(bi - 1) : 0xFFFFu;
+        v_p1 = bi + 1 < D ? get16(bi + 1) : 0xFFFFu;
+        poison(bi);
+        if (bi > 0) poison(bi - 1);
+        if (bi + 1 < D) poison(bi + 1);
     }
     // pass 2: second best outside best-1..best+1 (both lanes of a pixel
     // poisoned the same words, and LDS operations of a wave are in order)

@@ -119,6 +119,42 @@ def test_pipeline_sgm_wave_shapes(ctx, kind, w, h, D, over, sgm_dual):
     _check(g, o, FUSED_KEYS)
 
 
+# the fused schedule without S requested (the WTA stage keeps its sums in
+# LDS only): odd and even widths, narrow and wide images, negative and
+# positive min disparity, Fast (no Cross)
+SGM_WTA_CASES = [
+    (1, 160, 121, 128, 1, {}),
+    (2, 97, 64, 48, 1, {"min_disparity": -8}),
+    (0, 131, 77, 64, 1, {}),
+    (2, 200, 90, 112, 1, {"min_disparity": 10}),
+    (1, 1, 9, 16, 1, {}),
+    (0, 2, 5, 16, 1, {}),
+    (1, 3, 7, 16, 1, {}),
+    (2, 17, 11, 32, 1, {}),
+    (0, 95, 33, 16, 1, {}),
+    (1, 96, 20, 128, 1, {}),
+    (2, 129, 40, 80, 1, {"min_disparity": -20}),
+    (0, 400, 30, 128, 1, {}),
+    (1, 301, 25, 96, 0, {}),
+    (2, 777, 12, 64, 1, {"min_disparity": 5}),
+]
+WTA_KEYS = ["disp_right", "disp_wta", "best", "second", "after_refine", "disparity", "confidence",
+            "invalid_reason", "reliable"]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind,w,h,D,mode,over", SGM_WTA_CASES)
+def test_pipeline_fused_wta_outputs(ctx, kind, w, h, D, mode, over):
+    print(f"fused wta outputs kind={kind} {w}x{h} D={D} mode={mode} {over}", flush=True)
+    p, left, right, o = _case(kind, w, h, D, mode, over)
+    A.set_schedule(ctx, 0)
+    g = A.run_pipeline(ctx, p, left, right, intermediates=tuple(WTA_KEYS[:5]))
+    _check(g, o, WTA_KEYS)
+    # the same pair without intermediates (the WTA outputs go to the workspace)
+    g2 = A.run_pipeline(ctx, p, left, right)
+    _check(g2, o, WTA_KEYS[5:])
+
+
 @pytest.mark.gpu
 def test_pipeline_color_input(ctx):
     rng = np.random.default_rng(5)
