@@ -2,6 +2,8 @@
 // stage entry points and the device pipeline schedule.
 #include <string.h>
 
+#include <random>
+
 #include <string>
 #include <vector>
 
@@ -22,7 +24,7 @@ enum BufId {
     B_SRC_L, B_SRC_R, B_GRAY_L, B_GX_L, B_GY_L, B_GRAY_R, B_GX_R, B_GY_R, B_CEN_L, B_CEN_R,
     B_DMIN, B_DMAX, B_ARMS, B_COST, B_TMP, B_SUM, B_DR, B_BEST, B_SECOND, B_SCRATCH, B_LUT,
     B_RKEYS, B_DS0, B_DS1, B_DS2, B_DS3, B_DS4, B_DS5, B_DS6, B_DS7, B_PRIOR, B_CAND_D, B_CAND_U, B_GRID,
-    B_COUNT
+    B_REFINE, B_ENGINE, B_COUNT
 };
 
 enum StageId { S_AUX, S_PRIOR, S_COST, S_RWTA, S_CROSS, S_SGM, S_WTA, S_CONF, S_POST, S_COUNT };
@@ -63,6 +65,7 @@ struct apg_hip_ctx_s {
     bool capturing = false;   // stream capture in progress: no allocation, no host copy
     std::vector<float> prior_grid_host;   // source of the last prior grid upload
     std::vector<apg_hip_support> last_supports;   // the last prior's supports
+    size_t engine_len = 0;   // std::mt19937(12345) outputs held in B_ENGINE
 };
 
 namespace {
@@ -542,6 +545,7 @@ int apg_hip_refine(apg_hip_ctx ctx, const apg_hip_params* p, int w, int h, const
     if (!p->refine_enable) return 0;
     const bool sym = p->census_type == 1;
     if (sym ? (!d_c32l || !d_c32r) : (!d_c64l || !d_c64r)) return fail(ctx, 4, "census buffers");
+    if (int s = ensure_lut(ctx, p)) return s;
     return run_refine(ctx, p, w, h, d_gray_l, d_gx_l, d_gy_l, d_gray_r, d_gx_r, d_gy_r,
                       sym ? (const void*)d_c32l : (const void*)d_c64l, sym ? (const void*)d_c32r : (const void*)d_c64r,
                       d_dmin, d_dmax, d_reliable, d_disparity);
@@ -609,43 +613,36 @@ int run_prior(apg_hip_ctx ctx, const apg_hip_params* p, int w, int h, const uint
     return 0;
 }
 
-// Refiner::refine (reference src/refiner.cpp:90-98) on the host between the
-// confidence and the LR check; synchronises the context stream.
+// Refiner::refine (reference src/refiner.cpp:90-98) between the confidence
+// and the LR check, on the device.  The engine outputs are generated once on
+// the host with std::mt19937 and kept for later frames.
 int run_refine(apg_hip_ctx ctx, const apg_hip_params* p, int w, int h, const uint8_t* gl, const uint8_t* gxl,
                const uint8_t* gyl, const uint8_t* gr, const uint8_t* gxr, const uint8_t* gyr, const void* cl,
                const void* cr, const int16_t* dmin, const int16_t* dmax, const uint8_t* reliable, float* disp) {
-    if (ctx->capturing) return fail(ctx, 8, "the HighQuality refiner synchronises and cannot be captured");
+    if (ctx->capturing) return fail(ctx, 8, "the HighQuality refiner uploads its random stream and cannot be captured");
+    if (h > 4096) return fail(ctx, 6, "the refiner supports heights up to 4096");
     hipStream_t st = ctx->stream;
     const size_t n = (size_t)w * h;
-    const bool sym = p->census_type == 1;
-    std::vector<uint8_t> hgl, hgxl, hgyl, hgr, hgxr, hgyr, hrel;
-    std::vector<uint32_t> c32l, c32r;
-    std::vector<uint64_t> c64l, c64r;
-    std::vector<int16_t> hmin, hmax;
-    std::vector<float> hd;
-    HIPCHK(download_vec(st, hgl, gl, n), "refine download");
-    HIPCHK(download_vec(st, hgxl, gxl, n), "refine download");
-    HIPCHK(download_vec(st, hgyl, gyl, n), "refine download");
-    HIPCHK(download_vec(st, hgr, gr, n), "refine download");
-    HIPCHK(download_vec(st, hgxr, gxr, n), "refine download");
-    HIPCHK(download_vec(st, hgyr, gyr, n), "refine download");
-    HIPCHK(download_vec(st, hrel, reliable, n), "refine download");
-    HIPCHK(download_vec(st, hmin, dmin, n), "refine download");
-    HIPCHK(download_vec(st, hmax, dmax, n), "refine download");
-    HIPCHK(download_vec(st, hd, disp, n), "refine download");
-    if (sym) {
-        HIPCHK(download_vec(st, c32l, cl, n), "refine download");
-        HIPCHK(download_vec(st, c32r, cr, n), "refine download");
-    } else {
-        HIPCHK(download_vec(st, c64l, cl, n), "refine download");
-        HIPCHK(download_vec(st, c64r, cr, n), "refine download");
+    const size_t need = (size_t)(p->refine_iterations + 1) * (n + kRefineSlack);
+    if (ctx->engine_len < need) {
+        std::vector<uint32_t> e(need);
+        std::mt19937 gen(12345);
+        for (auto& v : e) v = (uint32_t)gen();
+        HIPCHK(ensure(ctx, B_ENGINE, need * 4), "alloc");
+        HIPCHK(hipMemcpyAsync(ctx->bufs[B_ENGINE].ptr, e.data(), need * 4, hipMemcpyHostToDevice, st), "engine upload");
+        HIPCHK(hipStreamSynchronize(st), "engine upload");
+        ctx->engine_len = need;
     }
-    HIPCHK(hipStreamSynchronize(st), "refine sync");
-    refine_host(*p, w, h, hgl.data(), hgxl.data(), hgyl.data(), hgr.data(), hgxr.data(), hgyr.data(),
-                sym ? c32l.data() : nullptr, sym ? c32r.data() : nullptr, sym ? nullptr : c64l.data(),
-                sym ? nullptr : c64r.data(), hmin.data(), hmax.data(), hrel.data(), hd.data());
-    HIPCHK(hipMemcpyAsync(disp, hd.data(), n * 4, hipMemcpyHostToDevice, st), "refine upload");
-    HIPCHK(hipStreamSynchronize(st), "refine sync");
+    const size_t sb = refine_scratch_bytes(w, h);
+    HIPCHK(ensure(ctx, B_REFINE, sb), "alloc");
+    const bool sym = p->census_type == 1;
+    const RefineInputs in{gl, gxl, gyl, gr, gxr, gyr, sym ? (const uint32_t*)cl : nullptr,
+                          sym ? (const uint32_t*)cr : nullptr, sym ? nullptr : (const uint64_t*)cl,
+                          sym ? nullptr : (const uint64_t*)cr};
+    HIPCHK(launch_refine(st, *p, cur_lut(ctx), w, h, in, dmin, dmax, reliable, disp,
+                         buf<uint32_t>(ctx, B_ENGINE), ctx->engine_len, ctx->bufs[B_REFINE].ptr,
+                         ctx->bufs[B_REFINE].bytes),
+           "refine");
     return 0;
 }
 

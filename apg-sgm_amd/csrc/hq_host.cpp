@@ -5,14 +5,12 @@
 #include <stdlib.h>
 
 #include <algorithm>
-#include <random>
 #include <utility>
 
 namespace apgk {
 
 namespace {
 
-int iclamp(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 float fclamp(float v, float lo, float hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
 using Pt = std::pair<float, float>;   // (disparity or deviation, weight)
@@ -165,73 +163,6 @@ int prior_build_grid(const apg_hip_params& p, int w, int h, const int16_t* cand_
         if (!changed) break;
     }
     return n;
-}
-
-void refine_host(const apg_hip_params& p, int w, int h, const uint8_t* gl, const uint8_t* gxl, const uint8_t* gyl,
-                 const uint8_t* gr, const uint8_t* gxr, const uint8_t* gyr, const uint32_t* c32l,
-                 const uint32_t* c32r, const uint64_t* c64l, const uint64_t* c64r, const int16_t* dmin,
-                 const int16_t* dmax, const uint8_t* reliable, float* disp) {
-    if (!p.refine_enable) return;
-    // the discrete cost LUT, unquantised (reference src/discrete_cost_lut.hpp:10-38)
-    float tc[65], ta[256] = {}, tg[511] = {};
-    const float lc = std::max(p.lambda_census, 1e-3f), la = std::max(p.lambda_ad, 1e-3f),
-                lg = std::max(p.lambda_grad, 1e-3f);
-    for (int i = 0; i <= 64; ++i) tc[i] = 1.f - expf(-(float)i / lc);
-    if (p.use_ad)
-        for (int i = 0; i < 256; ++i) ta[i] = p.eta_ad * (1.f - expf(-(float)i / la));
-    if (p.use_grad)
-        for (int i = 0; i < 511; ++i) tg[i] = p.mu_grad * (1.f - expf(-(float)i / lg));
-    auto state_cost = [&](size_t q, int x, int y, int d) -> float {
-        const int xr = x - d;
-        if (xr < 0 || xr >= w || d < dmin[q] || d >= dmax[q]) return 1e6f;
-        const size_t qr = (size_t)y * w + xr;
-        const int pop = c32l ? __builtin_popcount(c32l[q] ^ c32r[qr]) : __builtin_popcountll(c64l[q] ^ c64r[qr]);
-        float c = tc[pop];
-        if (p.use_ad) c += ta[abs((int)gl[q] - (int)gr[qr])];
-        if (p.use_grad) c += tg[abs((int)gxl[q] - (int)gxr[qr]) + abs((int)gyl[q] - (int)gyr[qr])];
-        return c;
-    };
-    std::mt19937 gen(12345);
-    for (int it = 0; it < p.refine_iterations; ++it) {
-        const bool fwd = (it & 1) == 0;
-        const int r = std::max(1, p.refine_random_radius >> it);
-        std::uniform_int_distribution<int> jitter(-r, r);
-        for (int row = 0; row < h; ++row) {
-            const int y = fwd ? row : h - 1 - row;
-            for (int col = 0; col < w; ++col) {
-                const int x = fwd ? col : w - 1 - col;
-                const size_t q = (size_t)y * w + x;
-                if (reliable[q]) continue;
-                const int lo = dmin[q], hi = dmax[q];
-                if (hi <= lo) continue;
-                const float cur = disp[q];
-                int cand[6], nc = 0;
-                auto take = [&](float d) {
-                    if (!(d < 0.f)) cand[nc++] = iclamp((int)roundf(d), lo, hi - 1);
-                };
-                take(cur);
-                // the neighbours this sweep has already visited
-                if (fwd) {
-                    if (x > 0) take(disp[q - 1]);
-                    if (y > 0) take(disp[q - w]);
-                } else {
-                    if (x + 1 < w) take(disp[q + 1]);
-                    if (y + 1 < h) take(disp[q + w]);
-                }
-                if (cur >= 0.f) take(cur + (float)jitter(gen));
-                float best = 1e9f;
-                int bd = cur >= 0.f ? iclamp((int)roundf(cur), lo, hi - 1) : lo;
-                for (int i = 0; i < nc; ++i) {
-                    const float c = state_cost(q, x, y, cand[i]);
-                    if (c < best) {
-                        best = c;
-                        bd = cand[i];
-                    }
-                }
-                disp[q] = (float)bd;
-            }
-        }
-    }
 }
 
 }  // namespace apgk

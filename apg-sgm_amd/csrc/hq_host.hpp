@@ -1,8 +1,7 @@
-// Host-side parts of the HighQuality stages: the support selection and grid
-// statistics of the prior (a few thousand supports on a 16-pixel grid: small,
-// sort-heavy, order-sensitive) and the refiner (a sequential raster sweep
-// whose random jitter comes from one std::mt19937 stream).  Everything per
-// pixel or per disparity state runs on the device (k_prior.hip).
+// Host-side part of the HighQuality prior: the support selection and grid
+// statistics (a few thousand supports on a 16-pixel grid: small, sort-heavy,
+// order-sensitive).  Everything per pixel or per disparity state runs on the
+// device (k_prior.hip, k_refine.hip).
 #pragma once
 
 #include <stdint.h>
@@ -33,12 +32,5 @@ struct PriorGrid {
 // not a candidate).  Returns the number of supports kept.
 int prior_build_grid(const apg_hip_params& p, int w, int h, const int16_t* cand_d, const float* cand_u,
                      PriorGrid& grid, std::vector<apg_hip_support>* supports = nullptr);
-
-// Refiner::refine (reference src/refiner.cpp:14-98) in place on disp.
-// census64 is used when census32 is null.
-void refine_host(const apg_hip_params& p, int w, int h, const uint8_t* gl, const uint8_t* gxl, const uint8_t* gyl,
-                 const uint8_t* gr, const uint8_t* gxr, const uint8_t* gyr, const uint32_t* c32l,
-                 const uint32_t* c32r, const uint64_t* c64l, const uint64_t* c64r, const int16_t* dmin,
-                 const int16_t* dmax, const uint8_t* reliable, float* disp);
 
 }  // namespace apgk

@@ -119,6 +119,19 @@ hipError_t launch_rwta_finalize(hipStream_t s, const apg_hip_params& p, const Co
 struct PlaneSet {
     uint8_t* p[8];
 };
+// Refiner (reference src/refiner.cpp:14-98) on the device.  The engine is
+// the std::mt19937(12345) output stream; it must hold (iterations + 1) *
+// (w * h + kRefineSlack) outputs.
+constexpr size_t kRefineSlack = 4096;   // engine outputs a sweep may reject beyond its draws
+struct RefineInputs {
+    const uint8_t *gl, *gxl, *gyl, *gr, *gxr, *gyr;
+    const uint32_t *c32l, *c32r;
+    const uint64_t *c64l, *c64r;
+};
+size_t refine_scratch_bytes(int w, int h);
+hipError_t launch_refine(hipStream_t st, const apg_hip_params& p, const CostLut* d_lut, int w, int h,
+                         const RefineInputs& in, const int16_t* dmin, const int16_t* dmax, const uint8_t* reliable,
+                         float* disp, const uint32_t* eng, size_t eng_len, void* scratch, size_t scratch_bytes);
 hipError_t launch_sgm_lines(hipStream_t s, const apg_hip_params& p, int w, int h, int D, const uint8_t* gray,
                             const void* cost, bool cost_u8, const PlaneSet& planes, const KTune& tn = KTune());
 hipError_t launch_wta_fused(hipStream_t s, const apg_hip_params& p, int w, int h, const void* cost, bool cost_u8,

@@ -199,9 +199,10 @@ APG_HIP_API int apg_hip_prior_estimate(apg_hip_ctx ctx, const apg_hip_params* p,
                                        int16_t* d_dmin, int16_t* d_dmax, float* d_prior, int* n_supports);
 
 /* Refiner::refine (reference src/refiner.cpp:14-98), in place on
- * d_disparity: a sequential raster sweep driven by one std::mt19937 stream,
- * run on the host (inputs are copied down and the result back up; the call
- * synchronises).  No-op unless p->refine_enable. */
+ * d_disparity, on the device and stream-ordered: the reference's raster sweeps
+ * with the std::mt19937(12345) jitter, bit-exact.  The first call on a context
+ * generates the engine outputs on the host and uploads them (synchronises);
+ * heights up to 4096.  No-op unless p->refine_enable. */
 APG_HIP_API int apg_hip_refine(apg_hip_ctx ctx, const apg_hip_params* p, int w, int h,
                                const uint8_t* d_gray_l, const uint8_t* d_gx_l, const uint8_t* d_gy_l,
                                const uint8_t* d_gray_r, const uint8_t* d_gx_r, const uint8_t* d_gy_r,
