@@ -168,6 +168,9 @@ void orc_refine(const orc_cfg* cfg, int w, int h, const uint8_t* gl, const uint8
                 const uint8_t* gr, const uint8_t* gxr, const uint8_t* gyr, const uint32_t* c32l,
                 const uint32_t* c32r, const uint64_t* c64l, const uint64_t* c64r, const int16_t* dmin,
                 const int16_t* dmax, const uint8_t* reliable, float* disp);
+/* draws of std::uniform_int_distribution<int>(-r, r) over a replayed 32-bit
+ * engine stream; the number of stream outputs consumed in *consumed */
+void orc_uniform_draws(const uint32_t* stream, size_t n, int r, int ndraw, int* out, size_t* consumed);
 
 /* Pipeline (src/pipeline.cpp:435-605): aux, prior (or the dense range
  * init), cost, right WTA, Cross, SGM, WTA, confidence, refine, LR check,

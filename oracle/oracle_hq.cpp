@@ -388,4 +388,22 @@ void orc_refine(const orc_cfg* cfg, int w, int h, const uint8_t* gl, const uint8
     }
 }
 
+// std::uniform_int_distribution<int>(-r, r) applied ndraw times to an engine
+// that replays `stream` (32-bit outputs; past its end it repeats the last):
+// the draws in out, the number of outputs consumed in *consumed.  The
+// refiner's jitter is this with the std::mt19937(12345) stream.
+void orc_uniform_draws(const uint32_t* stream, size_t n, int r, int ndraw, int* out, size_t* consumed) {
+    struct Replay {
+        using result_type = uint32_t;
+        const uint32_t* s;
+        size_t n, i;
+        static constexpr result_type min() { return 0u; }
+        static constexpr result_type max() { return 0xFFFFFFFFu; }
+        result_type operator()() { return s[i < n ? i++ : (i++, n - 1)]; }
+    } eng{stream, n, 0};
+    std::uniform_int_distribution<int> dist(-r, r);
+    for (int k = 0; k < ndraw; ++k) out[k] = dist(eng);
+    *consumed = eng.i;
+}
+
 }  // extern "C"

@@ -299,6 +299,18 @@ def ref_max_threads():
     return int(ref().ref_max_threads())
 
 
+def uniform_draws(stream: np.ndarray, r: int, ndraw: int):
+    """std::uniform_int_distribution<int>(-r, r) draws over a replayed engine
+    stream (libstdc++'s own algorithm): (draws, outputs consumed)."""
+    o = oracle()
+    s = np.ascontiguousarray(stream, dtype=np.uint32)
+    out = np.empty(ndraw, np.int32)
+    used = C.c_size_t()
+    o.orc_uniform_draws.argtypes = [C.c_void_p, C.c_size_t, C.c_int, C.c_int, C.c_void_p, C.POINTER(C.c_size_t)]
+    o.orc_uniform_draws(_p(s), s.size, r, ndraw, _p(out), C.byref(used))
+    return out, used.value
+
+
 def prior_supports(c: Cfg, gx, gy, census_l, census_r):
     """extract_supports -> list of (x, y, disparity, confidence)."""
     h, w = gx.shape

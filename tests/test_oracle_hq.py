@@ -2,6 +2,7 @@
 prior and the refiner), independent of the reference build; the golden
 vectors in tests/golden/ pin their exact outputs."""
 import numpy as np
+import pytest
 
 import pyoracle as O
 
@@ -74,3 +75,55 @@ def test_refiner_invariants():
     ooff = O.pipeline(coff, left, right, intermediates=True)
     assert np.array_equal(o0["disparity"], ooff["disparity"])
     assert np.array_equal(o["disp_wta"], o0["disp_wta"])   # d_before_refine does not depend on the refiner
+
+
+def _device_rule_draws(stream, r, ndraw):
+    """The refiner kernels' rule: with s = 2r + 1, an engine output e is
+    accepted iff (e * s) mod 2^32 >= 2^32 mod s; the k-th draw is the k-th
+    accepted output, mapped to (e * s) >> 32 - r."""
+    s = 2 * r + 1
+    t = (2**32 - s) % s
+    prod = stream.astype(np.uint64) * np.uint64(s)
+    ok = (prod & np.uint64(0xFFFFFFFF)) >= np.uint64(t)
+    pos = np.flatnonzero(ok)[:ndraw]
+    vals = (prod[pos] >> np.uint64(32)).astype(np.int64) - r
+    return vals, int(pos[-1]) + 1 if ndraw else 0
+
+
+@pytest.mark.parametrize("r", [1, 2, 4, 9, 100])
+def test_jitter_acceptance_rule_matches_libstdcxx(r):
+    """libstdc++'s uniform_int_distribution over a 32-bit engine equals the
+    data-independent accept-and-compact rule the device refiner uses,
+    including rejected outputs (planted: values whose (e * s) mod 2^32 falls
+    below 2^32 mod s, which a real mt19937 stream reaches about once in 1e9)."""
+    rng = np.random.default_rng(7 + r)
+    stream = rng.integers(0, 2**32, 4000, dtype=np.uint64).astype(np.uint32)
+    s = 2 * r + 1
+    t = (2**32 - s) % s
+    assert t > 0
+    # e with (e * s) mod 2^32 == 0 (< t): e = k * ceil(2^32 / s) adjusted; search a few
+    rejects = [e for e in range(0, 2**32, (2**32 + s - 1) // s)][:50]
+    rejects = [e for e in rejects if (e * s) % 2**32 < t]
+    assert rejects
+    for i, e in enumerate(rejects[:20]):
+        stream[97 * i + 3] = e
+        stream[97 * i + 4] = rejects[(i + 1) % len(rejects)]   # two rejections in a row
+    nd = 3500
+    got, used = O.uniform_draws(stream, r, nd)
+    exp, exp_used = _device_rule_draws(stream, r, nd)
+    assert used == exp_used
+    np.testing.assert_array_equal(got, exp)
+    assert got.min() >= -r and got.max() <= r
+
+
+def test_jitter_acceptance_rule_on_mt19937_stream():
+    """The same on the refiner's real engine: std::mt19937(12345) outputs
+    (numpy's MT19937 with the reference generator's integer seeding)."""
+    bg = np.random.MT19937(0)
+    bg._legacy_seeding(12345)
+    stream = np.array([bg.random_raw() for _ in range(5000)], dtype=np.uint64).astype(np.uint32)
+    for r in (4, 2, 1):
+        got, used = O.uniform_draws(stream, r, 4000)
+        exp, exp_used = _device_rule_draws(stream, r, 4000)
+        assert used == exp_used
+        np.testing.assert_array_equal(got, exp)
