@@ -44,18 +44,30 @@ __global__ __launch_bounds__(256) void cross_h_kernel(int w, int h, int d0, int 
     const int c_beg = wave * seg, c_end = min(NCOL, c_beg + seg);
     uint32_t acc = 0;
     if (wave == 0) P[lane] = 0;
-    for (int col = c_beg; col < c_end; ++col) {
-        const int xx = base + col;
-        uint32_t v = 0;
-        if (xx >= 0 && xx < w && di < D) {
-            const size_t q = (size_t)y * w + xx;
-            if (d >= dmin[q] && d < dmax[q]) {
-                const uint16_t c = cost[q * D + di];
-                if (c != kInv16) v = ((uint32_t)c << 8) | 1u;
-            }
+    // columns in batches of CH_B: every load is issued (clamped inside the
+    // image and the chunk), so a batch waits once
+    constexpr int CH_B = 8;
+    const int dic = min(di, D - 1);
+    for (int col0 = c_beg; col0 < c_end; col0 += CH_B) {
+        uint32_t cv[CH_B];
+        int lo[CH_B], hi[CH_B];
+#pragma unroll
+        for (int i = 0; i < CH_B; ++i) {
+            const int xc = min(max(base + col0 + i, 0), w - 1);
+            const size_t q = (size_t)y * w + xc;
+            cv[i] = cost[q * D + dic];
+            lo[i] = dmin[q];
+            hi[i] = dmax[q];
         }
-        acc += v;
-        P[(col + 1) * 64 + lane] = acc;
+#pragma unroll
+        for (int i = 0; i < CH_B; ++i) {
+            const int col = col0 + i;
+            if (col >= c_end) break;
+            const int xx = base + col;
+            const bool ok = xx >= 0 && xx < w && di < D && d >= lo[i] && d < hi[i] && cv[i] != kInv16;
+            acc += ok ? ((cv[i] << 8) | 1u) : 0u;
+            P[(col + 1) * 64 + lane] = acc;
+        }
     }
     __syncthreads();
     uint32_t off = 0;
