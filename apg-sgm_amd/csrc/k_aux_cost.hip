@@ -328,12 +328,26 @@ __global__ __launch_bounds__(256) void right_wta_kernel(int w, int h, int d0, in
     for (int dc = 0; dc < D; dc += RW_DC) {
         if (y < h) {
             // band[i][j] = C(x0 + d0 + dc + i - j + j ..) : pixel xl = x0 + d0 + dc + i, disparity index dc + j
-            for (int k = lane; k < (RW_TX + RW_DC) * RW_DC; k += 64) {
-                const int i = k / RW_DC, j = k % RW_DC;
-                const int xl = x0 + d0 + dc + i;
-                uint16_t v = kInv16;
-                if (xl >= 0 && xl < w && dc + j < D) v = cost[((size_t)y * w + xl) * D + dc + j];
-                band[row][i][j] = v;
+            // 8 loads in flight per lane (clamped inside the row and the
+            // volume, masked after), so a batch waits once
+            constexpr int NK = (RW_TX + RW_DC) * RW_DC;
+            for (int k0 = lane; k0 < NK; k0 += 64 * 8) {
+                uint16_t v[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int k = min(k0 + 64 * u, NK - 1);
+                    const int i = k / RW_DC, j = k % RW_DC;
+                    const int xl = min(max(x0 + d0 + dc + i, 0), w - 1);
+                    v[u] = cost[((size_t)y * w + xl) * D + min(dc + j, D - 1)];
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int k = k0 + 64 * u;
+                    if (k >= NK) break;
+                    const int i = k / RW_DC, j = k % RW_DC;
+                    const int xl = x0 + d0 + dc + i;
+                    band[row][i][j] = (xl >= 0 && xl < w && dc + j < D) ? v[u] : kInv16;
+                }
             }
         }
         __syncthreads();

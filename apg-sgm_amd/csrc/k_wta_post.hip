@@ -43,9 +43,19 @@ __global__ __launch_bounds__(64) void wta_kernel(int w, int h, int d0, int D, in
     uint64_t raw_prev = ~0ull;
     for (int dc = 0; dc < D; dc += WTA_DC) {
         const int nd = min(WTA_DC, D - dc);
-        for (int i = 0; i < WTA_PB; ++i) {
-            const size_t q = p_base + i;
-            if (q < npx && lane < nd) st[i][lane] = (uint32_t)vol[q * D + dc + lane];
+        // pixel rows in batches of 8: every load is issued (clamped inside the
+        // volume), so a batch waits once
+        const int lc = min(lane, nd - 1);
+        for (int i0 = 0; i0 < WTA_PB; i0 += 8) {
+            uint32_t v[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const size_t q = min(p_base + (size_t)(i0 + i), npx - 1);
+                v[i] = (uint32_t)vol[q * D + dc + lc];
+            }
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+                if (p_base + (size_t)(i0 + i) < npx && lane < nd) st[i0 + i][lane] = v[i];
         }
         __syncthreads();
         if (act) {
