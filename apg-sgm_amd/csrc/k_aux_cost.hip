@@ -400,9 +400,52 @@ __global__ void arms_kernel(const uint8_t* __restrict__ g, int w, int h, int Lma
     arms[(size_t)y * w + x] = (uint32_t)l | ((uint32_t)r << 8) | ((uint32_t)u << 16) | ((uint32_t)v << 24);
 }
 
+// The same arms with every neighbour loaded up front (LM >= Lmax values per
+// direction, clamped inside the image, all in flight together) and the four
+// scans run on registers: the first neighbour past the border, past Lmax or
+// off by more than tau ends an arm.
+template <int LM>
+__global__ __launch_bounds__(128) void arms_batch_kernel(const uint8_t* __restrict__ g, int w, int h, int Lmax,
+                                                         int tau, uint32_t* __restrict__ arms) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    const int xc = min(x, w - 1);
+    const uint8_t* row = g + (size_t)y * w;
+    const int c0 = row[xc];
+    uint8_t vl[LM], vr[LM], vu[LM], vd[LM];
+#pragma unroll
+    for (int k = 0; k < LM; ++k) {
+        vl[k] = row[max(xc - k - 1, 0)];
+        vr[k] = row[min(xc + k + 1, w - 1)];
+        vu[k] = g[(size_t)max(y - k - 1, 0) * w + xc];
+        vd[k] = g[(size_t)min(y + k + 1, h - 1) * w + xc];
+    }
+    if (x >= w) return;
+    int l = 0, r = 0, u = 0, v = 0;
+    bool gl = true, gr = true, gu = true, gd = true;
+#pragma unroll
+    for (int k = 0; k < LM; ++k) {
+        const bool in = k < Lmax;
+        gl = gl && in && x - k - 1 >= 0 && abs((int)vl[k] - c0) <= tau;
+        gr = gr && in && x + k + 1 < w && abs((int)vr[k] - c0) <= tau;
+        gu = gu && in && y - k - 1 >= 0 && abs((int)vu[k] - c0) <= tau;
+        gd = gd && in && y + k + 1 < h && abs((int)vd[k] - c0) <= tau;
+        l += gl;
+        r += gr;
+        u += gu;
+        v += gd;
+    }
+    arms[(size_t)y * w + x] = (uint32_t)l | ((uint32_t)r << 8) | ((uint32_t)u << 16) | ((uint32_t)v << 24);
+}
+
 hipError_t launch_cross_arms(hipStream_t s, const apg_hip_params& p, const uint8_t* gray, int w, int h, uint32_t* arms) {
     const int Lmax = p.max_arm_length > 1 ? p.max_arm_length : 1;
-    hipLaunchKernelGGL(arms_kernel, dim3((w + 127) / 128, h), dim3(128), 0, s, gray, w, h, Lmax, p.color_threshold, arms);
+    const dim3 grid((w + 127) / 128, h);
+    if (Lmax <= 17)
+        hipLaunchKernelGGL(arms_batch_kernel<17>, grid, dim3(128), 0, s, gray, w, h, Lmax, p.color_threshold, arms);
+    else if (Lmax <= 32)
+        hipLaunchKernelGGL(arms_batch_kernel<32>, grid, dim3(128), 0, s, gray, w, h, Lmax, p.color_threshold, arms);
+    else
+        hipLaunchKernelGGL(arms_kernel, grid, dim3(128), 0, s, gray, w, h, Lmax, p.color_threshold, arms);
     return hipGetLastError();
 }
 
