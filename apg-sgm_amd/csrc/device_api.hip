@@ -160,6 +160,10 @@ int finish_timing(apg_hip_ctx ctx) {
 // of each buffer it needs.
 struct WsPlan {
     bool fused = false, u8 = false;
+    // stage-by-stage Cost and Cross, then the fused schedule's SGM (delta
+    // planes) and WTA kernels over the per-pixel ranges: the HighQuality
+    // preset, whose refiner sits between the confidence and the LR check
+    bool planes_sgm = false;
     size_t bytes[B_COUNT] = {};
 };
 
@@ -199,13 +203,19 @@ WsPlan plan_workspace(const apg_hip_params& p, int w, int h, const CostLut& lut,
                p2max <= 254 &&
                npaths * (cbound + p2max) < 65535 && (!p.agg_enable || p.agg_iterations <= 1) &&
                nv * (pl.u8 ? 1 : 2) < (1ull << 31);
+    pl.planes_sgm = !pl.fused && schedule != 1 && !want_raw && (p.prior_enable || p.refine_enable) && D % 2 == 0 &&
+                    D <= 256 && p2max <= 254 && npaths * (cbound + p2max) < 65535 && nv * 2 < (1ull << 31);
     if (pl.fused) {
         pl.bytes[B_COST] = nv * (pl.u8 ? 1 : 2);
         if (p.agg_enable) pl.bytes[B_TMP] = nv * (pl.u8 ? 1 : 2);
         pl.bytes[B_RKEYS] = n * 4;
         for (int r = 0; r < npaths; ++r) pl.bytes[B_DS0 + r] = nv;
     } else {
-        pl.bytes[B_COST] = pl.bytes[B_TMP] = pl.bytes[B_SUM] = nv * 2;
+        pl.bytes[B_COST] = pl.bytes[B_TMP] = nv * 2;
+        if (pl.planes_sgm)
+            for (int r = 0; r < npaths; ++r) pl.bytes[B_DS0 + r] = nv;
+        else
+            pl.bytes[B_SUM] = nv * 2;
         pl.bytes[B_BEST] = pl.bytes[B_SECOND] = n * 4;
     }
     return pl;
@@ -761,22 +771,43 @@ int apg_hip_pipeline(apg_hip_ctx ctx, const apg_hip_params* p, const uint8_t* d_
     if (inter && inter->d_cost_agg)
         HIPCHK(hipMemcpyAsync(inter->d_cost_agg, cost, nv * 2, hipMemcpyDeviceToDevice, st), "copy C'");
     mark(ctx, S_SGM);
-    const int npath = p->paths == 8 ? 8 : 4;
-    for (int k = 0; k < npath; ++k)
-        HIPCHK(launch_sgm_dir(st, *p, w, h, D, gl, cost, sum, kDirs[k][0], kDirs[k][1], k == 0 ? 0 : 1), "sgm");
-    if (inter && inter->d_sum32) HIPCHK(launch_widen16(st, nv, sum, inter->d_sum32), "widen");
-    mark(ctx, S_WTA);
-    HIPCHK(launch_wta(st, *p, w, h, sum, 16, dmin, dmax, d_disparity, best, second), "wta");
-    if (inter && inter->d_disp_wta)
-        HIPCHK(hipMemcpyAsync(inter->d_disp_wta, d_disparity, n * 4, hipMemcpyDeviceToDevice, st), "copy wta");
-    if (inter && inter->d_disp_right)
-        HIPCHK(hipMemcpyAsync(inter->d_disp_right, dr, n * 4, hipMemcpyDeviceToDevice, st), "copy dr");
-    if (inter && inter->d_best) HIPCHK(hipMemcpyAsync(inter->d_best, best, n * 4, hipMemcpyDeviceToDevice, st), "copy best");
-    if (inter && inter->d_second)
-        HIPCHK(hipMemcpyAsync(inter->d_second, second, n * 4, hipMemcpyDeviceToDevice, st), "copy second");
-    mark(ctx, S_CONF);
-    HIPCHK(launch_confidence(st, *p, w, h, d_disparity, dr, gxl, gyl, best, second, d_confidence, d_reliable, d_reason),
-           "confidence");
+    if (plan.planes_sgm) {
+        PlaneSet planes{};
+        planes.general_ranges = 1;
+        for (int r = 0; r < npaths; ++r) planes.p[r] = buf<uint8_t>(ctx, (BufId)(B_DS0 + r));
+        HIPCHK(launch_sgm_lines(st, *p, w, h, D, gl, cost, false, planes, ctx->tune), "sgm_lines");
+        mark(ctx, S_WTA);
+        // WTA + confidence in one kernel; the LR check follows the refiner
+        apg_hip_params pw = *p;
+        pw.post_lr_check = 0;
+        HIPCHK(launch_wta_fused(st, pw, w, h, cost, false, planes, dmin, dmax, dr, gxl, gyl, d_disparity, d_confidence,
+                                d_reason, d_reliable, inter ? inter->d_disp_wta : nullptr,
+                                inter && inter->d_best ? inter->d_best : best,
+                                inter && inter->d_second ? inter->d_second : second, inter ? inter->d_sum32 : nullptr),
+               "wta_fused");
+        if (inter && inter->d_disp_right)
+            HIPCHK(hipMemcpyAsync(inter->d_disp_right, dr, n * 4, hipMemcpyDeviceToDevice, st), "copy dr");
+        mark(ctx, S_CONF);
+    } else {
+        const int npath = p->paths == 8 ? 8 : 4;
+        for (int k = 0; k < npath; ++k)
+            HIPCHK(launch_sgm_dir(st, *p, w, h, D, gl, cost, sum, kDirs[k][0], kDirs[k][1], k == 0 ? 0 : 1), "sgm");
+        if (inter && inter->d_sum32) HIPCHK(launch_widen16(st, nv, sum, inter->d_sum32), "widen");
+        mark(ctx, S_WTA);
+        HIPCHK(launch_wta(st, *p, w, h, sum, 16, dmin, dmax, d_disparity, best, second), "wta");
+        if (inter && inter->d_disp_wta)
+            HIPCHK(hipMemcpyAsync(inter->d_disp_wta, d_disparity, n * 4, hipMemcpyDeviceToDevice, st), "copy wta");
+        if (inter && inter->d_disp_right)
+            HIPCHK(hipMemcpyAsync(inter->d_disp_right, dr, n * 4, hipMemcpyDeviceToDevice, st), "copy dr");
+        if (inter && inter->d_best)
+            HIPCHK(hipMemcpyAsync(inter->d_best, best, n * 4, hipMemcpyDeviceToDevice, st), "copy best");
+        if (inter && inter->d_second)
+            HIPCHK(hipMemcpyAsync(inter->d_second, second, n * 4, hipMemcpyDeviceToDevice, st), "copy second");
+        mark(ctx, S_CONF);
+        HIPCHK(launch_confidence(st, *p, w, h, d_disparity, dr, gxl, gyl, best, second, d_confidence, d_reliable,
+                                 d_reason),
+               "confidence");
+    }
     if (p->refine_enable) {
         if (int s = run_refine(ctx, p, w, h, gl, gxl, gyl, gr, gxr, gyr, cl, cr, dmin, dmax, d_reliable, d_disparity))
             return s;

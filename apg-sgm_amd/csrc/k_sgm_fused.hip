@@ -263,6 +263,7 @@ __device__ __forceinline__ void sgm_job(const apg_hip_params& p, int w, int h, i
     const int dxA = f == 1 ? 0 : (chain ? -1 : 1);   // x step of chain A: rows and rays move +x forward
     const int xA0 = pA0 % w, xB0 = pB0 % w;
     auto batch_valid = [&](int k) {
+        if (planes.general_ranges) return false;
         const int t0 = k * SB, t1 = min(k * SB + SB, n) - 1;
         const int a0 = xA0 + dxA * t0, a1 = xA0 + dxA * t1;
         const int b0 = xB0 - dxA * t0, b1 = xB0 - dxA * t1;
@@ -474,6 +475,7 @@ __global__ __launch_bounds__(64) void sgm_half_kernel(apg_hip_params p, int w, i
     const int dxA = f == 1 ? 0 : 1;
     const int xA0 = pA0 % w, xB0 = pB0 % w;
     auto batch_valid = [&](int k) {
+        if (planes.general_ranges) return false;
         const int t0 = k * SB, t1 = min(k * SB + SB, n) - 1;
         const int a0 = xA0 + dxA * t0, a1 = xA0 + dxA * t1;
         const int b0 = xB0 - dxA * t0, b1 = xB0 - dxA * t1;
@@ -671,6 +673,7 @@ __global__ __launch_bounds__(64) void sgm_quad_kernel(apg_hip_params p, int w, i
     auto cstep = [&](int q) { return (q & 1) ? -step : step; };
     const int xlo = p.min_disparity + D - 1, xhi = w - 1 + p.min_disparity;
     auto batch_valid = [&](int k) {
+        if (planes.general_ranges) return false;
         const int t0 = k * SB, t1 = min(k * SB + SB, n) - 1;
         if (f == 1) {   // columns: x fixed per scanline
             const int xa = p0a % w, xb2 = p0b % w;

@@ -87,7 +87,7 @@ __global__ __launch_bounds__(64) void wta_fused_kernel(apg_hip_params p, int w, 
         // every state of the block valid: its pixels lie in one row with x >= the
         // largest disparity (column ranges: C'(x, d) is valid iff d <= x)
         const int xb0 = (int)(p_base % (size_t)w);
-        const bool allv = xb0 + npb <= w && xb0 >= d0 + D - 1 && d0 >= 0;
+        const bool allv = !planes.general_ranges && xb0 + npb <= w && xb0 >= d0 + D - 1 && d0 >= 0;
         auto stage = [&](int k0, auto allv_t) {
             constexpr bool AV = decltype(allv_t)::value;
             u32x4 cw[WF_WG], pw[WF_WG][NP];

@@ -118,6 +118,10 @@ hipError_t launch_rwta_finalize(hipStream_t s, const apg_hip_params& p, const Co
 // One u8 delta plane per SGM direction (2 per scanline family: forward, backward)
 struct PlaneSet {
     uint8_t* p[8];
+    // 0: the dense range init, C'(x, d) valid iff d <= x < W + d, so whole
+    // batches of columns are known valid; 1: per-pixel ranges (the prior's),
+    // validity only from C'
+    int general_ranges;
 };
 // Refiner (reference src/refiner.cpp:14-98) on the device.  The engine is
 // the std::mt19937(12345) output stream; it must hold (iterations + 1) *
