@@ -282,6 +282,8 @@ int apg_hip_validate_params(const apg_hip_params* c) {
     if (D > kMaxD || D > 2 * 64 * kMaxSgmK) return 5;
     if (c->agg_enable && c->max_arm_length > kMaxArm) return 5;
     if (c->post_median_radius > 3) return 5;
+    // the refiner's engine outputs are held on the device: (iterations + 1) per pixel
+    if (c->refine_enable && c->refine_iterations > kMaxRefineIterations) return 5;
     if (c->paths != 4 && c->paths != 8) return 4;
     return 0;
 }

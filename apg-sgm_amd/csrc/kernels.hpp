@@ -127,6 +127,7 @@ struct PlaneSet {
 // the std::mt19937(12345) output stream; it must hold (iterations + 1) *
 // (w * h + kRefineSlack) outputs.
 constexpr size_t kRefineSlack = 4096;   // engine outputs a sweep may reject beyond its draws
+constexpr int kMaxRefineIterations = 64;
 struct RefineInputs {
     const uint8_t *gl, *gxl, *gyl, *gr, *gxr, *gyr;
     const uint32_t *c32l, *c32r;

@@ -63,6 +63,9 @@ def test_validate_params_codes():
     assert v(cost_max=20000) == 5
     assert v(max_disparity=1024) == 5
     assert v(max_arm_length=100) == 5
+    assert v(refine_enable=1, refine_iterations=64) == 0
+    assert v(refine_enable=1, refine_iterations=65) == 5
+    assert v(refine_enable=0, refine_iterations=65) == 0
 
 
 def test_synth_deterministic_and_shaped():
