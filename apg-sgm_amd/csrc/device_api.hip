@@ -284,6 +284,10 @@ int apg_hip_validate_params(const apg_hip_params* c) {
     if (c->post_median_radius > 3) return 5;
     // the refiner's engine outputs are held on the device: (iterations + 1) per pixel
     if (c->refine_enable && c->refine_iterations > kMaxRefineIterations) return 5;
+    // each sweep's window holds kRefineSlack rejected engine outputs beyond its draws;
+    // a rejection has probability < (2r + 1) / 2^32, so r <= 1024 keeps a window of
+    // 2^31 draws far inside it (about 1000 rejections expected)
+    if (c->refine_enable && c->refine_random_radius > kMaxRefineRadius) return 5;
     if (c->paths != 4 && c->paths != 8) return 4;
     return 0;
 }

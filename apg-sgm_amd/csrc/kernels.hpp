@@ -128,6 +128,7 @@ struct PlaneSet {
 // (w * h + kRefineSlack) outputs.
 constexpr size_t kRefineSlack = 4096;   // engine outputs a sweep may reject beyond its draws
 constexpr int kMaxRefineIterations = 64;
+constexpr int kMaxRefineRadius = 1024;
 struct RefineInputs {
     const uint8_t *gl, *gxl, *gyl, *gr, *gxr, *gyr;
     const uint32_t *c32l, *c32r;

@@ -22,6 +22,13 @@ CASES = [
     (2, 120, 90, 32, {"prior_max_supports": 10, "prior_min_supports": 5}),
     (0, 96, 64, 24, {"prior_min_supports": 10 ** 6}),
     (1, 333, 101, 160, {}),
+    # the refiner alone: a tall image (the sweep's diagonals clamp on the width),
+    # one and four sweeps (last sweep backward / forward), none, radius 1 and 37
+    (2, 64, 200, 32, {}),
+    (0, 128, 96, 32, {"refine_iterations": 1}),
+    (1, 128, 96, 32, {"refine_iterations": 4, "refine_random_radius": 37}),
+    (2, 112, 80, 32, {"refine_iterations": 0}),
+    (0, 100, 70, 32, {"refine_random_radius": 1}),
 ]
 
 KEYS = ["cost", "cost_agg", "disp_right", "disp_wta", "best", "second", "dmin", "dmax", "d_prior", "after_refine",

@@ -66,6 +66,8 @@ def test_validate_params_codes():
     assert v(refine_enable=1, refine_iterations=64) == 0
     assert v(refine_enable=1, refine_iterations=65) == 5
     assert v(refine_enable=0, refine_iterations=65) == 0
+    assert v(refine_enable=1, refine_random_radius=1024) == 0
+    assert v(refine_enable=1, refine_random_radius=1025) == 5
 
 
 def test_synth_deterministic_and_shaped():
