@@ -364,7 +364,7 @@ int tune_pair(apg_hip_ctx ctx, const apg_hip_pair& q, apg_hip_tuning* out) {
         if (int s = time_shapes(ctx, q, apg_hip_tuning{seg, 0, 0}, reps, &t)) return s;
         if (t.cost_ms < 0.98 * bc) bc = t.cost_ms, r.cost_segment = seg;
     }
-    for (int cv : {1024, 2048, 8192, 16384}) {
+    for (int cv : {1024, 2048, 4096, 16384}) {
         if (int s = time_shapes(ctx, q, apg_hip_tuning{0, 0, cv}, reps, &t)) return s;
         if (t.cross_ms < 0.98 * bv) bv = t.cross_ms, r.cross_v_waves = cv;
     }

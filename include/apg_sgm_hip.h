@@ -305,7 +305,7 @@ typedef struct {
                             direction per wave, 3 both directions over 32 lanes each,
                             4 (4 paths, D <= 128) both directions of two scanlines
                             per wave, 16 lanes each */
-    int cross_v_waves;   /* target wave count of the Cross-V pass: 256..65536 (4096) */
+    int cross_v_waves;   /* target wave count of the Cross-V pass: 256..65536 (10240) */
 } apg_hip_tuning;
 
 /* NULL restores the built-in choice.  Applies to later pipeline calls. */
