@@ -302,7 +302,7 @@ int apg_hip_ctx_create(int device, apg_hip_ctx* out) {
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     for (int i = 0; e == hipSuccess && i <= S_COUNT; ++i) e = hipEventCreate(&ctx->ev[i]);
     if (e != hipSuccess) {
-        delete ctx;
+        apg_hip_ctx_destroy(ctx);
         return (e == hipErrorOutOfMemory) ? 7 : 8;
     }
     *out = ctx;
