@@ -271,12 +271,22 @@ __global__ __launch_bounds__(256) void fill_mask_kernel(int w, int h, int max_ga
     const int y = blockIdx.x;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const float* row = in + (size_t)y * w;
-    for (int s = wave; s < nseg; s += 4) {
-        const int x = s * 64 + lane;
-        const float v = x < w ? row[x] : -1.f;
-        if (x < w) srow[x] = v;
-        const uint64_t m = __ballot(v >= 0.f);
-        if (lane == 0) smask[s] = m;
+    // four segments per wave per trip, their loads issued unconditionally
+    // (clamped into the row) so that all four are in flight together
+    for (int s0 = wave; s0 < nseg; s0 += 16) {
+        float v[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) v[j] = row[min((s0 + 4 * j) * 64 + lane, w - 1)];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int s = s0 + 4 * j, x = s * 64 + lane;
+            if (s < nseg) {   // uniform across the wave: the ballot sees every lane
+                const float vv = x < w ? v[j] : -1.f;
+                if (x < w) srow[x] = vv;
+                const uint64_t m = __ballot(vv >= 0.f);
+                if (lane == 0) smask[s] = m;
+            }
+        }
     }
     __syncthreads();
     for (int x = tid; x < w; x += 256) {
