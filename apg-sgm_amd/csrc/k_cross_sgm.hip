@@ -275,7 +275,9 @@ __global__ __launch_bounds__(64) void cross_v2_kernel(int w, int h, int d0, int 
 // prefix of input rows [y0 - Lmax, y1 + Lmax) lives in an LDS ring of
 // 2*Lmax + 2 rows.  floor(s / n) = umulhi(2s, ceil(2^31 / n)), exact for
 // s * n < 2^31.
-constexpr int CV3_PF = 16;   // rows per prefetch batch (16: 83 us, 8: 87 us, 32: 98 us at 1280x720 D=128)
+// rows per prefetch batch; at 1280x720 D=128: 16: 77 us, 12: 84 us, 8: 83 us with 90-row
+// segments (with 180-row segments 16: 83 us, 8: 87 us, 32: 98 us)
+constexpr int CV3_PF = 16;
 
 __global__ __launch_bounds__(64) void cross_v3_kernel(int w, int h, int d0, int D, int Lmax, int seg,
                                                       const int16_t* __restrict__ dmin,
