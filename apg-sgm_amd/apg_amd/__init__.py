@@ -50,7 +50,7 @@ class Params(C.Structure):
 class Timing(C.Structure):
     _fields_ = [(n, C.c_double) for n in (
         "aux_ms", "prior_ms", "cost_ms", "right_wta_ms", "cross_ms", "sgm_ms", "wta_ms",
-        "confidence_ms", "post_ms", "total_ms")]
+        "confidence_ms", "refine_ms", "post_ms", "total_ms")]
 
 
 class Intermediates(C.Structure):

@@ -27,7 +27,7 @@ enum BufId {
     B_REFINE, B_ENGINE, B_COUNT
 };
 
-enum StageId { S_AUX, S_PRIOR, S_COST, S_RWTA, S_CROSS, S_SGM, S_WTA, S_CONF, S_POST, S_COUNT };
+enum StageId { S_AUX, S_PRIOR, S_COST, S_RWTA, S_CROSS, S_SGM, S_WTA, S_CONF, S_REFINE, S_POST, S_COUNT };
 
 // The parameters build_cost_lut reads.
 struct LutKey {
@@ -151,7 +151,7 @@ int finish_timing(apg_hip_ctx ctx) {
         apg_hip_timing& t = ctx->last;
         t.aux_ms = ms[S_AUX]; t.prior_ms = ms[S_PRIOR]; t.cost_ms = ms[S_COST]; t.right_wta_ms = ms[S_RWTA];
         t.cross_ms = ms[S_CROSS]; t.sgm_ms = ms[S_SGM]; t.wta_ms = ms[S_WTA]; t.confidence_ms = ms[S_CONF];
-        t.post_ms = ms[S_POST]; t.total_ms = tot;
+        t.refine_ms = ms[S_REFINE]; t.post_ms = ms[S_POST]; t.total_ms = tot;
     }
     return 0;
 }
@@ -188,6 +188,11 @@ WsPlan plan_workspace(const apg_hip_params& p, int w, int h, const CostLut& lut,
     pl.u8 = cbound <= 254;
     // the WTA stage rebuilds S in u16: npath * (C' + P2max) must stay below 65535
     const int npaths = p.paths == 8 ? 8 : 4;
+    if (p.refine_enable) {
+        // the refiner's random stream and its sweep scratch (run_refine)
+        pl.bytes[B_ENGINE] = refine_engine_len(p, w, h) * 4;
+        pl.bytes[B_REFINE] = refine_scratch_bytes(w, h);
+    }
     if (p.prior_enable) {
         const size_t ns = (size_t)prior_samples_x(w) * prior_samples_y(h);
         const size_t nc = (size_t)((w + kPriorCell - 1) / kPriorCell) * ((h + kPriorCell - 1) / kPriorCell);
@@ -346,6 +351,7 @@ void apg_hip_release_workspace(apg_hip_ctx ctx) {
     }
     for (bool& u : ctx->lut_used) u = false;
     ctx->lut_cur = -1;
+    ctx->engine_len = 0;   // B_ENGINE is gone with the rest
 }
 
 void apg_hip_enable_stage_timing(apg_hip_ctx ctx, int on) {
@@ -639,8 +645,11 @@ int run_refine(apg_hip_ctx ctx, const apg_hip_params* p, int w, int h, const uin
     if (h > 4096) return fail(ctx, 6, "the refiner supports heights up to 4096");
     hipStream_t st = ctx->stream;
     const size_t n = (size_t)w * h;
-    const size_t need = (size_t)(p->refine_iterations + 1) * (n + kRefineSlack);
-    if (ctx->engine_len < need) {
+    const size_t need = refine_engine_len(*p, w, h);
+    // draw offsets are 32-bit on the device
+    if (need > 0xFFFFFFFFull) return fail(ctx, 5, "the refiner's random stream exceeds 2^32 outputs");
+    (void)n;
+    if (ctx->engine_len < need || !ctx->bufs[B_ENGINE].ptr) {
         std::vector<uint32_t> e(need);
         std::mt19937 gen(12345);
         for (auto& v : e) v = (uint32_t)gen();
@@ -750,6 +759,7 @@ int apg_hip_pipeline(apg_hip_ctx ctx, const apg_hip_params* p, const uint8_t* d_
         if (inter && inter->d_disp_right)
             HIPCHK(hipMemcpyAsync(inter->d_disp_right, dr, n * 4, hipMemcpyDeviceToDevice, st), "copy dr");
         mark(ctx, S_CONF);
+        mark(ctx, S_REFINE);
         mark(ctx, S_POST);
         HIPCHK(launch_fill(st, *p, w, h, d_disparity, d_reason, gl, nullptr, scratch), "fill");
         HIPCHK(launch_median(st, *p, w, h, scratch, d_disparity), "median");
@@ -814,6 +824,7 @@ int apg_hip_pipeline(apg_hip_ctx ctx, const apg_hip_params* p, const uint8_t* d_
                                  d_reason),
                "confidence");
     }
+    mark(ctx, S_REFINE);
     if (p->refine_enable) {
         if (int s = run_refine(ctx, p, w, h, gl, gxl, gyl, gr, gxr, gyr, cl, cr, dmin, dmax, d_reliable, d_disparity))
             return s;
@@ -927,14 +938,16 @@ int apg_hip_bench_pipeline(apg_hip_ctx ctx, const apg_hip_params* p, const uint8
             const apg_hip_timing& t = ctx->last;
             acc.aux_ms += t.aux_ms; acc.prior_ms += t.prior_ms; acc.cost_ms += t.cost_ms;
             acc.right_wta_ms += t.right_wta_ms; acc.cross_ms += t.cross_ms; acc.sgm_ms += t.sgm_ms;
-            acc.wta_ms += t.wta_ms; acc.confidence_ms += t.confidence_ms; acc.post_ms += t.post_ms;
+            acc.wta_ms += t.wta_ms; acc.confidence_ms += t.confidence_ms; acc.refine_ms += t.refine_ms;
+            acc.post_ms += t.post_ms;
             acc.total_ms += t.total_ms;
         }
         const double k = steps > 0 ? 1.0 / steps : 0.0;
         stage_avg->aux_ms = acc.aux_ms * k; stage_avg->prior_ms = acc.prior_ms * k; stage_avg->cost_ms = acc.cost_ms * k;
         stage_avg->right_wta_ms = acc.right_wta_ms * k; stage_avg->cross_ms = acc.cross_ms * k;
         stage_avg->sgm_ms = acc.sgm_ms * k; stage_avg->wta_ms = acc.wta_ms * k;
-        stage_avg->confidence_ms = acc.confidence_ms * k; stage_avg->post_ms = acc.post_ms * k;
+        stage_avg->confidence_ms = acc.confidence_ms * k; stage_avg->refine_ms = acc.refine_ms * k;
+        stage_avg->post_ms = acc.post_ms * k;
         stage_avg->total_ms = acc.total_ms * k;
     }
     ctx->timing = was_timing;

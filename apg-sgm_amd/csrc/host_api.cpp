@@ -812,6 +812,7 @@ Status StereoMatcher::compute_impl(const ImageView8& left, const ImageView8& rig
             stats->timing.sgm_ms = t.sgm_ms;
             stats->timing.wta_ms = t.wta_ms;
             stats->timing.confidence_ms = t.confidence_ms;
+            stats->timing.refine_ms = t.refine_ms;
             stats->timing.post_ms = t.post_ms;
             stats->timing.total_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
             const size_t vox = n * static_cast<size_t>(cfg_.max_disparity - cfg_.min_disparity);

@@ -32,7 +32,11 @@ namespace apgk {
 
 namespace {
 
-constexpr float kNoCand = -1.f;   // no candidate (the reference pushes only d >= 0)
+// a neighbour with no candidate (the reference pushes only values >= 0, so
+// any negative value means none for a neighbour; the own and jitter
+// candidates, whose clamped disparity may be negative, carry flags instead)
+constexpr float kNoCand = -1.f;
+constexpr uint32_t kHasOwn = 1u, kHasJitter = 2u;
 
 __device__ __forceinline__ int sweep_index(bool fwd, int w, int h, int x, int y) {
     return fwd ? y * w + x : (h - 1 - y) * w + (w - 1 - x);
@@ -112,13 +116,13 @@ __device__ __forceinline__ float rec_cost(const uint4& l, const uint4& r, bool s
 
 // Per pixel, stored in anti-diagonal order: value, range, whether the sweep
 // changes it, the candidates that do not depend on the sweep (own value,
-// jitter: cost and disparity, disparity kNoCand when the reference pushes
-// none) and the left view's terms.
+// jitter: cost and clamped disparity, present when `has` says so) and the
+// left view's terms.
 struct SweepRec {
     float cur;
     int lohi;           // lo | hi << 16
     uint32_t act;       // unreliable with a non-empty range
-    uint32_t pad;
+    uint32_t has;       // kHasOwn | kHasJitter: the reference pushes that candidate
     float4 st;          // cost, disparity of the own value; of the jitter
     uint4 lt;           // left-view terms
 };
@@ -141,8 +145,8 @@ __global__ void refine_static_kernel(const CostLut* __restrict__ lut, int w, int
     o.cur = cur;
     o.lohi = (int)(uint16_t)lo | (hi << 16);
     o.act = (!reliable[q] && hi > lo) ? 1u : 0u;
-    o.pad = 0u;
-    o.st = make_float4(0.f, kNoCand, 0.f, kNoCand);
+    o.has = 0u;
+    o.st = make_float4(0.f, 0.f, 0.f, 0.f);
     o.lt = lrec[q];
     auto cost = [&](int d) {
         const int xr = x - d;
@@ -153,6 +157,7 @@ __global__ void refine_static_kernel(const CostLut* __restrict__ lut, int w, int
         const int dc = clampi((int)roundf(cur), lo, hi - 1);
         o.st.x = cost(dc);
         o.st.y = (float)dc;
+        o.has = kHasOwn;
         const uint32_t s = 2u * (uint32_t)r + 1u;
         const uint32_t e = eng[min(*base + acc[didx[sweep_index(fwd, w, h, x, y)]], eng_last)];
         const int jit = (int)(((uint64_t)e * s) >> 32) - r;
@@ -161,6 +166,7 @@ __global__ void refine_static_kernel(const CostLut* __restrict__ lut, int w, int
             const int d = clampi((int)roundf(dj), lo, hi - 1);
             o.st.z = cost(d);
             o.st.w = (float)d;
+            o.has |= kHasJitter;
         }
     }
     out[diag_index(xs, ys, w, h)] = o;
@@ -225,7 +231,7 @@ __global__ __launch_bounds__(kSweepThreads) void refine_sweep_kernel(const CostL
                 int bd = p.cur >= 0.f ? clampi((int)roundf(p.cur), lo, hi - 1) : lo;
                 // candidates in the reference's order: own value, the neighbour
                 // along the row, the neighbour across rows, the jitter
-                if (p.st.y != kNoCand && p.st.x < best) { best = p.st.x; bd = (int)p.st.y; }
+                if ((p.has & kHasOwn) && p.st.x < best) { best = p.st.x; bd = (int)p.st.y; }
                 const float nb[2] = {xs > 0 ? left[k] : kNoCand, ys > 0 ? up_r[ys - 1] : kNoCand};
                 float nc[2];
                 int nd[2];
@@ -240,7 +246,7 @@ __global__ __launch_bounds__(kSweepThreads) void refine_sweep_kernel(const CostL
 #pragma unroll
                 for (int i = 0; i < 2; ++i)
                     if (!(nb[i] < 0.f) && nc[i] < best) { best = nc[i]; bd = nd[i]; }
-                if (p.st.w != kNoCand && p.st.z < best) { best = p.st.z; bd = (int)p.st.w; }
+                if ((p.has & kHasJitter) && p.st.z < best) { best = p.st.z; bd = (int)p.st.w; }
                 res = (float)bd;
             }
             left[k] = res;
@@ -272,7 +278,9 @@ hipError_t launch_refine(hipStream_t st, const apg_hip_params& p, const CostLut*
     if (h > kSweepThreads * kSweepRows || n >= (1u << 31)) return hipErrorInvalidValue;
     if (scratch_bytes < refine_scratch_bytes(w, h)) return hipErrorInvalidValue;
     const int iters = p.refine_iterations;
-    if (eng_len < (size_t)(iters + 1) * nw) return hipErrorInvalidValue;
+    if (!eng || !scratch || eng_len < (size_t)(iters + 1) * nw) return hipErrorInvalidValue;
+    // engine offsets (base, acc, the clamp index) are 32-bit
+    if (eng_len > 0xFFFFFFFFull) return hipErrorInvalidValue;
     uint8_t* sp = (uint8_t*)scratch;
     uint32_t* dflag = (uint32_t*)sp; sp += n * 4;
     uint32_t* didx = (uint32_t*)sp; sp += n * 4;

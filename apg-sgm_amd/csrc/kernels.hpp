@@ -135,6 +135,10 @@ struct RefineInputs {
     const uint64_t *c64l, *c64r;
 };
 size_t refine_scratch_bytes(int w, int h);
+// engine outputs the refiner of p needs at w x h
+inline size_t refine_engine_len(const apg_hip_params& p, int w, int h) {
+    return (size_t)(p.refine_iterations + 1) * ((size_t)w * h + kRefineSlack);
+}
 hipError_t launch_refine(hipStream_t st, const apg_hip_params& p, const CostLut* d_lut, int w, int h,
                          const RefineInputs& in, const int16_t* dmin, const int16_t* dmax, const uint8_t* reliable,
                          float* disp, const uint32_t* eng, size_t eng_len, void* scratch, size_t scratch_bytes);

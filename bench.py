@@ -25,6 +25,9 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
 import numpy as np  # noqa: E402
 
+from apg_amd.workloads import (SEED_BATCH, SEED_SERVE, SERVE_CLASSES, batch_specs,  # noqa: E402
+                               serve_specs)
+
 HBM_PEAK_GBS = 8000.0
 SEED = 20260101
 W, H, DMAX = 1280, 720, 128
@@ -95,34 +98,7 @@ def measured_traffic(stage):
 
 # ------------------------------------------------------------ mixed forms ----
 
-SEED_BATCH = 20260202
-SEED_SERVE = 20260303
 TUNE_CACHE = os.path.join(ROOT, "apg-sgm_amd", "apg_amd", "launch_shapes_gfx950.json")
-
-
-def splitmix64(state):
-    """Integer-only seeded stream (splitmix64): yields 64-bit values."""
-    m = (1 << 64) - 1
-    while True:
-        state = (state + 0x9E3779B97F4A7C15) & m
-        z = state
-        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & m
-        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & m
-        yield z ^ (z >> 31)
-
-
-def batch_specs(n=64, seed=SEED_BATCH):
-    """Uneven batch: W uniform in [320, 1920] step 16, H in [240, 1080] step 8,
-    D from {64, 128, 192, 256} restricted to D < W/2; Balanced; pair i is
-    generated with seed + i by the noise + 4 rectangles generator."""
-    rng = splitmix64(seed)
-    specs = []
-    for i in range(n):
-        w = 320 + 16 * (next(rng) % 101)
-        h = 240 + 8 * (next(rng) % 106)
-        ds = [d for d in (64, 128, 192, 256) if d < w / 2]
-        specs.append((w, h, ds[next(rng) % len(ds)], seed + i))
-    return specs
 
 
 def bench_batched(args):
@@ -201,25 +177,6 @@ def bench_batched(args):
                                "how": "apg_hip_pipeline per pair, one stream, synchronize after each pair"},
             "speedup_vs_plain": round(value / plain, 3), "bit_identical_to_plain": same,
             "autotune_s": round(tune_s, 2), "launch_shapes": tuned, "workspace_bytes": ws}
-
-
-def serve_specs(n, rate, seed):
-    """Poisson arrivals (exponential gaps from the integer stream) at `rate`
-    per second; class 60% 640x480 D64 / 30% 1280x720 D128 / 10% 1920x1080
-    D256; mode 70% Balanced / 30% Fast."""
-    rng = splitmix64(seed)
-    t, out = 0.0, []
-    for _ in range(n):
-        u = (next(rng) >> 11) * (1.0 / (1 << 53))
-        t += -np.log1p(-u) / rate
-        c = next(rng) % 10
-        cls = 0 if c < 6 else (1 if c < 9 else 2)
-        mode = 1 if next(rng) % 10 < 7 else 0
-        out.append((t, cls, mode, next(rng) % 4))
-    return out
-
-
-SERVE_CLASSES = [(640, 480, 64), (1280, 720, 128), (1920, 1080, 256)]
 
 
 def bench_serving(args):
@@ -406,7 +363,7 @@ def main():
     value = states / (ms_step * 1e-3) / 1e6
 
     stages = {k: getattr(tim, k) for k in ("aux_ms", "prior_ms", "cost_ms", "right_wta_ms", "cross_ms",
-                                            "sgm_ms", "wta_ms", "confidence_ms", "post_ms", "total_ms")}
+                                            "sgm_ms", "wta_ms", "confidence_ms", "refine_ms", "post_ms", "total_ms")}
     dom = max(("cost", "cross", "sgm", "wta"), key=lambda k: stages[k + "_ms"])
     dom_ms = stages[dom + "_ms"]
     achieved = states * BYTES_PER_STATE[dom] / (dom_ms * 1e-3) / 1e9

@@ -64,7 +64,7 @@ typedef struct apg_hip_ctx_s* apg_hip_ctx;
  * follow StageTiming (reference include/apg_sgm/pipeline_stats.hpp:10-22). */
 typedef struct {
     double aux_ms, prior_ms, cost_ms, right_wta_ms, cross_ms, sgm_ms, wta_ms,
-           confidence_ms, post_ms, total_ms;
+           confidence_ms, refine_ms, post_ms, total_ms;
 } apg_hip_timing;
 
 /* PipelineConfig::from_mode (reference src/config.cpp:5-45). */

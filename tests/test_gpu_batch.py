@@ -200,3 +200,24 @@ def test_server_budget_admission(alone):
     assert np.all(lat > 0)
     for i, r in enumerate(reqs):
         _same(r.outputs(), alone[i], f"budget request {i}")
+
+
+@pytest.mark.gpu
+def test_server_budget_admission_with_highquality(ctx):
+    # the HighQuality workspace includes the refiner's random stream and sweep
+    # scratch: a budget of one HQ request's bytes admits it alone, and a
+    # Balanced request queued beside it waits for the HQ slot to be released
+    hq = A.HostRequest(A.params(2, 64), *_inputs(50, 2, 320, 240, 64))
+    bal = A.HostRequest(A.params(1, 32), *_inputs(51, 1, 160, 120, 32))
+    ws_hq = A.workspace_bytes(hq.req.params, 320, 240)
+    no_ref = A.params(2, 64, refine_enable=0)
+    assert ws_hq - A.workspace_bytes(no_ref, 320, 240) >= 4 * 4 * (320 * 240 + 4096)
+    budget = ws_hq + 320 * 240 * 16
+    print(f"server HQ budget {budget} bytes", flush=True)
+    with A.Server(0, 2, budget) as s:
+        s.replay([hq, bal], [0.0, 0.0])
+        assert s.resident_bytes() <= budget
+    for r, kind, w, h, D, mode, i in ((hq, 2, 320, 240, 64, 2, 50), (bal, 1, 160, 120, 32, 1, 51)):
+        left, right = _inputs(i, kind, w, h, D)
+        o = O.pipeline(O.cfg_from(A.params(mode, D).as_dict()), left, right)
+        _same(r.outputs(), (o["disparity"], o["confidence"], o["invalid_reason"], o["reliable"]), f"mode {mode}")

@@ -29,6 +29,8 @@ CASES = [
     (1, 128, 96, 32, {"refine_iterations": 4, "refine_random_radius": 37}),
     (2, 112, 80, 32, {"refine_iterations": 0}),
     (0, 100, 70, 32, {"refine_random_radius": 1}),
+    # ranges reaching below 0
+    (2, 128, 96, 32, {"min_disparity": -8}),
 ]
 
 KEYS = ["cost", "cost_agg", "disp_right", "disp_wta", "best", "second", "dmin", "dmax", "d_prior", "after_refine",
@@ -97,3 +99,21 @@ def test_prior_entry_point_matches_oracle(ctx):
     _cmp(dmin.download(), want_min, "dmin")
     _cmp(dmax.download(), want_max, "dmax")
     _cmp(dp.download(), want_dp, "d_prior")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("shift,maxd,over", [
+    (-2, 16, {"min_disparity": -8, "prior_search_radius": 1}),
+    (4, 0, {"min_disparity": -16, "refine_random_radius": 3}),
+])
+def test_hq_negative_ranges_match_oracle(ctx, shift, maxd, over):
+    # a scene at negative disparity, and a range that ends at 0
+    rng = np.random.default_rng(17 + shift)
+    left = rng.integers(0, 256, (48, 80), dtype=np.uint8)
+    right = np.roll(left, shift, axis=1)
+    p = A.params(2, maxd, **over)
+    print(f"HQ negative range shift={shift} max={maxd} {over}", flush=True)
+    o = O.pipeline(O.cfg_from(p.as_dict()), left, right, intermediates=True)
+    g = A.run_pipeline(ctx, p, left, right, intermediates=True)
+    for k in KEYS:
+        _cmp(getattr(g, k), o[k], k)

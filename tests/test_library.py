@@ -114,3 +114,12 @@ def test_pipeline_workspace_bytes_closed_form():
     lut = ws(1, 32, 200, 100) - (34 * n + 6 * nv)
     assert ws(1, 32, 200, 100, P2_base=300) == (34 - 4 + 8) * n + 6 * nv + lut
     assert A.workspace_bytes(A.params(1, 32, min_disparity=40), 10, 10) == 0   # max <= min
+
+
+def test_workspace_bytes_count_the_refiner():
+    # HighQuality admission must see the refiner's random stream
+    # (4 B x (iterations + 1) x (w * h + 4096)) and its sweep scratch
+    w, h = 640, 360
+    hq = A.workspace_bytes(A.params(2, 64), w, h)
+    no_ref = A.workspace_bytes(A.params(2, 64, refine_enable=0), w, h)
+    assert hq - no_ref >= 4 * 4 * (w * h + 4096) + 100 * w * h

@@ -284,3 +284,24 @@ def test_oracle_matches_reference_build(w, h, D, mode, over):
     for k in ("disparity", "confidence", "invalid_reason", "reliable", "disp_right", "dmin", "dmax"):
         np.testing.assert_array_equal(o[k], rp[k], err_msg=k)
     np.testing.assert_array_equal(o["disp_wta"], rp["d_before_refine"], err_msg="d_before_refine")
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="reference build (oracle/_ref) not present")
+@pytest.mark.parametrize("shift,maxd,over", [
+    (-2, 16, {"min_disparity": -8, "prior_search_radius": 1}),
+    (2, 16, {"min_disparity": -8}),
+    (4, 0, {"min_disparity": -16, "refine_random_radius": 3}),
+])
+def test_oracle_hq_negative_ranges_match_reference_build(shift, maxd, over):
+    """HighQuality with ranges reaching below 0: a scene at positive and at
+    negative disparity, and a range that ends at 0 (every pixel's range lies
+    below 0, so the refiner's clamped candidates would be negative)."""
+    rng = np.random.default_rng(17 + shift)
+    left = rng.integers(0, 256, (48, 80), dtype=np.uint8)
+    right = np.roll(left, shift, axis=1)
+    c = O.cfg(2, maxd, **over)
+    rp = O.ref_pipeline(c, left, right)
+    o = O.pipeline(c, left, right, intermediates=True)
+    for k in ("disparity", "confidence", "invalid_reason", "reliable", "disp_right", "dmin", "dmax"):
+        np.testing.assert_array_equal(o[k], rp[k], err_msg=k)
+    np.testing.assert_array_equal(o["disp_wta"], rp["d_before_refine"], err_msg="d_before_refine")
