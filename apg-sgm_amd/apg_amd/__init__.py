@@ -141,6 +141,10 @@ def load():
         "apg_hip_bench_pipeline": (i, [vp, P, vp, vp, i, i, i, i, vp, vp, vp, vp, i, i, C.POINTER(C.c_double),
                                        C.POINTER(Timing)]),
         "apg_hip_pipeline_workspace_bytes": (sz, [P, i, i]),
+        "apg_hip_pipeline_static": (i, [P, vp, vp, i, i, i, i, vp, vp, vp, vp, vp]),
+        "apg_hip_pipeline_static_bytes": (sz, [P, i, i]),
+        "apg_hip_static_scratch_bytes": (sz, []),
+        "apg_hip_static_last_error": (C.c_char_p, []),
         "apg_hip_prior_estimate": (i, [vp, P, i, i] + [vp] * 9 + [C.POINTER(i)]),
         "apg_hip_refine": (i, [vp, P, i, i] + [vp] * 14),
         "apg_hip_set_tuning": (i, [vp, C.POINTER(Tuning)]),

@@ -2,6 +2,7 @@
 // stage entry points and the device pipeline schedule.
 #include <string.h>
 
+#include <mutex>
 #include <random>
 
 #include <string>
@@ -955,5 +956,128 @@ int apg_hip_bench_pipeline(apg_hip_ctx ctx, const apg_hip_params* p, const uint8
 }
 
 const char* apg_hip_version_string(void) { return "1.0.0-gfx950"; }
+
+}  // extern "C"
+
+// ------------------------------------------------ allocation-free entry ----
+//
+// apg_hip_pipeline_static: the fused schedule with its whole workspace in a
+// file-scope device array instead of context buffers.  Each GPU the library
+// runs on holds its own copy of `scratch` from the moment the code object is
+// loaded until the process ends.  Host code never passes the array's name to
+// a kernel: on a device's first call the array's device address is taken
+// with hipGetSymbolAddress and kept per device, and every buffer handed to the
+// kernels is an offset into that address.
+
+namespace {
+
+constexpr size_t kStaticScratchBytes = size_t(768) << 20;
+constexpr int kStaticMaxDevices = 64;
+
+// [0, kStaticLutBytes): the cost LUT slots (their host-side cache survives
+// between calls); the rest: one call's workspace, laid out per call
+constexpr size_t kStaticLutBytes = (kLutSlots * sizeof(CostLut) + 255) & ~size_t(255);
+
+static __device__ __align__(256) uint8_t scratch[kStaticScratchBytes];
+
+struct StaticDevice {
+    std::mutex mu;          // serialises the host side of calls on one device
+    bool ready = false;
+    uint8_t* ptr = nullptr;   // device address of `scratch` on this device
+    apg_hip_ctx_s ctx;        // LUT cache and launch shapes; no owned buffers
+};
+StaticDevice g_static[kStaticMaxDevices];
+thread_local std::string g_static_error;
+
+int static_fail(int code, const std::string& msg) {
+    g_static_error = msg;
+    return code;
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t apg_hip_static_scratch_bytes(void) { return kStaticScratchBytes; }
+
+const char* apg_hip_static_last_error(void) { return g_static_error.c_str(); }
+
+size_t apg_hip_pipeline_static_bytes(const apg_hip_params* p, int w, int h) {
+    if (!p || w <= 0 || h <= 0 || apg_hip_validate_params(p)) return 0;
+    CostLut lut;
+    build_cost_lut(*p, &lut);
+    const WsPlan pl = plan_workspace(*p, w, h, lut, 0, nullptr);
+    if (!pl.fused) return 0;
+    size_t t = kStaticLutBytes;
+    for (int b = 0; b < B_COUNT; ++b)
+        if (b != B_LUT && pl.bytes[b]) t += (pl.bytes[b] + 255) & ~size_t(255);
+    return t;
+}
+
+int apg_hip_pipeline_static(const apg_hip_params* p, const uint8_t* d_left, const uint8_t* d_right, int w, int h,
+                            int stride, int channels, float* d_disparity, float* d_confidence, uint8_t* d_reason,
+                            uint8_t* d_reliable, void* stream) {
+    g_static_error.clear();
+    if (!p) return static_fail(4, "null params");
+    if (!d_left || !d_right || w <= 0 || h <= 0) return static_fail(1, "empty input image(left or right)");
+    if (!d_disparity || !d_confidence || !d_reason || !d_reliable) return static_fail(4, "null output");
+    if (stride < w * channels || (channels != 1 && channels != 3))
+        return static_fail(2, "invalid row stride (stride less than row byte length)");
+    if (int s = apg_hip_validate_params(p)) return static_fail(s, "invalid parameters");
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0) return static_fail(8, "no current HIP device");
+    if (dev >= kStaticMaxDevices) return static_fail(5, "device index beyond the static entry point's table");
+    StaticDevice& sd = g_static[dev];
+    std::lock_guard<std::mutex> lock(sd.mu);
+    apg_hip_ctx ctx = &sd.ctx;
+    if (!sd.ready) {
+        // once per device, with that device current
+        void* addr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&addr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !addr)
+            return static_fail(8, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        sd.ptr = static_cast<uint8_t*>(addr);
+        ctx->device = dev;
+        sd.ready = true;
+    }
+    ctx->stream = static_cast<hipStream_t>(stream);
+    ctx->timing = false;
+    ctx->schedule = 0;
+    // under capture the LUT upload would record a host copy: the parameter
+    // set's LUT must already be cached by an earlier eager call
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(ctx->stream, &cs) != hipSuccess) return static_fail(8, "hipStreamIsCapturing");
+    ctx->capturing = cs != hipStreamCaptureStatusNone;
+    ctx->bufs[B_LUT] = DevBuf{sd.ptr, kLutSlots * sizeof(CostLut)};
+    if (int s = ensure_lut(ctx, p)) return static_fail(s, ctx->last_error);
+    const WsPlan plan = plan_workspace(*p, w, h, ctx->lut_host[ctx->lut_cur], 0, nullptr);
+    if (!plan.fused)
+        return static_fail(5, "the static entry point runs the fused schedule only (no prior or refiner, even D <= 256, "
+                              "one Cross iteration)");
+    // this call's workspace, carved from the array after the LUT slots
+    size_t off = kStaticLutBytes;
+    for (int b = 0; b < B_COUNT; ++b) {
+        if (b == B_LUT) continue;
+        if (!plan.bytes[b]) {
+            ctx->bufs[b] = DevBuf{};
+            continue;
+        }
+        const size_t need = (plan.bytes[b] + 255) & ~size_t(255);
+        if (off + need > kStaticScratchBytes) {
+            for (int k = 0; k < B_COUNT; ++k)
+                if (k != B_LUT) ctx->bufs[k] = DevBuf{};
+            return static_fail(6, "pair needs " + std::to_string(apg_hip_pipeline_static_bytes(p, w, h)) +
+                                      " bytes of workspace; the static scratch holds " +
+                                      std::to_string(kStaticScratchBytes));
+        }
+        ctx->bufs[b] = DevBuf{sd.ptr + off, plan.bytes[b]};
+        off += need;
+    }
+    // every buffer fits, so the pipeline's ensure() allocates nothing
+    const int s = apg_hip_pipeline(ctx, p, d_left, d_right, w, h, stride, channels, d_disparity, d_confidence,
+                                   d_reason, d_reliable, nullptr);
+    if (s) g_static_error = ctx->last_error;
+    return s;
+}
 
 }  // extern "C"

@@ -291,6 +291,37 @@ APG_HIP_API int apg_hip_bench_pipeline(apg_hip_ctx ctx, const apg_hip_params* p,
  * reports it after the fact). */
 APG_HIP_API size_t apg_hip_pipeline_workspace_bytes(const apg_hip_params* p, int w, int h);
 
+/* ---- allocation-free entry point -------------------------------------- */
+
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): apg_hip_pipeline's fused schedule (Balanced / Fast: no
+ * prior or refiner, one Cross iteration, even D <= 256) on the caller's
+ * stream, with the whole workspace in a device array the library holds on
+ * each GPU it runs on until the process ends: apg_hip_static_scratch_bytes()
+ * = 768 MiB (1280x720 D=128 Balanced needs about 705 MiB).  A pair that needs
+ * more returns ResourceLimitExceeded (6), any other configuration
+ * UnsupportedBackend (5), before anything is launched.  No hipMalloc,
+ * hipFree or other allocating call is made.  Inputs and outputs are the
+ * caller's, as for apg_hip_pipeline; stream may be NULL (the null stream).
+ * Every call on one GPU shares that GPU's array, which keeps the last call's
+ * contents: calls that overlap on one GPU (other threads, other streams)
+ * race, so serialise them.  Under stream capture the cost LUT of the
+ * parameter set must already be cached by an earlier eager call (eight sets
+ * are kept).  Errors: the status code, with a message from
+ * apg_hip_static_last_error() (per calling thread). */
+APG_HIP_API int apg_hip_pipeline_static(const apg_hip_params* p,
+                                        const uint8_t* d_left, const uint8_t* d_right,
+                                        int w, int h, int stride, int channels,
+                                        float* d_disparity, float* d_confidence,
+                                        uint8_t* d_invalid_reason, uint8_t* d_reliable,
+                                        void* stream);
+/* Bytes of the static array one apg_hip_pipeline_static call on (w, h, p)
+ * uses; 0 when p is invalid or not on the fused schedule. */
+APG_HIP_API size_t apg_hip_pipeline_static_bytes(const apg_hip_params* p, int w, int h);
+APG_HIP_API size_t apg_hip_static_scratch_bytes(void);
+APG_HIP_API const char* apg_hip_static_last_error(void);
+
 /* ---- launch shapes ----------------------------------------------------- */
 
 /* Launch shapes of the fused schedule; 0 keeps the built-in choice.  They
