@@ -435,6 +435,66 @@ void PriorEstimator::estimate(const PipelineConfig& cfg, PipelineBuffers& buf) c
     buf.support_count = static_cast<size_t>(nsup);
 }
 
+std::vector<SupportMatch> PriorEstimator::extract_supports(const PipelineConfig& cfg,
+                                                           const PipelineBuffers& buf) const {
+    std::vector<SupportMatch> out;
+    if (!cfg.prior.enable) return out;
+    const int w = buf.left_gray.width(), h = buf.left_gray.height();
+    apg_hip_ctx ctx = stage_ctx();
+    // the device candidates and the host selection of estimate()
+    const apg_hip_params p = to_device_params(cfg);
+    const size_t n = npx(w, h);
+    const bool sym = cfg.cost.census == CensusType::SymmetricCensus9x7;
+    Dev gx(ctx, buf.left_gx.data(), n), gy(ctx, buf.left_gy.data(), n);
+    Dev cl(ctx, sym ? (const void*)buf.census_left.data() : (const void*)buf.census_left64.data(), n * (sym ? 4 : 8));
+    Dev cr(ctx, sym ? (const void*)buf.census_right.data() : (const void*)buf.census_right64.data(), n * (sym ? 4 : 8));
+    Dev dmin(ctx, n * 2), dmax(ctx, n * 2), dp(ctx, n * 4);
+    int nsup = 0;
+    check(ctx,
+          apg_hip_prior_estimate(ctx, &p, w, h, gx.as<uint8_t>(), gy.as<uint8_t>(), sym ? cl.as<uint32_t>() : nullptr,
+                                 sym ? cr.as<uint32_t>() : nullptr, sym ? nullptr : cl.as<uint64_t>(),
+                                 sym ? nullptr : cr.as<uint64_t>(), dmin.as<int16_t>(), dmax.as<int16_t>(),
+                                 dp.as<float>(), &nsup),
+          "prior");
+    std::vector<apg_hip_support> sup(static_cast<size_t>(nsup));
+    int cnt = 0;
+    check(ctx, apg_hip_last_supports(ctx, sup.data(), nsup, &cnt), "supports");
+    out.reserve(sup.size());
+    for (const apg_hip_support& s : sup) {
+        SupportMatch m;
+        m.x = s.x;
+        m.y = s.y;
+        m.disparity = s.disparity;
+        m.confidence = s.confidence;
+        out.push_back(m);
+    }
+    return out;
+}
+
+// ----------------------------------------------------------------- Refiner --
+void Refiner::refine(const PipelineConfig& cfg, PipelineBuffers& buf) const {
+    if (!cfg.refine.enable) return;
+    const int w = buf.disparity.width(), h = buf.disparity.height();
+    apg_hip_ctx ctx = stage_ctx();
+    const apg_hip_params p = to_device_params(cfg);
+    const size_t n = npx(w, h);
+    const bool sym = cfg.cost.census == CensusType::SymmetricCensus9x7;
+    Dev gl(ctx, buf.left_gray.data(), n), gxl(ctx, buf.left_gx.data(), n), gyl(ctx, buf.left_gy.data(), n);
+    Dev gr(ctx, buf.right_gray.data(), n), gxr(ctx, buf.right_gx.data(), n), gyr(ctx, buf.right_gy.data(), n);
+    Dev cl(ctx, sym ? (const void*)buf.census_left.data() : (const void*)buf.census_left64.data(), n * (sym ? 4 : 8));
+    Dev cr(ctx, sym ? (const void*)buf.census_right.data() : (const void*)buf.census_right64.data(), n * (sym ? 4 : 8));
+    Dev dmin(ctx, buf.range.dmin.data(), n * 2), dmax(ctx, buf.range.dmax.data(), n * 2);
+    Dev rel(ctx, buf.reliable_mask.data(), n), disp(ctx, buf.disparity.data(), n * 4);
+    check(ctx,
+          apg_hip_refine(ctx, &p, w, h, gl.as<uint8_t>(), gxl.as<uint8_t>(), gyl.as<uint8_t>(), gr.as<uint8_t>(),
+                         gxr.as<uint8_t>(), gyr.as<uint8_t>(), sym ? cl.as<uint32_t>() : nullptr,
+                         sym ? cr.as<uint32_t>() : nullptr, sym ? nullptr : cl.as<uint64_t>(),
+                         sym ? nullptr : cr.as<uint64_t>(), dmin.as<int16_t>(), dmax.as<int16_t>(),
+                         rel.as<uint8_t>(), disp.as<float>()),
+          "refine");
+    disp.get(buf.disparity.data(), n * 4);
+}
+
 // ---------------------------------------------------------- CostAggregator --
 void CostAggregator::aggregate(const PipelineConfig& cfg, PipelineBuffers& buf) const {
     if (!cfg.aggregation.enable) return;

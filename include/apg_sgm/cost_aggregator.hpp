@@ -1,0 +1,6 @@
+// CostAggregator (reference include/apg_sgm/cost_aggregator.hpp:9-30).
+// The stage classes share one declaration header; this path is the
+// reference's, so code written against it compiles unchanged.
+#pragma once
+
+#include "apg_sgm/stages.hpp"

@@ -1,0 +1,6 @@
+// MemoryPolicy and ExecutionOptions (reference include/apg_sgm/execution.hpp:7-16).
+// The session types share one declaration header; this path is the
+// reference's, so code written against it compiles unchanged.
+#pragma once
+
+#include "apg_sgm/pipeline.hpp"

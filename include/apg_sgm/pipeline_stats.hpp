@@ -1,0 +1,6 @@
+// StageTiming and PipelineStats (reference include/apg_sgm/pipeline_stats.hpp:10-22).
+// The session types share one declaration header; this path is the
+// reference's, so code written against it compiles unchanged.
+#pragma once
+
+#include "apg_sgm/pipeline.hpp"

@@ -1,0 +1,6 @@
+// PriorEstimator (reference include/apg_sgm/prior_estimator.hpp).
+// The stage classes share one declaration header; this path is the
+// reference's, so code written against it compiles unchanged.
+#pragma once
+
+#include "apg_sgm/stages.hpp"

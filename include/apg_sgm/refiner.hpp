@@ -1,0 +1,6 @@
+// Refiner (reference include/apg_sgm/refiner.hpp).
+// The stage classes share one declaration header; this path is the
+// reference's, so code written against it compiles unchanged.
+#pragma once
+
+#include "apg_sgm/stages.hpp"

@@ -1,0 +1,6 @@
+// StereoResult (reference include/apg_sgm/result.hpp).
+// The session types share one declaration header; this path is the
+// reference's, so code written against it compiles unchanged.
+#pragma once
+
+#include "apg_sgm/pipeline.hpp"

@@ -4,6 +4,8 @@
 //   CostAggregator  (reference include/apg_sgm/cost_aggregator.hpp:9-30)
 //   SgmOptimizer    (reference include/apg_sgm/sgm_optimizer.hpp:20-61)
 //   ConfidenceEstimator (reference include/apg_sgm/confidence_estimator.hpp)
+//   PriorEstimator  (reference include/apg_sgm/prior_estimator.hpp)
+//   Refiner         (reference include/apg_sgm/refiner.hpp)
 //   PostProcessor   (reference include/apg_sgm/postprocess.hpp:8-13)
 // They keep the reference's host-buffer contract (PipelineBuffers in, volumes
 // out on the host), so every call uploads its inputs, runs the device
@@ -45,10 +47,16 @@ public:
     static int popcount64(uint64_t x) { return __builtin_popcountll(x); }
 };
 
-// Search-range init with the prior off (reference src/prior_estimator.cpp:414-428).
+// Search ranges (reference src/prior_estimator.cpp:414-435): the dense init
+// [min, min(max, x+1)), narrowed by the sparse prior when cfg.prior.enable.
+// extract_supports (reference :29-166) returns the support matches the prior
+// would use, from the buffers' census and gradients.  The reference's two
+// middle steps (interpolate_prior, apply_search_range) run fused on the device
+// inside estimate and are not exposed separately.
 class PriorEstimator {
 public:
     void estimate(const PipelineConfig& cfg, PipelineBuffers& buf) const;
+    std::vector<SupportMatch> extract_supports(const PipelineConfig& cfg, const PipelineBuffers& buf) const;
 };
 
 class CostAggregator {
@@ -87,6 +95,14 @@ class ConfidenceEstimator {
 public:
     void estimate(const PipelineConfig& cfg, PipelineBuffers& buf, const Image32f& best_cost,
                   const Image32f& second_cost) const;
+};
+
+// Randomised raster refinement of the unreliable pixels (reference
+// src/refiner.cpp:42-98) over buf.disparity, using the buffers' gray,
+// gradients, census, search range and reliable mask.
+class Refiner {
+public:
+    void refine(const PipelineConfig& cfg, PipelineBuffers& buf) const;
 };
 
 class PostProcessor {

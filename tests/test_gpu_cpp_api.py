@@ -158,3 +158,29 @@ def test_packed_stage_classes_match_oracle(tmp_path, w, h, dmax):
     g, gx, gy, cen = O.aux(c, left)
     acc = O.sgm_path(c, g, o["cost"], np.zeros(o["cost"].shape, np.uint32), 1, 0)
     np.testing.assert_array_equal(rd("path.bin", np.uint32), _pack(acc, dmn, dmx, 0))
+
+
+@pytest.mark.gpu
+def test_client_on_reference_header_paths(tmp_path):
+    # a C++ client including every header by the reference's path: session,
+    # matcher with a retained workspace, HighQuality stage chain with the refiner
+    w, h, D = 120, 80, 32
+    left, right, _ = A.synth_pair(2, w, h, D, 31)
+    lp, rp = tmp_path / "l.raw", tmp_path / "r.raw"
+    left.tofile(lp)
+    right.tofile(rp)
+    print(f"reference-header client {w}x{h} D={D}", flush=True)
+    out = subprocess.run([os.path.join(BUILD, "reference_headers_main"), str(lp), str(rp), str(w), str(h), str(D)],
+                         capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    got = json.loads(out.stdout)
+    bal = O.pipeline(O.cfg(1, D), left, right)
+    for k in ("disparity", "confidence", "invalid_reason", "reliable"):
+        assert got[k] == hex(O.fnv1a64(bal[k])), k
+    assert got["matcher_disparity"] == hex(O.fnv1a64(bal["disparity"]))
+    hq = O.pipeline(O.cfg(2, D), left, right, intermediates=True)
+    assert got["hq_dmin"] == hex(O.fnv1a64(hq["dmin"]))
+    assert got["hq_dmax"] == hex(O.fnv1a64(hq["dmax"]))
+    assert got["hq_disp_wta"] == hex(O.fnv1a64(hq["disp_wta"]))
+    assert got["hq_after_refine"] == hex(O.fnv1a64(hq["after_refine"]))
+    assert got["hq_supports"] > 0

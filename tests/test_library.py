@@ -123,3 +123,23 @@ def test_workspace_bytes_count_the_refiner():
     hq = A.workspace_bytes(A.params(2, 64), w, h)
     no_ref = A.workspace_bytes(A.params(2, 64, refine_enable=0), w, h)
     assert hq - no_ref >= 4 * 4 * (w * h + 4096) + 100 * w * h
+
+
+REF_HEADER_PATHS = ["cost_computer", "cost_aggregator", "sgm_optimizer", "postprocess", "prior_estimator",
+                    "confidence_estimator", "refiner", "session", "workspace", "execution", "result",
+                    "pipeline_stats", "pipeline", "buffers", "config", "image", "image_view", "packed_volume",
+                    "search_range", "status", "types", "metrics", "ablation"]
+
+
+@pytest.mark.parametrize("name", REF_HEADER_PATHS)
+def test_reference_header_path_compiles_alone(tmp_path, name):
+    # each header the reference's clients include, by its path, in a TU of its own
+    src = tmp_path / "tu.cpp"
+    src.write_text(f'#include "apg_sgm/{name}.hpp"\nint main() {{ return 0; }}\n')
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I", os.path.join(ROOT, "include"), str(src)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+
+
+def test_reference_headers_client_is_built():
+    assert os.path.exists(os.path.join(ROOT, "apg-sgm_amd", "_build", "reference_headers_main"))
