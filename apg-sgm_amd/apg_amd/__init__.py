@@ -141,6 +141,8 @@ def load():
         "apg_hip_bench_pipeline": (i, [vp, P, vp, vp, i, i, i, i, vp, vp, vp, vp, i, i, C.POINTER(C.c_double),
                                        C.POINTER(Timing)]),
         "apg_hip_pipeline_workspace_bytes": (sz, [P, i, i]),
+        "apg_hip_bench_pipeline_steps": (i, [vp, P, vp, vp, i, i, i, i, vp, vp, vp, vp, i, i,
+                                             C.POINTER(C.c_float)]),
         "apg_hip_pipeline_static": (i, [P, vp, vp, i, i, i, i, vp, vp, vp, vp, vp]),
         "apg_hip_pipeline_static_bytes": (sz, [P, i, i]),
         "apg_hip_static_scratch_bytes": (sz, []),

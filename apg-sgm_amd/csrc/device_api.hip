@@ -955,6 +955,44 @@ int apg_hip_bench_pipeline(apg_hip_ctx ctx, const apg_hip_params* p, const uint8
     return status;
 }
 
+int apg_hip_bench_pipeline_steps(apg_hip_ctx ctx, const apg_hip_params* p, const uint8_t* d_left,
+                                 const uint8_t* d_right, int w, int h, int stride, int channels, float* d_disp,
+                                 float* d_conf, uint8_t* d_reason, uint8_t* d_rel, int warmup, int steps,
+                                 float* step_ms) {
+    if (!ctx) return 8;
+    if (steps <= 0 || !step_ms) return fail(ctx, 4, "steps and step_ms required");
+    const bool was_timing = ctx->timing;
+    ctx->timing = false;
+    int status = 0;
+    for (int i = 0; i < warmup && !status; ++i)
+        status = apg_hip_pipeline(ctx, p, d_left, d_right, w, h, stride, channels, d_disp, d_conf, d_reason, d_rel,
+                                  nullptr);
+    std::vector<hipEvent_t> ev((size_t)steps + 1, nullptr);
+    hipError_t e = hipSuccess;
+    for (auto& x : ev)
+        if (e == hipSuccess) e = hipEventCreate(&x);
+    if (!status && e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (!status && e == hipSuccess) {
+        // one event before each call and one after the last: back-to-back calls,
+        // each interval is one pair's device time on the context stream
+        (void)hipEventRecord(ev[0], ctx->stream);
+        for (int i = 0; i < steps && !status; ++i) {
+            status = apg_hip_pipeline(ctx, p, d_left, d_right, w, h, stride, channels, d_disp, d_conf, d_reason,
+                                      d_rel, nullptr);
+            (void)hipEventRecord(ev[(size_t)i + 1], ctx->stream);
+        }
+        e = hipEventSynchronize(ev[(size_t)steps]);
+        for (int i = 0; i < steps && e == hipSuccess && !status; ++i)
+            e = hipEventElapsedTime(&step_ms[i], ev[(size_t)i], ev[(size_t)i + 1]);
+    }
+    for (auto& x : ev)
+        if (x) (void)hipEventDestroy(x);
+    ctx->timing = was_timing;
+    if (status) return status;
+    if (e != hipSuccess) return hip_fail(ctx, e, "bench steps");
+    return 0;
+}
+
 const char* apg_hip_version_string(void) { return "1.0.0-gfx950"; }
 
 }  // extern "C"

@@ -5,12 +5,20 @@ Workload (N=1): Balanced mode, one 1280x720 pair, D=128 (the synthetic
 ramp + 8 rectangles + gain/bias generator), inputs resident in HBM.  One step =
 one full device pipeline (aux, range, cost, right WTA, Cross H+V, 4-path SGM,
 WTA + subpixel, confidence, LR check, fill, median).  Timed with HIP events on
-the library's stream.  Prints ONE JSON line.
+the library's stream.  Prints ONE JSON line; progress goes to stderr.
 
-The CPU baseline (reported, not the target) is the reference implementation
-compiled from its own sources (oracle/_ref) when present, else the repo's C
-oracle, on the same pair with all host cores: one parity run (bit-exact
-against the GPU disparity), then about 10 s of repeated runs, median.
+Beside the headline value the line carries:
+  latency_ms     p50 / p99 of the per-pair device time over back-to-back calls;
+  session        apg_session_process (the reference's C ABI: host images in,
+                 host results out) wall time per frame, both memory policies;
+  roofline       the dominant kernel against 8 TB/s;
+  cpu_baseline   the reference implementation compiled from its own sources
+                 (oracle/_ref) on the GPU box's host: dense and packed AVX2
+                 backends over all threads (warmup 2, repeat 5, median), the
+                 faster one as the headline, and a 1-thread figure;
+  forms          bounded runs of the batched form (64 uneven pairs), the
+                 serving form (Poisson mix, p50 / p99) and HighQuality
+                 1920x1080 D=256, each with its own parity check.
 """
 import argparse
 import ctypes as C
@@ -40,40 +48,79 @@ W, H, DMAX = 1280, 720, 128
 BYTES_PER_STATE = {"pipeline": 12.0, "sgm": 4.0, "cross": 4.0, "cost": 2.0, "wta": 2.0}
 
 
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_model():
+    """CPU model name of this host (lscpu, else /proc/cpuinfo)."""
+    import subprocess
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for ln in out.splitlines():
+            if ln.startswith("Model name:"):
+                return ln.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+CPU_WARMUP, CPU_REPEAT = 2, 5   # the reference benchmark's defaults (apps/benchmark_stereo.cpp:132-133)
+
+
 def cpu_baseline(left, right, p, gpu_disp):
-    """The reference pipeline (oracle/_ref, compiled from the reference's
-    sources) on the bench pair, OpenMP over all host threads: one run for
-    parity with the GPU disparity, then repeated runs for about 10 s, median
-    per-pair time.  The repo's C oracle stands in when the reference build is
-    absent."""
+    """The reference pipeline compiled from its own sources (oracle/_ref) on
+    the bench pair, on this host: the dense backend (what Balanced runs by
+    default) and the packed AVX2 backend (its fastest bit-exact path), each
+    over all OpenMP threads with warmup 2 and repeat 5 (median); the faster
+    is the headline.  Then the faster backend on one thread (warmup 1,
+    repeat 3: a bounded sample).  Parity: both backends' disparity against
+    the GPU's.  The repo's C oracle stands in when the reference build is
+    absent (one timing over all threads)."""
     import pyoracle as O
     c = O.cfg_from(p.as_dict())
     states = left.shape[0] * left.shape[1] * (p.max_disparity - p.min_disparity)
-    t0 = time.perf_counter()
-    if O.ref_available():
-        out = O.ref_pipeline(c, left, right)
-        kind, threads = "reference", O.ref_max_threads()
-    else:
+    mds = lambda ms: round(states / (ms * 1e-3) / 1e6, 1)  # noqa: E731
+    model = cpu_model()
+    if not O.ref_available():
         threads = os.cpu_count() or 1
         O.set_threads(threads)
         out = O.pipeline(c, left, right)
-        kind = "port"
-    t1 = time.perf_counter() - t0
-    parity = bool(np.array_equal(out["disparity"], gpu_disp))
-    reps = int(min(60, max(3, round(10.0 / max(t1, 1e-3)))))
-    if kind == "reference":
-        ms = O.ref_time_pipeline(c, left, right, threads=0, warmup=0, reps=reps)
-    else:
         ts = []
-        for _ in range(reps):
+        for _ in range(CPU_REPEAT):
             t = time.perf_counter()
             O.pipeline(c, left, right)
             ts.append((time.perf_counter() - t) * 1e3)
         ms = float(sorted(ts)[len(ts) // 2])
-    return {"value": round(states / (ms * 1e-3) / 1e6, 1), "unit": "Mdisp/s", "cores": threads, "kind": kind,
-            "sample": f"Balanced pipeline on the bench pair {left.shape[1]}x{left.shape[0]} D={DMAX}: median of "
-                      f"{reps} runs ({ms:.1f} ms per pair) after one parity run, OpenMP over {threads} host threads"
-            }, parity
+        return {"value": mds(ms), "unit": "Mdisp/s", "cores": threads, "kind": "port", "cpu_model": model,
+                "sample": f"C oracle on the bench pair, median of {CPU_REPEAT} runs ({ms:.1f} ms per pair)"
+                }, bool(np.array_equal(out["disparity"], gpu_disp))
+    threads = O.ref_max_threads()
+    backends = {}
+    parity = True
+    for name, be in (("dense", 1), ("packed_avx2", 2)):
+        log(f"cpu baseline: reference {name} backend, {threads} threads")
+        ref = O.ref_pipeline(c, left, right, backend=be)
+        parity = parity and bool(np.array_equal(ref["disparity"], gpu_disp))
+        ms = O.ref_time_pipeline(c, left, right, threads=threads, warmup=CPU_WARMUP, reps=CPU_REPEAT, backend=be)
+        backends[name] = {"ms": round(ms, 2), "mdisp_s": mds(ms)}
+    best = min(backends, key=lambda k: backends[k]["ms"])
+    log(f"cpu baseline: reference {best} backend, 1 thread")
+    ms1 = O.ref_time_pipeline(c, left, right, threads=1, warmup=1, reps=3, backend=1 if best == "dense" else 2)
+    O.set_threads(threads)   # the OpenMP thread count is process-wide: back to all threads
+    hb = backends[best]
+    return {"value": hb["mdisp_s"], "unit": "Mdisp/s", "cores": threads, "kind": "reference", "cpu_model": model,
+            "backend": best, "backends": backends, "warmup": CPU_WARMUP, "repeat": CPU_REPEAT,
+            "one_thread": {"backend": best, "ms": round(ms1, 2), "mdisp_s": mds(ms1), "warmup": 1, "repeat": 3},
+            "sample": f"Balanced pipeline on the bench pair {left.shape[1]}x{left.shape[0]} D={DMAX}, reference "
+                      f"build, {best} backend, median of {CPU_REPEAT} after {CPU_WARMUP} warmup runs "
+                      f"({hb['ms']:.1f} ms per pair), OpenMP over {threads} host threads"}, parity
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_balanced_1280x720_d128.md")
@@ -96,6 +143,62 @@ def measured_traffic(stage):
     return None
 
 
+# --------------------------------------------------------- session C ABI ----
+
+class _ImageView(C.Structure):
+    _fields_ = [("data", C.c_void_p), ("width", C.c_int), ("height", C.c_int), ("stride", C.c_int),
+                ("channels", C.c_int)]
+
+
+def session_timing(left, right, frames, gpu_disp):
+    """Wall time of apg_session_process (the reference's C ABI, Balanced,
+    max_disparity 128: host images in, H2D, device pipeline, D2H of the four
+    result planes) per frame, for ReleasePerFrame (the default: the device
+    workspace is allocated and freed every frame) and BoundedReuse.  Two
+    warmup frames each; parity of the last frame's disparity with the
+    device-resident run."""
+    import apg_amd as A
+    lib = A.load()
+    lib.apg_session_create.restype = C.c_void_p
+    lib.apg_session_create.argtypes = [C.c_int, C.c_int]
+    lib.apg_result_create.restype = C.c_void_p
+    lib.apg_session_process.argtypes = [C.c_void_p, C.POINTER(_ImageView), C.POINTER(_ImageView), C.c_void_p]
+    lib.apg_result_get_disparity.restype = C.POINTER(C.c_float)
+    lib.apg_result_get_disparity.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    lib.apg_session_destroy.argtypes = [C.c_void_p]
+    lib.apg_result_destroy.argtypes = [C.c_void_p]
+    h, w = left.shape
+    lv, rv = _ImageView(left.ctypes.data, w, h, w, 1), _ImageView(right.ctypes.data, w, h, w, 1)
+    out = {}
+    res = C.c_void_p(lib.apg_result_create())
+    parity = True
+    for name, policy in (("release_per_frame", 0), ("bounded_reuse", 1)):
+        sess = C.c_void_p(lib.apg_session_create(1, policy))
+        ts = []
+        for i in range(frames + 2):
+            t0 = time.perf_counter()
+            rc = lib.apg_session_process(sess, C.byref(lv), C.byref(rv), res)
+            t1 = time.perf_counter()
+            if rc:
+                raise RuntimeError(f"apg_session_process status {rc}")
+            if i >= 2:
+                ts.append((t1 - t0) * 1e3)
+        ww, hh = C.c_int(), C.c_int()
+        d = np.ctypeslib.as_array(lib.apg_result_get_disparity(res, C.byref(ww), C.byref(hh)), (h, w))
+        parity = parity and bool(np.array_equal(d, gpu_disp))
+        lib.apg_session_destroy(sess)
+        ts = np.asarray(ts)
+        out[name] = {"p50_ms": round(float(np.percentile(ts, 50)), 3), "p99_ms": round(float(np.percentile(ts, 99)), 3),
+                     "mean_ms": round(float(ts.mean()), 3),
+                     "mdisp_s": round(w * h * DMAX / (float(np.percentile(ts, 50)) * 1e-3) / 1e6, 1)}
+    lib.apg_result_destroy(res)
+    out["frames"] = frames
+    out["parity_vs_device_run"] = parity
+    out["how"] = ("apg_session_process wall time per frame (host inputs, upload, device pipeline, download of "
+                  "disparity / confidence / invalid_reason / reliable), one thread, back to back")
+    return out
+
+
 # ------------------------------------------------------------ mixed forms ----
 
 TUNE_CACHE = os.path.join(ROOT, "apg-sgm_amd", "apg_amd", "launch_shapes_gfx950.json")
@@ -109,13 +212,14 @@ def bench_batched(args):
     import apg_amd as A
     specs = batch_specs(args.pairs)
     states = sum(w * h * d for w, h, d, _ in specs)
+    log(f"batched form: {len(specs)} pairs")
     ctx = A.Context(0)
     pairs = []
     for w, h, d, sd in specs:
         left, right, _ = A.synth_pair(0, w, h, d, sd)
         pairs.append(A.DevicePair(ctx, A.params(1, d), left, right))
     # plain per-pair path
-    for q in pairs[: args.warmup]:
+    for q in pairs:   # one pass sizes the workspace for every shape
         ctx._check(ctx.lib.apg_hip_pipeline(ctx.h, C.byref(q.params), q.left.ptr, q.right.ptr, q.w, q.h, q.w, 1,
                                             q.disp.ptr, q.conf.ptr, q.reason.ptr, q.rel.ptr, None), "pipeline")
     ctx.sync()
@@ -185,6 +289,7 @@ def bench_serving(args):
     on their own streams, upload / pipeline / download per request).  The
     plain per-pair path is the same server with one slot (no overlap)."""
     import apg_amd as A
+    log(f"serving form: rates {args.rates}, {args.seconds} s each")
     pool = {}
     for ci, (w, h, d) in enumerate(SERVE_CLASSES):
         for v in range(4):
@@ -222,7 +327,7 @@ def bench_serving(args):
     srv.close()
     one = A.Server(0, 1, args.budget_gb << 30)
     one.replay([request(*k) for k in warm], [0.0] * len(warm))
-    sat_n = int(min(args.max_requests, 1000))
+    sat_n = int(min(args.max_requests, args.sat_requests))
     plain = run(one, 1e6, sat_n)
     one.close()
     srv = A.Server(0, args.slots, args.budget_gb << 30)
@@ -253,6 +358,7 @@ def bench_hq(args):
     import apg_amd as A
     import pyoracle as O
     w, h, d = 1920, 1080, 256
+    log("HighQuality 1920x1080 D=256")
     p = A.params(2, d)
     left, right, _ = A.synth_pair(2, w, h, d, SEED_HQ)
     ctx = A.Context(0)
@@ -291,7 +397,67 @@ def bench_hq(args):
                                "sample": f"one HighQuality run on the bench pair ({cpu_ms:.0f} ms), OpenMP over "
                                          f"{threads} host threads"}
         out["parity_vs_cpu"] = bool(np.array_equal(ref["disparity"], disp, equal_nan=True))
-    print(json.dumps(out))
+    return out
+
+
+def bench_pair(args):
+    """The headline: Balanced 1280x720 D=128 on device-resident inputs."""
+    import apg_amd as A
+    p = A.params(1, DMAX)
+    left, right, _ = A.synth_pair(1, W, H, DMAX, SEED)
+    states = W * H * DMAX
+    ctx = A.Context(0)
+    A.set_schedule(ctx, args.schedule)
+    if args.shapes:
+        A.set_tuning(ctx, tuple(int(v) for v in args.shapes.split(",")))
+    dl, dr = ctx.to_device(left), ctx.to_device(right)
+    od = ctx.array((H, W), np.float32)
+    oc = ctx.array((H, W), np.float32)
+    orr = ctx.array((H, W), np.uint8)
+    om = ctx.array((H, W), np.uint8)
+    ms = C.c_double()
+    tim = A.Timing()
+    log(f"pair: {args.warmup} warmup + {args.steps} timed steps")
+    rc = ctx.lib.apg_hip_bench_pipeline(ctx.h, C.byref(p), dl.ptr, dr.ptr, W, H, W, 1, od.ptr, oc.ptr, orr.ptr,
+                                        om.ptr, args.warmup, args.steps, C.byref(ms), C.byref(tim))
+    ctx._check(rc, "bench")
+    ms_step = ms.value / args.steps
+    gpu_disp = od.download()
+    value = states / (ms_step * 1e-3) / 1e6
+    # per-pair device time of back-to-back calls: p50 / p99
+    nlat = max(100, args.latency_steps)
+    lat = (C.c_float * nlat)()
+    ctx._check(ctx.lib.apg_hip_bench_pipeline_steps(ctx.h, C.byref(p), dl.ptr, dr.ptr, W, H, W, 1, od.ptr, oc.ptr,
+                                                    orr.ptr, om.ptr, 2, nlat, lat), "bench steps")
+    lat = np.ctypeslib.as_array(lat).astype(np.float64)
+
+    stages = {k: getattr(tim, k) for k in ("aux_ms", "prior_ms", "cost_ms", "right_wta_ms", "cross_ms",
+                                            "sgm_ms", "wta_ms", "confidence_ms", "refine_ms", "post_ms", "total_ms")}
+    dom = max(("cost", "cross", "sgm", "wta"), key=lambda k: stages[k + "_ms"])
+    dom_ms = stages[dom + "_ms"]
+    achieved = states * BYTES_PER_STATE[dom] / (dom_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "kernel": STAGE_KERNEL[dom], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom),
+                "pipeline_frac": round(states * BYTES_PER_STATE["pipeline"] / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    out = {"metric": "Mdisp/s (W*H*D per sec), Balanced 1280x720 D=128", "value": round(value, 1),
+           "unit": "Mdisp/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": round(ms_step, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "u16", "data": "synthetic",
+           "config": {"workload": "Balanced single pair 1280x720 D=128 (ramp + 8 rects, gain/bias)",
+                      "width": W, "height": H, "disparities": DMAX, "mode": "Balanced", "seed": SEED},
+           "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4), "p99": round(float(np.percentile(lat, 99)), 4),
+                          "mean": round(float(lat.mean()), 4), "steps": nlat,
+                          "how": "HIP events around each of back-to-back pipeline calls on the library stream"},
+           "stage_ms": {k: round(v, 4) for k, v in stages.items()}, "roofline": roofline}
+    ctx.close()
+    if not args.no_session:
+        log("session C ABI timing")
+        out["session"] = session_timing(left, right, args.session_frames, gpu_disp)
+    if not args.no_cpu:
+        cb, parity = cpu_baseline(left, right, p, gpu_disp)
+        out["cpu_baseline"] = cb
+        out["parity_vs_cpu"] = parity
+    return out
 
 
 def main():
@@ -300,20 +466,25 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-forms", action="store_true", help="pair form: leave out the batched / serving / hq runs")
+    ap.add_argument("--no-session", action="store_true", help="pair form: leave out the session C ABI timing")
+    ap.add_argument("--latency-steps", type=int, default=200, help="pair form: calls timed one by one for p50/p99")
+    ap.add_argument("--session-frames", type=int, default=50)
     ap.add_argument("--schedule", type=int, default=0, help="0 fused (default), 1 stage by stage")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MB",
                     help="first run one device-to-device copy of MB MiB (a kernel of known traffic, "
                          "to scale the FETCH_SIZE / WRITE_SIZE counters of a profiled run)")
     ap.add_argument("--form", choices=("pair", "batched", "serving", "hq"), default="pair",
-                    help="pair: the single-pair headline (default); batched / serving: the mixed-size forms")
+                    help="pair: the single-pair headline (default); batched / serving / hq: one form alone")
     ap.add_argument("--slots", type=int, default=4, help="concurrent pipelines of the batched / serving forms")
     ap.add_argument("--pairs", type=int, default=64, help="batched form: pairs per batch")
-    ap.add_argument("--reps", type=int, default=5, help="batched form: timed batches")
+    ap.add_argument("--reps", type=int, default=5, help="batched / hq forms: timed repetitions")
     ap.add_argument("--tune-cache", default=TUNE_CACHE, help="batched form: launch-shape cache (JSON)")
     ap.add_argument("--rates", type=float, nargs="+", default=[50, 200, 500, 1000, 2000],
                     help="serving form: arrival rates (requests/s)")
     ap.add_argument("--seconds", type=float, default=2.0, help="serving form: trace length per rate")
     ap.add_argument("--max-requests", type=int, default=2000)
+    ap.add_argument("--sat-requests", type=int, default=1000, help="serving form: requests of the saturated runs")
     ap.add_argument("--budget-gb", type=int, default=64, help="serving form: HBM admission budget")
     ap.add_argument("--shapes", default=None, metavar="SEG,DUAL,CVW",
                     help="pair form: launch shapes (cost_segment, sgm_dual, cross_v_waves; 0 = built-in)")
@@ -332,56 +503,24 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
-    if args.form == "hq":
-        bench_hq(args)
-        return
     if args.form != "pair":
-        out = bench_batched(args) if args.form == "batched" else bench_serving(args)
+        out = {"batched": bench_batched, "serving": bench_serving, "hq": bench_hq}[args.form](args)
         print(json.dumps(out))
         return
 
-    import apg_amd as A
-    p = A.params(1, DMAX)
-    left, right, _ = A.synth_pair(1, W, H, DMAX, SEED)
-    states = W * H * DMAX
-    ctx = A.Context(0)
-    A.set_schedule(ctx, args.schedule)
-    if args.shapes:
-        A.set_tuning(ctx, tuple(int(v) for v in args.shapes.split(",")))
-    dl, dr = ctx.to_device(left), ctx.to_device(right)
-    od = ctx.array((H, W), np.float32)
-    oc = ctx.array((H, W), np.float32)
-    orr = ctx.array((H, W), np.uint8)
-    om = ctx.array((H, W), np.uint8)
-    ms = C.c_double()
-    tim = A.Timing()
-    rc = ctx.lib.apg_hip_bench_pipeline(ctx.h, C.byref(p), dl.ptr, dr.ptr, W, H, W, 1, od.ptr, oc.ptr, orr.ptr,
-                                        om.ptr, args.warmup, args.steps, C.byref(ms), C.byref(tim))
-    ctx._check(rc, "bench")
-    ms_step = ms.value / args.steps
-    gpu_disp = od.download()
-    value = states / (ms_step * 1e-3) / 1e6
-
-    stages = {k: getattr(tim, k) for k in ("aux_ms", "prior_ms", "cost_ms", "right_wta_ms", "cross_ms",
-                                            "sgm_ms", "wta_ms", "confidence_ms", "refine_ms", "post_ms", "total_ms")}
-    dom = max(("cost", "cross", "sgm", "wta"), key=lambda k: stages[k + "_ms"])
-    dom_ms = stages[dom + "_ms"]
-    achieved = states * BYTES_PER_STATE[dom] / (dom_ms * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "kernel": STAGE_KERNEL[dom], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom),
-                "pipeline_frac": round(states * BYTES_PER_STATE["pipeline"] / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
-    out = {"metric": "Mdisp/s (W*H*D per sec), Balanced 1280x720 D=128", "value": round(value, 1),
-           "unit": "Mdisp/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-           "ms_per_step": round(ms_step, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-           "dtype": "u16", "data": "synthetic",
-           "config": {"workload": "Balanced single pair 1280x720 D=128 (ramp + 8 rects, gain/bias)",
-                      "width": W, "height": H, "disparities": DMAX, "mode": "Balanced", "seed": SEED},
-           "stage_ms": {k: round(v, 4) for k, v in stages.items()}, "roofline": roofline}
-    if not args.no_cpu:
-        cb, parity = cpu_baseline(left, right, p, gpu_disp)
-        out["cpu_baseline"] = cb
-        out["parity_vs_cpu"] = parity
-    ctx.close()
+    out = bench_pair(args)
+    if not args.no_forms:
+        # bounded runs of the other forms (about a minute together)
+        fa = argparse.Namespace(**vars(args))
+        fa.reps, fa.warmup = 3, 1
+        fa.rates, fa.seconds, fa.max_requests, fa.sat_requests = [500.0, 1000.0], 1.0, 1000, 400
+        forms = {}
+        for name, fn in (("batched", bench_batched), ("serving", bench_serving), ("hq", bench_hq)):
+            t0 = time.perf_counter()
+            r = fn(fa)
+            r["wall_s"] = round(time.perf_counter() - t0, 1)
+            forms[name] = r
+        out["forms"] = forms
     print(json.dumps(out))
 
 

@@ -284,6 +284,17 @@ APG_HIP_API int apg_hip_bench_pipeline(apg_hip_ctx ctx, const apg_hip_params* p,
                                        int warmup, int steps, double* ms_total,
                                        apg_hip_timing* stage_avg);
 
+/* Per-pair timing (no counterpart in the reference): `warmup` untimed calls,
+ * then `steps` back-to-back calls with a HIP event before each and after the
+ * last; step_ms[i] = device time of call i on the context stream.  Their
+ * percentiles are the per-pair latency at full load. */
+APG_HIP_API int apg_hip_bench_pipeline_steps(apg_hip_ctx ctx, const apg_hip_params* p,
+                                             const uint8_t* d_left, const uint8_t* d_right,
+                                             int w, int h, int stride, int channels,
+                                             float* d_disparity, float* d_confidence,
+                                             uint8_t* d_invalid_reason, uint8_t* d_reliable,
+                                             int warmup, int steps, float* step_ms);
+
 /* Device bytes the workspace of one apg_hip_pipeline call on a (w, h) pair
  * with parameters p needs (fused schedule when eligible); 0 for invalid
  * parameters.  Used for HBM admission.  No counterpart in the reference
