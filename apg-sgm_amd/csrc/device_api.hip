@@ -157,6 +157,15 @@ int finish_timing(apg_hip_ctx ctx) {
     return 0;
 }
 
+// The path sums fit u16 (reference include/apg_sgm/sgm_optimizer.hpp:9-18);
+// beyond that bound they are accumulated in u32, as the reference's dense
+// optimize always does (src/sgm_optimizer.cpp:261-274).
+bool u16_path_sums(const apg_hip_params& p) {
+    const long p2max = p.P2_base > p.P1 + 1 ? p.P2_base : p.P1 + 1;
+    const long npath = p.paths == 8 ? 8 : 4;
+    return npath * ((long)p.cost_max + p2max) < 65535;
+}
+
 // Workspace of one apg_hip_pipeline call: which schedule runs and the bytes
 // of each buffer it needs.
 struct WsPlan {
@@ -221,7 +230,7 @@ WsPlan plan_workspace(const apg_hip_params& p, int w, int h, const CostLut& lut,
         if (pl.planes_sgm)
             for (int r = 0; r < npaths; ++r) pl.bytes[B_DS0 + r] = nv;
         else
-            pl.bytes[B_SUM] = nv * 2;
+            pl.bytes[B_SUM] = nv * (u16_path_sums(p) ? 2 : 4);
         pl.bytes[B_BEST] = pl.bytes[B_SECOND] = n * 4;
     }
     return pl;
@@ -284,10 +293,14 @@ int apg_hip_validate_params(const apg_hip_params* c) {
     const long D = (long)c->max_disparity - c->min_disparity;
     const long p2max = c->P2_base > c->P1 + 1 ? c->P2_base : c->P1 + 1;
     const long npath = c->paths == 8 ? 8 : 4;
-    if (npath * ((long)c->cost_max + p2max) >= 65535) return 5;
+    (void)npath;
+    // one path's values (at most cost_max + P2max) are computed in u16; their
+    // sums go to u32 past the u16 bound
+    if ((long)c->cost_max + p2max >= 65535) return 5;
     if (D > kMaxD || D > 2 * 64 * kMaxSgmK) return 5;
     if (c->agg_enable && c->max_arm_length > kMaxArm) return 5;
-    if (c->post_median_radius > 3) return 5;
+    // the median's window tile (16 + 2r)^2 floats stays within 160 KB of LDS
+    if (c->post_median_radius > kMaxMedianRadius) return 5;
     // the refiner's engine outputs are held on the device: (iterations + 1) per pixel
     if (c->refine_enable && c->refine_iterations > kMaxRefineIterations) return 5;
     // each sweep's window holds kRefineSlack rejected engine outputs beyond its draws;
@@ -467,6 +480,17 @@ int apg_hip_sgm_optimize(apg_hip_ctx ctx, const apg_hip_params* p, int w, int h,
     if (int s = check_ctx(ctx, p)) return s;
     const int D = p->max_disparity - p->min_disparity;
     const int npath = p->paths == 8 ? 8 : 4;
+    if (!u16_path_sums(*p)) {
+        // u32 accumulation; d_sum16 (w * h * D u16) holds each path's values
+        if (!d_sum32 || !d_sum16) return fail(ctx, 4, "u32 path sums need d_sum32 and a u16 scratch in d_sum16");
+        const size_t nv = (size_t)w * h * D;
+        HIPCHK(launch_fill_u32(ctx->stream, nv, 0u, d_sum32), "sgm clear");
+        for (int k = 0; k < npath; ++k)
+            HIPCHK(launch_sgm_dir32_ws(ctx->stream, *p, w, h, D, d_gray, d_cost, d_sum32, kDirs[k][0], kDirs[k][1],
+                                       d_sum16),
+                   "sgm");
+        return 0;
+    }
     for (int k = 0; k < npath; ++k)
         HIPCHK(launch_sgm_dir(ctx->stream, *p, w, h, D, d_gray, d_cost, d_sum16, kDirs[k][0], kDirs[k][1], k == 0 ? 0 : 1),
                "sgm");
@@ -807,11 +831,23 @@ int apg_hip_pipeline(apg_hip_ctx ctx, const apg_hip_params* p, const uint8_t* d_
         mark(ctx, S_CONF);
     } else {
         const int npath = p->paths == 8 ? 8 : 4;
-        for (int k = 0; k < npath; ++k)
-            HIPCHK(launch_sgm_dir(st, *p, w, h, D, gl, cost, sum, kDirs[k][0], kDirs[k][1], k == 0 ? 0 : 1), "sgm");
-        if (inter && inter->d_sum32) HIPCHK(launch_widen16(st, nv, sum, inter->d_sum32), "widen");
+        const bool s16 = u16_path_sums(*p);
+        if (s16) {
+            for (int k = 0; k < npath; ++k)
+                HIPCHK(launch_sgm_dir(st, *p, w, h, D, gl, cost, sum, kDirs[k][0], kDirs[k][1], k == 0 ? 0 : 1),
+                       "sgm");
+            if (inter && inter->d_sum32) HIPCHK(launch_widen16(st, nv, sum, inter->d_sum32), "widen");
+        } else {
+            // u32 sums: each path into the free T buffer, added to a zeroed accumulator
+            uint32_t* sum32 = (uint32_t*)sum;
+            HIPCHK(launch_fill_u32(st, nv, 0u, sum32), "sgm clear");
+            for (int k = 0; k < npath; ++k)
+                HIPCHK(launch_sgm_dir32_ws(st, *p, w, h, D, gl, cost, sum32, kDirs[k][0], kDirs[k][1], tmp), "sgm");
+            if (inter && inter->d_sum32)
+                HIPCHK(hipMemcpyAsync(inter->d_sum32, sum32, nv * 4, hipMemcpyDeviceToDevice, st), "copy S");
+        }
         mark(ctx, S_WTA);
-        HIPCHK(launch_wta(st, *p, w, h, sum, 16, dmin, dmax, d_disparity, best, second), "wta");
+        HIPCHK(launch_wta(st, *p, w, h, sum, s16 ? 16 : 32, dmin, dmax, d_disparity, best, second), "wta");
         if (inter && inter->d_disp_wta)
             HIPCHK(hipMemcpyAsync(inter->d_disp_wta, d_disparity, n * 4, hipMemcpyDeviceToDevice, st), "copy wta");
         if (inter && inter->d_disp_right)

@@ -603,6 +603,15 @@ hipError_t launch_widen16(hipStream_t s, size_t n, const uint16_t* in, uint32_t*
     return hipGetLastError();
 }
 
+hipError_t launch_sgm_dir32_ws(hipStream_t s, const apg_hip_params& p, int w, int h, int D, const uint8_t* gray,
+                               const uint16_t* cost, uint32_t* acc, int dx, int dy, uint16_t* L) {
+    const size_t n = (size_t)w * h * D;
+    hipError_t e = launch_sgm_dir(s, p, w, h, D, gray, cost, L, dx, dy, 0);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(add_path32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, L, acc);
+    return hipGetLastError();
+}
+
 hipError_t launch_sgm_dir32(hipStream_t s, const apg_hip_params& p, int w, int h, int D, const uint8_t* gray,
                             const uint16_t* cost, uint32_t* acc, int dx, int dy) {
     // scratch L volume owned by the caller via acc's tail is not available;

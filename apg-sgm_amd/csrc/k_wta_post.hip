@@ -421,7 +421,74 @@ hipError_t launch_fill(hipStream_t s, const apg_hip_params& p, int w, int h, con
     return hipGetLastError();
 }
 
+// Median of any radius: a 16 x 16 tile of pixels with its radius-r halo in
+// LDS (invalid or outside the image as a key of all ones); each pixel selects
+// the (n/2)-th smallest of its n valid window values, which is the value
+// std::nth_element places at n/2.  Valid values are >= 0, so the low 31 bits
+// of a float order like the float (-0 folds onto +0); the selection fixes the
+// key one bit at a time from bit 30 down, counting the window values below
+// the candidate prefix.
+constexpr int kMedTile = 16;
+__global__ __launch_bounds__(kMedTile * kMedTile) void median_select_kernel(int w, int h, int r,
+                                                                             const float* __restrict__ in,
+                                                                             float* __restrict__ out) {
+    extern __shared__ uint32_t s_key[];
+    const int tw = kMedTile + 2 * r;
+    const int x0 = blockIdx.x * kMedTile - r, y0 = blockIdx.y * kMedTile - r;
+    const int tid = threadIdx.y * kMedTile + threadIdx.x;
+    for (int i = tid; i < tw * tw; i += kMedTile * kMedTile) {
+        const int xx = x0 + i % tw, yy = y0 + i / tw;
+        uint32_t k = 0xFFFFFFFFu;
+        if (xx >= 0 && yy >= 0 && xx < w && yy < h) {
+            const float d = in[(size_t)yy * w + xx];
+            if (d >= 0.f) k = __float_as_uint(d) & 0x7FFFFFFFu;
+        }
+        s_key[i] = k;
+    }
+    __syncthreads();
+    const int x = blockIdx.x * kMedTile + threadIdx.x, y = blockIdx.y * kMedTile + threadIdx.y;
+    if (x >= w || y >= h) return;
+    const int side = 2 * r + 1;
+    const uint32_t* win = s_key + threadIdx.y * tw + threadIdx.x;
+    int n = 0;
+    for (int oy = 0; oy < side; ++oy)
+        for (int ox = 0; ox < side; ++ox) n += win[oy * tw + ox] != 0xFFFFFFFFu ? 1 : 0;
+    const size_t q = (size_t)y * w + x;
+    if (n == 0) {
+        out[q] = in[q];
+        return;
+    }
+    int k = n / 2;   // rank among the valid keys
+    uint32_t prefix = 0u;
+    for (int b = 30; b >= 0; --b) {
+        const uint32_t hi_mask = ~((2u << b) - 1u) & 0x7FFFFFFFu;   // bits above b
+        int c = 0;   // valid keys matching the prefix above b with bit b clear
+        for (int oy = 0; oy < side; ++oy)
+            for (int ox = 0; ox < side; ++ox) {
+                const uint32_t v = win[oy * tw + ox];
+                c += (v != 0xFFFFFFFFu && (v & hi_mask) == prefix && !((v >> b) & 1u)) ? 1 : 0;
+            }
+        if (k >= c) {
+            k -= c;
+            prefix |= 1u << b;
+        }
+    }
+    out[q] = __uint_as_float(prefix);
+}
+
 hipError_t launch_median(hipStream_t s, const apg_hip_params& p, int w, int h, const float* disp, float* out) {
+    if (p.post_median_radius > kMaxMedianR) {
+        const int r = p.post_median_radius;
+        const size_t shm = (size_t)(kMedTile + 2 * r) * (kMedTile + 2 * r) * 4;
+        if (shm > 64 * 1024) {
+            hipError_t e = hipFuncSetAttribute((const void*)median_select_kernel,
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+            if (e != hipSuccess) return e;
+        }
+        hipLaunchKernelGGL(median_select_kernel, dim3((w + kMedTile - 1) / kMedTile, (h + kMedTile - 1) / kMedTile),
+                           dim3(kMedTile, kMedTile), shm, s, w, h, r, disp, out);
+        return hipGetLastError();
+    }
     if (p.post_median_radius == 1) {
         hipLaunchKernelGGL(median3_kernel, dim3((w + 255) / 256, h), dim3(256), 0, s, w, h, disp, out);
         return hipGetLastError();

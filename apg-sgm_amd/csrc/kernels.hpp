@@ -76,6 +76,10 @@ hipError_t launch_sgm_dir(hipStream_t s, const apg_hip_params& p, int w, int h, 
                           const uint16_t* cost, uint16_t* sum, int dx, int dy, int mode);
 // One SGM direction added into a u32 accumulator with the reference's
 // invalid rule (used by the per-path class API).
+// One path added into a u32 accumulator, the path values computed into the
+// caller's u16 scratch L (w * h * D); needs cost_max + max(P2, P1 + 1) < 65535.
+hipError_t launch_sgm_dir32_ws(hipStream_t s, const apg_hip_params& p, int w, int h, int D, const uint8_t* gray,
+                               const uint16_t* cost, uint32_t* acc, int dx, int dy, uint16_t* L);
 hipError_t launch_sgm_dir32(hipStream_t s, const apg_hip_params& p, int w, int h, int D, const uint8_t* gray,
                             const uint16_t* cost, uint32_t* acc, int dx, int dy);
 hipError_t launch_widen16(hipStream_t s, size_t n, const uint16_t* in, uint32_t* out);
@@ -89,6 +93,7 @@ hipError_t launch_lr_check(hipStream_t s, const apg_hip_params& p, int w, int h,
 hipError_t launch_fill(hipStream_t s, const apg_hip_params& p, int w, int h, const float* disp, const uint8_t* reason,
                        const uint8_t* gray, const float* prior, float* out);
 // median_radius 0 copies disp to out.
+constexpr int kMaxMedianRadius = 64;
 hipError_t launch_median(hipStream_t s, const apg_hip_params& p, int w, int h, const float* disp, float* out);
 
 // ---- fused schedule ----

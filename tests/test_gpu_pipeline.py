@@ -39,6 +39,13 @@ CASES = [
     (1, 420, 90, 192, 1, {"paths": 8}),
     # a negative AD weight: costs below 0 and above cost_max are clamped
     (2, 150, 90, 64, 1, {"eta_ad": -0.3, "cost_max": 200}),
+    # path sums past the u16 bound: u32 accumulation (4 and 8 paths)
+    (1, 120, 80, 40, 1, {"cost_max": 16383}),
+    (2, 96, 64, 32, 1, {"paths": 8, "cost_max": 9000}),
+    # median windows of radius 2, 5 and 9 (the general selection kernel)
+    (0, 110, 70, 32, 1, {"post_median_radius": 2}),
+    (1, 130, 90, 40, 1, {"post_median_radius": 5}),
+    (2, 75, 50, 24, 0, {"post_median_radius": 9}),
 ]
 
 FUSED_KEYS = ["sgm", "disp_right", "disp_wta", "best", "second", "after_refine",

@@ -33,3 +33,23 @@ def test_cross_arms_match_oracle(ctx, lmax, tau):
         assert np.array_equal(g, want[k]), f"{name} arm differs at {np.argwhere(g != want[k])[:3].tolist()}"
     dg.free()
     darms.free()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("paths,cost_max", [(4, 16383), (8, 9000), (4, 255)])
+def test_sgm_optimize_stage_u16_and_u32_sums(ctx, paths, cost_max):
+    # apg_hip_sgm_optimize on the oracle's aggregated volume: u16 sums widened,
+    # or past the u16 bound u32 accumulation (d_sum16 then holds one path)
+    p = A.params(1, 24, paths=paths, cost_max=cost_max)
+    c = O.cfg_from(p.as_dict())
+    left, right, _ = A.synth_pair(1, 90, 60, 24, 8)
+    o = O.pipeline(c, left, right, intermediates=True)
+    g = O.aux(c, left)[0]
+    h, w, D = o["cost_agg"].shape
+    print(f"sgm_optimize paths={paths} cost_max={cost_max}", flush=True)
+    dg, dc = ctx.to_device(g), ctx.to_device(o["cost_agg"])
+    s16, s32 = ctx.array((h, w, D), np.uint16), ctx.array((h, w, D), np.uint32)
+    ctx._check(ctx.lib.apg_hip_sgm_optimize(ctx.h, C.byref(p), w, h, dg.ptr, dc.ptr, s16.ptr, s32.ptr), "optimize")
+    np.testing.assert_array_equal(s32.download(), o["sgm"])
+    for a in (dg, dc, s16, s32):
+        a.free()

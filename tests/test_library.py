@@ -59,8 +59,10 @@ def test_validate_params_codes():
     assert v(P1=-1) == 4
     assert v(grad_t1=30, grad_t2=10) == 4
     assert v(cost_max=70000) == 4
-    # u16 path sums need npath*(cost_max + P2max) < 65535 (the reference's own u16 criterion)
-    assert v(cost_max=20000) == 5
+    # past the reference's u16 criterion npath*(cost_max + P2max) < 65535 the sums
+    # are u32; one path's values (cost_max + P2max) must still fit u16
+    assert v(cost_max=20000) == 0
+    assert v(cost_max=65415) == 5
     assert v(max_disparity=1024) == 5
     assert v(max_arm_length=100) == 5
     assert v(refine_enable=1, refine_iterations=64) == 0
@@ -143,3 +145,12 @@ def test_reference_header_path_compiles_alone(tmp_path, name):
 
 def test_reference_headers_client_is_built():
     assert os.path.exists(os.path.join(ROOT, "apg-sgm_amd", "_build", "reference_headers_main"))
+
+
+@pytest.mark.parametrize("over,ok", [
+    ({"cost_max": 16383}, 0), ({"paths": 8, "cost_max": 9000}, 0), ({"post_median_radius": 9}, 0),
+    ({"post_median_radius": 64}, 0), ({"post_median_radius": 65}, 5), ({"cost_max": 65500}, 5),
+])
+def test_validate_u32_sums_and_median_radius(over, ok):
+    # u32 path sums past the u16 bound; one path's values must still fit u16
+    assert A.load().apg_hip_validate_params(C.byref(A.params(1, 64, **over))) == ok

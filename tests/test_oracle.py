@@ -268,6 +268,9 @@ def test_sgm_first_pixel_and_restart():
     (64, 48, 16, 2, {}), (96, 64, 24, 2, {"census_type": 0}),
     (120, 80, 32, 2, {"prior_max_supports": 10, "prior_min_supports": 5}),
     (40, 30, 8, 2, {"prior_min_supports": 100000}),
+    # path sums past the u16 bound, wide median windows
+    (36, 24, 12, 1, {"cost_max": 16383}), (30, 22, 10, 1, {"paths": 8, "cost_max": 9000}),
+    (34, 26, 10, 1, {"post_median_radius": 5}), (29, 31, 8, 0, {"post_median_radius": 9}),
 ])
 def test_oracle_matches_reference_build(w, h, D, mode, over):
     """Edge sizes (1x1, thin, D > width, shifted / negative ranges) against
