@@ -174,6 +174,9 @@ struct WsPlan {
     // planes) and WTA kernels over the per-pixel ranges: the HighQuality
     // preset, whose refiner sits between the confidence and the LR check
     bool planes_sgm = false;
+    // planes_sgm with u8 T / C': the general-range forms of the fused Cost and
+    // Cross kernels (right-view keys in the cost pass, no u16 volumes)
+    bool planes_fused = false;
     size_t bytes[B_COUNT] = {};
 };
 
@@ -220,7 +223,13 @@ WsPlan plan_workspace(const apg_hip_params& p, int w, int h, const CostLut& lut,
                nv * (pl.u8 ? 1 : 2) < (1ull << 31);
     pl.planes_sgm = !pl.fused && schedule != 1 && !want_raw && (p.prior_enable || p.refine_enable) && D % 2 == 0 &&
                     D <= 256 && p2max <= 254 && npaths * (cbound + p2max) < 65535 && nv * 2 < (1ull << 31);
-    if (pl.fused) {
+    pl.planes_fused = pl.planes_sgm && pl.u8 && (!p.agg_enable || p.agg_iterations <= 1) && nv < (1ull << 31);
+    if (pl.planes_fused) {
+        pl.bytes[B_COST] = nv;
+        if (p.agg_enable) pl.bytes[B_TMP] = nv;
+        pl.bytes[B_RKEYS] = n * 4;
+        for (int r = 0; r < npaths; ++r) pl.bytes[B_DS0 + r] = nv;
+    } else if (pl.fused) {
         pl.bytes[B_COST] = nv * (pl.u8 ? 1 : 2);
         if (p.agg_enable) pl.bytes[B_TMP] = nv * (pl.u8 ? 1 : 2);
         pl.bytes[B_RKEYS] = n * 4;
@@ -730,7 +739,7 @@ int apg_hip_pipeline(apg_hip_ctx ctx, const apg_hip_params* p, const uint8_t* d_
     const bool sym = p->census_type == 1;
 
     // the fused schedule's right-view WTA keys start cleared, by the aux pass
-    uint32_t* rkeys = fused ? buf<uint32_t>(ctx, B_RKEYS) : nullptr;
+    uint32_t* rkeys = (fused || plan.planes_fused) ? buf<uint32_t>(ctx, B_RKEYS) : nullptr;
     mark(ctx, S_AUX);
     // both views and the dense range init in one launch
     const AuxViews av_l{d_left, gl, gxl, gyl, sym ? (uint32_t*)cl : nullptr, sym ? nullptr : (uint64_t*)cl};
@@ -748,8 +757,12 @@ int apg_hip_pipeline(apg_hip_ctx ctx, const apg_hip_params* p, const uint8_t* d_
         else HIPCHK(launch_fill_u32(st, n, 0xBF800000u, (uint32_t*)inter->d_prior), "fill prior");
     }
 
-    if (fused) {
+    if (fused || plan.planes_fused) {
+        // fused Cost + Cross: dense range init (column ranges) or, for the
+        // HighQuality preset, the prior's per-pixel ranges
+        const bool colr = fused;
         PlaneSet planes{};
+        planes.general_ranges = colr ? 0 : 1;
         for (int r = 0; r < npaths; ++r) planes.p[r] = buf<uint8_t>(ctx, (BufId)(B_DS0 + r));
         void* cp = ctx->bufs[B_COST].ptr;   // C' (u8 or u16)
         void* tt = ctx->bufs[B_TMP].ptr;    // T  (u8 or u16)
@@ -758,7 +771,7 @@ int apg_hip_pipeline(apg_hip_ctx ctx, const apg_hip_params* p, const uint8_t* d_
         HIPCHK(launch_cost_crossh(st, *p, cur_lut(ctx), w, h, gl, gxl, gyl, gr, gxr, gyr,
                                   sym ? (const uint32_t*)cl : nullptr, sym ? (const uint32_t*)cr : nullptr,
                                   sym ? nullptr : (const uint64_t*)cl, sym ? nullptr : (const uint64_t*)cr, dmin,
-                                  dmax, arms, p->agg_enable ? tt : cp, u8, rkeys, true,
+                                  dmax, arms, p->agg_enable ? tt : cp, u8, rkeys, colr,
                                   cost_clamp_free(ctx->lut_host[ctx->lut_cur], p->cost_max), ctx->tune),
                "cost_crossh");
         mark(ctx, S_RWTA);
@@ -768,30 +781,43 @@ int apg_hip_pipeline(apg_hip_ctx ctx, const apg_hip_params* p, const uint8_t* d_
                                     dmax, rkeys, dr),
                "rwta_finalize");
         mark(ctx, S_CROSS);
-        if (p->agg_enable) HIPCHK(launch_cross_v_t(st, *p, w, h, dmin, dmax, arms, tt, cp, u8, true, ctx->tune),
+        if (p->agg_enable) HIPCHK(launch_cross_v_t(st, *p, w, h, dmin, dmax, arms, tt, cp, u8, colr, ctx->tune),
                                     "cross_v");
         mark(ctx, S_SGM);
         HIPCHK(launch_sgm_lines(st, *p, w, h, D, gl, cp, u8, planes, ctx->tune), "sgm_lines");
         mark(ctx, S_WTA);
-        // without the refiner, the disparity after it is the WTA output
-        float* wta_out = inter ? (inter->d_disp_wta ? inter->d_disp_wta : inter->d_after_refine) : nullptr;
-        HIPCHK(launch_wta_fused(st, *p, w, h, cp, u8, planes, dmin, dmax, dr, gxl, gyl, d_disparity, d_confidence, d_reason,
-                                d_reliable, wta_out, inter ? inter->d_best : nullptr,
-                                inter ? inter->d_second : nullptr, inter ? inter->d_sum32 : nullptr),
+        if (fused) {
+            // without the refiner, the disparity after it is the WTA output
+            float* wta_out = inter ? (inter->d_disp_wta ? inter->d_disp_wta : inter->d_after_refine) : nullptr;
+            HIPCHK(launch_wta_fused(st, *p, w, h, cp, u8, planes, dmin, dmax, dr, gxl, gyl, d_disparity, d_confidence,
+                                    d_reason, d_reliable, wta_out, inter ? inter->d_best : nullptr,
+                                    inter ? inter->d_second : nullptr, inter ? inter->d_sum32 : nullptr),
+                   "wta_fused");
+            if (inter && inter->d_after_refine && wta_out != inter->d_after_refine)
+                HIPCHK(hipMemcpyAsync(inter->d_after_refine, wta_out, n * 4, hipMemcpyDeviceToDevice, st),
+                       "copy refined");
+            if (inter && inter->d_disp_right)
+                HIPCHK(hipMemcpyAsync(inter->d_disp_right, dr, n * 4, hipMemcpyDeviceToDevice, st), "copy dr");
+            mark(ctx, S_CONF);
+            mark(ctx, S_REFINE);
+            mark(ctx, S_POST);
+            HIPCHK(launch_fill(st, *p, w, h, d_disparity, d_reason, gl, nullptr, scratch), "fill");
+            HIPCHK(launch_median(st, *p, w, h, scratch, d_disparity), "median");
+            mark(ctx, S_COUNT);
+            return finish_timing(ctx);
+        }
+        // WTA + confidence in one kernel; the LR check follows the refiner
+        apg_hip_params pw = *p;
+        pw.post_lr_check = 0;
+        HIPCHK(launch_wta_fused(st, pw, w, h, cp, u8, planes, dmin, dmax, dr, gxl, gyl, d_disparity, d_confidence,
+                                d_reason, d_reliable, inter ? inter->d_disp_wta : nullptr,
+                                inter ? inter->d_best : nullptr, inter ? inter->d_second : nullptr,
+                                inter ? inter->d_sum32 : nullptr),
                "wta_fused");
-        if (inter && inter->d_after_refine && wta_out != inter->d_after_refine)
-            HIPCHK(hipMemcpyAsync(inter->d_after_refine, wta_out, n * 4, hipMemcpyDeviceToDevice, st), "copy refined");
         if (inter && inter->d_disp_right)
             HIPCHK(hipMemcpyAsync(inter->d_disp_right, dr, n * 4, hipMemcpyDeviceToDevice, st), "copy dr");
         mark(ctx, S_CONF);
-        mark(ctx, S_REFINE);
-        mark(ctx, S_POST);
-        HIPCHK(launch_fill(st, *p, w, h, d_disparity, d_reason, gl, nullptr, scratch), "fill");
-        HIPCHK(launch_median(st, *p, w, h, scratch, d_disparity), "median");
-        mark(ctx, S_COUNT);
-        return finish_timing(ctx);
-    }
-
+    } else {
     mark(ctx, S_COST);
     HIPCHK(launch_cost(st, *p, cur_lut(ctx), w, h, gl, gxl, gyl, gr, gxr, gyr,
                        sym ? (const uint32_t*)cl : nullptr, sym ? (const uint32_t*)cr : nullptr,
@@ -861,6 +887,7 @@ int apg_hip_pipeline(apg_hip_ctx ctx, const apg_hip_params* p, const uint8_t* d_
                                  d_reason),
                "confidence");
     }
+    }   // stage-by-stage Cost, Cross, SGM, WTA, confidence
     mark(ctx, S_REFINE);
     if (p->refine_enable) {
         if (int s = run_refine(ctx, p, w, h, gl, gxl, gyl, gr, gxr, gyr, cl, cr, dmin, dmax, d_reliable, d_disparity))
