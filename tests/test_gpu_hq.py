@@ -31,6 +31,9 @@ CASES = [
     (0, 100, 70, 32, {"refine_random_radius": 1}),
     # ranges reaching below 0
     (2, 128, 96, 32, {"min_disparity": -8}),
+    # heights around one wavefront of rows
+    (1, 90, 64, 24, {}),
+    (0, 90, 65, 24, {}),
 ]
 
 KEYS = ["cost", "cost_agg", "disp_right", "disp_wta", "best", "second", "dmin", "dmax", "d_prior", "after_refine",
@@ -117,3 +120,4 @@ def test_hq_negative_ranges_match_oracle(ctx, shift, maxd, over):
     g = A.run_pipeline(ctx, p, left, right, intermediates=True)
     for k in KEYS:
         _cmp(getattr(g, k), o[k], k)
+
