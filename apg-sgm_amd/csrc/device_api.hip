@@ -744,7 +744,9 @@ int apg_hip_pipeline(apg_hip_ctx ctx, const apg_hip_params* p, const uint8_t* d_
     // both views and the dense range init in one launch
     const AuxViews av_l{d_left, gl, gxl, gyl, sym ? (uint32_t*)cl : nullptr, sym ? nullptr : (uint64_t*)cl};
     const AuxViews av_r{d_right, gr, gxr, gyr, sym ? (uint32_t*)cr : nullptr, sym ? nullptr : (uint64_t*)cr};
-    HIPCHK(launch_aux_pair(st, *p, w, h, stride, channels, av_l, av_r, rkeys, dmin, dmax), "aux");
+    // the left view's Cross arms come from the same gray tiles where the arms are short enough
+    uint32_t* arms_aux = aux_pair_builds_arms(*p) ? arms : nullptr;
+    HIPCHK(launch_aux_pair(st, *p, w, h, stride, channels, av_l, av_r, rkeys, dmin, dmax, arms_aux), "aux");
     mark(ctx, S_PRIOR);
     float* d_prior = p->prior_enable ? buf<float>(ctx, B_PRIOR) : nullptr;
     if (d_prior) {
@@ -766,7 +768,7 @@ int apg_hip_pipeline(apg_hip_ctx ctx, const apg_hip_params* p, const uint8_t* d_
         for (int r = 0; r < npaths; ++r) planes.p[r] = buf<uint8_t>(ctx, (BufId)(B_DS0 + r));
         void* cp = ctx->bufs[B_COST].ptr;   // C' (u8 or u16)
         void* tt = ctx->bufs[B_TMP].ptr;    // T  (u8 or u16)
-        if (p->agg_enable) HIPCHK(launch_cross_arms(st, *p, gl, w, h, arms), "arms");
+        if (p->agg_enable && !arms_aux) HIPCHK(launch_cross_arms(st, *p, gl, w, h, arms), "arms");
         mark(ctx, S_COST);
         HIPCHK(launch_cost_crossh(st, *p, cur_lut(ctx), w, h, gl, gxl, gyl, gr, gxr, gyr,
                                   sym ? (const uint32_t*)cl : nullptr, sym ? (const uint32_t*)cr : nullptr,
@@ -828,7 +830,7 @@ int apg_hip_pipeline(apg_hip_ctx ctx, const apg_hip_params* p, const uint8_t* d_
     HIPCHK(launch_right_wta(st, *p, w, h, cost, dr), "right_wta");
     mark(ctx, S_CROSS);
     if (p->agg_enable) {
-        HIPCHK(launch_cross_arms(st, *p, gl, w, h, arms), "arms");
+        if (!arms_aux) HIPCHK(launch_cross_arms(st, *p, gl, w, h, arms), "arms");
         const int iters = p->agg_iterations > 1 ? p->agg_iterations : 1;
         for (int i = 0; i < iters; ++i) {
             HIPCHK(launch_cross_h(st, *p, w, h, dmin, dmax, arms, cost, tmp), "cross_h");

@@ -81,10 +81,235 @@ struct AuxPair {
 __device__ __forceinline__ void aux_pixel(const AuxImage& im, int w, int h, int stride, int ch, int census_type,
                                           int gmin, int gmax, uint8_t (&tile)[AUX_TY + 6][AUX_TX + 8]);
 
-__global__ __launch_bounds__(256) void aux_pair_kernel(AuxPair ap, int w, int h, int stride, int ch, int census_type,
-                                                       int gmin, int gmax) {
-    __shared__ uint8_t tile[AUX_TY + 6][AUX_TX + 8];
-    aux_pixel(ap.img[blockIdx.z], w, h, stride, ch, census_type, gmin, gmax, tile);
+// ------------------------------------------------------- aux, pair form ----
+// Both views in one launch (blockIdx.z = view), four horizontally adjacent
+// pixels per thread, one u32 of the gray image per tile word.  A block covers
+// 64 columns x 16 rows; its LDS tile holds the clamp-to-edge gray values of
+// 16 + 2*RH rows (RH = 3 for the census, max(3, Lmax) where the left view also
+// builds the Cross arms) and 104 columns (20 on each side: census 4, arms
+// Lmax <= 17 rounded up to whole words).  Tile rows are 48 words apart, so the
+// two or four rows that one LDS access of a wave touches fall in disjoint banks.
+//
+// Byte-parallel comparison: for bytes a, b of two words,
+//   t = (b | 0x80) - (a & 0x7f) has top bit [b & 0x7f >= a & 0x7f] (no borrow
+//   leaves a byte), and "b >= a" = top bit of ((a ^ b) & b) | (~(a ^ b) & t).
+// Each census comparison yields that bit for the thread's four pixels at bit 7
+// of each byte; acc = (acc >> 1) with the top bits replaced collects eight
+// comparisons per byte (first at bit 0), and a 4x4 byte transpose makes the
+// per-pixel census words.
+constexpr int AX_PX = 64, AX_ROWS = 16, AX_HALO = 20, AX_TW = (AX_PX + 2 * AX_HALO) / 4, AX_S = 48;
+constexpr int AX_LMAX = 17;   // longest arm the pair form builds
+
+__device__ __forceinline__ uint32_t bsel(uint32_t m, uint32_t a, uint32_t b) { return (m & a) | (~m & b); }
+// top bit of each byte: b >= a (unsigned bytes)
+__device__ __forceinline__ uint32_t byte_ge(uint32_t b, uint32_t a) {
+    const uint32_t t = (b | 0x80808080u) - (a & 0x7F7F7F7Fu);
+    return bsel(a ^ b, b, t);
+}
+// per-byte |a - b|
+__device__ __forceinline__ uint32_t byte_absdiff(uint32_t a, uint32_t b) {
+    const uint32_t one = (~byte_ge(b, a) >> 7) & 0x01010101u;   // 1 where a > b
+    const uint32_t m = (one << 8) - one;                         // 0xff where a > b
+    return bsel(m, a, b) - bsel(m, b, a);
+}
+// bytes j of g0..g3 -> word j (byte k from g_k)
+__device__ __forceinline__ void byte_transpose(const uint32_t (&g)[4], uint32_t (&c)[4]) {
+    const uint32_t x0 = __builtin_amdgcn_perm(g[1], g[0], 0x05010400u), x1 = __builtin_amdgcn_perm(g[1], g[0], 0x07030602u);
+    const uint32_t y0 = __builtin_amdgcn_perm(g[3], g[2], 0x05010400u), y1 = __builtin_amdgcn_perm(g[3], g[2], 0x07030602u);
+    c[0] = __builtin_amdgcn_perm(y0, x0, 0x05040100u);
+    c[1] = __builtin_amdgcn_perm(y0, x0, 0x07060302u);
+    c[2] = __builtin_amdgcn_perm(y1, x1, 0x05040100u);
+    c[3] = __builtin_amdgcn_perm(y1, x1, 0x07060302u);
+}
+
+template <int CT>
+__global__ __launch_bounds__(256) void aux4_pair_kernel(AuxPair ap, int w, int h, int stride, int ch, int gmin,
+                                                        int gmax, uint32_t* __restrict__ arms, int Lmax, int tau) {
+    __shared__ uint32_t tile[(AX_ROWS + 2 * AX_LMAX) * AX_S];
+    const int view = blockIdx.z;
+    const AuxImage im = ap.img[view];
+    const bool do_arms = view == 0 && arms != nullptr;
+    const int rh = do_arms ? max(3, Lmax) : 3;
+    const int th = AX_ROWS + 2 * rh;
+    const int x0 = blockIdx.x * AX_PX, y0 = blockIdx.y * AX_ROWS;
+    const int tid = threadIdx.x;
+    for (int i = tid; i < th * AX_TW; i += 256) {
+        const int r = i / AX_TW, c = i - r * AX_TW;
+        const int sy = clampi(y0 - rh + r, 0, h - 1);
+        const int xa = x0 - AX_HALO + 4 * c;
+        const uint8_t* rowp = im.src + (size_t)sy * stride;
+        uint32_t v = 0u;
+        if (ch == 1 && xa >= 0 && xa + 3 < w && (((uintptr_t)(rowp + xa)) & 3u) == 0) {
+            v = *(const uint32_t*)(rowp + xa);
+        } else {
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const uint8_t* px = rowp + (size_t)clampi(xa + b, 0, w - 1) * ch;
+                const uint32_t g = ch == 1 ? px[0] : ((77u * px[0] + 150u * px[1] + 29u * px[2]) >> 8);
+                v |= g << (8 * b);
+            }
+        }
+        tile[r * AX_S + c] = v;
+    }
+    __syncthreads();
+    const int tx = tid & 15, ty = tid >> 4;
+    const int x = x0 + 4 * tx, y = y0 + ty;
+    if (x >= w || y >= h) return;
+    const uint32_t* tc = tile + (rh + ty) * AX_S + AX_HALO / 4 + tx;   // the thread's word of its row
+    // rows -3..3, words -1..1 around the thread's word
+    uint32_t W[7][3];
+#pragma unroll
+    for (int r = 0; r < 7; ++r)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) W[r][k] = tc[(r - 3) * AX_S + k - 1];
+    // the four bytes at columns x + dx .. x + dx + 3 of row y + dy
+    auto win = [&](int dy, int dx) -> uint32_t {
+        const uint32_t* r = W[dy + 3];
+        if (dx == -4) return r[0];
+        if (dx == 0) return r[1];
+        if (dx == 4) return r[2];
+        return dx < 0 ? __builtin_amdgcn_alignbyte(r[1], r[0], (uint32_t)(4 + dx))
+                      : __builtin_amdgcn_alignbyte(r[2], r[1], (uint32_t)dx);
+    };
+    const uint32_t center = win(0, 0);
+    const uint32_t gxw = byte_absdiff(win(0, 1), win(0, -1));
+    const uint32_t gyw = byte_absdiff(win(1, 0), win(-1, 0));
+    const bool wide = (w & 3) == 0 && x + 3 < w;   // four pixels, word-aligned rows
+    const int nq = min(4, w - x);
+    const size_t pi = (size_t)y * w + x;
+    if (wide) {
+        *(uint32_t*)(im.gray + pi) = center;
+        *(uint32_t*)(im.gx + pi) = gxw;
+        *(uint32_t*)(im.gy + pi) = gyw;
+    } else {
+        for (int j = 0; j < nq; ++j) {
+            im.gray[pi + j] = (uint8_t)(center >> (8 * j));
+            im.gx[pi + j] = (uint8_t)(gxw >> (8 * j));
+            im.gy[pi + j] = (uint8_t)(gyw >> (8 * j));
+        }
+    }
+    if (im.keys) {
+        if (wide) *(uint4*)(im.keys + pi) = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu);
+        else for (int j = 0; j < nq; ++j) im.keys[pi + j] = 0xFFFFFFFFu;
+    }
+    if (im.dmin) {   // as range_kernel
+        int16_t lo[4], hi[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            lo[j] = (int16_t)gmin;
+            hi[j] = (int16_t)max(min(gmax, x + j + 1), gmin);
+        }
+        if (wide) {
+            *(uint2*)(im.dmin + pi) = make_uint2((uint16_t)lo[0] | ((uint32_t)(uint16_t)lo[1] << 16),
+                                                 (uint16_t)lo[2] | ((uint32_t)(uint16_t)lo[3] << 16));
+            *(uint2*)(im.dmax + pi) = make_uint2((uint16_t)hi[0] | ((uint32_t)(uint16_t)hi[1] << 16),
+                                                 (uint16_t)hi[2] | ((uint32_t)(uint16_t)hi[3] << 16));
+        } else {
+            for (int j = 0; j < nq; ++j) {
+                im.dmin[pi + j] = lo[j];
+                im.dmax[pi + j] = hi[j];
+            }
+        }
+    }
+    // census: "b >= a" bits are collected and inverted at the end (a > b)
+    if constexpr (CT == 1) {
+        // symmetric pairs (dy, dx) > (-dy, -dx) over the first half of the 9x7 window
+        uint32_t g[4], acc = 0u;
+        int b = 0;
+#pragma unroll
+        for (int dy = -3; dy <= 0; ++dy)
+#pragma unroll
+            for (int dx = -4; dx <= 4; ++dx) {
+                if (dy == 0 && dx >= 0) continue;
+                acc = bsel(0x80808080u, byte_ge(win(-dy, -dx), win(dy, dx)), acc >> 1);
+                ++b;
+                if ((b & 7) == 0) g[(b >> 3) - 1] = acc;
+            }
+        g[3] = (acc >> 1) | 0x80808080u;   // 31 comparisons: bit 31 of every census word is 0
+        uint32_t c[4];
+        byte_transpose(g, c);
+        const uint4 cw = make_uint4(~c[0], ~c[1], ~c[2], ~c[3]);
+        if (wide) *(uint4*)(im.c32 + pi) = cw;
+        else {
+            const uint32_t cv[4] = {cw.x, cw.y, cw.z, cw.w};
+            for (int j = 0; j < nq; ++j) im.c32[pi + j] = cv[j];
+        }
+    } else {
+        // every neighbour of the 9x7 window against the centre
+        uint32_t g[8], acc = 0u;
+        int b = 0;
+#pragma unroll
+        for (int dy = -3; dy <= 3; ++dy)
+#pragma unroll
+            for (int dx = -4; dx <= 4; ++dx) {
+                if (dy == 0 && dx == 0) continue;
+                acc = bsel(0x80808080u, byte_ge(center, win(dy, dx)), acc >> 1);
+                ++b;
+                if ((b & 7) == 0) g[(b >> 3) - 1] = acc;
+            }
+        g[7] = (acc >> 2) | 0xC0C0C0C0u;   // 62 comparisons: bits 62, 63 are 0
+        uint32_t lo[4], hi[4];
+        byte_transpose({g[0], g[1], g[2], g[3]}, lo);
+        byte_transpose({g[4], g[5], g[6], g[7]}, hi);
+        uint64_t cv[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) cv[j] = (uint64_t)~lo[j] | ((uint64_t)~hi[j] << 32);
+        if (wide) {
+            *(ulonglong2*)(im.c64 + pi) = make_ulonglong2(cv[0], cv[1]);
+            *(ulonglong2*)(im.c64 + pi + 2) = make_ulonglong2(cv[2], cv[3]);
+        } else {
+            for (int j = 0; j < nq; ++j) im.c64[pi + j] = cv[j];
+        }
+    }
+    if (!do_arms) return;
+    // CostAggregator::build_cross_arms (reference src/cost_aggregator.cpp:17-63):
+    // an arm grows while the next neighbour differs from the centre by at most
+    // tau, up to Lmax.  Neighbours past the border repeat the edge pixel in the
+    // tile; capping each arm at the distance to the border afterwards gives the
+    // reference's arm, which stops there.  |a - c| <= tau <=> a - (c - tau) <= 2 tau
+    // in unsigned arithmetic.
+    const uint32_t t2 = 2u * (uint32_t)tau;
+    uint32_t ct[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) ct[j] = ((center >> (8 * j)) & 0xFFu) - (uint32_t)tau;
+    uint32_t H[11];   // words -5..5 of the thread's row
+#pragma unroll
+    for (int k = 0; k < 11; ++k) H[k] = tc[k - 5];
+    int al[4], ar[4], au[4], ad[4];
+    bool gl[4], gr[4], gu[4], gd[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        al[j] = ar[j] = au[j] = ad[j] = 0;
+        gl[j] = gr[j] = gu[j] = gd[j] = true;
+    }
+    auto okb = [&](uint32_t word, int byte, int j) { return ((word >> (8 * byte)) & 0xFFu) - ct[j] <= t2; };
+#pragma unroll
+    for (int k = 1; k <= AX_LMAX; ++k) {
+        const bool in = k <= Lmax;
+        const int kr = in ? k : Lmax;   // rows past Lmax are not in the tile; their tests are masked
+        const uint32_t up = tc[-kr * AX_S], dn = tc[kr * AX_S];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int ql = j - k + 20, qr = j + k + 20;   // byte offsets from word -5
+            gl[j] = gl[j] & in & okb(H[ql >> 2], ql & 3, j);
+            gr[j] = gr[j] & in & okb(H[qr >> 2], qr & 3, j);
+            gu[j] = gu[j] & in & okb(up, j, j);
+            gd[j] = gd[j] & in & okb(dn, j, j);
+            al[j] += gl[j];
+            ar[j] += gr[j];
+            au[j] += gu[j];
+            ad[j] += gd[j];
+        }
+    }
+    uint32_t av[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int xj = x + j;
+        av[j] = (uint32_t)min(al[j], xj) | ((uint32_t)min(ar[j], w - 1 - xj) << 8) |
+                ((uint32_t)min(au[j], y) << 16) | ((uint32_t)min(ad[j], h - 1 - y) << 24);
+    }
+    if (wide) *(uint4*)(arms + pi) = make_uint4(av[0], av[1], av[2], av[3]);
+    else for (int j = 0; j < nq; ++j) arms[pi + j] = av[j];
 }
 
 __global__ __launch_bounds__(256) void aux_kernel(const uint8_t* __restrict__ src, int w, int h, int stride, int ch,
@@ -161,14 +386,25 @@ __device__ __forceinline__ void aux_pixel(const AuxImage& im, int w, int h, int 
 }
 
 hipError_t launch_aux_pair(hipStream_t s, const apg_hip_params& p, int w, int h, int stride, int ch,
-                           const AuxViews& L, const AuxViews& R, uint32_t* rkeys, int16_t* dmin, int16_t* dmax) {
+                           const AuxViews& L, const AuxViews& R, uint32_t* rkeys, int16_t* dmin, int16_t* dmax,
+                           uint32_t* arms) {
     AuxPair ap;
     ap.img[0] = AuxImage{L.src, L.gray, L.gx, L.gy, L.c32, L.c64, nullptr, dmin, dmax};
     ap.img[1] = AuxImage{R.src, R.gray, R.gx, R.gy, R.c32, R.c64, rkeys, nullptr, nullptr};
-    dim3 grid((w + AUX_TX - 1) / AUX_TX, (h + AUX_TY - 1) / AUX_TY, 2), block(AUX_TX, AUX_TY);
-    hipLaunchKernelGGL(aux_pair_kernel, grid, block, 0, s, ap, w, h, stride, ch, p.census_type, p.min_disparity,
-                       p.max_disparity);
+    const int Lmax = p.max_arm_length > 1 ? p.max_arm_length : 1;
+    if (arms && Lmax > AX_LMAX) return hipErrorInvalidValue;
+    dim3 grid((w + AX_PX - 1) / AX_PX, (h + AX_ROWS - 1) / AX_ROWS, 2), block(256);
+    if (p.census_type == 1)
+        hipLaunchKernelGGL(aux4_pair_kernel<1>, grid, block, 0, s, ap, w, h, stride, ch, p.min_disparity,
+                           p.max_disparity, arms, Lmax, p.color_threshold);
+    else
+        hipLaunchKernelGGL(aux4_pair_kernel<0>, grid, block, 0, s, ap, w, h, stride, ch, p.min_disparity,
+                           p.max_disparity, arms, Lmax, p.color_threshold);
     return hipGetLastError();
+}
+
+bool aux_pair_builds_arms(const apg_hip_params& p) {
+    return p.agg_enable && (p.max_arm_length > 1 ? p.max_arm_length : 1) <= AX_LMAX;
 }
 
 hipError_t launch_aux(hipStream_t s, const apg_hip_params& p, const uint8_t* src, int w, int h, int stride,

@@ -54,10 +54,13 @@ struct AuxViews {
     uint32_t* c32;
     uint64_t* c64;
 };
-// both views in one launch; also the dense range init (left) and the cleared
-// right-WTA keys (rkeys may be null)
+// both views in one launch; also the dense range init (left), the cleared
+// right-WTA keys (rkeys may be null) and, where arms is not null, the left
+// view's Cross arms (only where aux_pair_builds_arms(p))
 hipError_t launch_aux_pair(hipStream_t s, const apg_hip_params& p, int w, int h, int stride, int ch,
-                           const AuxViews& L, const AuxViews& R, uint32_t* rkeys, int16_t* dmin, int16_t* dmax);
+                           const AuxViews& L, const AuxViews& R, uint32_t* rkeys, int16_t* dmin, int16_t* dmax,
+                           uint32_t* arms);
+bool aux_pair_builds_arms(const apg_hip_params& p);
 hipError_t launch_range_init(hipStream_t s, const apg_hip_params& p, int w, int h, int16_t* dmin, int16_t* dmax);
 hipError_t launch_cost(hipStream_t s, const apg_hip_params& p, const CostLut* d_lut, int w, int h,
                        const uint8_t* gl, const uint8_t* gxl, const uint8_t* gyl,

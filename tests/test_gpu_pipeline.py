@@ -162,6 +162,53 @@ def test_pipeline_fused_wta_outputs(ctx, kind, w, h, D, mode, over):
     _check(g2, o, WTA_KEYS[5:])
 
 
+# the aux pass (four pixels per thread over a 64 x 16 tile, the left view's
+# Cross arms from the same tile up to arm length 17, the separate arms kernel
+# beyond): widths around the tile and word edges, heights around the row
+# tiles, arm lengths 1..30, thresholds 0 and 255, the 64-bit census
+AUX_CASES = [
+    (0, 63, 15, 16, 1, {"max_arm_length": 1}),
+    (1, 65, 17, 16, 1, {"max_arm_length": 2, "color_threshold": 0}),
+    (2, 66, 33, 16, 1, {"max_arm_length": 17, "color_threshold": 40}),
+    (0, 67, 16, 16, 1, {"max_arm_length": 18}),
+    (1, 130, 47, 24, 1, {"max_arm_length": 4, "color_threshold": 255}),
+    (2, 129, 50, 16, 1, {"census_type": 0, "max_arm_length": 16}),
+    (0, 4, 3, 4, 1, {"max_arm_length": 17}),
+    (1, 200, 31, 32, 1, {"max_arm_length": 30, "color_threshold": 12}),
+    (2, 6, 40, 4, 1, {"max_arm_length": 9}),
+]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("sched", [1, 0], ids=["stage_by_stage", "fused"])
+@pytest.mark.parametrize("kind,w,h,D,mode,over", AUX_CASES)
+def test_pipeline_aux_pair_edges(ctx, kind, w, h, D, mode, over, sched):
+    print(f"aux pair sched={sched} kind={kind} {w}x{h} D={D} {over}", flush=True)
+    p, left, right, o = _case(kind, w, h, D, mode, over)
+    A.set_schedule(ctx, sched)
+    try:
+        g = A.run_pipeline(ctx, p, left, right, intermediates=True if sched == 1 else A.FUSED_INTERMEDIATES)
+    finally:
+        A.set_schedule(ctx, 0)
+    _check(g, o, PLAIN_KEYS if sched == 1 else FUSED_KEYS)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("h,w", [(19, 67), (33, 130)])
+def test_pipeline_color_input_odd_widths(ctx, h, w):
+    rng = np.random.default_rng(w)
+    left = rng.integers(0, 256, (h, w, 3), dtype=np.uint8)
+    right = np.roll(left, -5, axis=1)
+    p = A.params(1, 16, census_type=0 if w > 100 else 1)
+    o = O.pipeline(O.cfg_from(p.as_dict()), left, right, intermediates=True)
+    for sched in (1, 0):
+        print(f"color {w}x{h} sched={sched}", flush=True)
+        A.set_schedule(ctx, sched)
+        g = A.run_pipeline(ctx, p, left, right, intermediates=True if sched == 1 else A.FUSED_INTERMEDIATES)
+        _check(g, o, PLAIN_KEYS if sched == 1 else FUSED_KEYS)
+    A.set_schedule(ctx, 0)
+
+
 @pytest.mark.gpu
 def test_pipeline_color_input(ctx):
     rng = np.random.default_rng(5)
