@@ -177,11 +177,14 @@ struct WsPlan {
     // planes_sgm with u8 T / C': the general-range forms of the fused Cost and
     // Cross kernels (right-view keys in the cost pass, no u16 volumes)
     bool planes_fused = false;
+    // the SGM keeps one plane per scanline family (the sum of its two
+    // directions' deltas) instead of one per direction
+    bool paired = false;
     size_t bytes[B_COUNT] = {};
 };
 
 WsPlan plan_workspace(const apg_hip_params& p, int w, int h, const CostLut& lut, int schedule,
-                      const apg_hip_intermediates* inter) {
+                      const apg_hip_intermediates* inter, const KTune& tn = KTune()) {
     WsPlan pl;
     const size_t n = (size_t)w * h;
     const int D = p.max_disparity - p.min_disparity;
@@ -224,16 +227,18 @@ WsPlan plan_workspace(const apg_hip_params& p, int w, int h, const CostLut& lut,
     pl.planes_sgm = !pl.fused && schedule != 1 && !want_raw && (p.prior_enable || p.refine_enable) && D % 2 == 0 &&
                     D <= 256 && p2max <= 254 && npaths * (cbound + p2max) < 65535 && nv * 2 < (1ull << 31);
     pl.planes_fused = pl.planes_sgm && pl.u8 && (!p.agg_enable || p.agg_iterations <= 1) && nv < (1ull << 31);
+    pl.paired = (pl.fused || pl.planes_fused) && sgm_pairs_planes(p, D, pl.u8, tn);
+    const int nplanes = pl.paired ? npaths / 2 : npaths;
     if (pl.planes_fused) {
         pl.bytes[B_COST] = nv;
         if (p.agg_enable) pl.bytes[B_TMP] = nv;
         pl.bytes[B_RKEYS] = n * 4;
-        for (int r = 0; r < npaths; ++r) pl.bytes[B_DS0 + r] = nv;
+        for (int r = 0; r < nplanes; ++r) pl.bytes[B_DS0 + r] = nv + (pl.paired ? kPlaneSpare : 0);
     } else if (pl.fused) {
         pl.bytes[B_COST] = nv * (pl.u8 ? 1 : 2);
         if (p.agg_enable) pl.bytes[B_TMP] = nv * (pl.u8 ? 1 : 2);
         pl.bytes[B_RKEYS] = n * 4;
-        for (int r = 0; r < npaths; ++r) pl.bytes[B_DS0 + r] = nv;
+        for (int r = 0; r < nplanes; ++r) pl.bytes[B_DS0 + r] = nv + (pl.paired ? kPlaneSpare : 0);
     } else {
         pl.bytes[B_COST] = pl.bytes[B_TMP] = nv * 2;
         if (pl.planes_sgm)
@@ -723,7 +728,7 @@ int apg_hip_pipeline(apg_hip_ctx ctx, const apg_hip_params* p, const uint8_t* d_
     const size_t n = (size_t)w * h;
     const int D = p->max_disparity - p->min_disparity;
     const size_t nv = n * (size_t)D;
-    const WsPlan plan = plan_workspace(*p, w, h, ctx->lut_host[ctx->lut_cur], ctx->schedule, inter);
+    const WsPlan plan = plan_workspace(*p, w, h, ctx->lut_host[ctx->lut_cur], ctx->schedule, inter, ctx->tune);
     for (int b = 0; b < B_COUNT; ++b)
         if (b != B_LUT && plan.bytes[b]) HIPCHK(ensure(ctx, (BufId)b, plan.bytes[b]), "alloc");
     const bool fused = plan.fused, u8 = plan.u8;
@@ -765,7 +770,8 @@ int apg_hip_pipeline(apg_hip_ctx ctx, const apg_hip_params* p, const uint8_t* d_
         const bool colr = fused;
         PlaneSet planes{};
         planes.general_ranges = colr ? 0 : 1;
-        for (int r = 0; r < npaths; ++r) planes.p[r] = buf<uint8_t>(ctx, (BufId)(B_DS0 + r));
+        planes.paired = plan.paired ? 1 : 0;
+        for (int r = 0; r < (plan.paired ? npaths / 2 : npaths); ++r) planes.p[r] = buf<uint8_t>(ctx, (BufId)(B_DS0 + r));
         void* cp = ctx->bufs[B_COST].ptr;   // C' (u8 or u16)
         void* tt = ctx->bufs[B_TMP].ptr;    // T  (u8 or u16)
         if (p->agg_enable && !arms_aux) HIPCHK(launch_cross_arms(st, *p, gl, w, h, arms), "arms");
@@ -791,7 +797,10 @@ int apg_hip_pipeline(apg_hip_ctx ctx, const apg_hip_params* p, const uint8_t* d_
         if (fused) {
             // without the refiner, the disparity after it is the WTA output
             float* wta_out = inter ? (inter->d_disp_wta ? inter->d_disp_wta : inter->d_after_refine) : nullptr;
-            HIPCHK(launch_wta_fused(st, *p, w, h, cp, u8, planes, dmin, dmax, dr, gxl, gyl, d_disparity, d_confidence,
+            // fill + median in one pass reads the WTA's disparity from scratch
+            const bool fm = fill_median_fusable(*p);
+            float* wta_disp = fm ? scratch : d_disparity;
+            HIPCHK(launch_wta_fused(st, *p, w, h, cp, u8, planes, dmin, dmax, dr, gxl, gyl, wta_disp, d_confidence,
                                     d_reason, d_reliable, wta_out, inter ? inter->d_best : nullptr,
                                     inter ? inter->d_second : nullptr, inter ? inter->d_sum32 : nullptr),
                    "wta_fused");
@@ -803,8 +812,12 @@ int apg_hip_pipeline(apg_hip_ctx ctx, const apg_hip_params* p, const uint8_t* d_
             mark(ctx, S_CONF);
             mark(ctx, S_REFINE);
             mark(ctx, S_POST);
-            HIPCHK(launch_fill(st, *p, w, h, d_disparity, d_reason, gl, nullptr, scratch), "fill");
-            HIPCHK(launch_median(st, *p, w, h, scratch, d_disparity), "median");
+            if (fm) {
+                HIPCHK(launch_fill_median(st, *p, w, h, scratch, d_reason, gl, nullptr, d_disparity), "fill_median");
+            } else {
+                HIPCHK(launch_fill(st, *p, w, h, d_disparity, d_reason, gl, nullptr, scratch), "fill");
+                HIPCHK(launch_median(st, *p, w, h, scratch, d_disparity), "median");
+            }
             mark(ctx, S_COUNT);
             return finish_timing(ctx);
         }
@@ -956,7 +969,7 @@ size_t apgk_ctx_growth_bytes(apg_hip_ctx ctx, const apg_hip_params* p, int w, in
     if (!ctx || !p || w <= 0 || h <= 0 || apg_hip_validate_params(p)) return 0;
     CostLut lut;
     build_cost_lut(*p, &lut);
-    const WsPlan pl = plan_workspace(*p, w, h, lut, ctx->schedule, nullptr);
+    const WsPlan pl = plan_workspace(*p, w, h, lut, ctx->schedule, nullptr, ctx->tune);
     size_t g = 0;
     for (int b = 0; b < B_COUNT; ++b) {
         const size_t want = pl.bytes[b] ? (pl.bytes[b] < 256 ? 256 : pl.bytes[b]) : 0;
@@ -1153,7 +1166,7 @@ int apg_hip_pipeline_static(const apg_hip_params* p, const uint8_t* d_left, cons
     ctx->capturing = cs != hipStreamCaptureStatusNone;
     ctx->bufs[B_LUT] = DevBuf{sd.ptr, kLutSlots * sizeof(CostLut)};
     if (int s = ensure_lut(ctx, p)) return static_fail(s, ctx->last_error);
-    const WsPlan plan = plan_workspace(*p, w, h, ctx->lut_host[ctx->lut_cur], 0, nullptr);
+    const WsPlan plan = plan_workspace(*p, w, h, ctx->lut_host[ctx->lut_cur], 0, nullptr, ctx->tune);
     if (!plan.fused)
         return static_fail(5, "the static entry point runs the fused schedule only (no prior or refiner, even D <= 256, "
                               "one Cross iteration)");

@@ -97,6 +97,11 @@ typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ u32x4 buf_ld128(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
     return __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, (int)soff, 0);
 }
+// the same at device scope (sc1): served by L2, never by a CU's L1, for data
+// that this kernel wrote earlier
+__device__ __forceinline__ u32x4 buf_ld128_dev(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    return __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, (int)soff, 16);
+}
 
 // u8 pair (bytes 0, 1 of `raw`) -> u16 halves
 __device__ __forceinline__ uint32_t bytes_to_halves(uint32_t raw) { return __builtin_amdgcn_perm(0u, raw, 0x0c010c00u); }

@@ -133,23 +133,39 @@ __global__ __launch_bounds__(256) void aux4_pair_kernel(AuxPair ap, int w, int h
     const int th = AX_ROWS + 2 * rh;
     const int x0 = blockIdx.x * AX_PX, y0 = blockIdx.y * AX_ROWS;
     const int tid = threadIdx.x;
-    for (int i = tid; i < th * AX_TW; i += 256) {
+    // every word's load is issued before any is stored (a word wholly inside
+    // the image of a 4-byte aligned gray row is one load; the edge words and
+    // colour images are gathered byte by byte)
+    constexpr int NT = ((AX_ROWS + 2 * AX_LMAX) * AX_TW + 255) / 256;
+    const bool aligned = ch == 1 && (((uintptr_t)im.src) & 3u) == 0 && (stride & 3) == 0;
+    uint32_t tv[NT];
+#pragma unroll
+    for (int k = 0; k < NT; ++k) {
+        const int i = min(tid + 256 * k, th * AX_TW - 1);
         const int r = i / AX_TW, c = i - r * AX_TW;
         const int sy = clampi(y0 - rh + r, 0, h - 1);
         const int xa = x0 - AX_HALO + 4 * c;
         const uint8_t* rowp = im.src + (size_t)sy * stride;
-        uint32_t v = 0u;
-        if (ch == 1 && xa >= 0 && xa + 3 < w && (((uintptr_t)(rowp + xa)) & 3u) == 0) {
-            v = *(const uint32_t*)(rowp + xa);
+        if (aligned && xa >= 0 && xa + 3 < w) {
+            tv[k] = *(const uint32_t*)(rowp + xa);
         } else {
+            uint32_t v = 0u;
 #pragma unroll
             for (int b = 0; b < 4; ++b) {
                 const uint8_t* px = rowp + (size_t)clampi(xa + b, 0, w - 1) * ch;
                 const uint32_t g = ch == 1 ? px[0] : ((77u * px[0] + 150u * px[1] + 29u * px[2]) >> 8);
                 v |= g << (8 * b);
             }
+            tv[k] = v;
         }
-        tile[r * AX_S + c] = v;
+    }
+#pragma unroll
+    for (int k = 0; k < NT; ++k) {
+        const int i = tid + 256 * k;
+        if (i < th * AX_TW) {
+            const int r = i / AX_TW, c = i - r * AX_TW;
+            tile[r * AX_S + c] = tv[k];
+        }
     }
     __syncthreads();
     const int tx = tid & 15, ty = tid >> 4;

@@ -274,52 +274,94 @@ __global__ __launch_bounds__(64) void cost_crossh2_kernel(
     const int nown = min(seg, w - x0);
     const size_t row = (size_t)y * w;
 
-    for (int i = lane; i < 65; i += 64) s_lc[i] = lut->census[i];
-    for (int i = lane; i < 256; i += 64) s_la[i] = lut->ad[i];
-    for (int i = lane; i < 511; i += 64) s_lg[i] = lut->grad[i];
-    for (int n = lane; n < 2 * kMaxArm + 2; n += 64) s_mag[n] = n ? (0x80000000u + (uint32_t)n - 1u) / (uint32_t)n : 0u;
-    for (int c = lane; c <= ncol + 1; c += 64) {
-        const int x = xb + c;
-        uint4 r = make_uint4(0u, 0u, 0u, (uint32_t)kFarLeft);
-        uint32_t hi = 0u;
-        if (c < ncol && x >= 0 && x < w) {
-            const size_t q = row + x;
-            if (SYM) r.x = c32l[q];
-            else { const uint64_t v = c64l[q]; r.x = (uint32_t)v; hi = (uint32_t)(v >> 32); }
-            r.y = rep4(gl[q]);
-            r.z = grad4(gxl[q], gyl[q]);
-            r.w = (uint32_t)x;
-        }
-        s_col[c] = r;
-        if (!SYM) s_chi[c] = hi;
+    // The staging below issues all of a batch's global loads before the first
+    // of them is used, so a wave waits for memory once per batch rather than
+    // once per loop trip.  The LUTs are contiguous floats in CostLut and in LDS.
+    {
+        constexpr int NLUT = 65 + 256 + 511, NL = (NLUT + 63) / 64;
+        float lv[NL];
+#pragma unroll
+        for (int k = 0; k < NL; ++k) lv[k] = lut->census[min(lane + 64 * k, NLUT - 1)];
+#pragma unroll
+        for (int k = 0; k < NL; ++k)
+            if (lane + 64 * k < NLUT) s_lc[lane + 64 * k] = lv[k];
     }
-    for (int o = lane; o <= seg; o += 64) {
-        const int x = x0 + o;
-        uint4 r = make_uint4(0u, 0u, 0u, 0u);
-        if (o < seg && x < w) {
-            const uint32_t a = arms[row + x];
-            const int La = a & 0xFF, Ra = (a >> 8) & 0xFF;
-            const int co = Lmax + o;   // column index of x; P(j) sums columns < j
-            r.x = (uint32_t)(((co + Ra + 1) % rs) * 256);
-            r.y = (uint32_t)(((co - La) % rs) * 256);
-            const uint32_t n = (uint32_t)(La + Ra + 1);
-            r.z = (0x80000000u + n - 1u) / n;
-            r.w = (uint32_t)(x - La) | ((uint32_t)(x + Ra) << 16);
+    for (int n = lane; n < 2 * kMaxArm + 2; n += 64) s_mag[n] = n ? (0x80000000u + (uint32_t)n - 1u) / (uint32_t)n : 0u;
+    // left records: columns xb .. xb + ncol - 1 (kFarLeft outside the image)
+    for (int c0 = 0; c0 <= ncol + 1; c0 += 256) {
+        uint32_t lo[4], hi[4], g[4], gx[4], gy[4];
+        bool in[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int c = c0 + lane + 64 * j, x = xb + c;
+            in[j] = c < ncol && x >= 0 && x < w;
+            const size_t q = row + clampi(x, 0, w - 1);
+            if (SYM) { lo[j] = c32l[q]; hi[j] = 0u; }
+            else { const uint64_t v = c64l[q]; lo[j] = (uint32_t)v; hi[j] = (uint32_t)(v >> 32); }
+            g[j] = gl[q];
+            gx[j] = gxl[q];
+            gy[j] = gyl[q];
         }
-        s_out[o] = r;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int c = c0 + lane + 64 * j;
+            if (c <= ncol + 1) {
+                s_col[c] = in[j] ? make_uint4(lo[j], rep4(g[j]), grad4(gx[j], gy[j]), (uint32_t)(xb + c))
+                                 : make_uint4(0u, 0u, 0u, (uint32_t)kFarLeft);
+                if (!SYM) s_chi[c] = in[j] ? hi[j] : 0u;
+            }
+        }
+    }
+    for (int o0 = 0; o0 <= seg; o0 += 256) {
+        uint32_t a[4];
+        bool in[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int o = o0 + lane + 64 * j, x = x0 + o;
+            in[j] = o < seg && x < w;
+            a[j] = arms[row + min(x, w - 1)];
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int o = o0 + lane + 64 * j;
+            if (o > seg) continue;
+            uint4 r = make_uint4(0u, 0u, 0u, 0u);
+            if (in[j]) {
+                const int x = x0 + o;
+                const int La = a[j] & 0xFF, Ra = (a[j] >> 8) & 0xFF;
+                const int co = Lmax + o;   // column index of x; P(j) sums columns < j
+                r.x = (uint32_t)(((co + Ra + 1) % rs) * 256);
+                r.y = (uint32_t)(((co - La) % rs) * 256);
+                const uint32_t n = (uint32_t)(La + Ra + 1);
+                r.z = (0x80000000u + n - 1u) / n;
+                r.w = (uint32_t)(x - La) | ((uint32_t)(x + Ra) << 16);
+            }
+            s_out[o] = r;
+        }
     }
     // right pixel xs + i -> parity array i & 1, entry i >> 1
-    for (int i = lane; i < ncol + D + 1; i += 64) {
-        const int xr = xs + i;
-        uint4 r = make_uint4(0u, 0u, 0u, 0u);
-        if (i < ncol + D - 1 && xr >= 0 && xr < w) {
-            const size_t q = row + xr;
-            if (SYM) r.x = c32r[q];
-            else { const uint64_t v = c64r[q]; r.x = (uint32_t)v; r.w = (uint32_t)(v >> 32); }
-            r.y = rep4(gr[q]);
-            r.z = grad4(gxr[q], gyr[q]);
+    for (int i0 = 0; i0 < ncol + D + 1; i0 += 256) {
+        uint32_t lo[4], hi[4], g[4], gx[4], gy[4];
+        bool in[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int i = i0 + lane + 64 * j, xr = xs + i;
+            in[j] = i < ncol + D - 1 && xr >= 0 && xr < w;
+            const size_t q = row + clampi(xr, 0, w - 1);
+            if (SYM) { lo[j] = c32r[q]; hi[j] = 0u; }
+            else { const uint64_t v = c64r[q]; lo[j] = (uint32_t)v; hi[j] = (uint32_t)(v >> 32); }
+            g[j] = gr[q];
+            gx[j] = gxr[q];
+            gy[j] = gyr[q];
         }
-        ((uint4*)(shm + ((i & 1) ? L.rro : L.rre)))[i >> 1] = r;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int i = i0 + lane + 64 * j;
+            if (i >= ncol + D + 1) continue;
+            const uint4 r = in[j] ? make_uint4(lo[j], rep4(g[j]), grad4(gx[j], gy[j]), SYM ? 0u : hi[j])
+                                  : make_uint4(0u, 0u, 0u, 0u);
+            ((uint4*)(shm + ((i & 1) ? L.rro : L.rre)))[i >> 1] = r;
+        }
     }
     for (int i = lane; i < seg + D - 1; i += 64) s_key[i] = 0xFFFFFFFFu;
     __syncthreads();

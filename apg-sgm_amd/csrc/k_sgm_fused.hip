@@ -20,6 +20,10 @@
 namespace apgk {
 
 constexpr int SL_PF = 8;   // steps per prefetch batch (two batches in flight)
+// paired planes: positions whose two arrivals are fewer steps apart keep the
+// first delta in LDS (a parked delta is stored at least four batches before
+// the second arrival's prefetch reads it back)
+constexpr int kPairGap = 48;
 
 // Raw C' words of one lane: K pairs of TC (2, 4 or 8 bytes)
 template <typename TC, int K>
@@ -451,7 +455,7 @@ __device__ __forceinline__ void static_for(F&& f) {
     static_for_impl<N>(f, std::make_integer_sequence<int, N>{});
 }
 
-template <bool FULL, int PF>
+template <bool FULL, int PF, bool PAIRED>
 __global__ __launch_bounds__(64) void sgm_half_kernel(apg_hip_params p, int w, int h, int D, int nfam,
                                                       const uint8_t* __restrict__ gray,
                                                       const uint8_t* __restrict__ cost, PlaneSet planes, int nmax) {
@@ -459,7 +463,9 @@ __global__ __launch_bounds__(64) void sgm_half_kernel(apg_hip_params p, int w, i
     constexpr int RB = 128;                  // staged row bytes (>= D)
     uint8_t* s_c = sh;                       // [chain][buffer][SB][RB] C' rows
     uint8_t* s_d = sh + 4 * SB * RB;         // [chain][buffer][SB][RB] delta rows
-    uint32_t* s_p2 = (uint32_t*)(sh + 8 * SB * RB);   // [chain][nmax] P2 | P2 << 16 per step
+    uint8_t* s_p2 = sh + 8 * SB * RB;        // [chain][nmax] P2 per step
+    uint8_t* s_mid = s_p2 + 2 * nmax;        // PAIRED: [kPairGap + 1][RB] first deltas near the crossing,
+                                             // then one spare row
     const int lane = threadIdx.x;
     const int hf = lane >> 5, hl = lane & 31;
     int b = blockIdx.x, f = 0;
@@ -481,23 +487,40 @@ __global__ __launch_bounds__(64) void sgm_half_kernel(apg_hip_params p, int w, i
         const int b0 = xB0 - dxA * t0, b1 = xB0 - dxA * t1;
         return min(a0, a1) >= xlo && max(a0, a1) <= xhi && min(b0, b1) >= xlo && max(b0, b1) <= xhi;
     };
-    const auto rpa = raw_rsrc(planes.p[2 * f]);
-    const auto rpb = raw_rsrc(planes.p[2 * f + 1]);
+    // PAIRED: one plane per family holds delta_A + delta_B (both <= P2 <= 127).
+    // The first chain to reach a position parks its delta there; the second
+    // adds its own and stores the sum.  Positions within kPairGap / 2 of the
+    // crossing keep the first delta in s_mid instead, so a parked delta is
+    // always stored at least four batches before the second chain loads it.
+    const auto rpa = raw_rsrc(planes.p[PAIRED ? f : 2 * f]);
+    const auto rpb = raw_rsrc(planes.p[PAIRED ? f : 2 * f + 1]);
     // P2 of every step of both chains: chain A at step t is pixel t after t-1,
     // chain B pixel n-1-t after n-t (the first step of each compares a pixel
-    // with itself)
-    for (int t = lane; t < n; t += 64) {
-        const int ga = gray[pA0 + t * step], gpa = gray[pA0 + max(t - 1, 0) * step];
-        const int u = n - 1 - t;
-        const int gb = gray[pA0 + u * step], gpb = gray[pA0 + min(u + 1, n - 1) * step];
-        const uint32_t a = (uint32_t)adaptive_p2(p, ga - gpa), bb = (uint32_t)adaptive_p2(p, gb - gpb);
-        s_p2[t] = a | (a << 16);
-        s_p2[nmax + t] = bb | (bb << 16);
+    // with itself).  Each trip issues the gray loads of eight steps per lane
+    // before using any of them.
+    for (int t0 = 0; t0 < n; t0 += 512) {
+        int ga[8], gpa[8], gb[8], gpb[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int t = min(t0 + lane + 64 * j, n - 1), u = n - 1 - t;
+            ga[j] = gray[pA0 + t * step];
+            gpa[j] = gray[pA0 + max(t - 1, 0) * step];
+            gb[j] = gray[pA0 + u * step];
+            gpb[j] = gray[pA0 + min(u + 1, n - 1) * step];
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int t = t0 + lane + 64 * j;
+            if (t < n) {
+                s_p2[t] = (uint8_t)adaptive_p2(p, ga[j] - gpa[j]);
+                s_p2[nmax + t] = (uint8_t)adaptive_p2(p, gb[j] - gpb[j]);
+            }
+        }
     }
     // the P2 words past the last step are read with their batch (never used)
     for (int t = n + lane; t < ((n + SB - 1) / SB) * SB; t += 64) {
-        s_p2[t] = 0u;
-        s_p2[nmax + t] = 0u;
+        s_p2[t] = 0;
+        s_p2[nmax + t] = 0;
     }
     // batch transfer: lane -> (row j, 16-byte chunk c); one instruction per chain covers SB rows
     const int lpx = D / 16;
@@ -513,11 +536,24 @@ __global__ __launch_bounds__(64) void sgm_half_kernel(apg_hip_params p, int w, i
     auto voffB = [&](int k, int j) { return (uint32_t)(pB0 - min(k * SB + j, n - 1) * step) * (uint32_t)D + cl * 16u; };
     // batch k's rows arrive in register slot k % PF, PF batches ahead of use
     u32x4 gA[PF], gB[PF];
-    auto gload = [&](int k, auto slot_t) {
+    // PAIRED: parked deltas of batch k (register slot k % PF), loaded with the
+    // C' rows of batch k + PF - 2 (two batches before the flush that needs them)
+    u32x4 pkA[PF], pkB[PF];
+    // kp = k - 2: PAIRED, the batch whose parked deltas are loaded into slot kp % PF.
+    // Only a far second arrival's row is read; the other lanes read the plane's
+    // first 16 bytes (inside every plane: D >= 16), one cache line per load.
+    auto gload = [&](int k, int kp, auto slot_t) {
         constexpr int q = decltype(slot_t)::value;
         const int j = min(jl, SB - 1);
         gA[q] = buf_ld128(rc, voffA(k, j), 0u);
         gB[q] = buf_ld128(rc, voffB(k, j), 0u);
+        if constexpr (PAIRED) {
+            constexpr int qp = (q + PF - 2) % PF;
+            const int t = kp * SB + jl;
+            const bool far2 = jl < SB && t < n && 2 * t - (n - 1) >= kPairGap;
+            pkA[qp] = buf_ld128_dev(rpa, far2 ? voffA(kp, jl) : 0u, 0u);
+            pkB[qp] = buf_ld128_dev(rpb, far2 ? voffB(kp, jl) : 0u, 0u);
+        }
     };
     auto lstage = [&](int k, auto slot_t) {   // batch k's rows -> LDS buffer k & 1
         constexpr int q = decltype(slot_t)::value;
@@ -527,38 +563,73 @@ __global__ __launch_bounds__(64) void sgm_half_kernel(apg_hip_params p, int w, i
             *(u32x4*)(s_c + ((1 * 2 + bf) * SB + jl) * RB + cl * 16) = gB[q];
         }
     };
-    auto flush = [&](int k) {    // deltas of batch k (LDS buffer k & 1) -> planes
+    const int nb = (n + SB - 1) / SB;
+    auto flush = [&](int k, auto slot_t) {    // deltas of batch k (LDS buffer k & 1) -> planes
+        constexpr int q = decltype(slot_t)::value;
         const int bf = k & 1;
-        if (jl < SB && k * SB + jl < n) {
-            const u32x4 va = *(const u32x4*)(s_d + ((0 * 2 + bf) * SB + jl) * RB + cl * 16);
-            const u32x4 vb = *(const u32x4*)(s_d + ((1 * 2 + bf) * SB + jl) * RB + cl * 16);
-            __builtin_amdgcn_raw_buffer_store_b128(va, rpa, (int)voffA(k, jl), 0, 0);
-            __builtin_amdgcn_raw_buffer_store_b128(vb, rpb, (int)voffB(k, jl), 0, 0);
+        const int t = k * SB + jl;
+        if (jl < SB && t < n) {
+            u32x4 va = *(const u32x4*)(s_d + ((0 * 2 + bf) * SB + jl) * RB + cl * 16);
+            u32x4 vb = *(const u32x4*)(s_d + ((1 * 2 + bf) * SB + jl) * RB + cl * 16);
+            if constexpr (PAIRED) {
+                // A reaches its row's position t first iff e <= 0, B reaches
+                // n-1-t first iff e < 0.  A first arrival parks its delta in the
+                // plane (far from the crossing) or copies it to s_mid (near; the
+                // plane store goes to the spare bytes).  A second arrival stores
+                // its delta plus the first's: the parked one (far), the other
+                // chain's staged row when both arrivals fall in this batch, else
+                // the s_mid copy made by an earlier flush.
+                const int e = 2 * t - (n - 1);
+                const bool near = e > -kPairGap && e < kPairGap;
+                const int t1 = n - 1 - t;                    // the other chain's step at this row's positions
+                const bool same = t1 >= k * SB && t1 < k * SB + SB;
+                const int midA = t - ((n - 1) >> 1) + kPairGap / 2;        // s_mid row of position t
+                const int midB = (n - 1 - t) - ((n - 1) >> 1) + kPairGap / 2;
+                const int spare_row = kPairGap + 1;
+                const uint8_t* srcA = same ? s_d + ((1 * 2 + bf) * SB + (t1 - k * SB)) * RB
+                                           : s_mid + (near && e > 0 ? midA : spare_row) * RB;
+                const uint8_t* srcB = same ? s_d + ((0 * 2 + bf) * SB + (t1 - k * SB)) * RB
+                                           : s_mid + (near && e >= 0 ? midB : spare_row) * RB;
+                const u32x4 oa = *(const u32x4*)(srcA + cl * 16), ob = *(const u32x4*)(srcB + cl * 16);
+                *(u32x4*)(s_mid + (near && e <= 0 ? midA : spare_row) * RB + cl * 16) = va;
+                *(u32x4*)(s_mid + (near && e < 0 ? midB : spare_row) * RB + cl * 16) = vb;
+                const u32x4 z = {0u, 0u, 0u, 0u};
+                va = va + (e <= 0 ? z : (near ? oa : pkA[q]));
+                vb = vb + (e < 0 ? z : (near ? ob : pkB[q]));
+                const uint32_t spare = (uint32_t)((size_t)w * h * D) + cl * 16u;
+                __builtin_amdgcn_raw_buffer_store_b128(va, rpa, (int)(near && e <= 0 ? spare : voffA(k, jl)), 0, 0);
+                __builtin_amdgcn_raw_buffer_store_b128(vb, rpb, (int)(near && e < 0 ? spare : voffB(k, jl)), 0, 0);
+            } else {
+                __builtin_amdgcn_raw_buffer_store_b128(va, rpa, (int)voffA(k, jl), 0, 0);
+                __builtin_amdgcn_raw_buffer_store_b128(vb, rpb, (int)voffB(k, jl), 0, 0);
+            }
         }
     };
     // the lane's C' words (4 d of its chain) and P2 pairs of all SB steps of a
     // batch are read into register set k & 1 one batch ahead, so no LDS
     // latency lies on the recurrence
     const uint32_t lofs = (uint32_t)(hf * 2 * SB * RB + 4 * hl);
-    const uint32_t* p2h = s_p2 + hf * nmax;
+    const uint8_t* p2h = s_p2 + hf * nmax;
     uint32_t W[2][SB], P[2][SB];
     auto wread = [&](int k, auto set_t) {
         constexpr int e = decltype(set_t)::value;
 #pragma unroll
         for (int i = 0; i < SB; ++i) W[e][i] = *(const uint32_t*)(s_c + lofs + ((k & 1) * SB + i) * RB);
-        const u32x4* pp = (const u32x4*)(p2h + k * SB);
-        const u32x4 a = pp[0], b = pp[1];
+        const uint2 pb = *(const uint2*)(p2h + k * SB);   // P2 of the batch's steps, one byte each
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            P[e][i] = a[i];
-            P[e][4 + i] = b[i];
+            const uint32_t sel = 0x0c000c00u | (uint32_t)i | ((uint32_t)i << 16);   // byte i -> both halves
+            P[e][i] = __builtin_amdgcn_perm(0u, pb.x, sel);
+            P[e][4 + i] = __builtin_amdgcn_perm(0u, pb.y, sel);
         }
     };
-    const int nb = (n + SB - 1) / SB;
     // loads are issued unconditionally (past the end: the last batch again), so
     // the wait before staging a batch counts the loads issued after it and
     // leaves them in flight
-    static_for<PF>([&](auto q_t) { gload(min((int)decltype(q_t)::value, nb - 1), q_t); });
+    static_for<PF>([&](auto q_t) {
+        constexpr int q = decltype(q_t)::value;
+        gload(min(q, nb - 1), q - 2, q_t);   // parked deltas of batches -2 .. 1: never far second arrivals
+    });
     lstage(0, std::integral_constant<int, 0>{});
     __syncthreads();
     wread(0, std::integral_constant<int, 0>{});
@@ -574,8 +645,9 @@ __global__ __launch_bounds__(64) void sgm_half_kernel(apg_hip_params p, int w, i
         }
         // the previous batch's deltas: in an inner batch after half its steps, so
         // the wait for the LDS reads of the flush finds the word reads done
-        if (!INNER && k > 0) flush(k - 1);
-        gload(min(k + PF, nb - 1), q_t);
+        using PrevSlot = std::integral_constant<int, (q + PF - 1) % PF>;   // slot of batch k - 1
+        if (!INNER && k > 0) flush(k - 1, PrevSlot{});
+        gload(min(k + PF, nb - 1), k + PF - 2, q_t);
         const int bf = k & 1;
         auto run_batch = [&](auto allv_tag) {
             constexpr bool ALLV = decltype(allv_tag)::value;
@@ -583,7 +655,7 @@ __global__ __launch_bounds__(64) void sgm_half_kernel(apg_hip_params p, int w, i
             for (int i = 0; i < SB; ++i) {
                 const int t = k * SB + i;
                 if (!INNER && t >= n) break;
-                if (INNER && i == SB / 2 && k > 0) flush(k - 1);
+                if (INNER && i == SB / 2 && k > 0) flush(k - 1, PrevSlot{});
                 const uint32_t wc = W[q & 1][i], p2p = P[q & 1][i];
                 CRaw<uint8_t, 2> x;
                 x.w[0] = wc;
@@ -629,7 +701,9 @@ __global__ __launch_bounds__(64) void sgm_half_kernel(apg_hip_params p, int w, i
         const int k = k0 + decltype(q_t)::value;
         if (k < nb) batch(k, q_t, std::false_type{});
     });
-    flush(nb - 1);
+    static_for<PF>([&](auto q_t) {
+        if ((nb - 1) % PF == decltype(q_t)::value) flush(nb - 1, q_t);
+    });
 }
 
 // Quarter-wave form for u8 C', 4 paths, D % 16 == 0, D <= 128: one wave runs
@@ -841,6 +915,11 @@ __global__ __launch_bounds__(64) void sgm_quad_kernel(apg_hip_params p, int w, i
 // a batch's words (k & 1) is static within the PF-batch trip
 constexpr int kHalfPF = 4;
 
+bool sgm_pairs_planes(const apg_hip_params& p, int D, bool cost_u8, const KTune& tn) {
+    const int p2max = p.P2_base > p.P1 + 1 ? p.P2_base : p.P1 + 1;
+    return cost_u8 && D % 16 == 0 && D > 0 && D <= 128 && (tn.sgm_dual == 0 || tn.sgm_dual == 3) && p2max <= 127;
+}
+
 template <typename TC>
 static hipError_t launch_lines_t(hipStream_t s, const apg_hip_params& p, int w, int h, int D, const uint8_t* gray,
                                  const TC* cost, const PlaneSet& planes, const KTune& tn) {
@@ -870,22 +949,31 @@ static hipError_t launch_lines_t(hipStream_t s, const apg_hip_params& p, int w, 
     }
     if (sizeof(TC) == 1 && D % 16 == 0 && K == 1 && (tn.sgm_dual == 0 || tn.sgm_dual == 3)) {
         // both chains of a scanline per wave, 32 lanes each; P2 of every step staged
-        const int nmax = ((w > h ? w : h) + SB - 1) / SB * SB;   // whole batches of P2 words
-        const size_t shmh = (size_t)8 * SB * 128 + (size_t)2 * nmax * 4;
+        const int nmax = ((w > h ? w : h) + SB - 1) / SB * SB;   // whole batches of P2 bytes
+        const size_t shmh =
+            (size_t)8 * SB * 128 + (size_t)2 * nmax + (planes.paired ? (size_t)(kPairGap + 2) * 128 : 0);
+        const void* kf = planes.paired ? (full ? (const void*)sgm_half_kernel<true, kHalfPF, true>
+                                               : (const void*)sgm_half_kernel<false, kHalfPF, true>)
+                                       : (full ? (const void*)sgm_half_kernel<true, kHalfPF, false>
+                                               : (const void*)sgm_half_kernel<false, kHalfPF, false>);
         if (shmh > 64 * 1024) {
-            hipError_t e = hipFuncSetAttribute(full ? (const void*)sgm_half_kernel<true, kHalfPF>
-                                                    : (const void*)sgm_half_kernel<false, kHalfPF>,
-                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmh);
+            hipError_t e = hipFuncSetAttribute(kf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmh);
             if (e != hipSuccess) return e;
         }
-        if (full)
-            hipLaunchKernelGGL((sgm_half_kernel<true, kHalfPF>), dim3(nblk), dim3(64), shmh, s, p, w, h, D, nfam, gray,
-                               (const uint8_t*)cost, planes, nmax);
-        else
-            hipLaunchKernelGGL((sgm_half_kernel<false, kHalfPF>), dim3(nblk), dim3(64), shmh, s, p, w, h, D, nfam, gray,
-                               (const uint8_t*)cost, planes, nmax);
+#define APG_HALF(FF, PP)                                                                                      \
+    hipLaunchKernelGGL((sgm_half_kernel<FF, kHalfPF, PP>), dim3(nblk), dim3(64), shmh, s, p, w, h, D, nfam, gray, \
+                       (const uint8_t*)cost, planes, nmax)
+        if (planes.paired) {
+            if (full) APG_HALF(true, true);
+            else APG_HALF(false, true);
+        } else {
+            if (full) APG_HALF(true, false);
+            else APG_HALF(false, false);
+        }
+#undef APG_HALF
         return hipGetLastError();
     }
+    if (planes.paired) return hipErrorInvalidValue;   // only the half-wave form pairs its planes
     if (sizeof(TC) == 1 && D % 16 == 0 && K <= 2) {
         const size_t shm4 = (size_t)8 * SB * 128 * K + shm;
         const int nblk4 = 2 * nblk;

@@ -25,21 +25,23 @@ __device__ __forceinline__ float clampf_(float v, float lo, float hi) { return v
 // S of one (d, d+1) pair from C' and the planes' deltas, all as u16 halves
 // (u8 C' undecoded: 255 = invalid): npath * C' + sum of the deltas (< 65535
 // for valid C', checked by the host), 65535 where C' is invalid.
-// ALLV: the caller knows no C' entry of the pair is invalid.
-template <typename TC, int NP, bool ALLV = false>
+// ALLV: the caller knows no C' entry of the pair is invalid.  MUL is the
+// number of paths: NP for one plane per direction, 2 NP where each plane
+// holds the sum of a scanline's two directions.
+template <typename TC, int NP, bool ALLV = false, int MUL = NP>
 __device__ __forceinline__ uint32_t s_pair(uint32_t ch, const uint32_t (&ph)[NP]) {
     if (ALLV) {
         u16x2 ds = as_v2(ph[0]);
 #pragma unroll
         for (int f = 1; f < NP; ++f) ds = ds + as_v2(ph[f]);
-        return as_u32(as_v2(ch) * splat2(NP) + ds);
+        return as_u32(as_v2(ch) * splat2(MUL) + ds);
     }
     const uint32_t c16 = sizeof(TC) == 1 ? as_u32(as_v2(ch) - (as_v2((ch + 0x00010001u) & 0xFF00FF00u)))   // 255 -> 65535
                                          : ch;
     u16x2 ds = as_v2(ph[0]);
 #pragma unroll
     for (int f = 1; f < NP; ++f) ds = ds + as_v2(ph[f]);
-    const uint32_t sv = as_u32(as_v2(c16) * splat2(NP) + ds);
+    const uint32_t sv = as_u32(as_v2(c16) * splat2(MUL) + ds);
     const uint32_t inv = as_u32((as_v2(c16) + splat2(1)) == splat2(0) ? splat2(0xFFFF) : splat2(0));
     return sv | inv;
 }
@@ -54,7 +56,7 @@ constexpr int WF_WG = 4;
 // scans half of its pixel's row and the two halves are combined by a DPP
 // swap of adjacent lanes; half the LDS per wave, so twice the waves per CU
 // keep loads in flight while others scan.
-template <typename TC, int NP, bool WIDE, int LP>
+template <typename TC, int NP, bool WIDE, int LP, int MUL>
 __global__ __launch_bounds__(64) void wta_fused_kernel(apg_hip_params p, int w, int h, int D, int rw,
                                                        const TC* __restrict__ cost, PlaneSet planes,
                                                        const int16_t* __restrict__ dmin,
@@ -114,7 +116,7 @@ __global__ __launch_bounds__(64) void wta_fused_kernel(apg_hip_params p, int w, 
                         uint32_t ph[NP];
 #pragma unroll
                         for (int f = 0; f < NP; ++f) ph[f] = __builtin_amdgcn_perm(0u, pw[g][f][q], sel);
-                        dst[q * 2 + hh] = s_pair<TC, NP, AV>(__builtin_amdgcn_perm(0u, cw[g][q], sel), ph);
+                        dst[q * 2 + hh] = s_pair<TC, NP, AV, MUL>(__builtin_amdgcn_perm(0u, cw[g][q], sel), ph);
                     }
                 }
             }
@@ -147,7 +149,7 @@ __global__ __launch_bounds__(64) void wta_fused_kernel(apg_hip_params p, int w, 
                 uint32_t ph[NP];
 #pragma unroll
                 for (int f = 0; f < NP; ++f) ph[f] = bytes_to_halves(rd[g][f]);
-                const uint32_t S = s_pair<TC, NP>(sizeof(TC) == 1 ? bytes_to_halves(rcw[g]) : rcw[g], ph);
+                const uint32_t S = s_pair<TC, NP, false, MUL>(sizeof(TC) == 1 ? bytes_to_halves(rcw[g]) : rcw[g], ph);
                 st[i * rw + jw] = S;
                 if (sum32) {
                     const size_t q = (p_base + i) * (size_t)D + 2 * jw;
@@ -274,18 +276,28 @@ hipError_t launch_wta_fused(hipStream_t s, const apg_hip_params& p, int w, int h
     const int rw = (wide && D % 64 == 0) ? D / 2 + 2 : (D / 2) | 1;
     const size_t shm = (size_t)(WF_PB / lp) * rw * 4;
     dim3 grid((unsigned)((npx + WF_PB / lp - 1) / (WF_PB / lp)));
-#define APG_WTAF(TC, NP, WD)                                                                                   \
-    hipLaunchKernelGGL((wta_fused_kernel<TC, NP, WD, (WD ? 2 : 1)>), grid, dim3(64), shm, s, p, w, h, D, rw,         \
+#define APG_WTAF(TC, NP, WD, MUL)                                                                              \
+    hipLaunchKernelGGL((wta_fused_kernel<TC, NP, WD, (WD ? 2 : 1), MUL>), grid, dim3(64), shm, s, p, w, h, D, rw,    \
                        (const TC*)cost, planes, dmin, dmax, dr, gx, gy, disp, conf, reason, reliable, disp_wta, best,   \
                        second, sum32)
-    if (p.paths == 8) {
-        if (wide) APG_WTAF(uint8_t, 8, true);
-        else if (cost_u8) APG_WTAF(uint8_t, 8, false);
-        else APG_WTAF(uint16_t, 8, false);
+    if (planes.paired) {
+        // one plane per scanline family, each the sum of its two directions (u8 C')
+        if (!cost_u8) return hipErrorInvalidValue;
+        if (p.paths == 8) {
+            if (wide) APG_WTAF(uint8_t, 4, true, 8);
+            else APG_WTAF(uint8_t, 4, false, 8);
+        } else {
+            if (wide) APG_WTAF(uint8_t, 2, true, 4);
+            else APG_WTAF(uint8_t, 2, false, 4);
+        }
+    } else if (p.paths == 8) {
+        if (wide) APG_WTAF(uint8_t, 8, true, 8);
+        else if (cost_u8) APG_WTAF(uint8_t, 8, false, 8);
+        else APG_WTAF(uint16_t, 8, false, 8);
     } else {
-        if (wide) APG_WTAF(uint8_t, 4, true);
-        else if (cost_u8) APG_WTAF(uint8_t, 4, false);
-        else APG_WTAF(uint16_t, 4, false);
+        if (wide) APG_WTAF(uint8_t, 4, true, 4);
+        else if (cost_u8) APG_WTAF(uint8_t, 4, false, 4);
+        else APG_WTAF(uint16_t, 4, false, 4);
     }
 #undef APG_WTAF
     return hipGetLastError();

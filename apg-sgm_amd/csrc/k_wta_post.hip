@@ -199,44 +199,51 @@ hipError_t launch_lr_check(hipStream_t s, const apg_hip_params& p, int w, int h,
 
 // A hole pixel x of row y once the gap scans have stopped at xl (left) and
 // xr (right), as the reference's loops leave them (reference
-// src/postprocess.cpp:60-130).
+// src/postprocess.cpp:60-130).  in_at / gray_at read the input disparity and
+// the gray image at an in-image pixel; cur is the pixel's own (invalid) value.
+template <typename InAt, typename GrayAt>
+__device__ __forceinline__ float fill_hole_at(int w, int h, int x, int y, int xl, int xr, float cur, bool occl, int c0,
+                                              float prior_v, InAt in_at, GrayAt gray_at) {
+    float o = cur;
+    const float dl = (xl >= 0 && in_at(xl, y) >= 0.f) ? in_at(xl, y) : -1.f;
+    const float drr = (xr < w && in_at(xr, y) >= 0.f) ? in_at(xr, y) : -1.f;
+    if (occl) {
+        if (dl >= 0.f && drr >= 0.f) o = drr < dl ? drr : dl;
+        else if (dl >= 0.f) o = dl;
+        else if (drr >= 0.f) o = drr;
+    } else {
+        float best_d = -1.f;
+        int best_abs = 999;
+        const int cx[4] = {xl, xr, x, x};
+        const int cy[4] = {y, y, y - 1, y + 1};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int xx = cx[k], yy = cy[k];
+            if (xx < 0 || yy < 0 || xx >= w || yy >= h) continue;
+            const float dd = in_at(xx, yy);
+            if (dd < 0.f) continue;
+            const int ad = abs(gray_at(xx, yy) - c0);
+            if (ad < best_abs) { best_abs = ad; best_d = dd; }
+        }
+        if (best_d >= 0.f && best_abs <= 18) o = best_d;
+        else if (prior_v >= 0.f) o = prior_v;
+        else if (dl >= 0.f && drr >= 0.f) {
+            const float t = (float)(x - xl) / (float)(xr - xl);
+            o = dl * (1.f - t) + drr * t;
+        } else if (dl >= 0.f) o = dl;
+        else if (drr >= 0.f) o = drr;
+    }
+    return o;
+}
+
 __device__ __forceinline__ float fill_hole(int w, int h, int x, int y, int xl, int xr, const float* row,
                                            const float* __restrict__ in, const uint8_t* __restrict__ reason,
                                            const uint8_t* __restrict__ gray, const float* __restrict__ prior) {
     const size_t q = (size_t)y * w + x;
-    float o = row[x];
-    {
-        const float dl = (xl >= 0 && row[xl] >= 0.f) ? row[xl] : -1.f;
-        const float drr = (xr < w && row[xr] >= 0.f) ? row[xr] : -1.f;
-        if (reason[q] == R_OCCLUSION) {
-            if (dl >= 0.f && drr >= 0.f) o = drr < dl ? drr : dl;
-            else if (dl >= 0.f) o = dl;
-            else if (drr >= 0.f) o = drr;
-        } else {
-            const int c0 = gray[q];
-            float best_d = -1.f;
-            int best_abs = 999;
-            const int cx[4] = {xl, xr, x, x};
-            const int cy[4] = {y, y, y - 1, y + 1};
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const int xx = cx[k], yy = cy[k];
-                if (xx < 0 || yy < 0 || xx >= w || yy >= h) continue;
-                const float dd = in[(size_t)yy * w + xx];
-                if (dd < 0.f) continue;
-                const int ad = abs((int)gray[(size_t)yy * w + xx] - c0);
-                if (ad < best_abs) { best_abs = ad; best_d = dd; }
-            }
-            if (best_d >= 0.f && best_abs <= 18) o = best_d;
-            else if (prior && prior[q] >= 0.f) o = prior[q];
-            else if (dl >= 0.f && drr >= 0.f) {
-                const float t = (float)(x - xl) / (float)(xr - xl);
-                o = dl * (1.f - t) + drr * t;
-            } else if (dl >= 0.f) o = dl;
-            else if (drr >= 0.f) o = drr;
-        }
-    }
-    return o;
+    return fill_hole_at(
+        w, h, x, y, xl, xr, row[x], reason[q] == R_OCCLUSION, (int)gray[q], prior ? prior[q] : -1.f,
+        [&](int xx, int yy) { return yy == y ? row[xx] : in[(size_t)yy * w + xx]; },
+        [&](int xx, int yy) { return (int)gray[(size_t)yy * w + xx]; });
 }
 
 __device__ __forceinline__ void fill_pixel(int w, int h, int max_gap, int x, int y, const float* row,
@@ -404,6 +411,155 @@ __global__ __launch_bounds__(256) void median3_kernel(int w, int h, const float*
 #pragma unroll
     for (int k = 1; k < 9; ++k) m = (n / 2 == k) ? v[k] : m;
     out[q] = m;
+}
+
+// Fill and the radius-1 median in one pass over 64 x 16 output tiles.  The
+// tile's input disparity and gray image are staged in LDS for rows y0-2 ..
+// y0+17 and columns x0-64 .. x0+127 (outside the image: invalid), with one
+// 64-bit mask of valid pixels per row and 64 columns (a wavefront ballot).
+// The filled values of rows y0-1 .. y0+16, columns x0-1 .. x0+64 (every pixel
+// the tile's 3x3 windows read) are then computed in LDS: a hole's gap scans
+// stay inside [x - max_gap - 1, x + max_gap + 1], which the 64-column halo
+// holds for max_gap <= 62.  The median of the filled values follows as in
+// median3_kernel (radius 0: the filled value itself).
+constexpr int FM_W = 64, FM_H = 16, FM_TC = 192, FM_TR = FM_H + 4, FM_FC = FM_W + 2, FM_FR = FM_H + 2;
+constexpr int FM_GMAX = 62;
+
+__global__ __launch_bounds__(256) void fill_median_kernel(int w, int h, int max_gap, int radius,
+                                                          const float* __restrict__ in,
+                                                          const uint8_t* __restrict__ reason,
+                                                          const uint8_t* __restrict__ gray,
+                                                          const float* __restrict__ prior, float* __restrict__ out) {
+    __shared__ float s_in[FM_TR * FM_TC];
+    __shared__ uint8_t s_g[FM_TR * FM_TC];
+    __shared__ uint64_t s_m[FM_TR * 3];
+    __shared__ float s_f[FM_FR * FM_FC];
+    const int x0 = blockIdx.x * FM_W, y0 = blockIdx.y * FM_H;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int xb = x0 - 64, yb = y0 - 2;   // image coordinates of tile cell (0, 0)
+    // 60 row segments of 64 columns, 15 per wave; all loads issued first
+    constexpr int NSEG = FM_TR * 3 / 4;
+    float v[NSEG];
+    uint32_t g[NSEG];
+    uint32_t outside = 0u;   // bit i: segment i's cell lies outside the image
+#pragma unroll
+    for (int i = 0; i < NSEG; ++i) {
+        const int sg = wave + 4 * i, r = sg / 3, c = (sg % 3) * 64 + lane;
+        const int xx = xb + c, yy = yb + r;
+        outside |= (uint32_t)!(xx >= 0 && xx < w && yy >= 0 && yy < h) << i;
+        const size_t q = (size_t)clampi(yy, 0, h - 1) * w + clampi(xx, 0, w - 1);
+        v[i] = in[q];
+        g[i] = gray[q];
+    }
+    // the loaded values are first used here, after every load was issued
+#pragma unroll
+    for (int i = 0; i < NSEG; ++i) {
+        const int sg = wave + 4 * i, r = sg / 3, c = (sg % 3) * 64 + lane;
+        const float vv = (outside >> i) & 1u ? -1.f : v[i];
+        s_in[r * FM_TC + c] = vv;
+        s_g[r * FM_TC + c] = (uint8_t)g[i];
+        const uint64_t m = __ballot(vv >= 0.f);
+        if (lane == 0) s_m[sg] = m;
+    }
+    __syncthreads();
+    auto in_at = [&](int xx, int yy) { return s_in[(yy - yb) * FM_TC + (xx - xb)]; };
+    auto gray_at = [&](int xx, int yy) { return (int)s_g[(yy - yb) * FM_TC + (xx - xb)]; };
+    // the reason codes (and prior values) of the thread's cells, all in flight
+    // before the first hole needs one
+    constexpr int NF = (FM_FR * FM_FC + 255) / 256;
+    uint8_t rs[NF];
+    float pv[NF];
+#pragma unroll
+    for (int k = 0; k < NF; ++k) {
+        const int i = min(tid + 256 * k, FM_FR * FM_FC - 1);
+        const int fr = i / FM_FC, fc = i - fr * FM_FC;
+        const size_t q = (size_t)clampi(yb + fr + 1, 0, h - 1) * w + clampi(xb + fc + 63, 0, w - 1);
+        rs[k] = reason[q];
+        pv[k] = prior ? prior[q] : -1.f;
+    }
+#pragma unroll
+    for (int k = 0; k < NF; ++k) {
+        const int i = tid + 256 * k;
+        if (i >= FM_FR * FM_FC) continue;
+        const int fr = i / FM_FC, fc = i - fr * FM_FC;
+        const int r = fr + 1, c = fc + 63;   // tile cell
+        const int x = xb + c, y = yb + r;
+        float o = -1.f;
+        if (x >= 0 && x < w && y >= 0 && y < h) {
+            o = s_in[r * FM_TC + c];
+            if (o < 0.f) {
+                const uint64_t* rm = s_m + r * 3;
+                const int clo = c - max_gap - 1, chi = c + max_gap + 1;
+                int Lc = -1;
+                {
+                    int sg = c >> 6;
+                    uint64_t m = rm[sg] & ((1ull << (c & 63)) - 1ull);
+                    while (m == 0ull && sg > 0 && sg * 64 > clo) m = rm[--sg];
+                    if (m != 0ull) Lc = sg * 64 + 63 - __clzll((long long)m);
+                    if (Lc < clo) Lc = -1;
+                }
+                int Rc = -1;
+                {
+                    int sg = c >> 6;
+                    const int b = c & 63;
+                    uint64_t m = b == 63 ? 0ull : rm[sg] & ~((2ull << b) - 1ull);
+                    while (m == 0ull && sg < 2 && (sg + 1) * 64 <= chi) m = rm[++sg];
+                    if (m != 0ull) Rc = sg * 64 + __ffsll((long long)m) - 1;
+                    if (Rc > chi) Rc = -1;
+                }
+                const int xl = Lc >= 0 ? xb + Lc : max(x - max_gap - 1, -1);
+                const int xr = Rc >= 0 ? xb + Rc : min(x + max_gap + 1, w);
+                o = fill_hole_at(w, h, x, y, xl, xr, o, rs[k] == R_OCCLUSION, gray_at(x, y), pv[k], in_at, gray_at);
+            }
+        }
+        s_f[i] = o;
+    }
+    __syncthreads();
+    for (int i = tid; i < FM_W * FM_H; i += 256) {
+        const int ry = i / FM_W, rx = i - ry * FM_W;
+        const int x = x0 + rx, y = y0 + ry;
+        if (x >= w || y >= h) continue;
+        const float* fc = s_f + (ry + 1) * FM_FC + rx + 1;   // the pixel's filled value
+        float res = fc[0];
+        if (radius == 1) {
+            float vv[9];
+            int n = 0;
+#pragma unroll
+            for (int k = 0; k < 9; ++k) {
+                const int ox = k % 3 - 1, oy = k / 3 - 1;
+                const int xx = x + ox, yy = y + oy;
+                float d = -1.f;
+                if (xx >= 0 && yy >= 0 && xx < w && yy < h) d = fc[oy * FM_FC + ox];
+                const bool ok = d >= 0.f;
+                n += ok ? 1 : 0;
+                vv[k] = ok ? d : INFINITY;
+            }
+            if (n > 0) {
+#pragma unroll
+                for (int rr = 0; rr < 9; ++rr)
+#pragma unroll
+                    for (int k = rr & 1; k + 1 < 9; k += 2) cswap(vv[k], vv[k + 1]);
+                float m = vv[0];
+#pragma unroll
+                for (int k = 1; k < 9; ++k) m = (n / 2 == k) ? vv[k] : m;
+                res = m;
+            }
+        }
+        out[(size_t)y * w + x] = res;
+    }
+}
+
+bool fill_median_fusable(const apg_hip_params& p) {
+    return p.post_median_radius >= 0 && p.post_median_radius <= 1 && p.post_max_fill_gap >= 0 &&
+           p.post_max_fill_gap <= FM_GMAX;
+}
+
+hipError_t launch_fill_median(hipStream_t s, const apg_hip_params& p, int w, int h, const float* disp,
+                              const uint8_t* reason, const uint8_t* gray, const float* prior, float* out) {
+    if (!fill_median_fusable(p)) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(fill_median_kernel, dim3((w + FM_W - 1) / FM_W, (h + FM_H - 1) / FM_H), dim3(256), 0, s, w, h,
+                       p.post_max_fill_gap, p.post_median_radius, disp, reason, gray, prior, out);
+    return hipGetLastError();
 }
 
 hipError_t launch_fill(hipStream_t s, const apg_hip_params& p, int w, int h, const float* disp, const uint8_t* reason,

@@ -98,6 +98,11 @@ hipError_t launch_fill(hipStream_t s, const apg_hip_params& p, int w, int h, con
 // median_radius 0 copies disp to out.
 constexpr int kMaxMedianRadius = 64;
 hipError_t launch_median(hipStream_t s, const apg_hip_params& p, int w, int h, const float* disp, float* out);
+// fill_holes then median in one launch (disp and out distinct), for median
+// radius 0 or 1 and max_fill_gap <= 62 (fill_median_fusable)
+bool fill_median_fusable(const apg_hip_params& p);
+hipError_t launch_fill_median(hipStream_t s, const apg_hip_params& p, int w, int h, const float* disp,
+                              const uint8_t* reason, const uint8_t* gray, const float* prior, float* out);
 
 // ---- fused schedule ----
 // Largest quantised cost any state can take (bound from the LUT maxima);
@@ -130,7 +135,13 @@ struct PlaneSet {
     // batches of columns are known valid; 1: per-pixel ranges (the prior's),
     // validity only from C'
     int general_ranges;
+    // 1: p[f] holds, per scanline family f (rows, columns, the two diagonals),
+    // the sum of its two directions' deltas (both <= P2 <= 127, so the sum
+    // fits u8), followed by kPlaneSpare bytes the SGM may overwrite; 0: p[r]
+    // holds direction r's deltas
+    int paired;
 };
+constexpr size_t kPlaneSpare = 256;
 // Refiner (reference src/refiner.cpp:14-98) on the device.  The engine is
 // the std::mt19937(12345) output stream; it must hold (iterations + 1) *
 // (w * h + kRefineSlack) outputs.
@@ -150,6 +161,9 @@ inline size_t refine_engine_len(const apg_hip_params& p, int w, int h) {
 hipError_t launch_refine(hipStream_t st, const apg_hip_params& p, const CostLut* d_lut, int w, int h,
                          const RefineInputs& in, const int16_t* dmin, const int16_t* dmax, const uint8_t* reliable,
                          float* disp, const uint32_t* eng, size_t eng_len, void* scratch, size_t scratch_bytes);
+// whether launch_sgm_lines pairs each scanline family's two directions into
+// one plane (the half-wave form with max(P2, P1 + 1) <= 127)
+bool sgm_pairs_planes(const apg_hip_params& p, int D, bool cost_u8, const KTune& tn);
 hipError_t launch_sgm_lines(hipStream_t s, const apg_hip_params& p, int w, int h, int D, const uint8_t* gray,
                             const void* cost, bool cost_u8, const PlaneSet& planes, const KTune& tn = KTune());
 hipError_t launch_wta_fused(hipStream_t s, const apg_hip_params& p, int w, int h, const void* cost, bool cost_u8,

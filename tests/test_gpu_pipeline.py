@@ -193,6 +193,33 @@ def test_pipeline_aux_pair_edges(ctx, kind, w, h, D, mode, over, sched):
     _check(g, o, PLAIN_KEYS if sched == 1 else FUSED_KEYS)
 
 
+# fill + median: one tiled pass for median radius 0 / 1 and max_fill_gap <=
+# 62 (a 64-column halo), the row fill and the median kernels otherwise; gaps
+# 0, 5, 40, 62, 63, 100, widths that leave partial tiles, a strict LR check
+# (more holes)
+POST_CASES = [
+    (0, 150, 40, 32, 1, {"post_max_fill_gap": 0}),
+    (1, 130, 35, 48, 1, {"post_max_fill_gap": 62, "post_lr_max_diff": 0}),
+    (2, 140, 33, 48, 1, {"post_max_fill_gap": 63, "post_lr_max_diff": 0}),
+    (0, 200, 20, 64, 1, {"post_max_fill_gap": 5, "post_median_radius": 0}),
+    (1, 65, 17, 16, 1, {"post_max_fill_gap": 40}),
+    (2, 300, 48, 96, 1, {"post_lr_max_diff": 0}),
+    (0, 190, 70, 64, 0, {"post_max_fill_gap": 100, "post_lr_max_diff": 0}),
+]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind,w,h,D,mode,over", POST_CASES)
+def test_pipeline_fill_median_fused(ctx, kind, w, h, D, mode, over):
+    print(f"fill+median kind={kind} {w}x{h} D={D} {over}", flush=True)
+    p, left, right, o = _case(kind, w, h, D, mode, over)
+    A.set_schedule(ctx, 0)
+    g = A.run_pipeline(ctx, p, left, right)
+    _check(g, o, WTA_KEYS[5:])
+    g = A.run_pipeline(ctx, p, left, right, intermediates=A.FUSED_INTERMEDIATES)
+    _check(g, o, FUSED_KEYS)
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("h,w", [(19, 67), (33, 130)])
 def test_pipeline_color_input_odd_widths(ctx, h, w):

@@ -102,18 +102,25 @@ def test_pipeline_workspace_bytes_closed_form():
     # host-only sizing used by the serving form's HBM admission:
     # per pixel 6 (gray/gx/gy x2) + 8 (symmetric census x2) + 4 (range) +
     # 12 (arms, right disparity, post scratch) + 4 (right-WTA keys) = 34 B;
-    # per state C' (u8) + T (u8, Cross only) + one u8 delta plane per path
+    # per state C' (u8) + T (u8, Cross only) + one u8 delta plane per path, or,
+    # where the half-wave SGM pairs a scanline's two directions (D % 16 == 0,
+    # D <= 128, P2 <= 127), one plane per scanline family
     def ws(mode, D, w, h, **over):
         return A.workspace_bytes(A.params(mode, D, **over), w, h)
-    for (w, h, D) in [(640, 480, 64), (1280, 720, 128), (1920, 1080, 256), (333, 101, 48)]:
+    for (w, h, D) in [(640, 480, 64), (1280, 720, 128), (1920, 1080, 256), (333, 101, 48), (200, 90, 40)]:
         n, nv = w * h, w * h * D
-        lut = ws(1, D, w, h) - (34 * n + 6 * nv)
+        pairs = D % 16 == 0 and D <= 128
+        planes4, planes8 = (2, 4) if pairs else (4, 8)
+        spare = 256 if pairs else 0   # bytes past each paired plane's states
+        lut = ws(1, D, w, h) - (34 * n + (2 + planes4) * nv + planes4 * spare)
         assert 0 < lut < 64 * 1024
-        assert ws(0, D, w, h) == 34 * n + 5 * nv + lut          # Fast: no Cross, no T
-        assert ws(1, D, w, h, paths=8) == 34 * n + 10 * nv + lut
+        assert ws(0, D, w, h) == 34 * n + (1 + planes4) * nv + planes4 * spare + lut          # Fast: no Cross, no T
+        assert ws(1, D, w, h, paths=8) == 34 * n + (2 + planes8) * nv + planes8 * spare + lut
+        # P2 above 127: one plane per direction
+        assert ws(1, D, w, h, P2_base=200) == 34 * n + 6 * nv + lut
     # stage-by-stage schedule (max(P2, P1+1) > 254): three u16 volumes, best / second
     n, nv = 200 * 100, 200 * 100 * 32
-    lut = ws(1, 32, 200, 100) - (34 * n + 6 * nv)
+    lut = ws(1, 32, 200, 100) - (34 * n + 4 * nv + 2 * 256)   # paired planes at D = 32
     assert ws(1, 32, 200, 100, P2_base=300) == (34 - 4 + 8) * n + 6 * nv + lut
     assert A.workspace_bytes(A.params(1, 32, min_disparity=40), 10, 10) == 0   # max <= min
 
