@@ -177,9 +177,9 @@ struct WsPlan {
     // planes_sgm with u8 T / C': the general-range forms of the fused Cost and
     // Cross kernels (right-view keys in the cost pass, no u16 volumes)
     bool planes_fused = false;
-    // the SGM keeps one plane per scanline family (the sum of its two
-    // directions' deltas) instead of one per direction
-    bool paired = false;
+    // scanline families whose SGM planes hold the sum of their two
+    // directions' deltas (one plane instead of two), as PlaneSet::paired
+    int pair_mask = 0;
     size_t bytes[B_COUNT] = {};
 };
 
@@ -227,18 +227,19 @@ WsPlan plan_workspace(const apg_hip_params& p, int w, int h, const CostLut& lut,
     pl.planes_sgm = !pl.fused && schedule != 1 && !want_raw && (p.prior_enable || p.refine_enable) && D % 2 == 0 &&
                     D <= 256 && p2max <= 254 && npaths * (cbound + p2max) < 65535 && nv * 2 < (1ull << 31);
     pl.planes_fused = pl.planes_sgm && pl.u8 && (!p.agg_enable || p.agg_iterations <= 1) && nv < (1ull << 31);
-    pl.paired = (pl.fused || pl.planes_fused) && sgm_pairs_planes(p, D, pl.u8, tn);
-    const int nplanes = pl.paired ? npaths / 2 : npaths;
+    pl.pair_mask = (pl.fused || pl.planes_fused) ? sgm_pair_mask(p, D, pl.u8, tn) : 0;
+    const int nplanes = npaths - __builtin_popcount((unsigned)pl.pair_mask);
+    const size_t spare = pl.pair_mask ? kPlaneSpare : 0;
     if (pl.planes_fused) {
         pl.bytes[B_COST] = nv;
         if (p.agg_enable) pl.bytes[B_TMP] = nv;
         pl.bytes[B_RKEYS] = n * 4;
-        for (int r = 0; r < nplanes; ++r) pl.bytes[B_DS0 + r] = nv + (pl.paired ? kPlaneSpare : 0);
+        for (int r = 0; r < nplanes; ++r) pl.bytes[B_DS0 + r] = nv + spare;
     } else if (pl.fused) {
         pl.bytes[B_COST] = nv * (pl.u8 ? 1 : 2);
         if (p.agg_enable) pl.bytes[B_TMP] = nv * (pl.u8 ? 1 : 2);
         pl.bytes[B_RKEYS] = n * 4;
-        for (int r = 0; r < nplanes; ++r) pl.bytes[B_DS0 + r] = nv + (pl.paired ? kPlaneSpare : 0);
+        for (int r = 0; r < nplanes; ++r) pl.bytes[B_DS0 + r] = nv + spare;
     } else {
         pl.bytes[B_COST] = pl.bytes[B_TMP] = nv * 2;
         if (pl.planes_sgm)
@@ -770,8 +771,9 @@ int apg_hip_pipeline(apg_hip_ctx ctx, const apg_hip_params* p, const uint8_t* d_
         const bool colr = fused;
         PlaneSet planes{};
         planes.general_ranges = colr ? 0 : 1;
-        planes.paired = plan.paired ? 1 : 0;
-        for (int r = 0; r < (plan.paired ? npaths / 2 : npaths); ++r) planes.p[r] = buf<uint8_t>(ctx, (BufId)(B_DS0 + r));
+        planes.paired = plan.pair_mask;
+        for (int r = 0; r < npaths - __builtin_popcount((unsigned)plan.pair_mask); ++r)
+            planes.p[r] = buf<uint8_t>(ctx, (BufId)(B_DS0 + r));
         void* cp = ctx->bufs[B_COST].ptr;   // C' (u8 or u16)
         void* tt = ctx->bufs[B_TMP].ptr;    // T  (u8 or u16)
         if (p->agg_enable && !arms_aux) HIPCHK(launch_cross_arms(st, *p, gl, w, h, arms), "arms");

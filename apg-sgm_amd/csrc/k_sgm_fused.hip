@@ -455,11 +455,18 @@ __device__ __forceinline__ void static_for(F&& f) {
     static_for_impl<N>(f, std::make_integer_sequence<int, N>{});
 }
 
+// plane of (family f, chain c): families whose bit is set in planes.paired
+// hold one plane (both chains), the others two
+__device__ __forceinline__ uint8_t* family_plane(const PlaneSet& planes, int f, int c) {
+    int base = 0;
+    for (int g = 0; g < f; ++g) base += (planes.paired >> g) & 1 ? 1 : 2;
+    return planes.p[base + ((planes.paired >> f) & 1 ? 0 : c)];
+}
+
 template <bool FULL, int PF, bool PAIRED>
-__global__ __launch_bounds__(64) void sgm_half_kernel(apg_hip_params p, int w, int h, int D, int nfam,
-                                                      const uint8_t* __restrict__ gray,
-                                                      const uint8_t* __restrict__ cost, PlaneSet planes, int nmax) {
-    extern __shared__ __align__(16) uint8_t sh[];
+__device__ __forceinline__ void sgm_half_body(const apg_hip_params& p, int w, int h, int D, int f, int b,
+                                              const uint8_t* __restrict__ gray, const uint8_t* __restrict__ cost,
+                                              const PlaneSet& planes, int nmax, uint8_t* sh) {
     constexpr int RB = 128;                  // staged row bytes (>= D)
     uint8_t* s_c = sh;                       // [chain][buffer][SB][RB] C' rows
     uint8_t* s_d = sh + 4 * SB * RB;         // [chain][buffer][SB][RB] delta rows
@@ -468,11 +475,6 @@ __global__ __launch_bounds__(64) void sgm_half_kernel(apg_hip_params p, int w, i
                                              // then one spare row
     const int lane = threadIdx.x;
     const int hf = lane >> 5, hl = lane & 31;
-    int b = blockIdx.x, f = 0;
-    {
-        const int cnt[4] = {h, w, w + h - 1, w + h - 1};
-        while (f < nfam - 1 && b >= cnt[f]) { b -= cnt[f]; ++f; }
-    }
     int p0, step, n;
     family_line(f, b, w, h, p0, step, n);
     const auto rc = raw_rsrc(cost);
@@ -492,8 +494,8 @@ __global__ __launch_bounds__(64) void sgm_half_kernel(apg_hip_params p, int w, i
     // adds its own and stores the sum.  Positions within kPairGap / 2 of the
     // crossing keep the first delta in s_mid instead, so a parked delta is
     // always stored at least four batches before the second chain loads it.
-    const auto rpa = raw_rsrc(planes.p[PAIRED ? f : 2 * f]);
-    const auto rpb = raw_rsrc(planes.p[PAIRED ? f : 2 * f + 1]);
+    const auto rpa = raw_rsrc(family_plane(planes, f, 0));
+    const auto rpb = raw_rsrc(family_plane(planes, f, 1));
     // P2 of every step of both chains: chain A at step t is pixel t after t-1,
     // chain B pixel n-1-t after n-t (the first step of each compares a pixel
     // with itself).  Each trip issues the gray loads of eight steps per lane
@@ -706,6 +708,22 @@ __global__ __launch_bounds__(64) void sgm_half_kernel(apg_hip_params p, int w, i
     });
 }
 
+// Scanline family f, member b of block blockIdx.x; PAIRED: the families whose
+// bit is set in planes.paired run the paired body, the others the plain one.
+template <bool FULL, int PF, bool PAIRED>
+__global__ __launch_bounds__(64) void sgm_half_kernel(apg_hip_params p, int w, int h, int D, int nfam,
+                                                      const uint8_t* __restrict__ gray,
+                                                      const uint8_t* __restrict__ cost, PlaneSet planes, int nmax) {
+    extern __shared__ __align__(16) uint8_t sh[];
+    int b = blockIdx.x, f = 0;
+    {
+        const int cnt[4] = {h, w, w + h - 1, w + h - 1};
+        while (f < nfam - 1 && b >= cnt[f]) { b -= cnt[f]; ++f; }
+    }
+    if (PAIRED && ((planes.paired >> f) & 1)) sgm_half_body<FULL, PF, true>(p, w, h, D, f, b, gray, cost, planes, nmax, sh);
+    else sgm_half_body<FULL, PF, false>(p, w, h, D, f, b, gray, cost, planes, nmax, sh);
+}
+
 // Quarter-wave form for u8 C', 4 paths, D % 16 == 0, D <= 128: one wave runs
 // the forward and backward chains of two scanlines of one family (rows 2i,
 // 2i+1 or columns 2j, 2j+1), 16 lanes per chain (chain q = lane >> 4: q & 1
@@ -915,9 +933,13 @@ __global__ __launch_bounds__(64) void sgm_quad_kernel(apg_hip_params p, int w, i
 // a batch's words (k & 1) is static within the PF-batch trip
 constexpr int kHalfPF = 4;
 
-bool sgm_pairs_planes(const apg_hip_params& p, int D, bool cost_u8, const KTune& tn) {
+int sgm_pair_mask(const apg_hip_params& p, int D, bool cost_u8, const KTune& tn) {
     const int p2max = p.P2_base > p.P1 + 1 ? p.P2_base : p.P1 + 1;
-    return cost_u8 && D % 16 == 0 && D > 0 && D <= 128 && (tn.sgm_dual == 0 || tn.sgm_dual == 3) && p2max <= 127;
+    if (!(cost_u8 && D % 16 == 0 && D > 0 && D <= 128 && (tn.sgm_dual == 0 || tn.sgm_dual == 3) && p2max <= 127))
+        return 0;
+    // every family: at 1280x720 D=128 the SGM and WTA took 227 + 138 us with
+    // no family paired, 250 + 107 with the columns only, 280 + 73 with all
+    return p.paths == 8 ? 0xF : 0x3;
 }
 
 template <typename TC>

@@ -281,14 +281,29 @@ hipError_t launch_wta_fused(hipStream_t s, const apg_hip_params& p, int w, int h
                        (const TC*)cost, planes, dmin, dmax, dr, gx, gy, disp, conf, reason, reliable, disp_wta, best,   \
                        second, sum32)
     if (planes.paired) {
-        // one plane per scanline family, each the sum of its two directions (u8 C')
+        // paired families keep one plane (u8 C' only): NP planes for MUL paths
         if (!cost_u8) return hipErrorInvalidValue;
+        const int np = (p.paths == 8 ? 8 : 4) - __builtin_popcount((unsigned)planes.paired);
         if (p.paths == 8) {
-            if (wide) APG_WTAF(uint8_t, 4, true, 8);
-            else APG_WTAF(uint8_t, 4, false, 8);
+            if (np == 5) {
+                if (wide) APG_WTAF(uint8_t, 5, true, 8);
+                else APG_WTAF(uint8_t, 5, false, 8);
+            } else if (np == 4) {
+                if (wide) APG_WTAF(uint8_t, 4, true, 8);
+                else APG_WTAF(uint8_t, 4, false, 8);
+            } else {
+                return hipErrorInvalidValue;
+            }
         } else {
-            if (wide) APG_WTAF(uint8_t, 2, true, 4);
-            else APG_WTAF(uint8_t, 2, false, 4);
+            if (np == 3) {
+                if (wide) APG_WTAF(uint8_t, 3, true, 4);
+                else APG_WTAF(uint8_t, 3, false, 4);
+            } else if (np == 2) {
+                if (wide) APG_WTAF(uint8_t, 2, true, 4);
+                else APG_WTAF(uint8_t, 2, false, 4);
+            } else {
+                return hipErrorInvalidValue;
+            }
         }
     } else if (p.paths == 8) {
         if (wide) APG_WTAF(uint8_t, 8, true, 8);

@@ -135,10 +135,11 @@ struct PlaneSet {
     // batches of columns are known valid; 1: per-pixel ranges (the prior's),
     // validity only from C'
     int general_ranges;
-    // 1: p[f] holds, per scanline family f (rows, columns, the two diagonals),
-    // the sum of its two directions' deltas (both <= P2 <= 127, so the sum
-    // fits u8), followed by kPlaneSpare bytes the SGM may overwrite; 0: p[r]
-    // holds direction r's deltas
+    // bit f set: scanline family f (0 rows, 1 columns, 2 and 3 the diagonals)
+    // has one plane, the sum of its two directions' deltas (both <= P2 <= 127,
+    // so the sum fits u8), followed by kPlaneSpare bytes the SGM may
+    // overwrite; otherwise two, one per direction.  Planes are listed by
+    // family, the two of a plain family forward then backward.
     int paired;
 };
 constexpr size_t kPlaneSpare = 256;
@@ -161,9 +162,10 @@ inline size_t refine_engine_len(const apg_hip_params& p, int w, int h) {
 hipError_t launch_refine(hipStream_t st, const apg_hip_params& p, const CostLut* d_lut, int w, int h,
                          const RefineInputs& in, const int16_t* dmin, const int16_t* dmax, const uint8_t* reliable,
                          float* disp, const uint32_t* eng, size_t eng_len, void* scratch, size_t scratch_bytes);
-// whether launch_sgm_lines pairs each scanline family's two directions into
-// one plane (the half-wave form with max(P2, P1 + 1) <= 127)
-bool sgm_pairs_planes(const apg_hip_params& p, int D, bool cost_u8, const KTune& tn);
+// the scanline families (bit f: 0 rows, 1 columns, 2 and 3 the diagonals)
+// whose two directions launch_sgm_lines sums into one plane (the half-wave
+// form with max(P2, P1 + 1) <= 127), 0 for none
+int sgm_pair_mask(const apg_hip_params& p, int D, bool cost_u8, const KTune& tn);
 hipError_t launch_sgm_lines(hipStream_t s, const apg_hip_params& p, int w, int h, int D, const uint8_t* gray,
                             const void* cost, bool cost_u8, const PlaneSet& planes, const KTune& tn = KTune());
 hipError_t launch_wta_fused(hipStream_t s, const apg_hip_params& p, int w, int h, const void* cost, bool cost_u8,
