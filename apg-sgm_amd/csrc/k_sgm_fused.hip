@@ -708,8 +708,8 @@ __device__ __forceinline__ void sgm_half_body(const apg_hip_params& p, int w, in
     });
 }
 
-// Scanline family f, member b of block blockIdx.x; PAIRED: the families whose
-// bit is set in planes.paired run the paired body, the others the plain one.
+// Scanline family f, member b of block blockIdx.x.  PAIRED: every family's
+// bit is set in planes.paired (sgm_pair_mask pairs all or none).
 template <bool FULL, int PF, bool PAIRED>
 __global__ __launch_bounds__(64) void sgm_half_kernel(apg_hip_params p, int w, int h, int D, int nfam,
                                                       const uint8_t* __restrict__ gray,
@@ -720,8 +720,7 @@ __global__ __launch_bounds__(64) void sgm_half_kernel(apg_hip_params p, int w, i
         const int cnt[4] = {h, w, w + h - 1, w + h - 1};
         while (f < nfam - 1 && b >= cnt[f]) { b -= cnt[f]; ++f; }
     }
-    if (PAIRED && ((planes.paired >> f) & 1)) sgm_half_body<FULL, PF, true>(p, w, h, D, f, b, gray, cost, planes, nmax, sh);
-    else sgm_half_body<FULL, PF, false>(p, w, h, D, f, b, gray, cost, planes, nmax, sh);
+    sgm_half_body<FULL, PF, PAIRED>(p, w, h, D, f, b, gray, cost, planes, nmax, sh);
 }
 
 // Quarter-wave form for u8 C', 4 paths, D % 16 == 0, D <= 128: one wave runs
@@ -937,8 +936,9 @@ int sgm_pair_mask(const apg_hip_params& p, int D, bool cost_u8, const KTune& tn)
     const int p2max = p.P2_base > p.P1 + 1 ? p.P2_base : p.P1 + 1;
     if (!(cost_u8 && D % 16 == 0 && D > 0 && D <= 128 && (tn.sgm_dual == 0 || tn.sgm_dual == 3) && p2max <= 127))
         return 0;
-    // every family: at 1280x720 D=128 the SGM and WTA took 227 + 138 us with
-    // no family paired, 250 + 107 with the columns only, 280 + 73 with all
+    // every family (all or none: the half-wave kernel's PAIRED form assumes
+    // it).  At 1280x720 D=128 the SGM and WTA took 227 + 138 us with no family
+    // paired, 250 + 107 with the columns only, 280 + 73 with all.
     return p.paths == 8 ? 0xF : 0x3;
 }
 
