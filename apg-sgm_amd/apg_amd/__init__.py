@@ -150,6 +150,7 @@ def load():
         "apg_hip_prior_estimate": (i, [vp, P, i, i] + [vp] * 9 + [C.POINTER(i)]),
         "apg_hip_refine": (i, [vp, P, i, i] + [vp] * 14),
         "apg_hip_set_tuning": (i, [vp, C.POINTER(Tuning)]),
+        "apg_hip_set_refine_wait": (i, [vp, C.c_longlong]),
         "apg_hip_get_tuning": (i, [vp, C.POINTER(Tuning)]),
         "apg_hip_batch_create": (i, [i, i, C.POINTER(vp)]),
         "apg_hip_batch_destroy": (None, [vp]),

@@ -67,6 +67,11 @@ struct apg_hip_ctx_s {
     std::vector<float> prior_grid_host;   // source of the last prior grid upload
     std::vector<apg_hip_support> last_supports;   // the last prior's supports
     size_t engine_len = 0;   // std::mt19937(12345) outputs held in B_ENGINE
+    // the refiner's banded sweep: resident bands (CUs), wall-clock ticks per
+    // second, and the test override of its hand-off wait (0: 2 s)
+    int cu_count = 0;
+    long long wall_hz = 0;
+    long long refine_wait = 0;
 };
 
 namespace {
@@ -704,10 +709,29 @@ int run_refine(apg_hip_ctx ctx, const apg_hip_params* p, int w, int h, const uin
     const RefineInputs in{gl, gxl, gyl, gr, gxr, gyr, sym ? (const uint32_t*)cl : nullptr,
                           sym ? (const uint32_t*)cr : nullptr, sym ? nullptr : (const uint64_t*)cl,
                           sym ? nullptr : (const uint64_t*)cr};
+    if (!ctx->cu_count) {
+        int cus = 0, khz = 0;
+        HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device), "device attribute");
+        HIPCHK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ctx->device), "device attribute");
+        ctx->cu_count = cus > 0 ? cus : 1;
+        ctx->wall_hz = khz > 0 ? (long long)khz * 1000 : 100000000LL;
+    }
+    // the banded sweep's hand-offs: a bounded wait (2 s of wall clock), whose
+    // give-up is reported below; under stream capture the result cannot be
+    // read back, so the single-workgroup sweep runs instead
+    const RefineBanding bd{ctx->capturing ? 0 : 1, ctx->cu_count,
+                           ctx->refine_wait ? ctx->refine_wait : 2 * ctx->wall_hz};
+    uint32_t* band_err = nullptr;
     HIPCHK(launch_refine(st, *p, cur_lut(ctx), w, h, in, dmin, dmax, reliable, disp,
                          buf<uint32_t>(ctx, B_ENGINE), ctx->engine_len, ctx->bufs[B_REFINE].ptr,
-                         ctx->bufs[B_REFINE].bytes),
+                         ctx->bufs[B_REFINE].bytes, bd, &band_err),
            "refine");
+    if (band_err) {
+        uint32_t gave_up = 0;
+        HIPCHK(hipMemcpyAsync(&gave_up, band_err, 4, hipMemcpyDeviceToHost, st), "refine status");
+        HIPCHK(hipStreamSynchronize(st), "refine status");
+        if (gave_up) return fail(ctx, 8, "refiner: a band's hand-off wait gave up");
+    }
     return 0;
 }
 
@@ -933,6 +957,12 @@ size_t apg_hip_pipeline_workspace_bytes(const apg_hip_params* p, int w, int h) {
     for (size_t b : pl.bytes)
         if (b) t += b < 256 ? 256 : b;
     return t;
+}
+
+int apg_hip_set_refine_wait(apg_hip_ctx ctx, long long ticks) {
+    if (!ctx) return 8;
+    ctx->refine_wait = ticks;
+    return 0;
 }
 
 int apg_hip_set_tuning(apg_hip_ctx ctx, const apg_hip_tuning* t) {

@@ -22,6 +22,7 @@
 //     sweeps), so one workgroup walks the anti-diagonals x + y = s, a thread
 //     per row, the upper neighbours' results passed through LDS.
 #include <algorithm>
+#include <utility>
 
 #include <hipcub/hipcub.hpp>
 
@@ -31,6 +32,16 @@
 namespace apgk {
 
 namespace {
+
+// f(integral_constant<int, 0>) ... f(integral_constant<int, N-1>)
+template <typename F, int... I>
+__device__ __forceinline__ void static_for_b_impl(F&& f, std::integer_sequence<int, I...>) {
+    (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, typename F>
+__device__ __forceinline__ void static_for_b(F&& f) {
+    static_for_b_impl(f, std::make_integer_sequence<int, N>{});
+}
 
 // a neighbour with no candidate (the reference pushes only values >= 0, so
 // any negative value means none for a neighbour; the own and jitter
@@ -260,20 +271,306 @@ __global__ __launch_bounds__(kSweepThreads) void refine_sweep_kernel(const CostL
     }
 }
 
+// Banded form of the same sweep: the rows of the sweep order are split into
+// bands of 64, one wavefront (one workgroup) per band, lane l holding row
+// b * 64 + l.  At local step k lane l handles sweep column xs = k - l, so the
+// band's anti-diagonals advance one per step; a pixel's upper neighbour is the
+// lane below's result of the previous step (a DPP shift), its left neighbour
+// the lane's own; no barrier is needed.
+//  * Records are band-major, slot ((b * kb + k) << 6) + l, written by
+//    refine_band_static_kernel for every slot (slots outside the image hold a
+//    record whose own candidate always wins with value -1), so each step
+//    loads a coalesced block at a wave-uniform base.  A record carries the
+//    static candidates already resolved: cost sc, float value sd and its
+//    rounded value sdi of the winner between the own value and the jitter
+//    (a jitter winner's cost is stored as the next float up, so that a
+//    neighbour with an equal cost, which the reference tries first, wins).
+//  * The neighbours' values travel as rounded integers (negative: none), so
+//    the sweep never rounds.
+//  * A band's first row takes its upper neighbours from the band above's last
+//    row through `edge`: lane 63 stores each result, tagged (bit 31 set), with
+//    an agent-scope relaxed atomic store; the band below loads it, also
+//    relaxed and agent-scope, PD steps ahead of its use, and re-polls only if
+//    the tag is not there yet.  Each value is its own flag, so no fence is
+//    needed.  The wait is bounded (bs.timeout ticks of the wall clock from the
+//    kernel's start and an iteration cap per wait); a band that gives up sets
+//    *bs.err, takes no neighbour from above for the rest of the sweep and
+//    runs to its end, so every wave exits.  A negative timeout makes every
+//    band but the first give up at once.  Bands wait only on bands with
+//    lower indices, which each XCD dispatches first, and the grid holds at
+//    most one band per CU.
+//  * The right-view terms a pixel's candidates need, columns x - d for d in
+//    [d0, d0 + D), sit in an LDS ring per row (RS = max(64, 2^ceil(log2 D))
+//    entries of 8 bytes): the window slides by one column per step, the
+//    entering column loaded PD steps ahead from a copy of the right-view
+//    terms padded by kBandPadCols entries at both ends, so every load is in
+//    bounds without a clamp.  Columns outside the image hold another row's
+//    terms; no candidate reads them (their cost is the out-of-range 1e6).
+//    Lanes before the start of their row write the columns their window will
+//    hold anyway (RS >= 64 keeps those writes off the other columns' slots).
+constexpr int kBandRows = 64;
+constexpr int kBandPD = 8;          // steps of prefetch (records, entering columns, edge values)
+constexpr int kBandSlack = 96;      // local steps past w in a band's slots: the skew (63) and two prefetch spans
+constexpr int kBandPadCols = 2048;  // right-view terms padding at both ends
+
+__host__ __device__ constexpr int band_steps(int w) { return w + kBandSlack; }
+
+// slot records: A = {sc, sd, sdi, lo}, B = {hi - 1, census, gray, gx << 8 | gy << 16}
+__global__ void refine_band_static_kernel(const CostLut* __restrict__ lut, int w, int h, int fwd, int r, int kb,
+                                          int use_ad, int use_grad, const uint4* __restrict__ lrec,
+                                          const uint4* __restrict__ rrec, const uint8_t* __restrict__ reliable,
+                                          const int16_t* __restrict__ dmin, const int16_t* __restrict__ dmax,
+                                          const float* __restrict__ disp, const uint32_t* __restrict__ eng,
+                                          uint32_t eng_last, const uint32_t* __restrict__ base,
+                                          const uint32_t* __restrict__ acc, const uint32_t* __restrict__ didx,
+                                          size_t nslots, float4* __restrict__ ra, uint4* __restrict__ rb) {
+    const size_t sl = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (sl >= nslots) return;
+    const int l = (int)(sl & 63);
+    const size_t bk = sl >> 6;
+    const int k = (int)(bk % (size_t)kb), b = (int)(bk / (size_t)kb);
+    const int xs = k - l, ys = b * kBandRows + l;
+    if (xs < 0 || xs >= w || ys >= h) {
+        ra[sl] = make_float4(-1.f, -1.f, __int_as_float(-1), __int_as_float(0));
+        rb[sl] = make_uint4(0u, 0u, 0u, 0u);
+        return;
+    }
+    const int x = fwd ? xs : w - 1 - xs, y = fwd ? ys : h - 1 - ys;
+    const size_t q = (size_t)y * w + x;
+    const int lo = dmin[q], hi = dmax[q];
+    const float cur = disp[q];
+    const uint4 lt = lrec[q];
+    auto cost = [&](int d) {
+        const int xr = x - d;
+        if (xr < 0 || xr >= w || d < lo || d >= hi) return 1e6f;
+        return rec_cost(lt, rrec[(size_t)y * w + xr], true, use_ad, use_grad, lut->census, lut->ad, lut->grad);
+    };
+    float sc, sd;
+    int sdi;
+    if (reliable[q] || hi <= lo) {   // not swept: the value stays, nothing beats it
+        sc = -1.f;
+        sd = cur;
+        sdi = cur >= 0.f ? (int)roundf(cur) : -1;
+    } else {
+        float best = 1e9f;
+        int bd = cur >= 0.f ? clampi((int)roundf(cur), lo, hi - 1) : lo;
+        bool jit_wins = false;
+        if (cur >= 0.f) {
+            const int dc = clampi((int)roundf(cur), lo, hi - 1);
+            best = cost(dc);
+            bd = dc;
+            const uint32_t s = 2u * (uint32_t)r + 1u;
+            const uint32_t e = eng[min(*base + acc[didx[sweep_index(fwd, w, h, x, y)]], eng_last)];
+            const int jit = (int)(((uint64_t)e * s) >> 32) - r;
+            const float dj = cur + (float)jit;
+            if (!(dj < 0.f)) {
+                const int d = clampi((int)roundf(dj), lo, hi - 1);
+                const float cj = cost(d);
+                if (cj < best) {
+                    best = cj;
+                    bd = d;
+                    jit_wins = true;
+                }
+            }
+        }
+        // the jitter is tried after the neighbours, so they win its ties
+        // (costs are finite and >= 0: the next float up is the next bit pattern)
+        sc = jit_wins ? __int_as_float(__float_as_int(best + 0.f) + 1) : best;
+        sd = (float)bd;
+        sdi = bd;
+    }
+    ra[sl] = make_float4(sc, sd, __int_as_float(sdi), __int_as_float(lo));
+    rb[sl] = make_uint4((uint32_t)(hi - 1), lt.x, lt.z & 0xFFu, lt.z & 0xFFFF00u);
+}
+
+// the right view's terms (census, gray | gx << 8 | gy << 16) at [kBandPadCols + q]
+__global__ void refine_band_terms_kernel(int n, const uint4* __restrict__ rrec, uint2* __restrict__ rt) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const uint4 r = rrec[q];
+    rt[kBandPadCols + q] = make_uint2(r.x, r.z);
+}
+
+// sweep results, band-major, back to the image
+__global__ void refine_band_unpack_kernel(int w, int h, int fwd, int kb, const float* __restrict__ res,
+                                          float* __restrict__ disp) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= w * h) return;
+    const int x = q % w, y = q / w;
+    const int xs = fwd ? x : w - 1 - x, ys = fwd ? y : h - 1 - y;
+    const int b = ys / kBandRows, l = ys % kBandRows;
+    disp[q] = res[(((size_t)b * kb + (size_t)(xs + l)) << 6) + l];
+}
+
+struct BandSync {
+    uint32_t* edge;     // [bands + 1][kb]: row b + 1 the tagged last-row values of band b (zeroed before the sweep)
+    uint32_t* err;      // set when a wait gave up
+    long long timeout;  // wall-clock ticks the waits may take; < 0: give up at once
+};
+
+template <int PD, bool AD, bool GRAD>
+__global__ __launch_bounds__(kBandRows) void refine_band_kernel(const CostLut* __restrict__ lut, int w, int h,
+                                                                 int fwd, int d0, int D, int rs_log2, int kb,
+                                                                 int krun, const float4* __restrict__ ra,
+                                                                 const uint4* __restrict__ rb,
+                                                                 const uint2* __restrict__ rt,
+                                                                 float* __restrict__ out, BandSync bs) {
+    extern __shared__ __align__(16) unsigned char shb[];
+    float* s_lc = (float*)shb;
+    float* s_la = s_lc + 65;
+    float* s_lg = s_la + 256;
+    const int RS = 1 << rs_log2;
+    const int lane = threadIdx.x, b = blockIdx.x;
+    const int nb = gridDim.x;
+    uint2* wrow = (uint2*)(shb + 3328) + lane * RS;   // the lane's window ring
+    {
+        constexpr int NLUT = 65 + 256 + 511, NL = (NLUT + 63) / 64;
+        float lv[NL];
+#pragma unroll
+        for (int k = 0; k < NL; ++k) lv[k] = lut->census[min(lane + 64 * k, NLUT - 1)];
+#pragma unroll
+        for (int k = 0; k < NL; ++k)
+            if (lane + 64 * k < NLUT) s_lc[lane + 64 * k] = lv[k];
+    }
+    const int ys = b * kBandRows + lane;
+    const int yc = min(ys, h - 1);
+    const int y = fwd ? yc : h - 1 - yc;
+    // right-view terms of column c of the lane's row: rtp[c] (c in [-kBandPadCols, w + kBandPadCols))
+    const uint2* rtp = rt + kBandPadCols + (size_t)y * w;
+    // the window of the row's first pixel: columns [x0 - d0 - D + 1, x0 - d0]
+    {
+        const int x0 = fwd ? 0 : w - 1;
+        const int c0 = x0 - d0 - D + 1;
+        for (int j0 = 0; j0 < D; j0 += 32) {
+            uint2 t[32];
+#pragma unroll
+            for (int j = 0; j < 32; ++j) t[j] = rtp[c0 + min(j0 + j, D - 1)];
+#pragma unroll
+            for (int j = 0; j < 32; ++j)
+                if (j0 + j < D) wrow[(c0 + j0 + j) & (RS - 1)] = t[j];
+        }
+    }
+    // entering column at step k: fwd x - d0 with x = k - lane, backward
+    // x - d0 - D + 1 with x = w - 1 - (k - lane); its offset in rt is
+    // col_lane + k (fwd) or col_lane - k (backward)
+    const uint32_t col_lane = (uint32_t)(kBandPadCols + y * w + (fwd ? -lane - d0 : w - d0 - D + lane));
+    const uint32_t col_b = col_lane * 8u;   // the padded terms stay below 4 GB
+    const size_t band0 = (size_t)b * kb;
+    const float4* ra_b = ra + (band0 << 6);
+    const uint4* rb_b = rb + (band0 << 6);
+    float* out_b = out + (band0 << 6);
+    const uint32_t* edge_in = bs.edge + (size_t)b * kb;         // written by band b - 1
+    uint32_t* edge_out = bs.edge + (size_t)(b + 1) * kb;        // read by band b + 1
+    float4 pa[PD];
+    uint4 pb[PD];
+    uint2 pc[PD];
+    uint32_t pe[PD];
+    auto issue = [&](int k, int q) {
+        pa[q] = ra_b[((size_t)k << 6) + lane];
+        pb[q] = rb_b[((size_t)k << 6) + lane];
+        pc[q] = *(const uint2*)((const char*)(fwd ? rt + k : rt - k) + col_b);
+        pe[q] = __hip_atomic_load(edge_in + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    };
+#pragma unroll
+    for (int j = 0; j < PD; ++j) issue(j, j);
+    const long long t_start = (long long)wall_clock64();
+    bool give_up = bs.timeout < 0 && b > 0;
+    int ri_prev = -1;                                  // the lane's result of the previous step, rounded
+    int x = fwd ? -lane : w - 1 + lane;               // the lane's image column at step 0
+    const int xdir = fwd ? 1 : -1;
+    const int ent = fwd ? -d0 : -d0 - D + 1;          // entering column - x
+    for (int k0 = 0; k0 < krun; k0 += PD) {
+        static_for_b<PD>([&](auto j_t) {
+            constexpr int q = decltype(j_t)::value;
+            const int k = k0 + q;
+            const float4 A = pa[q];
+            const uint4 B = pb[q];
+            const uint2 C = pc[q];
+            // the window takes the entering column
+            wrow[(x + ent) & (RS - 1)] = C;
+            // lane 0's upper neighbour: column k of the band above's last row
+            int up0 = -1;
+            if (b > 0 && k < w && !give_up) {
+                uint32_t e = (uint32_t)__builtin_amdgcn_readfirstlane((int)pe[q]);
+                if (!(e >> 31)) {
+                    for (int it = 0; it < (1 << 22); ++it) {
+                        e = (uint32_t)__builtin_amdgcn_readfirstlane((int)__hip_atomic_load(
+                            edge_in + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+                        if ((e >> 31) || (long long)wall_clock64() - t_start > bs.timeout) break;
+                        __builtin_amdgcn_s_sleep(1);
+                    }
+                    if (!(e >> 31)) give_up = true;
+                }
+                if (e >> 31) up0 = (int)(e & 0x7FFFFFFFu) - 1;
+            }
+            const int up = __builtin_amdgcn_update_dpp(up0, ri_prev, kDppWaveShr1, 0xF, 0xF, false);
+            const float sc = A.x, sd = A.y;
+            const int sdi = __float_as_int(A.z), lo = __float_as_int(A.w), hm1 = (int)B.x;
+            auto cand = [&](int nbv, float& c, int& nd) {
+                nd = min(max(nbv, lo), hm1);
+                const int xr = x - nd;
+                const uint2 t = wrow[xr & (RS - 1)];
+                float v = s_lc[__popc(B.y ^ t.x)];
+                if (AD) v += s_la[__builtin_amdgcn_sad_u8(B.z, t.y & 0xFFu, 0u)];
+                if (GRAD) v += s_lg[__builtin_amdgcn_sad_u8(B.w, t.y & 0xFFFF00u, 0u)];
+                v = (uint32_t)xr < (uint32_t)w ? v : 1e6f;
+                c = nbv >= 0 ? v : __builtin_inff();
+            };
+            float cl, cu;
+            int nl, nu;
+            cand(ri_prev, cl, nl);
+            cand(up, cu, nu);
+            // candidates in the reference's order: own value, the neighbour
+            // along the row, the neighbour across rows, the jitter
+            const bool wl = cl < sc;
+            const float b1 = wl ? cl : sc;
+            int ri = wl ? nl : sdi;
+            const bool wu = cu < b1;
+            ri = wu ? nu : ri;
+            const float res = (wl || wu) ? (float)ri : sd;
+            ri_prev = ri;
+            out_b[((size_t)k << 6) + lane] = res;
+            if (lane == kBandRows - 1 && b + 1 < nb && k >= kBandRows - 1 && k - (kBandRows - 1) < w)
+                __hip_atomic_store(edge_out + (k - (kBandRows - 1)), (uint32_t)(ri + 1) | 0x80000000u,
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            x += xdir;
+            issue(k + PD, q);
+        });
+    }
+    if (give_up && lane == 0) __hip_atomic_store(bs.err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 }  // namespace
 
 size_t refine_scratch_bytes(int w, int h) {
     const size_t n = (size_t)w * h, nw = n + kRefineSlack;
     size_t temp = 0;
     (void)hipcub::DeviceScan::ExclusiveSum(nullptr, temp, (const uint32_t*)nullptr, (uint32_t*)nullptr, (int)nw);
-    // flags, draw indices, sweep results, acceptance flags, their positions,
-    // accepted offsets, left and right terms, sweep records, engine base, scan temp
-    return 3 * n * 4 + 3 * nw * 4 + 2 * n * 16 + n * sizeof(SweepRec) + 256 + ((temp + 255) & ~(size_t)255);
+    const size_t nbands = (size_t)(h + kBandRows - 1) / kBandRows;
+    const size_t kb = (size_t)band_steps(w), slots = nbands * kb * kBandRows;
+    auto r256 = [](size_t v) { return (v + 255) & ~(size_t)255; };
+    // flags, draw indices, sweep results (image or band-major), acceptance
+    // flags, their positions, accepted offsets, left and right terms, the
+    // sweep records (diagonal or band-major), padded right terms, engine base,
+    // edge values and the give-up flag, scan temp
+    return r256(2 * n * 4) + r256(std::max(n, slots) * 4) + r256(3 * nw * 4) + r256(2 * n * 16) +
+           r256(std::max(n * sizeof(SweepRec), slots * 32)) + r256((n + 2 * (size_t)kBandPadCols) * 8) + 256 +
+           r256(((nbands + 1) * kb + 1) * 4) + r256(temp);
+}
+
+template <int PD>
+using BandKernel = void (*)(const CostLut*, int, int, int, int, int, int, int, int, const float4*, const uint4*,
+                            const uint2*, float*, BandSync);
+template <int PD>
+BandKernel<PD> band_kernel(int use_ad, int use_grad) {
+    if (use_ad) return use_grad ? refine_band_kernel<PD, true, true> : refine_band_kernel<PD, true, false>;
+    return use_grad ? refine_band_kernel<PD, false, true> : refine_band_kernel<PD, false, false>;
 }
 
 hipError_t launch_refine(hipStream_t st, const apg_hip_params& p, const CostLut* d_lut, int w, int h,
                          const RefineInputs& in, const int16_t* dmin, const int16_t* dmax, const uint8_t* reliable,
-                         float* disp, const uint32_t* eng, size_t eng_len, void* scratch, size_t scratch_bytes) {
+                         float* disp, const uint32_t* eng, size_t eng_len, void* scratch, size_t scratch_bytes,
+                         const RefineBanding& bd, uint32_t** band_err) {
     const size_t n = (size_t)w * h, nw = n + kRefineSlack;
     if (h > kSweepThreads * kSweepRows || n >= (1u << 31)) return hipErrorInvalidValue;
     if (scratch_bytes < refine_scratch_bytes(w, h)) return hipErrorInvalidValue;
@@ -281,17 +578,31 @@ hipError_t launch_refine(hipStream_t st, const apg_hip_params& p, const CostLut*
     if (!eng || !scratch || eng_len < (size_t)(iters + 1) * nw) return hipErrorInvalidValue;
     // engine offsets (base, acc, the clamp index) are 32-bit
     if (eng_len > 0xFFFFFFFFull) return hipErrorInvalidValue;
+    const int nbands = (h + kBandRows - 1) / kBandRows;
+    const int kb = band_steps(w);
+    const size_t slots = (size_t)nbands * kb * kBandRows;
+    auto r256 = [](size_t v) { return (v + 255) & ~(size_t)255; };
     uint8_t* sp = (uint8_t*)scratch;
-    uint32_t* dflag = (uint32_t*)sp; sp += n * 4;
-    uint32_t* didx = (uint32_t*)sp; sp += n * 4;
-    float* sres = (float*)sp; sp += n * 4;
-    uint32_t* aflag = (uint32_t*)sp; sp += nw * 4;
-    uint32_t* apos = (uint32_t*)sp; sp += nw * 4;
-    uint32_t* acc = (uint32_t*)sp; sp += nw * 4;
-    uint4* lrec = (uint4*)sp; sp += n * 16;
-    uint4* rrec = (uint4*)sp; sp += n * 16;
-    SweepRec* srec = (SweepRec*)sp; sp += n * sizeof(SweepRec);
+    uint32_t* dflag = (uint32_t*)sp;
+    uint32_t* didx = dflag + n;
+    sp += r256(2 * n * 4);
+    float* sres = (float*)sp; sp += r256(std::max(n, slots) * 4);
+    uint32_t* aflag = (uint32_t*)sp;
+    uint32_t* apos = aflag + nw;
+    uint32_t* acc = apos + nw;
+    sp += r256(3 * nw * 4);
+    uint4* lrec = (uint4*)sp;
+    uint4* rrec = lrec + n;
+    sp += r256(2 * n * 16);
+    SweepRec* srec = (SweepRec*)sp;           // single-workgroup sweep: records in diagonal order
+    float4* bra = (float4*)sp;                // banded sweep: band-major records
+    uint4* brb = (uint4*)(bra + slots);
+    sp += r256(std::max(n * sizeof(SweepRec), slots * 32));
+    uint2* rt = (uint2*)sp; sp += r256((n + 2 * (size_t)kBandPadCols) * 8);
     uint32_t* base = (uint32_t*)sp; sp += 256;
+    uint32_t* edge = (uint32_t*)sp;
+    uint32_t* berr = edge + (size_t)(nbands + 1) * kb;
+    sp += r256(((size_t)(nbands + 1) * kb + 1) * 4);
     void* temp = sp;
     size_t temp_bytes = scratch_bytes - (size_t)(sp - (uint8_t*)scratch);
     const int sym = in.c32l ? 1 : 0;
@@ -303,6 +614,28 @@ hipError_t launch_refine(hipStream_t st, const apg_hip_params& p, const CostLut*
     hipLaunchKernelGGL(refine_terms_kernel, dim3(gb), dim3(256), 0, st, (int)n, in.gr, in.gxr, in.gyr, in.c32r,
                        in.c64r, rrec);
     const size_t shm = (65 + 256 + 511 + 2 * (size_t)h) * sizeof(float);
+    // the banded sweep: 32-bit census, D <= 256 (a row's window in LDS), the
+    // entering columns within the padding, one band per CU at most (all bands
+    // resident)
+    const int D = p.max_disparity - p.min_disparity, d0 = p.min_disparity;
+    int rs_log2 = 6;
+    while ((1 << rs_log2) < D) ++rs_log2;
+    const size_t bshm = 3328 + (size_t)kBandRows * ((size_t)1 << rs_log2) * 8;
+    const bool banded = bd.enable && sym && D > 0 && D <= 256 && std::abs(d0) + D + 2 * kBandSlack <= kBandPadCols &&
+                        n + 2 * (size_t)kBandPadCols < ((size_t)1 << 29) &&
+                        nbands <= bd.max_bands;
+    const BandKernel<kBandPD> bk = band_kernel<kBandPD>(p.use_ad, p.use_grad);
+    if (banded) {
+        if (bshm > 64 * 1024 &&
+            (e = hipFuncSetAttribute((const void*)bk, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bshm)) !=
+                hipSuccess)
+            return e;
+        hipLaunchKernelGGL(refine_band_terms_kernel, dim3(gb), dim3(256), 0, st, (int)n, rrec, rt);
+    }
+    if (band_err) *band_err = banded ? berr : nullptr;
+    // a band runs whole prefetch spans: the last lane's last column is step w + 62
+    const int krun = (w + kBandRows - 1 + kBandPD - 1) / kBandPD * kBandPD;
+    const unsigned gs = (unsigned)((slots + 255) / 256);
     for (int it = 0; it < iters; ++it) {
         const int fwd = (it % 2) == 0 ? 1 : 0;
         const int r = std::max(1, p.refine_random_radius >> it);
@@ -313,12 +646,23 @@ hipError_t launch_refine(hipStream_t st, const apg_hip_params& p, const CostLut*
         hipLaunchKernelGGL(jitter_accept_kernel, dim3(gw), dim3(256), 0, st, eng, base, nw, s, t, aflag);
         if ((e = hipcub::DeviceScan::ExclusiveSum(temp, temp_bytes, aflag, apos, (int)nw, st)) != hipSuccess) return e;
         hipLaunchKernelGGL(jitter_compact_kernel, dim3(gw), dim3(256), 0, st, aflag, apos, nw, acc);
-        hipLaunchKernelGGL(refine_static_kernel, dim3(gb), dim3(256), 0, st, d_lut, w, h, fwd, r, sym, p.use_ad,
-                           p.use_grad, lrec, rrec, reliable, dmin, dmax, disp, eng, (uint32_t)(eng_len - 1), base,
-                           acc, didx, srec);
-        hipLaunchKernelGGL(refine_sweep_kernel, dim3(1), dim3(kSweepThreads), shm, st, d_lut, w, h, fwd, sym,
-                           p.use_ad, p.use_grad, srec, rrec, sres);
-        hipLaunchKernelGGL(refine_unpack_kernel, dim3(gb), dim3(256), 0, st, w, h, fwd, sres, disp);
+        if (banded) {
+            hipLaunchKernelGGL(refine_band_static_kernel, dim3(gs), dim3(256), 0, st, d_lut, w, h, fwd, r, kb,
+                               p.use_ad, p.use_grad, lrec, rrec, reliable, dmin, dmax, disp, eng,
+                               (uint32_t)(eng_len - 1), base, acc, didx, slots, bra, brb);
+            if ((e = launch_fill_u32(st, (size_t)(nbands + 1) * kb + 1, 0u, edge)) != hipSuccess) return e;
+            const BandSync bs{edge, berr, bd.wait_ticks};
+            hipLaunchKernelGGL(bk, dim3(nbands), dim3(kBandRows), bshm, st, d_lut, w, h, fwd, d0, D, rs_log2, kb, krun,
+                               bra, brb, rt, sres, bs);
+            hipLaunchKernelGGL(refine_band_unpack_kernel, dim3(gb), dim3(256), 0, st, w, h, fwd, kb, sres, disp);
+        } else {
+            hipLaunchKernelGGL(refine_static_kernel, dim3(gb), dim3(256), 0, st, d_lut, w, h, fwd, r, sym, p.use_ad,
+                               p.use_grad, lrec, rrec, reliable, dmin, dmax, disp, eng, (uint32_t)(eng_len - 1), base,
+                               acc, didx, srec);
+            hipLaunchKernelGGL(refine_sweep_kernel, dim3(1), dim3(kSweepThreads), shm, st, d_lut, w, h, fwd, sym,
+                               p.use_ad, p.use_grad, srec, rrec, sres);
+            hipLaunchKernelGGL(refine_unpack_kernel, dim3(gb), dim3(256), 0, st, w, h, fwd, sres, disp);
+        }
         hipLaunchKernelGGL(jitter_advance_kernel, dim3(1), dim3(1), 0, st, base, dflag, didx, n, acc);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }

@@ -159,9 +159,18 @@ size_t refine_scratch_bytes(int w, int h);
 inline size_t refine_engine_len(const apg_hip_params& p, int w, int h) {
     return (size_t)(p.refine_iterations + 1) * ((size_t)w * h + kRefineSlack);
 }
+// The banded sweep (64-row bands, one wave each, hand-offs through HBM with
+// a bounded wait) when enable is set and it applies; wait_ticks: wall-clock
+// ticks a hand-off may wait, < 0 gives up at the first wait (for tests).
+struct RefineBanding {
+    int enable;
+    int max_bands;          // bands that can be resident together (one per CU)
+    long long wait_ticks;
+};
 hipError_t launch_refine(hipStream_t st, const apg_hip_params& p, const CostLut* d_lut, int w, int h,
                          const RefineInputs& in, const int16_t* dmin, const int16_t* dmax, const uint8_t* reliable,
-                         float* disp, const uint32_t* eng, size_t eng_len, void* scratch, size_t scratch_bytes);
+                         float* disp, const uint32_t* eng, size_t eng_len, void* scratch, size_t scratch_bytes,
+                         const RefineBanding& bd, uint32_t** band_err);
 // the scanline families (bit f: 0 rows, 1 columns, 2 and 3 the diagonals)
 // whose two directions launch_sgm_lines sums into one plane (the half-wave
 // form with max(P2, P1 + 1) <= 127), 0 for none

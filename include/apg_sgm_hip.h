@@ -352,6 +352,14 @@ typedef struct {
 
 /* NULL restores the built-in choice.  Applies to later pipeline calls. */
 APG_HIP_API int apg_hip_set_tuning(apg_hip_ctx ctx, const apg_hip_tuning* t);
+
+/* Hand-off wait of the refiner's banded sweep, in wall-clock ticks of the
+ * device (0: 2 seconds; negative: give up at the first wait).  A band whose
+ * wait gives up ends its sweep, the bands after it give up in turn, and the
+ * pipeline call returns APG_STATUS_INTERNAL_ERROR (8).  No counterpart in
+ * the reference (its refiner runs on one host thread); for tests of the
+ * give-up path. */
+APG_HIP_API int apg_hip_set_refine_wait(apg_hip_ctx ctx, long long ticks);
 APG_HIP_API int apg_hip_get_tuning(apg_hip_ctx ctx, apg_hip_tuning* out);
 
 /* ---- batched form ------------------------------------------------------ */

@@ -34,6 +34,14 @@ CASES = [
     # heights around one wavefront of rows
     (1, 90, 64, 24, {}),
     (0, 90, 65, 24, {}),
+    # the banded sweep (64-row bands handing their last row over column by
+    # column): narrow, many bands, D = 256 windows, each cost-term combination
+    (0, 20, 300, 16, {}),
+    (2, 97, 1000, 32, {}),
+    (1, 200, 300, 256, {}),
+    (2, 160, 200, 64, {"use_ad": 0}),
+    (0, 150, 130, 48, {"use_grad": 0}),
+    (1, 140, 70, 32, {"use_ad": 0, "use_grad": 0, "min_disparity": 5}),
 ]
 
 KEYS = ["cost", "cost_agg", "disp_right", "disp_wta", "best", "second", "dmin", "dmax", "d_prior", "after_refine",
@@ -121,3 +129,25 @@ def test_hq_negative_ranges_match_oracle(ctx, shift, maxd, over):
     for k in KEYS:
         _cmp(getattr(g, k), o[k], k)
 
+
+
+@pytest.mark.gpu
+def test_refiner_band_wait_gives_up_and_recovers(ctx):
+    """The banded sweep's hand-off wait, made to give up by the host: the call
+    fails with status 8 and every wave exits (the next call, with the default
+    wait, runs bit-exact against the oracle)."""
+    p = A.params(2, 64)
+    left, right, _ = A.synth_pair(0, 200, 150, 64, 3)
+    print("refiner band wait: forced give-up", flush=True)
+    ctx._check(ctx.lib.apg_hip_set_refine_wait(ctx.h, -1), "set_refine_wait")
+    try:
+        with pytest.raises(A.ApgError) as ei:
+            A.run_pipeline(ctx, p, left, right)
+        assert ei.value.code == 8
+    finally:
+        ctx._check(ctx.lib.apg_hip_set_refine_wait(ctx.h, 0), "set_refine_wait")
+    print("refiner band wait: default", flush=True)
+    g = A.run_pipeline(ctx, p, left, right)
+    o = O.pipeline(O.cfg_from(p.as_dict()), left, right)
+    for k in ("disparity", "invalid_reason", "reliable"):
+        assert np.array_equal(getattr(g, k), o[k], equal_nan=True), k
