@@ -64,8 +64,12 @@ struct apg_hip_ctx_s {
     int schedule = 0;   // 0 automatic (fused when eligible), 1 stage-by-stage
     KTune tune;
     bool capturing = false;   // stream capture in progress: no allocation, no host copy
-    std::vector<float> prior_grid_host;   // source of the last prior grid upload
-    std::vector<apg_hip_support> last_supports;   // the last prior's supports
+    // the last prior's supports: read back from its device layout on the
+    // first request after it ran
+    std::vector<apg_hip_support> last_supports;
+    bool supports_pending = false;
+    PriorLayout prior_lay{};
+    int prior_sw = 0;
     size_t engine_len = 0;   // std::mt19937(12345) outputs held in B_ENGINE
     // the refiner's banded sweep: resident bands (CUs), wall-clock ticks per
     // second, and the test override of its hand-off wait (0: 2 s)
@@ -216,11 +220,10 @@ WsPlan plan_workspace(const apg_hip_params& p, int w, int h, const CostLut& lut,
     }
     if (p.prior_enable) {
         const size_t ns = (size_t)prior_samples_x(w) * prior_samples_y(h);
-        const size_t nc = (size_t)((w + kPriorCell - 1) / kPriorCell) * ((h + kPriorCell - 1) / kPriorCell);
         pl.bytes[B_PRIOR] = n * 4;
         pl.bytes[B_CAND_D] = ns * 2;
         pl.bytes[B_CAND_U] = ns * 4;
-        pl.bytes[B_GRID] = nc * 5 * 4;
+        pl.bytes[B_GRID] = prior_work_bytes(w, h, p.prior_max_supports);
     }
     // the fused schedule assumes the dense range init and runs confidence and
     // the LR check in one kernel, so the prior and the refiner (which sits
@@ -596,6 +599,28 @@ int apg_hip_prior_estimate(apg_hip_ctx ctx, const apg_hip_params* p, int w, int 
 
 int apg_hip_last_supports(apg_hip_ctx ctx, apg_hip_support* out, int capacity, int* count) {
     if (!ctx) return 8;
+    if (ctx->supports_pending) {
+        // each cell's first taken[] candidates, cells in index order
+        if (ctx->capturing) return fail(ctx, 8, "the prior's supports cannot be read back under stream capture");
+        const PriorLayout& L = ctx->prior_lay;
+        const size_t nc = (size_t)L.ncell, ns = nc * (size_t)L.per_cell;
+        std::vector<int> taken(nc), sidx(ns);
+        std::vector<float> key(ns), wt(ns);
+        hipStream_t st = ctx->stream;
+        HIPCHK(hipMemcpyAsync(taken.data(), L.taken, nc * 4, hipMemcpyDeviceToHost, st), "supports download");
+        HIPCHK(hipMemcpyAsync(sidx.data(), L.sidx, ns * 4, hipMemcpyDeviceToHost, st), "supports download");
+        HIPCHK(hipMemcpyAsync(key.data(), L.key, ns * 4, hipMemcpyDeviceToHost, st), "supports download");
+        HIPCHK(hipMemcpyAsync(wt.data(), L.wt, ns * 4, hipMemcpyDeviceToHost, st), "supports download");
+        HIPCHK(hipStreamSynchronize(st), "supports download");
+        ctx->last_supports.clear();
+        for (size_t i = 0; i < nc; ++i)
+            for (int k = 0; k < taken[i]; ++k) {
+                const size_t o = i * (size_t)L.per_cell + (size_t)k;
+                ctx->last_supports.push_back(apg_hip_support{2 + 2 * (sidx[o] % ctx->prior_sw),
+                                                             2 + 2 * (sidx[o] / ctx->prior_sw), key[o], wt[o]});
+            }
+        ctx->supports_pending = false;
+    }
     const int n = (int)ctx->last_supports.size();
     if (count) *count = n;
     if (out && capacity > 0) std::copy_n(ctx->last_supports.begin(), std::min(n, capacity), out);
@@ -632,51 +657,42 @@ hipError_t download_vec(hipStream_t st, std::vector<T>& v, const void* d, size_t
 }  // namespace
 
 // PriorEstimator::estimate (reference src/prior_estimator.cpp:414-435) after
-// the dense range init: device candidates, host support selection and cell
-// grid, device per-pixel prior and ranges.  d_prior is -1 where there is no
-// prior.  Synchronises the context stream once.
+// the dense range init, on the device: per-sample candidates, support
+// selection, cell grid, per-pixel prior and ranges.  d_prior is -1 where
+// there is no prior.  Synchronises only when n_supports is requested.
 int run_prior(apg_hip_ctx ctx, const apg_hip_params* p, int w, int h, const uint8_t* gxl, const uint8_t* gyl,
               const void* cl, const void* cr, int16_t* dmin, int16_t* dmax, float* d_prior, int* n_supports) {
     hipStream_t st = ctx->stream;
     const size_t n = (size_t)w * h;
     if (n_supports) *n_supports = 0;
     ctx->last_supports.clear();
+    ctx->supports_pending = false;
     HIPCHK(launch_fill_u32(st, n, 0xBF800000u /* -1.0f */, (uint32_t*)d_prior), "prior init");
     const int sw = prior_samples_x(w), sh = prior_samples_y(h);
     if (!p->prior_enable || sw <= 0 || sh <= 0) return 0;
-    if (ctx->capturing) return fail(ctx, 8, "the HighQuality prior synchronises and cannot be captured");
+    if (ctx->capturing && n_supports) return fail(ctx, 8, "the support count cannot be read back under stream capture");
     const bool sym = p->census_type == 1;
-    const size_t ncell = (size_t)((w + kPriorCell - 1) / kPriorCell) * ((h + kPriorCell - 1) / kPriorCell);
     HIPCHK(ensure(ctx, B_CAND_D, (size_t)sw * sh * 2), "prior alloc");
     HIPCHK(ensure(ctx, B_CAND_U, (size_t)sw * sh * 4), "prior alloc");
-    HIPCHK(ensure(ctx, B_GRID, ncell * 5 * 4), "prior alloc");
+    HIPCHK(ensure(ctx, B_GRID, prior_work_bytes(w, h, p->prior_max_supports)), "prior alloc");
+    const PriorLayout L = prior_layout(w, h, p->prior_max_supports, ctx->bufs[B_GRID].ptr);
     int16_t* cd = buf<int16_t>(ctx, B_CAND_D);
     float* cu = buf<float>(ctx, B_CAND_U);
     HIPCHK(launch_prior_candidates(st, *p, w, h, gxl, gyl, sym ? (const uint32_t*)cl : nullptr,
                                    sym ? (const uint32_t*)cr : nullptr, sym ? nullptr : (const uint64_t*)cl,
                                    sym ? nullptr : (const uint64_t*)cr, cd, cu),
            "prior candidates");
-    std::vector<int16_t> hd;
-    std::vector<float> hu;
-    const size_t ns = (size_t)sw * sh;
-    HIPCHK(download_vec(st, hd, cd, ns), "prior download");
-    HIPCHK(download_vec(st, hu, cu, ns), "prior download");
-    HIPCHK(hipStreamSynchronize(st), "prior sync");
-    PriorGrid g;
-    const int nsup = prior_build_grid(*p, w, h, hd.data(), hu.data(), g, &ctx->last_supports);
-    if (n_supports) *n_supports = nsup;
-    if (nsup < p->prior_min_supports || nsup == 0) return 0;
-    const size_t nc = (size_t)g.gw * g.gh;
-    std::vector<float>& planes = ctx->prior_grid_host;
-    planes.resize(nc * 5);
-    std::copy(g.d.begin(), g.d.end(), planes.begin());
-    std::copy(g.c.begin(), g.c.end(), planes.begin() + nc);
-    std::copy(g.s.begin(), g.s.end(), planes.begin() + 2 * nc);
-    std::copy(g.lo.begin(), g.lo.end(), planes.begin() + 3 * nc);
-    std::copy(g.hi.begin(), g.hi.end(), planes.begin() + 4 * nc);
-    float* dg = buf<float>(ctx, B_GRID);
-    HIPCHK(hipMemcpyAsync(dg, planes.data(), nc * 5 * sizeof(float), hipMemcpyHostToDevice, st), "prior upload");
-    HIPCHK(launch_prior_ranges(st, *p, w, h, g.gw, g.gh, dg, dmin, dmax, d_prior), "prior ranges");
+    HIPCHK(launch_prior_grid(st, *p, w, h, cd, cu, L), "prior grid");
+    HIPCHK(launch_prior_ranges(st, *p, w, h, L, dmin, dmax, d_prior), "prior ranges");
+    ctx->prior_lay = L;
+    ctx->prior_sw = sw;
+    ctx->supports_pending = true;
+    if (n_supports) {
+        int ns = 0;
+        HIPCHK(hipMemcpyAsync(&ns, L.nsel, 4, hipMemcpyDeviceToHost, st), "support count");
+        HIPCHK(hipStreamSynchronize(st), "support count");
+        *n_supports = ns;
+    }
     return 0;
 }
 

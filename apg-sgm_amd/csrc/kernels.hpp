@@ -182,13 +182,33 @@ hipError_t launch_wta_fused(hipStream_t s, const apg_hip_params& p, int w, int h
                             const uint8_t* gx, const uint8_t* gy, float* disp, float* conf, uint8_t* reason,
                             uint8_t* reliable, float* disp_wta, float* best, float* second, uint32_t* sum32);
 
-// HighQuality prior (k_prior.hip): per-sample census candidates, then the
-// per-pixel prior and search range from the host-built cell grid
-// (5 planes of gw*gh floats: disparity, confidence, spread, lowest, highest).
+// HighQuality prior (k_prior.hip): per-sample census candidates; the
+// support selection, cell statistics and hole filling; the per-pixel prior
+// and search range.  The work buffer holds, per cell, its candidates (at
+// most per_cell, raster order: disparity, weight, sample index), their count
+// and how many became supports; the grid (5 planes of gw*gh floats:
+// disparity, confidence, spread, lowest, highest) and its second buffer for
+// the filling sweeps; sort scratch; the number of supports.
+struct KeyW;
+struct PriorLayout {
+    int gw, gh, ncell, per_cell;
+    float *grid, *grid2;
+    int *cnt, *taken;
+    float *key, *wt;
+    int* sidx;
+    KeyW* tmp;
+    int* nsel;
+    size_t bytes;
+};
+int prior_per_cell(int ncell, int budget);
+PriorLayout prior_layout(int w, int h, int budget, void* work);
+size_t prior_work_bytes(int w, int h, int budget);
 hipError_t launch_prior_candidates(hipStream_t s, const apg_hip_params& p, int w, int h, const uint8_t* gx,
                                    const uint8_t* gy, const uint32_t* c32l, const uint32_t* c32r,
                                    const uint64_t* c64l, const uint64_t* c64r, int16_t* cand_d, float* cand_u);
-hipError_t launch_prior_ranges(hipStream_t s, const apg_hip_params& p, int w, int h, int gw, int gh,
-                               const float* grid, int16_t* dmin, int16_t* dmax, float* d_prior);
+hipError_t launch_prior_grid(hipStream_t s, const apg_hip_params& p, int w, int h, const int16_t* cand_d,
+                             const float* cand_u, const PriorLayout& L);
+hipError_t launch_prior_ranges(hipStream_t s, const apg_hip_params& p, int w, int h, const PriorLayout& L,
+                               int16_t* dmin, int16_t* dmax, float* d_prior);
 
 }  // namespace apgk

@@ -61,3 +61,18 @@ def test_packed_prefix_difference_is_exact_modulo_2_16():
         diff = (wrapped[win:] - wrapped[:-win]) & mask
         np.testing.assert_array_equal(diff, exact)
     assert WIN_MAX * 255 * 2 < 2**16   # the doubled sum also fits a u16 half
+
+
+def test_prior_sort_leaves_ties_where_std_sort_does(tmp_path):
+    # k_prior.hip sums support weights in std::sort order (key-only
+    # comparator), so its device sort must reproduce libstdc++'s placement of
+    # equal keys; tests/cpp/prior_sort_main.cpp compares the two over random,
+    # structured and adversarial (heapsort-branch) inputs.
+    import os
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = str(tmp_path / "prior_sort_main")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-I", os.path.join(root, "apg-sgm_amd", "csrc"),
+                    os.path.join(root, "tests", "cpp", "prior_sort_main.cpp"), "-o", exe], check=True)
+    out = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0 and out.stdout.startswith("ok"), out.stdout + out.stderr
