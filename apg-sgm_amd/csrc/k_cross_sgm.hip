@@ -360,6 +360,96 @@ __global__ __launch_bounds__(64) void cross_v3_kernel(int w, int h, int d0, int 
     }
 }
 
+// Vertical Cross pass over u8 T for per-pixel ranges (the prior's): the grid
+// of cross_v3_kernel (column, row segment, 128-wide d chunk; the prefix of
+// input rows [y0 - Lmax, y1 + Lmax) in an LDS ring of 2*Lmax + 2 rows) with
+// cross_v2_kernel's packed (sum << 8 | count) u32 prefixes, since the number
+// of valid entries in a window varies with d.  Window sums stay below
+// 125 * 254 * 256 + 125 < 2^32, so prefix differences are exact modulo 2^32.
+// Per step the ring slots it touches and the output pixel's range come from
+// a table built once per segment.  Rows past the image bottom add to prefixes
+// no output reads (a window ends at row h - 1 at most).
+constexpr int CV4_PF = 16;
+
+__global__ __launch_bounds__(64) void cross_v4_kernel(int w, int h, int d0, int D, int Lmax, int seg,
+                                                      const int16_t* __restrict__ dmin,
+                                                      const int16_t* __restrict__ dmax,
+                                                      const uint32_t* __restrict__ arms,
+                                                      const uint8_t* __restrict__ tmp, uint8_t* __restrict__ cost) {
+    extern __shared__ uint32_t sh[];
+    const int lane = threadIdx.x;
+    const int x = blockIdx.x;
+    const int y0 = blockIdx.y * seg, y1 = min(h, y0 + seg);
+    if (y0 >= h) return;
+    const int di = blockIdx.z * 128 + 2 * lane;
+    const int rs = 2 * Lmax + 2;
+    const int ya = max(0, y0 - Lmax), yb = min(h, y1 + Lmax), kend = y1 + Lmax, nk = kend - ya;
+    uint32_t* magic = sh;                              // [2*kMaxArm + 2]
+    uint2* ring = (uint2*)(sh + 2 * kMaxArm + 2);      // [rs][64]: the lane's two prefixes
+    uint2* meta = ring + rs * 64;                      // [nk]: slots P(k+1) | P(y+Dn+1) << 8 | P(y-U) << 16, range
+    for (int c = lane; c < 2 * kMaxArm + 2; c += 64) magic[c] = magic_for((uint32_t)c);
+    for (int r = lane; r < nk; r += 64) {
+        const int k = ya + r, y = k - Lmax;
+        uint2 m;
+        m.x = (uint32_t)((r + 1) % rs);
+        m.y = 0u;
+        if (y >= y0) {
+            const size_t q = (size_t)y * w + x;
+            const uint32_t a = arms[q];
+            const int U = (a >> 16) & 0xFF, Dn = a >> 24;
+            m.x |= (uint32_t)((y + Dn + 1 - ya) % rs) << 8 | (uint32_t)((y - U - ya) % rs) << 16;
+            m.y = (uint32_t)(uint16_t)dmin[q] | ((uint32_t)(uint16_t)dmax[q] << 16);
+        }
+        meta[r] = m;
+    }
+    ring[lane] = make_uint2(0u, 0u);   // P(ya)
+    __syncthreads();
+    const int dA = d0 + di;
+    const bool inA = di < D;   // D even: the pair is inside or outside as a whole
+    const uint32_t voff = inA ? (uint32_t)di : 0u;
+    const auto rt = raw_rsrc(tmp);
+    const auto rc = raw_rsrc(cost);
+    const uint32_t rowb = (uint32_t)w * (uint32_t)D, colb = (uint32_t)x * (uint32_t)D;
+    const int kout = y0 + Lmax;   // first step with an output row
+    uint32_t accA = 0, accB = 0;
+    auto step = [&](int k, uint32_t raw, uint2 m) {
+        const uint32_t a = raw & 0xFFu, b = (raw >> 8) & 0xFFu;
+        accA += (inA && a != 0xFFu) ? (a << 8) | 1u : 0u;
+        accB += (inA && b != 0xFFu) ? (b << 8) | 1u : 0u;
+        ring[(m.x & 0xFFu) * 64 + lane] = make_uint2(accA, accB);
+        if (k < kout) return;
+        const uint2 p1 = ring[((m.x >> 8) & 0xFFu) * 64 + lane];
+        const uint2 p0 = ring[(m.x >> 16) * 64 + lane];
+        const uint32_t sa = p1.x - p0.x, sb = p1.y - p0.y;
+        const int lo = (int16_t)(m.y & 0xFFFFu), hi = (int16_t)(m.y >> 16);
+        const uint32_t ca = sa & 0xFFu, cb = sb & 0xFFu;
+        const uint32_t oa = (dA >= lo && dA < hi && ca) ? div_small(sa >> 8, ca, magic[ca]) : 0xFFu;
+        const uint32_t ob = (dA + 1 >= lo && dA + 1 < hi && cb) ? div_small(sb >> 8, cb, magic[cb]) : 0xFFu;
+        if (inA) buf_st16(rc, oa | (ob << 8), voff, (uint32_t)(k - Lmax) * rowb + colb);
+    };
+    uint32_t X[CV4_PF], Y[CV4_PF];
+    auto load = [&](uint32_t (&bf)[CV4_PF], int kb) {
+#pragma unroll
+        for (int i = 0; i < CV4_PF; ++i) bf[i] = buf_ld16(rt, voff, (uint32_t)min(kb + i, yb - 1) * rowb + colb);
+    };
+    auto batch = [&](const uint32_t (&bf)[CV4_PF], int kb) {
+        uint2 m[CV4_PF];
+#pragma unroll
+        for (int i = 0; i < CV4_PF; ++i) m[i] = meta[min(kb + i - ya, nk - 1)];
+#pragma unroll
+        for (int i = 0; i < CV4_PF; ++i)
+            if (kb + i < kend) step(kb + i, bf[i], m[i]);
+    };
+    load(X, ya);
+    load(Y, ya + CV4_PF);
+    for (int kb = ya; kb < kend; kb += 2 * CV4_PF) {
+        batch(X, kb);
+        load(X, kb + 2 * CV4_PF);
+        batch(Y, kb + CV4_PF);
+        load(Y, kb + 3 * CV4_PF);
+    }
+}
+
 hipError_t launch_cross_v_t(hipStream_t s, const apg_hip_params& p, int w, int h, const int16_t* dmin,
                             const int16_t* dmax, const uint32_t* arms, const void* tmp, void* cost, bool u8,
                             bool column_ranges, const KTune& tn) {
@@ -377,6 +467,24 @@ hipError_t launch_cross_v_t(hipStream_t s, const apg_hip_params& p, int w, int h
             nseg = (h + seg - 1) / seg;
             const size_t shm = (size_t)(2 * Lmax + 2) * 64 * sizeof(uint32_t) + (size_t)(seg + 2 * Lmax) * 16;
             hipLaunchKernelGGL(cross_v3_kernel, dim3(w, nseg, dch), dim3(64), shm, s, w, h, p.min_disparity, D, Lmax,
+                               seg, dmin, dmax, arms, (const uint8_t*)tmp, (uint8_t*)cost);
+            return hipGetLastError();
+        }
+    }
+    {
+        // per-pixel ranges: the segmented grid with counted prefixes
+        const int D = p.max_disparity - p.min_disparity;
+        const int Lmax = p.max_arm_length > 1 ? p.max_arm_length : 1;
+        if (u8 && D % 2 == 0 && (size_t)w * h * D < (1ull << 31)) {
+            const int dch = (D + 127) / 128;
+            const int target = tn.cross_v_waves > 0 ? tn.cross_v_waves : 16384;
+            int nseg = (target + w * dch - 1) / (w * dch);
+            nseg = max(1, min(nseg, (h + 63) / 64));
+            const int seg = (h + nseg - 1) / nseg;
+            nseg = (h + seg - 1) / seg;
+            const size_t shm = (2 * kMaxArm + 2) * sizeof(uint32_t) + (size_t)(2 * Lmax + 2) * 64 * 8 +
+                               (size_t)(seg + 2 * Lmax) * 8;
+            hipLaunchKernelGGL(cross_v4_kernel, dim3(w, nseg, dch), dim3(64), shm, s, w, h, p.min_disparity, D, Lmax,
                                seg, dmin, dmax, arms, (const uint8_t*)tmp, (uint8_t*)cost);
             return hipGetLastError();
         }

@@ -70,6 +70,24 @@ def test_hq_pipeline_matches_oracle(ctx, kind, w, h, D, over):
     _cmp(g.sgm, o["sgm"], "sgm")
 
 
+FUSED_KEYS = ["dmin", "dmax", "d_prior", "disp_wta", "after_refine", "disparity", "confidence", "invalid_reason",
+              "reliable"]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind,w,h,D,over", CASES)
+def test_hq_fused_schedule_matches_oracle(ctx, kind, w, h, D, over):
+    """The same cases without raw volumes requested, so the fused general-range
+    Cost + Cross kernels and the delta-plane SGM run where they apply."""
+    print(f"HQ fused kind={kind} {w}x{h} D={D} {over}", flush=True)
+    p = A.params(2, D, **over)
+    left, right, _ = A.synth_pair(kind, w, h, D, 4321 + w)
+    o = O.pipeline(O.cfg_from(p.as_dict()), left, right, intermediates=True)
+    g = A.run_pipeline(ctx, p, left, right, intermediates=tuple(k for k in FUSED_KEYS[:5]))
+    for k in FUSED_KEYS:
+        _cmp(getattr(g, k), o[k], k)
+
+
 def _golden_small():
     z = np.load(os.path.join(GOLD, "golden_small.npz"))
     return z, sorted({k.split("__")[0] for k in z.files if k.startswith("hq_")})
