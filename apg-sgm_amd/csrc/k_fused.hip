@@ -213,10 +213,12 @@ __global__ __launch_bounds__(256) void cost_crossh_kernel(
 constexpr int K2_SEG = 64;   // measured best at 1280x720 D=128 (more waves; the halo costs less)
 
 struct K2Shm {
-    size_t lut, mag, col, chi, outr, rre, rro, key, ring, total;
+    size_t lut, mag, col, chi, rng, outr, rre, rro, key, ring, cring, total;
 };
 
-__host__ __device__ inline K2Shm k2_layout(int D, int Lmax, int seg, bool sym) {
+// gr: per-pixel ranges (the general form): each column's [dmin, dmax) and a
+// second ring of u8-pair valid counts
+__host__ __device__ inline K2Shm k2_layout(int D, int Lmax, int seg, bool sym, bool gr = false) {
     const int ncol = seg + 2 * Lmax;
     const int nrr = (ncol + D + 2) / 2 + 1;   // records per parity array
     K2Shm s;
@@ -226,6 +228,7 @@ __host__ __device__ inline K2Shm k2_layout(int D, int Lmax, int seg, bool sym) {
     o = (o + 15) & ~(size_t)15;
     s.col = o; o += (size_t)(ncol + 2) * 16;   // + the records read ahead of the last column
     s.chi = o; o += sym ? 0 : (size_t)(ncol + 2) * 4;
+    s.rng = o; o += gr ? (size_t)(ncol + 2) * 4 : 0;
     o = (o + 15) & ~(size_t)15;
     s.outr = o; o += (size_t)(seg + 1) * 16;
     s.rre = o; o += (size_t)nrr * 16;
@@ -233,6 +236,7 @@ __host__ __device__ inline K2Shm k2_layout(int D, int Lmax, int seg, bool sym) {
     s.key = o; o += (size_t)(seg + D - 1) * 4;
     o = (o + 15) & ~(size_t)15;
     s.ring = o; o += (size_t)(2 * Lmax + 2) * 256;
+    s.cring = o; o += gr ? (size_t)(2 * Lmax + 2) * 128 : 0;
     s.total = o;
     return s;
 }
@@ -243,26 +247,34 @@ constexpr int kFarLeft = -(1 << 24);   // x of a column outside the image: x - d
 __device__ __forceinline__ uint32_t rep4(uint32_t g) { return g * 0x01010101u; }
 __device__ __forceinline__ uint32_t grad4(uint32_t gx, uint32_t gy) { return (gx << 2) | (gy << 18); }
 
-template <bool SYM, bool FULL, bool NOCLAMP>
+// GR: ranges per pixel (dmin, dmax: the HighQuality prior's).  A state then
+// also needs d in its own pixel's range, a Cross-H window counts its valid
+// states from u8-pair prefix counts (a segment has fewer than 256 columns),
+// and an output pixel's T is valid for d in its own range with a non-empty
+// window (reference src/cost_aggregator.cpp:74-105).
+template <bool SYM, bool FULL, bool NOCLAMP, bool GR>
 __global__ __launch_bounds__(64) void cost_crossh2_kernel(
     const CostLut* __restrict__ lut, int w, int h, int d0, int D, int cost_max, int Lmax, int seg,
     const uint8_t* __restrict__ gl, const uint8_t* __restrict__ gxl, const uint8_t* __restrict__ gyl,
     const uint8_t* __restrict__ gr, const uint8_t* __restrict__ gxr, const uint8_t* __restrict__ gyr,
     const uint32_t* __restrict__ c32l, const uint32_t* __restrict__ c32r, const uint64_t* __restrict__ c64l,
-    const uint64_t* __restrict__ c64r, const uint32_t* __restrict__ arms, uint8_t* __restrict__ out,
-    uint32_t* __restrict__ rkeys) {
+    const uint64_t* __restrict__ c64r, const int16_t* __restrict__ dmin, const int16_t* __restrict__ dmax,
+    const uint32_t* __restrict__ arms, uint8_t* __restrict__ out, uint32_t* __restrict__ rkeys) {
     extern __shared__ __align__(16) unsigned char shm[];
-    const K2Shm L = k2_layout(D, Lmax, seg, SYM);
+    const K2Shm L = k2_layout(D, Lmax, seg, SYM, GR);
     float* s_lc = (float*)(shm + L.lut);
     float* s_la = s_lc + 65;
     float* s_lg = s_la + 256;
     uint32_t* s_mag = (uint32_t*)(shm + L.mag);
     uint4* s_col = (uint4*)(shm + L.col);    // census lo, rep4(g), grad4(gx, gy), x (kFarLeft outside)
     uint32_t* s_chi = (uint32_t*)(shm + L.chi);   // census hi (64-bit census)
+    uint32_t* s_rng = (uint32_t*)(shm + L.rng);   // GR: dmin | dmax << 16 (empty outside the image)
     uint4* s_out = (uint4*)(shm + L.outr);   // ring byte offsets of P(x+R+1) and of P(x-L),
                                              // ceil(2^31/(L+R+1)), (x-L) | (x+R) << 16
+                                             // (GR: the pixel's dmin | dmax << 16)
     uint32_t* s_key = (uint32_t*)(shm + L.key);
     unsigned char* s_ring = shm + L.ring;    // [2*Lmax+2][64 lanes] u16-pair doubled prefix sums
+    unsigned char* s_cring = shm + L.cring;  // GR: [2*Lmax+2][64 lanes] u8-pair prefix valid counts
 
     const int lane = threadIdx.x;
     const int y = blockIdx.y;
@@ -289,7 +301,7 @@ __global__ __launch_bounds__(64) void cost_crossh2_kernel(
     for (int n = lane; n < 2 * kMaxArm + 2; n += 64) s_mag[n] = n ? (0x80000000u + (uint32_t)n - 1u) / (uint32_t)n : 0u;
     // left records: columns xb .. xb + ncol - 1 (kFarLeft outside the image)
     for (int c0 = 0; c0 <= ncol + 1; c0 += 256) {
-        uint32_t lo[4], hi[4], g[4], gx[4], gy[4];
+        uint32_t lo[4], hi[4], g[4], gx[4], gy[4], rg[4];
         bool in[4];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
@@ -301,6 +313,7 @@ __global__ __launch_bounds__(64) void cost_crossh2_kernel(
             g[j] = gl[q];
             gx[j] = gxl[q];
             gy[j] = gyl[q];
+            if (GR) rg[j] = (uint32_t)(uint16_t)dmin[q] | ((uint32_t)(uint16_t)dmax[q] << 16);
         }
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
@@ -309,17 +322,20 @@ __global__ __launch_bounds__(64) void cost_crossh2_kernel(
                 s_col[c] = in[j] ? make_uint4(lo[j], rep4(g[j]), grad4(gx[j], gy[j]), (uint32_t)(xb + c))
                                  : make_uint4(0u, 0u, 0u, (uint32_t)kFarLeft);
                 if (!SYM) s_chi[c] = in[j] ? hi[j] : 0u;
+                if (GR) s_rng[c] = in[j] ? rg[j] : 0u;
             }
         }
     }
     for (int o0 = 0; o0 <= seg; o0 += 256) {
-        uint32_t a[4];
+        uint32_t a[4], rg[4];
         bool in[4];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const int o = o0 + lane + 64 * j, x = x0 + o;
             in[j] = o < seg && x < w;
-            a[j] = arms[row + min(x, w - 1)];
+            const size_t q = row + min(x, w - 1);
+            a[j] = arms[q];
+            if (GR) rg[j] = (uint32_t)(uint16_t)dmin[q] | ((uint32_t)(uint16_t)dmax[q] << 16);
         }
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
@@ -334,7 +350,7 @@ __global__ __launch_bounds__(64) void cost_crossh2_kernel(
                 r.y = (uint32_t)(((co - La) % rs) * 256);
                 const uint32_t n = (uint32_t)(La + Ra + 1);
                 r.z = (0x80000000u + n - 1u) / n;
-                r.w = (uint32_t)(x - La) | ((uint32_t)(x + Ra) << 16);
+                r.w = GR ? rg[j] : (uint32_t)(x - La) | ((uint32_t)(x + Ra) << 16);
             }
             s_out[o] = r;
         }
@@ -388,12 +404,14 @@ __global__ __launch_bounds__(64) void cost_crossh2_kernel(
         // s_key entry of the right pixel xr = x - dA is xr - kbase
         uint32_t* key_lane = s_key - dA - kbase;
         *(uint32_t*)(s_ring + lofs) = 0u;   // P(0)
-        uint32_t acc = 0u;
+        if (GR) *(uint16_t*)(s_cring + lofs / 2) = 0u;
+        uint32_t acc = 0u, cacc = 0u;
         uint32_t ringw = 256u;              // ring byte offset of P(c + 1)
         const uint32_t ringe = (uint32_t)rs * 256u;
         // pipeline registers: records of the next two columns, LUT terms of the next one
         uint4 cr_n = s_col[0], r0_n = rr_at(0), r1_n = rr_at(-1), or_n = s_out[0];
         uint32_t ch_n = SYM ? 0u : s_chi[0];
+        uint32_t rg_n = GR ? s_rng[0] : 0u, rg_c = 0u;
         float tc0, ta0, tg0, tc1, ta1, tg1;
         // LDS byte addresses of the AD and gradient LUT entries come out of the
         // SAD instructions themselves (the table base is their accumulator)
@@ -412,9 +430,13 @@ __global__ __launch_bounds__(64) void cost_crossh2_kernel(
         uint4 cr_c = cr_n;                   // column 0's record (its x)
         cr_n = s_col[1];
         if (!SYM) ch_n = s_chi[1];
+        if (GR) {
+            rg_c = rg_n;
+            rg_n = s_rng[1];
+        }
         r0_n = rr_at(1);
         r1_n = rr_at(0);
-        uint32_t pp1 = 0u, pp0 = 0u;
+        uint32_t pp1 = 0u, pp0 = 0u, cp1 = 0u, cp0 = 0u;
         uint4 por = make_uint4(0u, 0u, 0u, 0u);
         int pxo = 0;
         bool pend = false;
@@ -428,6 +450,21 @@ __global__ __launch_bounds__(64) void cost_crossh2_kernel(
             if (FAST) {
                 q0 = __umulhi(sw & 0xFFFFu, por.z);
                 q1 = __umulhi(sw >> 16, por.z);
+            } else if (GR) {
+                // own range of the output pixel, valid counts of the window
+                const int lo = (int)(int16_t)(por.w & 0xFFFFu), hi = (int)(int16_t)(por.w >> 16);
+                const uint32_t wc = cp1 - cp0;   // per byte: prefix counts never fall
+                uint32_t qq[2];
+#pragma unroll
+                for (int e = 0; e < 2; ++e) {
+                    const int d = dA + e;
+                    const uint32_t cnt = (wc >> (8 * e)) & 0xFFu;
+                    const uint32_t se = e ? (sw >> 16) : (sw & 0xFFFFu);
+                    const bool v = d >= lo && d < hi && cnt > 0;
+                    qq[e] = v ? __umulhi(se, s_mag[cnt]) : 0xFFu;
+                }
+                q0 = qq[0];
+                q1 = qq[1];
             } else {
                 const int xlo = (int)(por.w & 0xFFFFu), xhi = (int)(por.w >> 16);
                 uint32_t qq[2];
@@ -453,6 +490,7 @@ __global__ __launch_bounds__(64) void cost_crossh2_kernel(
             constexpr bool keys = decltype(keys_t)::value, outp = decltype(outp_t)::value;
             constexpr bool allv = decltype(allv_t)::value;
             const uint4 cr = cr_c;
+            const uint32_t rg = rg_c;
             // same operation order as state_cost (reference src/cost_computer.cpp:176-199)
             const float f0 = (tc0 + ta0) + tg0;
             const float f1 = (tc1 + ta1) + tg1;
@@ -461,6 +499,10 @@ __global__ __launch_bounds__(64) void cost_crossh2_kernel(
             cr_c = cr_n;
             cr_n = s_col[c + 2];
             if (!SYM) ch_n = s_chi[c + 2];
+            if (GR) {
+                rg_c = rg_n;
+                rg_n = s_rng[c + 2];
+            }
             // d + 1 at column c + 2 pairs with the right pixel of d at column c + 1
             r1_n = r0_n;
             r0_n = rr_at(c + 2);
@@ -472,8 +514,13 @@ __global__ __launch_bounds__(64) void cost_crossh2_kernel(
             const uint32_t cst1 = NOCLAMP ? (uint32_t)(int)(f1 * scale + 0.5f)
                                           : (uint32_t)clampi((int)(f1 * scale + 0.5f), 0, cost_max);
             const int xr0 = (int)cr.w - dA;   // right pixel of d; d + 1 has xr0 - 1
-            const bool v0 = allv ? act : act && (uint32_t)xr0 < uw;
-            const bool v1 = allv ? act : act && (uint32_t)(xr0 - 1) < uw;
+            bool v0 = allv ? act : act && (uint32_t)xr0 < uw;
+            bool v1 = allv ? act : act && (uint32_t)(xr0 - 1) < uw;
+            if (GR) {   // d in the column's own range [dmin, dmax)
+                const int lo = (int)(int16_t)(rg & 0xFFFFu), hi = (int)(int16_t)(rg >> 16);
+                v0 = v0 && dA >= lo && dA < hi;
+                v1 = v1 && dA + 1 >= lo && dA + 1 < hi;
+            }
             if (keys) {
                 uint32_t* k0p = FULL ? key_lane + (int)cr.w : s_key + max(xr0 - kbase, 1);
                 atomicMin(k0p, v0 ? (cst0 << 16) | (uint32_t)di0 : 0xFFFFFFFFu);
@@ -482,6 +529,10 @@ __global__ __launch_bounds__(64) void cost_crossh2_kernel(
             const uint32_t pair = (v0 ? cst0 : 0u) | ((v1 ? cst1 : 0u) << 16);
             acc = as_u32(as_v2(pair) * splat2(2) + as_v2(acc));
             *(uint32_t*)(s_ring + ringw + lofs) = acc;   // P(c + 1)
+            if (GR) {
+                cacc += (v0 ? 1u : 0u) | (v1 ? 0x100u : 0u);
+                *(uint16_t*)(s_cring + ringw / 2 + lofs / 2) = (uint16_t)cacc;
+            }
             ringw = ringw + 256u == ringe ? 0u : ringw + 256u;
             pend = outp;
             if (outp) {
@@ -490,6 +541,10 @@ __global__ __launch_bounds__(64) void cost_crossh2_kernel(
                 pxo = x0 + c - 2 * Lmax;
                 pp1 = *(const uint32_t*)(s_ring + por.x + lofs);
                 pp0 = *(const uint32_t*)(s_ring + por.y + lofs);
+                if (GR) {
+                    cp1 = *(const uint16_t*)(s_cring + por.x / 2 + lofs / 2);
+                    cp0 = *(const uint16_t*)(s_cring + por.y / 2 + lofs / 2);
+                }
             }
         };
 
@@ -503,7 +558,7 @@ __global__ __launch_bounds__(64) void cost_crossh2_kernel(
         const int kb = Lmax, ke = Lmax + nown, ob = 2 * Lmax, oe = 2 * Lmax + nown;
         const int vx0 = max(dtop, 0), vx1 = min(w - 1 + d0 + dc, w - 1);
         const int cvb = min(oe, max(0, vx0 - xb)), cve = max(cvb, min(oe, vx1 - xb + 1));
-        const int cf = nonneg ? min(oe, max(ob + 1, dtop + Lmax - x0 + 2 * Lmax + 1)) : oe;
+        const int cf = nonneg && !GR ? min(oe, max(ob + 1, dtop + Lmax - x0 + 2 * Lmax + 1)) : oe;
         using TT = std::true_type;
         using FF = std::false_type;
         int c = 0;
@@ -565,27 +620,30 @@ hipError_t launch_cost_crossh(hipStream_t s, const apg_hip_params& p, const Cost
     const int D = p.max_disparity - p.min_disparity;
     const bool cross = p.agg_enable != 0;
     const int Lmax = cross ? (p.max_arm_length > 1 ? p.max_arm_length : 1) : 0;
-    if (cross && out_u8 && column_ranges && D % 2 == 0 && (size_t)w * h * D < (1ull << 31)) {
-        int seg = tn.cost_segment > 0 ? tn.cost_segment : K2_SEG;
-        seg = w < seg ? w : seg;
-        const K2Shm L2 = k2_layout(D, Lmax, seg, p.census_type == 1);
+    int seg2 = tn.cost_segment > 0 ? tn.cost_segment : K2_SEG;
+    seg2 = w < seg2 ? w : seg2;
+    // per-pixel ranges: the valid counts are u8 prefixes over a segment's columns
+    const bool granges = !column_ranges;
+    if (cross && out_u8 && D % 2 == 0 && (size_t)w * h * D < (1ull << 31) && (!granges || seg2 + 2 * Lmax <= 255)) {
+        const int seg = seg2;
+        const K2Shm L2 = k2_layout(D, Lmax, seg, p.census_type == 1, granges);
         const bool full = D % 128 == 0;
-        decltype(&cost_crossh2_kernel<true, true, true>) k;
+        decltype(&cost_crossh2_kernel<true, true, true, false>) k;
+#define APG_K2(S, F, N) (granges ? cost_crossh2_kernel<S, F, N, true> : cost_crossh2_kernel<S, F, N, false>)
         if (clamp_free)
-            k = p.census_type == 1
-                    ? (full ? cost_crossh2_kernel<true, true, true> : cost_crossh2_kernel<true, false, true>)
-                    : (full ? cost_crossh2_kernel<false, true, true> : cost_crossh2_kernel<false, false, true>);
+            k = p.census_type == 1 ? (full ? APG_K2(true, true, true) : APG_K2(true, false, true))
+                                   : (full ? APG_K2(false, true, true) : APG_K2(false, false, true));
         else
-            k = p.census_type == 1
-                    ? (full ? cost_crossh2_kernel<true, true, false> : cost_crossh2_kernel<true, false, false>)
-                    : (full ? cost_crossh2_kernel<false, true, false> : cost_crossh2_kernel<false, false, false>);
+            k = p.census_type == 1 ? (full ? APG_K2(true, true, false) : APG_K2(true, false, false))
+                                   : (full ? APG_K2(false, true, false) : APG_K2(false, false, false));
+#undef APG_K2
         if (L2.total > 64 * 1024) {
             hipError_t e =
                 hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)L2.total);
             if (e != hipSuccess) return e;
         }
         hipLaunchKernelGGL(k, dim3((w + seg - 1) / seg, h), dim3(64), L2.total, s, d_lut, w, h, p.min_disparity, D,
-                           p.cost_max, Lmax, seg, gl, gxl, gyl, gr, gxr, gyr, c32l, c32r, c64l, c64r, arms,
+                           p.cost_max, Lmax, seg, gl, gxl, gyl, gr, gxr, gyr, c32l, c32r, c64l, c64r, dmin, dmax, arms,
                            (uint8_t*)out, rkeys);
         return hipGetLastError();
     }
