@@ -23,6 +23,8 @@ void build_cost_lut(const apg_hip_params& p, CostLut* out) {
     const float norm = 1.f + (p.use_ad ? p.eta_ad : 0.f) + (p.use_grad ? p.mu_grad : 0.f);
     out->scale = (float)p.cost_max / (norm > 1e-6f ? norm : 1e-6f);
     out->pad[0] = 0;
+    out->mag31[0] = 0u;
+    for (uint32_t n = 1; n < 2 * kMaxArm + 2; ++n) out->mag31[n] = (0x80000000u + n - 1u) / n;
 }
 
 // Upper bound of every quantised cost: float addition and the final

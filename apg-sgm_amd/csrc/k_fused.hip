@@ -298,7 +298,8 @@ __global__ __launch_bounds__(64) void cost_crossh2_kernel(
         for (int k = 0; k < NL; ++k)
             if (lane + 64 * k < NLUT) s_lc[lane + 64 * k] = lv[k];
     }
-    for (int n = lane; n < 2 * kMaxArm + 2; n += 64) s_mag[n] = n ? (0x80000000u + (uint32_t)n - 1u) / (uint32_t)n : 0u;
+    for (int n = lane; n < 2 * kMaxArm + 2; n += 64) s_mag[n] = lut->mag31[n];
+    __syncthreads();   // the arms' records below read s_mag
     // left records: columns xb .. xb + ncol - 1 (kFarLeft outside the image)
     for (int c0 = 0; c0 <= ncol + 1; c0 += 256) {
         uint32_t lo[4], hi[4], g[4], gx[4], gy[4], rg[4];
@@ -348,8 +349,7 @@ __global__ __launch_bounds__(64) void cost_crossh2_kernel(
                 const int co = Lmax + o;   // column index of x; P(j) sums columns < j
                 r.x = (uint32_t)(((co + Ra + 1) % rs) * 256);
                 r.y = (uint32_t)(((co - La) % rs) * 256);
-                const uint32_t n = (uint32_t)(La + Ra + 1);
-                r.z = (0x80000000u + n - 1u) / n;
+                r.z = s_mag[La + Ra + 1];
                 r.w = GR ? rg[j] : (uint32_t)(x - La) | ((uint32_t)(x + Ra) << 16);
             }
             s_out[o] = r;
@@ -479,7 +479,9 @@ __global__ __launch_bounds__(64) void cost_crossh2_kernel(
                 q0 = qq[0];
                 q1 = qq[1];
             }
-            if (act) buf_st16(rout, q0 | (q1 << 8), (uint32_t)di0, (uint32_t)((int)row + pxo) * (uint32_t)D);
+            // bytes q0, q1 (each < 256) -> one 16-bit store
+            if (act) buf_st16(rout, __builtin_amdgcn_perm(q1, q0, 0x0c0c0400u), (uint32_t)di0,
+                              (uint32_t)((int)row + pxo) * (uint32_t)D);
         };
 
         // keys: own column (right-view WTA candidates); outp: an output column's
@@ -527,7 +529,12 @@ __global__ __launch_bounds__(64) void cost_crossh2_kernel(
                 atomicMin(k0p - 1, v1 ? (cst1 << 16) | (uint32_t)(di0 + 1) : 0xFFFFFFFFu);
             }
             const uint32_t pair = (v0 ? cst0 : 0u) | ((v1 ? cst1 : 0u) << 16);
-            acc = as_u32(as_v2(pair) * splat2(2) + as_v2(acc));
+            // acc += 2 * pair per u16 lane in one v_pk_mad_u16 (no carry between halves)
+            {
+                uint32_t nacc;
+                asm("v_pk_mad_u16 %0, %1, %2, %3" : "=v"(nacc) : "v"(pair), "s"(0x00020002u), "v"(acc));
+                acc = nacc;
+            }
             *(uint32_t*)(s_ring + ringw + lofs) = acc;   // P(c + 1)
             if (GR) {
                 cacc += (v0 ? 1u : 0u) | (v1 ? 0x100u : 0u);

@@ -25,6 +25,9 @@ struct CostLut {
     float grad[511];
     float scale;
     int pad[1];
+    // ceil(2^31 / n) for the window sizes n = 1 .. 2 * kMaxArm + 1 (entry 0 unused):
+    // floor(s / n) = umulhi(2s, mag31[n]) for s * n < 2^31
+    uint32_t mag31[2 * 62 + 2];
 };
 
 // Launch-shape choices of the fused schedule (0 = default): output columns per
@@ -41,6 +44,7 @@ void build_cost_lut(const apg_hip_params& p, CostLut* out);
 
 // Device-side limits of this build (checked by apg_hip_validate_params).
 constexpr int kMaxArm = 62;           // Cross arm length; window <= 125 columns/rows
+static_assert(sizeof(CostLut::mag31) / sizeof(uint32_t) == 2 * kMaxArm + 2, "one magic per window size");
 constexpr int kMaxD = 1024;           // disparity levels
 constexpr int kMaxSgmK = 4;           // packed u16 pairs per lane in the SGM scan (D <= 512)
 
