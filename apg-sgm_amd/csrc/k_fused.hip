@@ -214,11 +214,14 @@ constexpr int K2_SEG = 64;   // measured best at 1280x720 D=128 (more waves; the
 
 struct K2Shm {
     size_t lut, mag, col, chi, rng, outr, rre, rro, key, ring, cring, total;
+    size_t col_st, chi_st, rng_st, outr_st, rr_st, key_st;   // per-row copy strides
 };
 
 // gr: per-pixel ranges (the general form): each column's [dmin, dmax) and a
-// second ring of u8-pair valid counts
-__host__ __device__ inline K2Shm k2_layout(int D, int Lmax, int seg, bool sym, bool gr = false) {
+// second ring of u8-pair valid counts.  nr: image rows per wave (2 when
+// D <= 64: each half-wave runs one row); the per-row arrays hold nr copies,
+// copy r at offset r * (the array's stride member).
+__host__ __device__ inline K2Shm k2_layout(int D, int Lmax, int seg, bool sym, bool gr = false, int nr = 1) {
     const int ncol = seg + 2 * Lmax;
     const int nrr = (ncol + D + 2) / 2 + 1;   // records per parity array
     K2Shm s;
@@ -226,14 +229,20 @@ __host__ __device__ inline K2Shm k2_layout(int D, int Lmax, int seg, bool sym, b
     s.lut = o; o += (65 + 256 + 511) * 4;
     s.mag = o; o += (2 * kMaxArm + 2) * 4;
     o = (o + 15) & ~(size_t)15;
-    s.col = o; o += (size_t)(ncol + 2) * 16;   // + the records read ahead of the last column
-    s.chi = o; o += sym ? 0 : (size_t)(ncol + 2) * 4;
-    s.rng = o; o += gr ? (size_t)(ncol + 2) * 4 : 0;
+    s.col_st = (size_t)(ncol + 2) * 16;        // + the records read ahead of the last column
+    s.col = o; o += nr * s.col_st;
+    s.chi_st = sym ? 0 : (size_t)(ncol + 2) * 4;
+    s.chi = o; o += nr * s.chi_st;
+    s.rng_st = gr ? (size_t)(ncol + 2) * 4 : 0;
+    s.rng = o; o += nr * s.rng_st;
     o = (o + 15) & ~(size_t)15;
-    s.outr = o; o += (size_t)(seg + 1) * 16;
-    s.rre = o; o += (size_t)nrr * 16;
-    s.rro = o; o += (size_t)nrr * 16;
-    s.key = o; o += (size_t)(seg + D - 1) * 4;
+    s.outr_st = (size_t)(seg + 1) * 16;
+    s.outr = o; o += nr * s.outr_st;
+    s.rr_st = (size_t)nrr * 16;
+    s.rre = o; o += nr * s.rr_st;
+    s.rro = o; o += nr * s.rr_st;
+    s.key_st = ((size_t)(seg + D - 1) * 4 + 15) & ~(size_t)15;
+    s.key = o; o += nr * s.key_st;
     o = (o + 15) & ~(size_t)15;
     s.ring = o; o += (size_t)(2 * Lmax + 2) * 256;
     s.cring = o; o += gr ? (size_t)(2 * Lmax + 2) * 128 : 0;
@@ -252,7 +261,10 @@ __device__ __forceinline__ uint32_t grad4(uint32_t gx, uint32_t gy) { return (gx
 // states from u8-pair prefix counts (a segment has fewer than 256 columns),
 // and an output pixel's T is valid for d in its own range with a non-empty
 // window (reference src/cost_aggregator.cpp:74-105).
-template <bool SYM, bool FULL, bool NOCLAMP, bool GR>
+// R2 (D <= 64): lanes 0-31 run row 2 * blockIdx.y and lanes 32-63 the row
+// below, each half over the D disparities (FULL: D == 64), so no lane idles
+// for want of disparities; each row has its own records, arms and keys.
+template <bool SYM, bool FULL, bool NOCLAMP, bool GR, bool R2>
 __global__ __launch_bounds__(64) void cost_crossh2_kernel(
     const CostLut* __restrict__ lut, int w, int h, int d0, int D, int cost_max, int Lmax, int seg,
     const uint8_t* __restrict__ gl, const uint8_t* __restrict__ gxl, const uint8_t* __restrict__ gyl,
@@ -261,30 +273,34 @@ __global__ __launch_bounds__(64) void cost_crossh2_kernel(
     const uint64_t* __restrict__ c64r, const int16_t* __restrict__ dmin, const int16_t* __restrict__ dmax,
     const uint32_t* __restrict__ arms, uint8_t* __restrict__ out, uint32_t* __restrict__ rkeys) {
     extern __shared__ __align__(16) unsigned char shm[];
-    const K2Shm L = k2_layout(D, Lmax, seg, SYM, GR);
+    constexpr int NR = R2 ? 2 : 1;
+    const K2Shm L = k2_layout(D, Lmax, seg, SYM, GR, NR);
     float* s_lc = (float*)(shm + L.lut);
     float* s_la = s_lc + 65;
     float* s_lg = s_la + 256;
     uint32_t* s_mag = (uint32_t*)(shm + L.mag);
-    uint4* s_col = (uint4*)(shm + L.col);    // census lo, rep4(g), grad4(gx, gy), x (kFarLeft outside)
-    uint32_t* s_chi = (uint32_t*)(shm + L.chi);   // census hi (64-bit census)
-    uint32_t* s_rng = (uint32_t*)(shm + L.rng);   // GR: dmin | dmax << 16 (empty outside the image)
-    uint4* s_out = (uint4*)(shm + L.outr);   // ring byte offsets of P(x+R+1) and of P(x-L),
-                                             // ceil(2^31/(L+R+1)), (x-L) | (x+R) << 16
-                                             // (GR: the pixel's dmin | dmax << 16)
+    // Per-row arrays (NR copies, strides in L):
+    //   col: left records: census lo, rep4(g), grad4(gx, gy), x (kFarLeft outside)
+    //   chi: census hi (64-bit census); rng (GR): dmin | dmax << 16 (empty outside the image)
+    //   outr: ring byte offsets of P(x+R+1) and of P(x-L), ceil(2^31/(L+R+1)),
+    //         (x-L) | (x+R) << 16 (GR: the pixel's dmin | dmax << 16)
+    //   rre / rro: right-pixel records by index parity; key: right-view WTA keys
     uint32_t* s_key = (uint32_t*)(shm + L.key);
     unsigned char* s_ring = shm + L.ring;    // [2*Lmax+2][64 lanes] u16-pair doubled prefix sums
     unsigned char* s_cring = shm + L.cring;  // GR: [2*Lmax+2][64 lanes] u8-pair prefix valid counts
 
     const int lane = threadIdx.x;
-    const int y = blockIdx.y;
+    const int rsub = R2 ? lane >> 5 : 0;    // the lane's row of the wave
+    const int ln = R2 ? lane & 31 : lane;   // the lane within its row's group
+    const int yw = blockIdx.y * NR;         // the wave's first row
+    const bool yin = yw + rsub < h;
     const int x0 = blockIdx.x * seg;
     const int ncol = seg + 2 * Lmax, rs = 2 * Lmax + 2;
     const int xb = x0 - Lmax;              // column 0
     const int xs = xb - (d0 + D - 1);      // right pixel of record 0
     const int kbase = x0 - (d0 + D - 1);   // right pixel of s_key[0]
     const int nown = min(seg, w - x0);
-    const size_t row = (size_t)y * w;
+    const size_t roww = (size_t)yw * w;   // the wave's first row
 
     // The staging below issues all of a batch's global loads before the first
     // of them is used, so a wave waits for memory once per batch rather than
@@ -300,6 +316,13 @@ __global__ __launch_bounds__(64) void cost_crossh2_kernel(
     }
     for (int n = lane; n < 2 * kMaxArm + 2; n += 64) s_mag[n] = lut->mag31[n];
     __syncthreads();   // the arms' records below read s_mag
+    for (int r = 0; r < NR && yw + r < h; ++r) {
+    const size_t row = roww + (size_t)r * w;
+    uint4* s_colw = (uint4*)(shm + L.col + r * L.col_st);
+    uint32_t* s_chiw = (uint32_t*)(shm + L.chi + r * L.chi_st);
+    uint32_t* s_rngw = (uint32_t*)(shm + L.rng + r * L.rng_st);
+    uint4* s_outw = (uint4*)(shm + L.outr + r * L.outr_st);
+    const size_t rro_w = L.rro + r * L.rr_st, rre_w = L.rre + r * L.rr_st;
     // left records: columns xb .. xb + ncol - 1 (kFarLeft outside the image)
     for (int c0 = 0; c0 <= ncol + 1; c0 += 256) {
         uint32_t lo[4], hi[4], g[4], gx[4], gy[4], rg[4];
@@ -320,10 +343,10 @@ __global__ __launch_bounds__(64) void cost_crossh2_kernel(
         for (int j = 0; j < 4; ++j) {
             const int c = c0 + lane + 64 * j;
             if (c <= ncol + 1) {
-                s_col[c] = in[j] ? make_uint4(lo[j], rep4(g[j]), grad4(gx[j], gy[j]), (uint32_t)(xb + c))
-                                 : make_uint4(0u, 0u, 0u, (uint32_t)kFarLeft);
-                if (!SYM) s_chi[c] = in[j] ? hi[j] : 0u;
-                if (GR) s_rng[c] = in[j] ? rg[j] : 0u;
+                s_colw[c] = in[j] ? make_uint4(lo[j], rep4(g[j]), grad4(gx[j], gy[j]), (uint32_t)(xb + c))
+                                  : make_uint4(0u, 0u, 0u, (uint32_t)kFarLeft);
+                if (!SYM) s_chiw[c] = in[j] ? hi[j] : 0u;
+                if (GR) s_rngw[c] = in[j] ? rg[j] : 0u;
             }
         }
     }
@@ -352,7 +375,7 @@ __global__ __launch_bounds__(64) void cost_crossh2_kernel(
                 r.z = s_mag[La + Ra + 1];
                 r.w = GR ? rg[j] : (uint32_t)(x - La) | ((uint32_t)(x + Ra) << 16);
             }
-            s_out[o] = r;
+            s_outw[o] = r;
         }
     }
     // right pixel xs + i -> parity array i & 1, entry i >> 1
@@ -376,10 +399,11 @@ __global__ __launch_bounds__(64) void cost_crossh2_kernel(
             if (i >= ncol + D + 1) continue;
             const uint4 r = in[j] ? make_uint4(lo[j], rep4(g[j]), grad4(gx[j], gy[j]), SYM ? 0u : hi[j])
                                   : make_uint4(0u, 0u, 0u, 0u);
-            ((uint4*)(shm + ((i & 1) ? L.rro : L.rre)))[i >> 1] = r;
+            ((uint4*)(shm + ((i & 1) ? rro_w : rre_w)))[i >> 1] = r;
         }
     }
-    for (int i = lane; i < seg + D - 1; i += 64) s_key[i] = 0xFFFFFFFFu;
+    }
+    for (int i = lane; i < NR * (int)(L.key_st / 4); i += 64) s_key[i] = 0xFFFFFFFFu;
     __syncthreads();
 
     const float scale = lut->scale;
@@ -388,10 +412,19 @@ __global__ __launch_bounds__(64) void cost_crossh2_kernel(
     const uint32_t uw = (uint32_t)w;
     const uint32_t la_base = (uint32_t)(uintptr_t)(__attribute__((address_space(3))) float*)s_la;
     const uint32_t lg_base = (uint32_t)(uintptr_t)(__attribute__((address_space(3))) float*)s_lg;
+    // the lane's row's copies of the per-row arrays
+    const uint4* s_colr = (const uint4*)(shm + L.col + rsub * L.col_st);
+    const uint32_t* s_chir = (const uint32_t*)(shm + L.chi + rsub * L.chi_st);
+    const uint32_t* s_rngr = (const uint32_t*)(shm + L.rng + rsub * L.rng_st);
+    const uint4* s_outr = (const uint4*)(shm + L.outr + rsub * L.outr_st);
+    uint32_t* s_keyr = (uint32_t*)(shm + L.key + rsub * L.key_st);
+    const size_t rr_r = rsub * L.rr_st;
+    // T of the lane's row: its byte offset from the wave's first row
+    const uint32_t voff_row = (uint32_t)rsub * uw * (uint32_t)D;
     for (int dc = 0; dc < D; dc += 128) {
-        const int di0 = dc + 2 * lane;   // the lane's pair (di0, di0 + 1); D even
+        const int di0 = dc + 2 * ln;   // the lane's pair (di0, di0 + 1); D even
         const int dA = d0 + di0;
-        const bool act = FULL || di0 < D;
+        const bool act = (FULL || di0 < D) && (!R2 || yin);
         const int dtop = d0 + min(dc + 127, D - 1);   // largest d of the chunk
         const bool nonneg = d0 >= 0;
         // record of the right pixel x - d for (c, di) is c + D - 1 - di = c + 2m + 1
@@ -399,19 +432,19 @@ __global__ __launch_bounds__(64) void cost_crossh2_kernel(
         const int m = act ? D / 2 - 1 - di0 / 2 : 0;
         auto rr_at = [&](int j) -> uint4 {   // record 2m + 1 + j
             const int u = j + 1;
-            return ((const uint4*)(shm + ((u & 1) ? L.rro : L.rre)))[(u >> 1) + m];
+            return ((const uint4*)(shm + ((u & 1) ? L.rro : L.rre) + rr_r))[(u >> 1) + m];
         };
         // s_key entry of the right pixel xr = x - dA is xr - kbase
-        uint32_t* key_lane = s_key - dA - kbase;
+        uint32_t* key_lane = s_keyr - dA - kbase;
         *(uint32_t*)(s_ring + lofs) = 0u;   // P(0)
         if (GR) *(uint16_t*)(s_cring + lofs / 2) = 0u;
         uint32_t acc = 0u, cacc = 0u;
         uint32_t ringw = 256u;              // ring byte offset of P(c + 1)
         const uint32_t ringe = (uint32_t)rs * 256u;
         // pipeline registers: records of the next two columns, LUT terms of the next one
-        uint4 cr_n = s_col[0], r0_n = rr_at(0), r1_n = rr_at(-1), or_n = s_out[0];
-        uint32_t ch_n = SYM ? 0u : s_chi[0];
-        uint32_t rg_n = GR ? s_rng[0] : 0u, rg_c = 0u;
+        uint4 cr_n = s_colr[0], r0_n = rr_at(0), r1_n = rr_at(-1), or_n = s_outr[0];
+        uint32_t ch_n = SYM ? 0u : s_chir[0];
+        uint32_t rg_n = GR ? s_rngr[0] : 0u, rg_c = 0u;
         float tc0, ta0, tg0, tc1, ta1, tg1;
         // LDS byte addresses of the AD and gradient LUT entries come out of the
         // SAD instructions themselves (the table base is their accumulator)
@@ -428,11 +461,11 @@ __global__ __launch_bounds__(64) void cost_crossh2_kernel(
         };
         lut_terms(cr_n, ch_n, r0_n, r1_n);   // column 0
         uint4 cr_c = cr_n;                   // column 0's record (its x)
-        cr_n = s_col[1];
-        if (!SYM) ch_n = s_chi[1];
+        cr_n = s_colr[1];
+        if (!SYM) ch_n = s_chir[1];
         if (GR) {
             rg_c = rg_n;
-            rg_n = s_rng[1];
+            rg_n = s_rngr[1];
         }
         r0_n = rr_at(1);
         r1_n = rr_at(0);
@@ -480,8 +513,8 @@ __global__ __launch_bounds__(64) void cost_crossh2_kernel(
                 q1 = qq[1];
             }
             // bytes q0, q1 (each < 256) -> one 16-bit store
-            if (act) buf_st16(rout, __builtin_amdgcn_perm(q1, q0, 0x0c0c0400u), (uint32_t)di0,
-                              (uint32_t)((int)row + pxo) * (uint32_t)D);
+            if (act) buf_st16(rout, __builtin_amdgcn_perm(q1, q0, 0x0c0c0400u), (uint32_t)di0 + voff_row,
+                              (uint32_t)((int)roww + pxo) * (uint32_t)D);
         };
 
         // keys: own column (right-view WTA candidates); outp: an output column's
@@ -499,11 +532,11 @@ __global__ __launch_bounds__(64) void cost_crossh2_kernel(
             // LUT terms of column c + 1 and records of column c + 2 (padded past the end)
             lut_terms(cr_n, ch_n, r0_n, r1_n);
             cr_c = cr_n;
-            cr_n = s_col[c + 2];
-            if (!SYM) ch_n = s_chi[c + 2];
+            cr_n = s_colr[c + 2];
+            if (!SYM) ch_n = s_chir[c + 2];
             if (GR) {
                 rg_c = rg_n;
-                rg_n = s_rng[c + 2];
+                rg_n = s_rngr[c + 2];
             }
             // d + 1 at column c + 2 pairs with the right pixel of d at column c + 1
             r1_n = r0_n;
@@ -524,7 +557,7 @@ __global__ __launch_bounds__(64) void cost_crossh2_kernel(
                 v1 = v1 && dA + 1 >= lo && dA + 1 < hi;
             }
             if (keys) {
-                uint32_t* k0p = FULL ? key_lane + (int)cr.w : s_key + max(xr0 - kbase, 1);
+                uint32_t* k0p = FULL ? key_lane + (int)cr.w : s_keyr + max(xr0 - kbase, 1);
                 atomicMin(k0p, v0 ? (cst0 << 16) | (uint32_t)di0 : 0xFFFFFFFFu);
                 atomicMin(k0p - 1, v1 ? (cst1 << 16) | (uint32_t)(di0 + 1) : 0xFFFFFFFFu);
             }
@@ -544,7 +577,7 @@ __global__ __launch_bounds__(64) void cost_crossh2_kernel(
             pend = outp;
             if (outp) {
                 por = or_n;
-                or_n = s_out[c - 2 * Lmax + 1];
+                or_n = s_outr[c - 2 * Lmax + 1];
                 pxo = x0 + c - 2 * Lmax;
                 pp1 = *(const uint32_t*)(s_ring + por.x + lofs);
                 pp0 = *(const uint32_t*)(s_ring + por.y + lofs);
@@ -611,10 +644,13 @@ __global__ __launch_bounds__(64) void cost_crossh2_kernel(
         if (pend) flush(FF{});
     }
     __syncthreads();
-    for (int i = lane; i < seg + D - 1; i += 64) {
-        const int xr = kbase + i;
-        const uint32_t k = s_key[i];
-        if (xr >= 0 && xr < w && k != 0xFFFFFFFFu) atomicMin(&rkeys[row + xr], k);
+    for (int r = 0; r < NR && yw + r < h; ++r) {
+        const uint32_t* s_keyw = (const uint32_t*)(shm + L.key + r * L.key_st);
+        for (int i = lane; i < seg + D - 1; i += 64) {
+            const int xr = kbase + i;
+            const uint32_t k = s_keyw[i];
+            if (xr >= 0 && xr < w && k != 0xFFFFFFFFu) atomicMin(&rkeys[roww + (size_t)r * w + xr], k);
+        }
     }
 }
 
@@ -633,10 +669,12 @@ hipError_t launch_cost_crossh(hipStream_t s, const apg_hip_params& p, const Cost
     const bool granges = !column_ranges;
     if (cross && out_u8 && D % 2 == 0 && (size_t)w * h * D < (1ull << 31) && (!granges || seg2 + 2 * Lmax <= 255)) {
         const int seg = seg2;
-        const K2Shm L2 = k2_layout(D, Lmax, seg, p.census_type == 1, granges);
-        const bool full = D % 128 == 0;
-        decltype(&cost_crossh2_kernel<true, true, true, false>) k;
-#define APG_K2(S, F, N) (granges ? cost_crossh2_kernel<S, F, N, true> : cost_crossh2_kernel<S, F, N, false>)
+        const bool r2 = D <= 64;   // two rows per wave
+        const K2Shm L2 = k2_layout(D, Lmax, seg, p.census_type == 1, granges, r2 ? 2 : 1);
+        const bool full = r2 ? D == 64 : D % 128 == 0;
+        decltype(&cost_crossh2_kernel<true, true, true, false, false>) k;
+#define APG_K2R(S, F, N, G) (r2 ? cost_crossh2_kernel<S, F, N, G, true> : cost_crossh2_kernel<S, F, N, G, false>)
+#define APG_K2(S, F, N) (granges ? APG_K2R(S, F, N, true) : APG_K2R(S, F, N, false))
         if (clamp_free)
             k = p.census_type == 1 ? (full ? APG_K2(true, true, true) : APG_K2(true, false, true))
                                    : (full ? APG_K2(false, true, true) : APG_K2(false, false, true));
@@ -644,12 +682,14 @@ hipError_t launch_cost_crossh(hipStream_t s, const apg_hip_params& p, const Cost
             k = p.census_type == 1 ? (full ? APG_K2(true, true, false) : APG_K2(true, false, false))
                                    : (full ? APG_K2(false, true, false) : APG_K2(false, false, false));
 #undef APG_K2
+#undef APG_K2R
         if (L2.total > 64 * 1024) {
             hipError_t e =
                 hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)L2.total);
             if (e != hipSuccess) return e;
         }
-        hipLaunchKernelGGL(k, dim3((w + seg - 1) / seg, h), dim3(64), L2.total, s, d_lut, w, h, p.min_disparity, D,
+        hipLaunchKernelGGL(k, dim3((w + seg - 1) / seg, r2 ? (h + 1) / 2 : h), dim3(64), L2.total, s, d_lut, w, h,
+                           p.min_disparity, D,
                            p.cost_max, Lmax, seg, gl, gxl, gyl, gr, gxr, gyr, c32l, c32r, c64l, c64r, dmin, dmax, arms,
                            (uint8_t*)out, rkeys);
         return hipGetLastError();
