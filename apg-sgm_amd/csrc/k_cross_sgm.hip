@@ -362,12 +362,13 @@ __global__ __launch_bounds__(64) void cross_v3_kernel(int w, int h, int d0, int 
 
 // Vertical Cross pass over u8 T for per-pixel ranges (the prior's): the grid
 // of cross_v3_kernel (column, row segment, 128-wide d chunk; the prefix of
-// input rows [y0 - Lmax, y1 + Lmax) in an LDS ring of 2*Lmax + 2 rows) with
-// cross_v2_kernel's packed (sum << 8 | count) u32 prefixes, since the number
-// of valid entries in a window varies with d.  Window sums stay below
-// 125 * 254 * 256 + 125 < 2^32, so prefix differences are exact modulo 2^32.
-// Per step the ring slots it touches and the output pixel's range come from
-// a table built once per segment.  Rows past the image bottom add to prefixes
+// input rows [y0 - Lmax, y1 + Lmax) in LDS rings of 2*Lmax + 2 rows).  Since
+// the number of valid entries of a window varies with d, a second ring keeps
+// u8-pair prefix counts beside the doubled u16-pair prefix sums (a segment
+// has at most 255 input rows, so a count byte never carries; window sums stay
+// below 125 * 254 < 2^15, so doubled prefixes are exact modulo 2^16).  Per
+// step the ring slots it touches and the output pixel's range come from a
+// table built once per segment.  Rows past the image bottom add to prefixes
 // no output reads (a window ends at row h - 1 at most).
 constexpr int CV4_PF = 16;
 
@@ -384,10 +385,11 @@ __global__ __launch_bounds__(64) void cross_v4_kernel(int w, int h, int d0, int 
     const int di = blockIdx.z * 128 + 2 * lane;
     const int rs = 2 * Lmax + 2;
     const int ya = max(0, y0 - Lmax), yb = min(h, y1 + Lmax), kend = y1 + Lmax, nk = kend - ya;
-    uint32_t* magic = sh;                              // [2*kMaxArm + 2]
-    uint2* ring = (uint2*)(sh + 2 * kMaxArm + 2);      // [rs][64]: the lane's two prefixes
-    uint2* meta = ring + rs * 64;                      // [nk]: slots P(k+1) | P(y+Dn+1) << 8 | P(y-U) << 16, range
-    for (int c = lane; c < 2 * kMaxArm + 2; c += 64) magic[c] = magic_for((uint32_t)c);
+    uint32_t* mag = sh;                                // [2*kMaxArm + 2] ceil(2^31 / n)
+    uint32_t* ring = sh + 2 * kMaxArm + 2;             // [rs][64]: doubled u16-pair prefix sums
+    uint16_t* cring = (uint16_t*)(ring + rs * 64);     // [rs][64]: u8-pair prefix valid counts
+    uint2* meta = (uint2*)(cring + ((rs * 64 + 3) & ~3));   // [nk]: slots P(k+1) | P(y+Dn+1) << 8 | P(y-U) << 16, range
+    for (int c = lane; c < 2 * kMaxArm + 2; c += 64) mag[c] = c ? (0x80000000u + (uint32_t)c - 1u) / (uint32_t)c : 0u;
     for (int r = lane; r < nk; r += 64) {
         const int k = ya + r, y = k - Lmax;
         uint2 m;
@@ -402,7 +404,8 @@ __global__ __launch_bounds__(64) void cross_v4_kernel(int w, int h, int d0, int 
         }
         meta[r] = m;
     }
-    ring[lane] = make_uint2(0u, 0u);   // P(ya)
+    ring[lane] = 0u;   // P(ya)
+    cring[lane] = 0u;
     __syncthreads();
     const int dA = d0 + di;
     const bool inA = di < D;   // D even: the pair is inside or outside as a whole
@@ -411,21 +414,28 @@ __global__ __launch_bounds__(64) void cross_v4_kernel(int w, int h, int d0, int 
     const auto rc = raw_rsrc(cost);
     const uint32_t rowb = (uint32_t)w * (uint32_t)D, colb = (uint32_t)x * (uint32_t)D;
     const int kout = y0 + Lmax;   // first step with an output row
-    uint32_t accA = 0, accB = 0;
+    uint32_t acc = 0, cacc = 0;
     auto step = [&](int k, uint32_t raw, uint2 m) {
-        const uint32_t a = raw & 0xFFu, b = (raw >> 8) & 0xFFu;
-        accA += (inA && a != 0xFFu) ? (a << 8) | 1u : 0u;
-        accB += (inA && b != 0xFFu) ? (b << 8) | 1u : 0u;
-        ring[(m.x & 0xFFu) * 64 + lane] = make_uint2(accA, accB);
+        const bool va = inA && (raw & 0xFFu) != 0xFFu, vb = inA && (raw & 0xFF00u) != 0xFF00u;
+        // the two T bytes as u16 halves, the invalid ones zeroed; acc += 2 * them
+        const uint32_t pr = bytes_to_halves(raw) & ((va ? 0xFFFFu : 0u) | (vb ? 0xFFFF0000u : 0u));
+        {
+            uint32_t nacc;
+            asm("v_pk_mad_u16 %0, %1, %2, %3" : "=v"(nacc) : "v"(pr), "s"(0x00020002u), "v"(acc));
+            acc = nacc;
+        }
+        cacc += (va ? 1u : 0u) | (vb ? 0x100u : 0u);
+        ring[(m.x & 0xFFu) * 64 + lane] = acc;
+        cring[(m.x & 0xFFu) * 64 + lane] = (uint16_t)cacc;
         if (k < kout) return;
-        const uint2 p1 = ring[((m.x >> 8) & 0xFFu) * 64 + lane];
-        const uint2 p0 = ring[(m.x >> 16) * 64 + lane];
-        const uint32_t sa = p1.x - p0.x, sb = p1.y - p0.y;
+        const uint32_t s1 = (m.x >> 8) & 0xFFu, s0 = m.x >> 16;
+        const uint32_t sw = as_u32(as_v2(ring[s1 * 64 + lane]) - as_v2(ring[s0 * 64 + lane]));
+        const uint32_t wc = (uint32_t)cring[s1 * 64 + lane] - (uint32_t)cring[s0 * 64 + lane];
         const int lo = (int16_t)(m.y & 0xFFFFu), hi = (int16_t)(m.y >> 16);
-        const uint32_t ca = sa & 0xFFu, cb = sb & 0xFFu;
-        const uint32_t oa = (dA >= lo && dA < hi && ca) ? div_small(sa >> 8, ca, magic[ca]) : 0xFFu;
-        const uint32_t ob = (dA + 1 >= lo && dA + 1 < hi && cb) ? div_small(sb >> 8, cb, magic[cb]) : 0xFFu;
-        if (inA) buf_st16(rc, oa | (ob << 8), voff, (uint32_t)(k - Lmax) * rowb + colb);
+        const uint32_t ca = wc & 0xFFu, cb = (wc >> 8) & 0xFFu;
+        const uint32_t oa = (dA >= lo && dA < hi && ca) ? __umulhi(sw & 0xFFFFu, mag[ca]) : 0xFFu;
+        const uint32_t ob = (dA + 1 >= lo && dA + 1 < hi && cb) ? __umulhi(sw >> 16, mag[cb]) : 0xFFu;
+        if (inA) buf_st16(rc, __builtin_amdgcn_perm(ob, oa, 0x0c0c0400u), voff, (uint32_t)(k - Lmax) * rowb + colb);
     };
     uint32_t X[CV4_PF], Y[CV4_PF];
     auto load = [&](uint32_t (&bf)[CV4_PF], int kb) {
@@ -480,9 +490,12 @@ hipError_t launch_cross_v_t(hipStream_t s, const apg_hip_params& p, int w, int h
             const int target = tn.cross_v_waves > 0 ? tn.cross_v_waves : 16384;
             int nseg = (target + w * dch - 1) / (w * dch);
             nseg = max(1, min(nseg, (h + 63) / 64));
+            // a segment's input rows (its own plus the arm halo) fit a count byte
+            nseg = max(nseg, (h + (255 - 2 * Lmax) - 1) / (255 - 2 * Lmax));
             const int seg = (h + nseg - 1) / nseg;
             nseg = (h + seg - 1) / seg;
-            const size_t shm = (2 * kMaxArm + 2) * sizeof(uint32_t) + (size_t)(2 * Lmax + 2) * 64 * 8 +
+            const size_t rs = (size_t)(2 * Lmax + 2);
+            const size_t shm = (2 * kMaxArm + 2) * sizeof(uint32_t) + rs * 64 * 4 + ((rs * 64 + 3) & ~(size_t)3) * 2 +
                                (size_t)(seg + 2 * Lmax) * 8;
             hipLaunchKernelGGL(cross_v4_kernel, dim3(w, nseg, dch), dim3(64), shm, s, w, h, p.min_disparity, D, Lmax,
                                seg, dmin, dmax, arms, (const uint8_t*)tmp, (uint8_t*)cost);
