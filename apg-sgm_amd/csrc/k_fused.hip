@@ -358,7 +358,7 @@ __global__ __launch_bounds__(64) void cost_crossh2_kernel(
             const int o = o0 + lane + 64 * j, x = x0 + o;
             in[j] = o < seg && x < w;
             const size_t q = row + min(x, w - 1);
-            a[j] = arms[q];
+            a[j] = arms ? arms[q] : 0u;   // no Cross pass: windows of one column
             if (GR) rg[j] = (uint32_t)(uint16_t)dmin[q] | ((uint32_t)(uint16_t)dmax[q] << 16);
         }
 #pragma unroll
@@ -669,7 +669,9 @@ hipError_t launch_cost_crossh(hipStream_t s, const apg_hip_params& p, const Cost
     seg2 = w < seg2 ? w : seg2;
     // per-pixel ranges: the valid counts are u8 prefixes over a segment's columns
     const bool granges = !column_ranges;
-    if (cross && out_u8 && D % 2 == 0 && (size_t)w * h * D < (1ull << 31) && (!granges || seg2 + 2 * Lmax <= 255)) {
+    // without the Cross pass (Fast) the same kernel runs with one-column
+    // windows (Lmax 0, no arms): T is then the cost itself, 255 where invalid
+    if (out_u8 && D % 2 == 0 && (size_t)w * h * D < (1ull << 31) && (!granges || seg2 + 2 * Lmax <= 255)) {
         const int seg = seg2;
         const bool r2 = D <= 64;   // two rows per wave
         const K2Shm L2 = k2_layout(D, Lmax, seg, p.census_type == 1, granges, r2 ? 2 : 1);
@@ -692,8 +694,8 @@ hipError_t launch_cost_crossh(hipStream_t s, const apg_hip_params& p, const Cost
         }
         hipLaunchKernelGGL(k, dim3((w + seg - 1) / seg, r2 ? (h + 1) / 2 : h), dim3(64), L2.total, s, d_lut, w, h,
                            p.min_disparity, D,
-                           p.cost_max, Lmax, seg, gl, gxl, gyl, gr, gxr, gyr, c32l, c32r, c64l, c64r, dmin, dmax, arms,
-                           (uint8_t*)out, rkeys);
+                           p.cost_max, Lmax, seg, gl, gxl, gyl, gr, gxr, gyr, c32l, c32r, c64l, c64r, dmin, dmax,
+                           cross ? arms : nullptr, (uint8_t*)out, rkeys);
         return hipGetLastError();
     }
     const FusedShm L = fused_layout(D, Lmax);
