@@ -2,6 +2,8 @@
 // (sum << 8 | count) prefix sums) and the SGM path scans (one wavefront per
 // scanline, disparities across lanes as packed u16 pairs, DPP neighbour
 // exchange and wave-min reduction).
+#include <type_traits>
+
 #include "device_common.hpp"
 #include "kernels.hpp"
 
@@ -338,24 +340,61 @@ __global__ __launch_bounds__(64) void cross_v3_kernel(int w, int h, int d0, int 
         if (inA) buf_st16(rc, out, voff, (uint32_t)(k - Lmax) * rowb + colb);
     };
     uint32_t X[CV3_PF], Y[CV3_PF];
+    // a batch's rows past the image bottom reload its last row (their prefix
+    // entries are never read); whole batches inside it skip the clamps
     auto load = [&](uint32_t (&b)[CV3_PF], int kb) {
+        if (kb + CV3_PF <= yb) {
 #pragma unroll
-        for (int i = 0; i < CV3_PF; ++i) b[i] = buf_ld16(rt, voff, (uint32_t)min(kb + i, yb - 1) * rowb + colb);
+            for (int i = 0; i < CV3_PF; ++i) b[i] = buf_ld16(rt, voff, (uint32_t)(kb + i) * rowb + colb);
+        } else {
+#pragma unroll
+            for (int i = 0; i < CV3_PF; ++i) b[i] = buf_ld16(rt, voff, (uint32_t)min(kb + i, yb - 1) * rowb + colb);
+        }
     };
-    auto batch = [&](const uint32_t (&b)[CV3_PF], int kb) {
+    // OUT 0: no step of the batch has an output row; 1: every step has one and
+    // lies before kend; 2: either may vary (checked per step)
+    auto batch = [&](const uint32_t (&b)[CV3_PF], int kb, auto out_t) {
+        constexpr int OUT = decltype(out_t)::value;
         uint4 m[CV3_PF];
+        if (OUT == 2) {
 #pragma unroll
-        for (int i = 0; i < CV3_PF; ++i) m[i] = meta[min(kb + i - ya, nk - 1)];
+            for (int i = 0; i < CV3_PF; ++i) m[i] = meta[min(kb + i - ya, nk - 1)];
+        } else {
+            const uint4* mb = meta + (kb - ya);
 #pragma unroll
-        for (int i = 0; i < CV3_PF; ++i)
-            if (kb + i < kend) step(kb + i, b[i], m[i]);
+            for (int i = 0; i < CV3_PF; ++i) m[i] = mb[i];
+        }
+#pragma unroll
+        for (int i = 0; i < CV3_PF; ++i) {
+            if (OUT == 2) {
+                if (kb + i < kend) step(kb + i, b[i], m[i]);
+            } else {
+                acc = as_u32(as_v2(acc) + as_v2(bytes_to_halves(b[i])));
+                *(uint32_t*)((uint8_t*)ring + m[i].x + lofs) = acc;
+                if (OUT == 1) {
+                    const uint32_t p1 = *(const uint32_t*)((const uint8_t*)ring + m[i].y + lofs);
+                    const uint32_t p0 = *(const uint32_t*)((const uint8_t*)ring + m[i].z + lofs);
+                    const uint32_t sd = as_u32(as_v2(p1) - as_v2(p0));
+                    const uint32_t s2 = as_u32(as_v2(sd) + as_v2(sd));
+                    const uint32_t qa = __umulhi(s2 & 0xFFFFu, m[i].w), qb = __umulhi(s2 >> 16, m[i].w);
+                    if (inA)
+                        buf_st16(rc, __builtin_amdgcn_perm(qb, qa, 0x0c0c0400u) | omask, voff,
+                                 (uint32_t)(kb + i - Lmax) * rowb + colb);
+                }
+            }
+        }
+    };
+    auto run = [&](const uint32_t (&b)[CV3_PF], int kb) {
+        if (kb + CV3_PF <= kout) batch(b, kb, std::integral_constant<int, 0>{});
+        else if (kb >= kout && kb + CV3_PF <= kend) batch(b, kb, std::integral_constant<int, 1>{});
+        else batch(b, kb, std::integral_constant<int, 2>{});
     };
     load(X, ya);
     load(Y, ya + CV3_PF);
     for (int kb = ya; kb < kend; kb += 2 * CV3_PF) {
-        batch(X, kb);
+        run(X, kb);
         load(X, kb + 2 * CV3_PF);
-        batch(Y, kb + CV3_PF);
+        run(Y, kb + CV3_PF);
         load(Y, kb + 3 * CV3_PF);
     }
 }
