@@ -454,7 +454,8 @@ __global__ __launch_bounds__(64) void cross_v4_kernel(int w, int h, int d0, int 
     const uint32_t rowb = (uint32_t)w * (uint32_t)D, colb = (uint32_t)x * (uint32_t)D;
     const int kout = y0 + Lmax;   // first step with an output row
     uint32_t acc = 0, cacc = 0;
-    auto step = [&](int k, uint32_t raw, uint2 m) {
+    // kt decides whether the step has an output row (kt >= kout); k is its input row
+    auto step = [&](int kt, uint32_t raw, uint2 m, int k) {
         const bool va = inA && (raw & 0xFFu) != 0xFFu, vb = inA && (raw & 0xFF00u) != 0xFF00u;
         // the two T bytes as u16 halves, the invalid ones zeroed; acc += 2 * them
         const uint32_t pr = bytes_to_halves(raw) & ((va ? 0xFFFFu : 0u) | (vb ? 0xFFFF0000u : 0u));
@@ -466,7 +467,7 @@ __global__ __launch_bounds__(64) void cross_v4_kernel(int w, int h, int d0, int 
         cacc += (va ? 1u : 0u) | (vb ? 0x100u : 0u);
         ring[(m.x & 0xFFu) * 64 + lane] = acc;
         cring[(m.x & 0xFFu) * 64 + lane] = (uint16_t)cacc;
-        if (k < kout) return;
+        if (kt < kout) return;
         const uint32_t s1 = (m.x >> 8) & 0xFFu, s0 = m.x >> 16;
         const uint32_t sw = as_u32(as_v2(ring[s1 * 64 + lane]) - as_v2(ring[s0 * 64 + lane]));
         const uint32_t wc = (uint32_t)cring[s1 * 64 + lane] - (uint32_t)cring[s0 * 64 + lane];
@@ -477,24 +478,48 @@ __global__ __launch_bounds__(64) void cross_v4_kernel(int w, int h, int d0, int 
         if (inA) buf_st16(rc, __builtin_amdgcn_perm(ob, oa, 0x0c0c0400u), voff, (uint32_t)(k - Lmax) * rowb + colb);
     };
     uint32_t X[CV4_PF], Y[CV4_PF];
+    // as in cross_v3_kernel: unclamped loads for whole batches inside the
+    // image, and batches classified once by their output rows
     auto load = [&](uint32_t (&bf)[CV4_PF], int kb) {
+        if (kb + CV4_PF <= yb) {
 #pragma unroll
-        for (int i = 0; i < CV4_PF; ++i) bf[i] = buf_ld16(rt, voff, (uint32_t)min(kb + i, yb - 1) * rowb + colb);
+            for (int i = 0; i < CV4_PF; ++i) bf[i] = buf_ld16(rt, voff, (uint32_t)(kb + i) * rowb + colb);
+        } else {
+#pragma unroll
+            for (int i = 0; i < CV4_PF; ++i) bf[i] = buf_ld16(rt, voff, (uint32_t)min(kb + i, yb - 1) * rowb + colb);
+        }
     };
-    auto batch = [&](const uint32_t (&bf)[CV4_PF], int kb) {
+    auto batch = [&](const uint32_t (&bf)[CV4_PF], int kb, auto out_t) {
+        constexpr int OUT = decltype(out_t)::value;   // 0 none, 1 every step, 2 checked
         uint2 m[CV4_PF];
+        if (OUT == 2) {
 #pragma unroll
-        for (int i = 0; i < CV4_PF; ++i) m[i] = meta[min(kb + i - ya, nk - 1)];
+            for (int i = 0; i < CV4_PF; ++i) m[i] = meta[min(kb + i - ya, nk - 1)];
+        } else {
+            const uint2* mb = meta + (kb - ya);
 #pragma unroll
-        for (int i = 0; i < CV4_PF; ++i)
-            if (kb + i < kend) step(kb + i, bf[i], m[i]);
+            for (int i = 0; i < CV4_PF; ++i) m[i] = mb[i];
+        }
+#pragma unroll
+        for (int i = 0; i < CV4_PF; ++i) {
+            if (OUT == 2) {
+                if (kb + i < kend) step(kb + i, bf[i], m[i], kb + i);
+            } else {
+                step(OUT == 1 ? kout : kout - 1, bf[i], m[i], kb + i);
+            }
+        }
+    };
+    auto run = [&](const uint32_t (&bf)[CV4_PF], int kb) {
+        if (kb + CV4_PF <= kout) batch(bf, kb, std::integral_constant<int, 0>{});
+        else if (kb >= kout && kb + CV4_PF <= kend) batch(bf, kb, std::integral_constant<int, 1>{});
+        else batch(bf, kb, std::integral_constant<int, 2>{});
     };
     load(X, ya);
     load(Y, ya + CV4_PF);
     for (int kb = ya; kb < kend; kb += 2 * CV4_PF) {
-        batch(X, kb);
+        run(X, kb);
         load(X, kb + 2 * CV4_PF);
-        batch(Y, kb + CV4_PF);
+        run(Y, kb + CV4_PF);
         load(Y, kb + 3 * CV4_PF);
     }
 }
