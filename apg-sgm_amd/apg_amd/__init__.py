@@ -85,6 +85,7 @@ class Request(C.Structure):
 
 
 BATCH_GRAPH = 1
+BATCH_PER_PAIR = 2
 
 
 class ApgError(RuntimeError):
@@ -159,6 +160,8 @@ def load():
         "apg_hip_batch_workspace_bytes": (sz, [vp]),
         "apg_hip_batch_last_error": (C.c_char_p, [vp]),
         "apg_hip_batch_run": (i, [vp, C.POINTER(Pair), i, i]),
+        "apg_hip_batch_group_stats": (i, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+        "apg_hip_batch_set_group_bytes": (i, [vp, C.c_size_t]),
         "apg_hip_batch_autotune": (i, [vp, C.POINTER(Pair), i, C.c_char_p]),
         "apg_hip_batch_tuning_for": (i, [vp, P, i, i, C.POINTER(Tuning)]),
         "apg_hip_server_create": (i, [i, i, sz, C.POINTER(vp)]),
@@ -409,9 +412,21 @@ class Batch:
             arr[i] = q.as_pair() if isinstance(q, DevicePair) else q
         return arr
 
-    def run(self, pairs, graph=False):
+    def run(self, pairs, graph=False, grouped=True):
         arr = pairs if isinstance(pairs, C.Array) else self.pair_array(pairs)
-        self._check(self.lib.apg_hip_batch_run(self.h, arr, len(arr), BATCH_GRAPH if graph else 0), "batch_run")
+        flags = (BATCH_GRAPH if graph else 0) | (0 if grouped else BATCH_PER_PAIR)
+        self._check(self.lib.apg_hip_batch_run(self.h, arr, len(arr), flags), "batch_run")
+
+    def set_group_bytes(self, nbytes):
+        """Workspace bytes per chunk of group launches (0: the default)."""
+        self._check(self.lib.apg_hip_batch_set_group_bytes(self.h, int(nbytes)), "set_group_bytes")
+
+    def group_stats(self):
+        """(pairs in group launches, group launches, pairs run one at a time)
+        of the last pair list run with group launches."""
+        g, n, s = C.c_int(), C.c_int(), C.c_int()
+        self._check(self.lib.apg_hip_batch_group_stats(self.h, C.byref(g), C.byref(n), C.byref(s)), "group_stats")
+        return g.value, n.value, s.value
 
     def autotune(self, pairs, cache_path=None):
         arr = pairs if isinstance(pairs, C.Array) else self.pair_array(pairs)

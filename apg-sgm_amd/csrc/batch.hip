@@ -1,10 +1,14 @@
 // Batched and serving forms over the device pipeline (host code).
 //
-// Batched form: `slots` pipelines, each a context with its own stream and
-// workspace, forked from and joined back to one origin stream.  Pairs are
-// spread over the slots in longest-processing-time order of W*H*D, each slot
-// runs its pairs back to back, and the kernels of different slots overlap on
-// the device.  A repeated pair list replays as one hipGraph launch.  Launch
+// Batched form: `slots` pipelines, each with its own stream, forked from and
+// joined back to one origin stream.  Pairs are spread over the slots in
+// longest-processing-time order of W*H*D and the kernels of different slots
+// overlap on the device.  Within a slot, the pairs of the fused column-range
+// schedule run as group launches: step by step (aux, Cost + Cross-H,
+// right-view WTA, Cross-V, SGM, WTA, fill + median), one launch per kernel
+// instance spans all of a chunk's pairs that use it (group.hpp), each pair
+// on a workspace of its own; the other pairs run one at a time on the slot's
+// context.  A repeated pair list replays as one hipGraph launch.  Launch
 // shapes come per size bucket from an autotuner with a JSON cache.
 //
 // Serving form: a FIFO of host-memory requests admitted into free slots as
@@ -131,6 +135,39 @@ int run_pair(apg_hip_ctx ctx, const apg_hip_pair& q) {
 
 // ================================================================= batch ====
 
+namespace {
+
+// Workspace bytes of the pairs one slot runs as one chunk of group launches
+// (default).  Smaller chunks keep the slots at different steps, so that
+// latency-bound steps (SGM) of one slot overlap issue-bound ones (Cost) of
+// another: 64 uneven pairs over 4 slots took 41.3 ms in 8 GiB chunks, 37.5 ms
+// in 1 GiB chunks; 64 pairs of 640x480 D=64 9.99 against 10.40 ms with each
+// pair its own kernels.
+constexpr size_t kGroupChunkBytes = size_t(1) << 30;
+
+// One group launch: its kernel instance and the offset of its chunk records
+// in the plan's device buffer.
+struct GroupLaunch {
+    const void* kernel;
+    int threads;
+    size_t shm;
+    uint32_t nblocks;
+    size_t recs_off;
+};
+
+// The group launches of a pair list, per slot in stream order, and the
+// pairs each slot runs one at a time after them.
+struct GroupPlan {
+    bool valid = false;
+    std::vector<apg_hip_pair> pairs;
+    std::vector<std::vector<GroupLaunch>> launches;
+    std::vector<std::vector<int>> single;
+    void* d_tables = nullptr;
+    int grouped = 0, nlaunch = 0, nsingle = 0;
+};
+
+}  // namespace
+
 struct apg_hip_batch_s {
     int device = 0;
     std::string arch;
@@ -141,8 +178,13 @@ struct apg_hip_batch_s {
     std::string last_error;
     std::map<BucketKey, apg_hip_tuning> tuned;
     std::vector<apg_hip_pair> graph_pairs;   // pair list of the cached graph
+    bool graph_grouped = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    // group launches: per slot, one context (workspace) per chunk position
+    std::vector<std::vector<apg_hip_ctx>> gctx;
+    GroupPlan plan;
+    size_t group_bytes = kGroupChunkBytes;
 };
 
 namespace {
@@ -164,6 +206,18 @@ void drop_graph(apg_hip_batch b) {
     b->exec = nullptr;
     b->graph = nullptr;
     b->graph_pairs.clear();
+}
+
+// The plan's tables are read by its launches and by a graph captured from
+// them: both go.
+void drop_plan(apg_hip_batch b) {
+    drop_graph(b);
+    if (b->plan.d_tables) {
+        (void)hipStreamSynchronize(b->origin);
+        for (apg_hip_ctx c : b->slots) (void)apg_hip_synchronize(c);
+        (void)hipFree(b->plan.d_tables);
+    }
+    b->plan = GroupPlan();
 }
 
 // Longest-processing-time assignment: pairs in decreasing W*H*(D+16) (the
@@ -214,6 +268,201 @@ int enqueue(apg_hip_batch b, const apg_hip_pair* pairs, int n) {
     return status;
 }
 
+void set_pair_tuning(apg_hip_batch b, apg_hip_ctx ctx, const apg_hip_pair& q) {
+    const auto it = b->tuned.find(bucket_of(q.params, q.width, q.height));
+    (void)apg_hip_set_tuning(ctx, it == b->tuned.end() ? nullptr : &it->second);
+}
+
+// Host image of the plan's device tables.
+struct TableImage {
+    std::vector<uint8_t> bytes;
+    size_t put(const void* src, size_t n) {
+        const size_t off = (bytes.size() + 255) & ~size_t(255);
+        bytes.resize(off + n);
+        memcpy(bytes.data() + off, src, n);
+        return off;
+    }
+};
+
+// The group launches of one step of a chunk: records with the same kernel
+// instance form one launch (split where its grid would pass 2^31 threads).
+int group_step(apg_hip_batch b, const std::vector<apgk::GroupRec>& recs, TableImage& img,
+               std::vector<GroupLaunch>& out) {
+    std::vector<bool> used(recs.size(), false);
+    for (size_t i = 0; i < recs.size(); ++i) {
+        if (used[i]) continue;
+        const apgk::GroupRec& r0 = recs[i];
+        std::vector<size_t> members;
+        for (size_t j = i; j < recs.size(); ++j)
+            if (!used[j] && recs[j].kernel == r0.kernel && recs[j].threads == r0.threads) {
+                if (recs[j].args.size() != r0.args.size() || recs[j].rec_bytes != r0.rec_bytes)
+                    return bfail(b, 8, "group records of one kernel differ");
+                members.push_back(j);
+                used[j] = true;
+            }
+        size_t m = 0;
+        while (m < members.size()) {
+            // one record per chunk of kGroupChunk blocks: its place in the
+            // pair's grid, that grid's shape and the pair's arguments
+            std::vector<uint8_t> recbuf;
+            size_t shm = 0;
+            uint64_t nblk = 0;
+            for (; m < members.size(); ++m) {
+                const apgk::GroupRec& r = recs[members[m]];
+                const uint64_t count = (uint64_t)r.grid.x * r.grid.y * r.grid.z;
+                const uint64_t nch = (count + apgk::kGroupChunk - 1) >> apgk::kGroupChunkLog2;
+                if (count == 0) continue;
+                const uint64_t next = nblk + (nch << apgk::kGroupChunkLog2);
+                if (nblk && next * (uint64_t)r0.threads >= (1ull << 31)) break;
+                if (next * (uint64_t)r0.threads >= (1ull << 31)) return bfail(b, 4, "pair grid too large");
+                for (uint64_t k = 0; k < nch; ++k) {
+                    const size_t at = recbuf.size();
+                    recbuf.resize(at + r.rec_bytes, 0);
+                    const apgk::GroupChunk c{(uint32_t)(k << apgk::kGroupChunkLog2), (uint32_t)count, r.grid.x,
+                                             r.grid.x * r.grid.y};
+                    memcpy(recbuf.data() + at, &c, sizeof c);
+                    memcpy(recbuf.data() + at + r.args_at, r.args.data(), r.args.size());
+                }
+                shm = std::max(shm, r.shm);
+                nblk = next;
+            }
+            if (!nblk) continue;
+            GroupLaunch g;
+            g.kernel = r0.kernel;
+            g.threads = r0.threads;
+            g.shm = shm;
+            g.nblocks = (uint32_t)nblk;
+            g.recs_off = img.put(recbuf.data(), recbuf.size());
+            if (shm > 64 * 1024)
+                if (int s = bhip(b, hipFuncSetAttribute(g.kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm),
+                                 "group LDS size"))
+                    return s;
+            out.push_back(g);
+        }
+    }
+    return 0;
+}
+
+// Builds the group plan of a pair list (not under capture): sizes one
+// workspace per chunk position of each slot, records every step of every
+// eligible pair and uploads the tables.
+int build_plan(apg_hip_batch b, const apg_hip_pair* pairs, int n) {
+    drop_plan(b);
+    const int nslots = (int)b->slots.size();
+    const auto per = plan_slots(pairs, n, nslots);
+    GroupPlan P;
+    P.launches.resize(nslots);
+    P.single.resize(nslots);
+    if (b->gctx.size() < (size_t)nslots) b->gctx.resize(nslots);
+    TableImage img;
+    for (int s = 0; s < nslots; ++s) {
+        hipStream_t st = (hipStream_t)apg_hip_stream(b->slots[s]);
+        // chunks: the slot's pairs in order, up to kGroupChunkBytes of workspace each
+        std::vector<std::vector<int>> chunks;
+        size_t acc = 0;
+        for (int i : per[s]) {
+            const apg_hip_pair& q = pairs[i];
+            const size_t ws = apg_hip_pipeline_workspace_bytes(&q.params, q.width, q.height);
+            if (chunks.empty() || acc + ws > b->group_bytes) {
+                chunks.emplace_back();
+                acc = 0;
+            }
+            chunks.back().push_back(i);
+            acc += ws;
+        }
+        size_t npos = 0;
+        for (const auto& c : chunks) npos = std::max(npos, c.size());
+        while (b->gctx[s].size() < npos) {
+            apg_hip_ctx c = nullptr;
+            if (int e = apgk_ctx_create_on(b->device, st, &c)) return bfail(b, e, "group context");
+            b->gctx[s].push_back(c);
+        }
+        // every workspace grows to its size before any launch is recorded:
+        // recorded launches hold the buffer addresses
+        std::vector<std::vector<bool>> ok(chunks.size());
+        for (size_t c = 0; c < chunks.size(); ++c)
+            for (size_t k = 0; k < chunks[c].size(); ++k) {
+                apg_hip_ctx ctx = b->gctx[s][k];
+                const apg_hip_pair& q = pairs[chunks[c][k]];
+                set_pair_tuning(b, ctx, q);
+                const int e = apgk_group_prepare(ctx, q);
+                if (e == 7) return bfail(b, 7, std::string("group workspace: ") + apg_hip_last_error(ctx));
+                ok[c].push_back(e == 0);
+            }
+        for (size_t c = 0; c < chunks.size(); ++c) {
+            // every step of each pair, recorded first, so that a pair with a
+            // step that has no group form runs one at a time instead
+            std::vector<std::vector<std::vector<apgk::GroupRec>>> rec(chunks[c].size());
+            for (size_t k = 0; k < chunks[c].size(); ++k) {
+                const int i = chunks[c][k];
+                if (ok[c][k]) {
+                    apg_hip_ctx ctx = b->gctx[s][k];
+                    set_pair_tuning(b, ctx, pairs[i]);
+                    rec[k].resize(kGroupSteps);
+                    for (int step = 0; step < kGroupSteps && ok[c][k]; ++step) {
+                        apgk::LaunchSink sink;
+                        if (apgk_group_record(ctx, pairs[i], step, &sink)) ok[c][k] = false;
+                        else rec[k][step] = std::move(sink.recs);
+                    }
+                }
+                if (!ok[c][k]) P.single[s].push_back(i);
+                else ++P.grouped;
+            }
+            for (int step = 0; step < kGroupSteps; ++step) {
+                std::vector<apgk::GroupRec> recs;
+                for (size_t k = 0; k < chunks[c].size(); ++k)
+                    if (ok[c][k])
+                        for (auto& r : rec[k][step]) recs.push_back(std::move(r));
+                if (int e = group_step(b, recs, img, P.launches[s])) return e;
+            }
+        }
+        P.nlaunch += (int)P.launches[s].size();
+        P.nsingle += (int)P.single[s].size();
+    }
+    if (!img.bytes.empty()) {
+        if (int e = bhip(b, hipMalloc(&P.d_tables, img.bytes.size()), "group tables")) return e;
+        if (int e = bhip(b, hipMemcpy(P.d_tables, img.bytes.data(), img.bytes.size(), hipMemcpyHostToDevice),
+                         "group tables upload")) {
+            (void)hipFree(P.d_tables);
+            return e;
+        }
+    }
+    P.pairs.assign(pairs, pairs + n);
+    P.valid = true;
+    b->plan = std::move(P);
+    return 0;
+}
+
+// Enqueues the plan: per slot its group launches, then its single pairs.
+int enqueue_plan(apg_hip_batch b, const apg_hip_pair* pairs) {
+    const GroupPlan& P = b->plan;
+    if (int s = bhip(b, hipEventRecord(b->fork, b->origin), "fork record")) return s;
+    int status = 0;
+    for (size_t s = 0; s < b->slots.size(); ++s) {
+        if (P.launches[s].empty() && P.single[s].empty()) continue;
+        apg_hip_ctx ctx = b->slots[s];
+        hipStream_t st = (hipStream_t)apg_hip_stream(ctx);
+        if (int e = bhip(b, hipStreamWaitEvent(st, b->fork, 0), "fork wait")) return e;
+        for (const GroupLaunch& g : P.launches[s]) {
+            if (status) break;
+            const void* rp = (const uint8_t*)P.d_tables + g.recs_off;
+            void* kargs[] = {(void*)&rp};
+            status = bhip(b, hipLaunchKernel(g.kernel, dim3(g.nblocks), dim3(g.threads), kargs, g.shm, st),
+                          "group launch");
+        }
+        for (int i : P.single[s]) {
+            if (status) break;
+            set_pair_tuning(b, ctx, pairs[i]);
+            status = run_pair(ctx, pairs[i]);
+            if (status) b->last_error = "pair " + std::to_string(i) + ": " + apg_hip_last_error(ctx);
+        }
+        hipError_t e = hipEventRecord(b->join[s], st);
+        if (e == hipSuccess) e = hipStreamWaitEvent(b->origin, b->join[s], 0);
+        if (e != hipSuccess && !status) status = bhip(b, e, "join");
+    }
+    return status;
+}
+
 }  // namespace
 
 extern "C" {
@@ -251,7 +500,9 @@ void apg_hip_batch_destroy(apg_hip_batch b) {
     if (!b) return;
     (void)hipSetDevice(b->device);
     if (b->origin) (void)hipStreamSynchronize(b->origin);
-    drop_graph(b);
+    drop_plan(b);
+    for (auto& v : b->gctx)
+        for (apg_hip_ctx c : v) apg_hip_ctx_destroy(c);
     for (apg_hip_ctx c : b->slots) apg_hip_ctx_destroy(c);
     for (hipEvent_t e : b->join) (void)hipEventDestroy(e);
     if (b->fork) (void)hipEventDestroy(b->fork);
@@ -270,6 +521,8 @@ size_t apg_hip_batch_workspace_bytes(apg_hip_batch b) {
     if (!b) return 0;
     size_t t = 0;
     for (apg_hip_ctx c : b->slots) t += apg_hip_workspace_bytes(c);
+    for (const auto& v : b->gctx)
+        for (apg_hip_ctx c : v) t += apg_hip_workspace_bytes(c);
     return t;
 }
 
@@ -282,21 +535,27 @@ int apg_hip_batch_run(apg_hip_batch b, const apg_hip_pair* pairs, int n, int fla
     if (int s = bhip(b, hipSetDevice(b->device), "hipSetDevice")) return s;
     bool host_steps = false;   // the prior and the refiner synchronise: no capture
     for (int i = 0; i < n; ++i) host_steps = host_steps || pairs[i].params.prior_enable || pairs[i].params.refine_enable;
+    const bool grouped = !(flags & APG_HIP_BATCH_PER_PAIR) && !host_steps;
     if (!(flags & APG_HIP_BATCH_GRAPH) || host_steps) {
         // a direct run may grow slot workspaces, which a cached graph points into
         drop_graph(b);
-        return enqueue(b, pairs, n);
+        if (!grouped) return enqueue(b, pairs, n);
+        if (!b->plan.valid || !same_pairs(b->plan.pairs, pairs, n))
+            if (int s = build_plan(b, pairs, n)) return s;
+        return enqueue_plan(b, pairs);
     }
-    if (b->exec && same_pairs(b->graph_pairs, pairs, n))
+    if (b->exec && same_pairs(b->graph_pairs, pairs, n) && b->graph_grouped == grouped)
         return bhip(b, hipGraphLaunch(b->exec, b->origin), "hipGraphLaunch");
     drop_graph(b);
-    if (int s = enqueue(b, pairs, n)) return s;
+    if (grouped && (!b->plan.valid || !same_pairs(b->plan.pairs, pairs, n)))
+        if (int s = build_plan(b, pairs, n)) return s;
+    if (int s = grouped ? enqueue_plan(b, pairs) : enqueue(b, pairs, n)) return s;
     // The direct run sized every workspace and cached every LUT, so the
     // capture below records kernels only.  It is not executed: the next call
     // with this pair list replays it.
     if (int s = bhip(b, hipStreamBeginCapture(b->origin, hipStreamCaptureModeRelaxed), "begin capture")) return s;
     for (apg_hip_ctx c : b->slots) apgk_ctx_set_capturing(c, true);
-    const int st = enqueue(b, pairs, n);
+    const int st = grouped ? enqueue_plan(b, pairs) : enqueue(b, pairs, n);
     hipGraph_t g = nullptr;
     const hipError_t ee = hipStreamEndCapture(b->origin, &g);
     for (apg_hip_ctx c : b->slots) apgk_ctx_set_capturing(c, false);
@@ -313,6 +572,22 @@ int apg_hip_batch_run(apg_hip_batch b, const apg_hip_pair* pairs, int n, int fla
     b->graph = g;
     b->exec = ex;
     b->graph_pairs.assign(pairs, pairs + n);
+    b->graph_grouped = grouped;
+    return 0;
+}
+
+int apg_hip_batch_set_group_bytes(apg_hip_batch b, size_t bytes) {
+    if (!b) return 8;
+    b->group_bytes = bytes ? bytes : kGroupChunkBytes;
+    drop_plan(b);
+    return 0;
+}
+
+int apg_hip_batch_group_stats(apg_hip_batch b, int* grouped_pairs, int* group_launches, int* single_pairs) {
+    if (!b) return 8;
+    if (grouped_pairs) *grouped_pairs = b->plan.valid ? b->plan.grouped : 0;
+    if (group_launches) *group_launches = b->plan.valid ? b->plan.nlaunch : 0;
+    if (single_pairs) *single_pairs = b->plan.valid ? b->plan.nsingle : 0;
     return 0;
 }
 
@@ -383,7 +658,7 @@ extern "C" int apg_hip_batch_autotune(apg_hip_batch b, const apg_hip_pair* pairs
     if (n < 0 || (n > 0 && !pairs)) return bfail(b, 4, "invalid pair list");
     if (int s = bhip(b, hipSetDevice(b->device), "hipSetDevice")) return s;
     if (cache_path) load_cache(cache_path, b->arch, b->tuned);
-    drop_graph(b);
+    drop_plan(b);
     // tuning runs on slot 0 after everything queued on the batch
     apg_hip_ctx ctx = b->slots[0];
     if (int s = bhip(b, hipStreamSynchronize(b->origin), "sync")) return s;

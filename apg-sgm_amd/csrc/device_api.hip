@@ -50,6 +50,7 @@ constexpr int kLutSlots = 8;
 struct apg_hip_ctx_s {
     int device = 0;
     hipStream_t stream = nullptr;
+    bool own_stream = true;   // false: the stream belongs to a batch slot
     DevBuf bufs[B_COUNT];
     std::string last_error;
     bool timing = false;
@@ -268,6 +269,127 @@ int check_ctx(apg_hip_ctx ctx, const apg_hip_params* p) {
     return 0;
 }
 
+// One pair's images and outputs.
+struct PairIO {
+    const uint8_t *left, *right;
+    int w, h, stride, ch;
+    float* disp;
+    float* conf;
+    uint8_t *reason, *reliable;
+};
+
+enum FusedStep { F_AUX, F_COST, F_RWTA, F_CROSS, F_SGM, F_WTA, F_POST, F_STEPS };
+
+// Launch step `step` of the fused schedules on the context's workspace
+// (allocated for `plan`): F_AUX both views' aux pass (every schedule), then
+// for the fused schedules the Cost + Cross-H pass, the right-view WTA,
+// Cross-V, the SGM planes (column ranges when plan.fused, the prior's
+// per-pixel ranges otherwise), the WTA with confidence (and the LR check when
+// plan.fused) and fill + median (plan.fused).  With a sink every launch is
+// recorded there and nothing is enqueued; a launch with no group form fails
+// the step.
+int fused_step(apg_hip_ctx ctx, hipStream_t st, const apg_hip_params* p, const PairIO& io, const WsPlan& plan,
+               int step, const apg_hip_intermediates* inter, LaunchSink* sink) {
+    if (sink && inter) return fail(ctx, 4, "intermediates have no group launch");
+    const int w = io.w, h = io.h;
+    const size_t n = (size_t)w * h;
+    const int D = p->max_disparity - p->min_disparity;
+    const bool fused = plan.fused, u8 = plan.u8, sym = p->census_type == 1;
+    const int npaths = p->paths == 8 ? 8 : 4;
+    uint8_t *gl = buf<uint8_t>(ctx, B_GRAY_L), *gxl = buf<uint8_t>(ctx, B_GX_L), *gyl = buf<uint8_t>(ctx, B_GY_L);
+    uint8_t *gr = buf<uint8_t>(ctx, B_GRAY_R), *gxr = buf<uint8_t>(ctx, B_GX_R), *gyr = buf<uint8_t>(ctx, B_GY_R);
+    void *cl = ctx->bufs[B_CEN_L].ptr, *cr = ctx->bufs[B_CEN_R].ptr;
+    const uint32_t *c32l = sym ? (const uint32_t*)cl : nullptr, *c32r = sym ? (const uint32_t*)cr : nullptr;
+    const uint64_t *c64l = sym ? nullptr : (const uint64_t*)cl, *c64r = sym ? nullptr : (const uint64_t*)cr;
+    int16_t *dmin = buf<int16_t>(ctx, B_DMIN), *dmax = buf<int16_t>(ctx, B_DMAX);
+    uint32_t* arms = buf<uint32_t>(ctx, B_ARMS);
+    float* dr = buf<float>(ctx, B_DR);
+    float* scratch = buf<float>(ctx, B_SCRATCH);
+    // the fused schedules' right-view WTA keys start cleared, by the aux pass
+    uint32_t* rkeys = (fused || plan.planes_fused) ? buf<uint32_t>(ctx, B_RKEYS) : nullptr;
+    // the left view's Cross arms come from the aux pass's gray tiles where the arms are short enough
+    uint32_t* arms_aux = aux_pair_builds_arms(*p) ? arms : nullptr;
+    // fused Cost + Cross: dense range init (column ranges) or, for the
+    // HighQuality preset, the prior's per-pixel ranges
+    const bool colr = fused;
+    PlaneSet planes{};
+    planes.general_ranges = colr ? 0 : 1;
+    planes.paired = plan.pair_mask;
+    for (int r = 0; r < npaths - __builtin_popcount((unsigned)plan.pair_mask); ++r)
+        planes.p[r] = buf<uint8_t>(ctx, (BufId)(B_DS0 + r));
+    void* cp = ctx->bufs[B_COST].ptr;   // C' (u8 or u16)
+    void* tt = ctx->bufs[B_TMP].ptr;    // T  (u8 or u16)
+    switch (step) {
+    case F_AUX: {
+        // both views and the dense range init in one launch
+        const AuxViews av_l{io.left, gl, gxl, gyl, sym ? (uint32_t*)cl : nullptr, sym ? nullptr : (uint64_t*)cl};
+        const AuxViews av_r{io.right, gr, gxr, gyr, sym ? (uint32_t*)cr : nullptr, sym ? nullptr : (uint64_t*)cr};
+        HIPCHK(launch_aux_pair(st, *p, w, h, io.stride, io.ch, av_l, av_r, rkeys, dmin, dmax, arms_aux, sink),
+               "aux");
+        return 0;
+    }
+    case F_COST:
+        if (p->agg_enable && !arms_aux) {
+            if (sink) return fail(ctx, 4, "Cross arms longer than the aux pass builds have no group launch");
+            HIPCHK(launch_cross_arms(st, *p, gl, w, h, arms), "arms");
+        }
+        HIPCHK(launch_cost_crossh(st, *p, cur_lut(ctx), w, h, gl, gxl, gyl, gr, gxr, gyr, c32l, c32r, c64l, c64r,
+                                  dmin, dmax, arms, p->agg_enable ? tt : cp, u8, rkeys, colr,
+                                  cost_clamp_free(ctx->lut_host[ctx->lut_cur], p->cost_max), ctx->tune, sink),
+               "cost_crossh");
+        return 0;
+    case F_RWTA:
+        HIPCHK(launch_rwta_finalize(st, *p, cur_lut(ctx), w, h, gl, gxl, gyl, gr, gxr, gyr, c32l, c32r, c64l, c64r,
+                                    dmin, dmax, rkeys, dr, sink),
+               "rwta_finalize");
+        return 0;
+    case F_CROSS:
+        if (p->agg_enable)
+            HIPCHK(launch_cross_v_t(st, *p, w, h, dmin, dmax, arms, tt, cp, u8, colr, ctx->tune, sink), "cross_v");
+        return 0;
+    case F_SGM:
+        HIPCHK(launch_sgm_lines(st, *p, w, h, D, gl, cp, u8, planes, ctx->tune, sink), "sgm_lines");
+        return 0;
+    case F_WTA:
+        if (fused) {
+            // without the refiner, the disparity after it is the WTA output
+            float* wta_out = inter ? (inter->d_disp_wta ? inter->d_disp_wta : inter->d_after_refine) : nullptr;
+            // fill + median in one pass reads the WTA's disparity from scratch
+            float* wta_disp = fill_median_fusable(*p) ? scratch : io.disp;
+            HIPCHK(launch_wta_fused(st, *p, w, h, cp, u8, planes, dmin, dmax, dr, gxl, gyl, wta_disp, io.conf,
+                                    io.reason, io.reliable, wta_out, inter ? inter->d_best : nullptr,
+                                    inter ? inter->d_second : nullptr, inter ? inter->d_sum32 : nullptr, sink),
+                   "wta_fused");
+            if (inter && inter->d_after_refine && wta_out != inter->d_after_refine)
+                HIPCHK(hipMemcpyAsync(inter->d_after_refine, wta_out, n * 4, hipMemcpyDeviceToDevice, st),
+                       "copy refined");
+        } else {
+            // WTA + confidence in one kernel; the LR check follows the refiner
+            apg_hip_params pw = *p;
+            pw.post_lr_check = 0;
+            HIPCHK(launch_wta_fused(st, pw, w, h, cp, u8, planes, dmin, dmax, dr, gxl, gyl, io.disp, io.conf,
+                                    io.reason, io.reliable, inter ? inter->d_disp_wta : nullptr,
+                                    inter ? inter->d_best : nullptr, inter ? inter->d_second : nullptr,
+                                    inter ? inter->d_sum32 : nullptr, sink),
+                   "wta_fused");
+        }
+        if (inter && inter->d_disp_right)
+            HIPCHK(hipMemcpyAsync(inter->d_disp_right, dr, n * 4, hipMemcpyDeviceToDevice, st), "copy dr");
+        return 0;
+    case F_POST:
+        if (fill_median_fusable(*p)) {
+            HIPCHK(launch_fill_median(st, *p, w, h, scratch, io.reason, gl, nullptr, io.disp, sink), "fill_median");
+        } else {
+            if (sink) return fail(ctx, 4, "separate fill and median have no group launch");
+            HIPCHK(launch_fill(st, *p, w, h, io.disp, io.reason, gl, nullptr, scratch), "fill");
+            HIPCHK(launch_median(st, *p, w, h, scratch, io.disp), "median");
+        }
+        return 0;
+    default:
+        return fail(ctx, 4, "invalid step");
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -359,7 +481,7 @@ void apg_hip_ctx_destroy(apg_hip_ctx ctx) {
         if (b.ptr) (void)hipFree(b.ptr);
     for (int i = 0; i <= S_COUNT; ++i)
         if (ctx->ev[i]) (void)hipEventDestroy(ctx->ev[i]);
-    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    if (ctx->stream && ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
 
@@ -772,8 +894,9 @@ int apg_hip_pipeline(apg_hip_ctx ctx, const apg_hip_params* p, const uint8_t* d_
     const WsPlan plan = plan_workspace(*p, w, h, ctx->lut_host[ctx->lut_cur], ctx->schedule, inter, ctx->tune);
     for (int b = 0; b < B_COUNT; ++b)
         if (b != B_LUT && plan.bytes[b]) HIPCHK(ensure(ctx, (BufId)b, plan.bytes[b]), "alloc");
-    const bool fused = plan.fused, u8 = plan.u8;
+    const bool fused = plan.fused;
     const int npaths = p->paths == 8 ? 8 : 4;
+    const PairIO io{d_left, d_right, w, h, stride, channels, d_disparity, d_confidence, d_reason, d_reliable};
     uint8_t *gl = buf<uint8_t>(ctx, B_GRAY_L), *gxl = buf<uint8_t>(ctx, B_GX_L), *gyl = buf<uint8_t>(ctx, B_GY_L);
     uint8_t *gr = buf<uint8_t>(ctx, B_GRAY_R), *gxr = buf<uint8_t>(ctx, B_GX_R), *gyr = buf<uint8_t>(ctx, B_GY_R);
     void *cl = ctx->bufs[B_CEN_L].ptr, *cr = ctx->bufs[B_CEN_R].ptr;
@@ -783,16 +906,10 @@ int apg_hip_pipeline(apg_hip_ctx ctx, const apg_hip_params* p, const uint8_t* d_
     float *dr = buf<float>(ctx, B_DR), *best = buf<float>(ctx, B_BEST), *second = buf<float>(ctx, B_SECOND);
     float* scratch = buf<float>(ctx, B_SCRATCH);
     const bool sym = p->census_type == 1;
+    const bool arms_aux = aux_pair_builds_arms(*p);
 
-    // the fused schedule's right-view WTA keys start cleared, by the aux pass
-    uint32_t* rkeys = (fused || plan.planes_fused) ? buf<uint32_t>(ctx, B_RKEYS) : nullptr;
     mark(ctx, S_AUX);
-    // both views and the dense range init in one launch
-    const AuxViews av_l{d_left, gl, gxl, gyl, sym ? (uint32_t*)cl : nullptr, sym ? nullptr : (uint64_t*)cl};
-    const AuxViews av_r{d_right, gr, gxr, gyr, sym ? (uint32_t*)cr : nullptr, sym ? nullptr : (uint64_t*)cr};
-    // the left view's Cross arms come from the same gray tiles where the arms are short enough
-    uint32_t* arms_aux = aux_pair_builds_arms(*p) ? arms : nullptr;
-    HIPCHK(launch_aux_pair(st, *p, w, h, stride, channels, av_l, av_r, rkeys, dmin, dmax, arms_aux), "aux");
+    if (int s = fused_step(ctx, st, p, io, plan, F_AUX, inter, nullptr)) return s;
     mark(ctx, S_PRIOR);
     float* d_prior = p->prior_enable ? buf<float>(ctx, B_PRIOR) : nullptr;
     if (d_prior) {
@@ -806,73 +923,19 @@ int apg_hip_pipeline(apg_hip_ctx ctx, const apg_hip_params* p, const uint8_t* d_
     }
 
     if (fused || plan.planes_fused) {
-        // fused Cost + Cross: dense range init (column ranges) or, for the
-        // HighQuality preset, the prior's per-pixel ranges
-        const bool colr = fused;
-        PlaneSet planes{};
-        planes.general_ranges = colr ? 0 : 1;
-        planes.paired = plan.pair_mask;
-        for (int r = 0; r < npaths - __builtin_popcount((unsigned)plan.pair_mask); ++r)
-            planes.p[r] = buf<uint8_t>(ctx, (BufId)(B_DS0 + r));
-        void* cp = ctx->bufs[B_COST].ptr;   // C' (u8 or u16)
-        void* tt = ctx->bufs[B_TMP].ptr;    // T  (u8 or u16)
-        if (p->agg_enable && !arms_aux) HIPCHK(launch_cross_arms(st, *p, gl, w, h, arms), "arms");
-        mark(ctx, S_COST);
-        HIPCHK(launch_cost_crossh(st, *p, cur_lut(ctx), w, h, gl, gxl, gyl, gr, gxr, gyr,
-                                  sym ? (const uint32_t*)cl : nullptr, sym ? (const uint32_t*)cr : nullptr,
-                                  sym ? nullptr : (const uint64_t*)cl, sym ? nullptr : (const uint64_t*)cr, dmin,
-                                  dmax, arms, p->agg_enable ? tt : cp, u8, rkeys, colr,
-                                  cost_clamp_free(ctx->lut_host[ctx->lut_cur], p->cost_max), ctx->tune),
-               "cost_crossh");
-        mark(ctx, S_RWTA);
-        HIPCHK(launch_rwta_finalize(st, *p, cur_lut(ctx), w, h, gl, gxl, gyl, gr, gxr, gyr,
-                                    sym ? (const uint32_t*)cl : nullptr, sym ? (const uint32_t*)cr : nullptr,
-                                    sym ? nullptr : (const uint64_t*)cl, sym ? nullptr : (const uint64_t*)cr, dmin,
-                                    dmax, rkeys, dr),
-               "rwta_finalize");
-        mark(ctx, S_CROSS);
-        if (p->agg_enable) HIPCHK(launch_cross_v_t(st, *p, w, h, dmin, dmax, arms, tt, cp, u8, colr, ctx->tune),
-                                    "cross_v");
-        mark(ctx, S_SGM);
-        HIPCHK(launch_sgm_lines(st, *p, w, h, D, gl, cp, u8, planes, ctx->tune), "sgm_lines");
-        mark(ctx, S_WTA);
-        if (fused) {
-            // without the refiner, the disparity after it is the WTA output
-            float* wta_out = inter ? (inter->d_disp_wta ? inter->d_disp_wta : inter->d_after_refine) : nullptr;
-            // fill + median in one pass reads the WTA's disparity from scratch
-            const bool fm = fill_median_fusable(*p);
-            float* wta_disp = fm ? scratch : d_disparity;
-            HIPCHK(launch_wta_fused(st, *p, w, h, cp, u8, planes, dmin, dmax, dr, gxl, gyl, wta_disp, d_confidence,
-                                    d_reason, d_reliable, wta_out, inter ? inter->d_best : nullptr,
-                                    inter ? inter->d_second : nullptr, inter ? inter->d_sum32 : nullptr),
-                   "wta_fused");
-            if (inter && inter->d_after_refine && wta_out != inter->d_after_refine)
-                HIPCHK(hipMemcpyAsync(inter->d_after_refine, wta_out, n * 4, hipMemcpyDeviceToDevice, st),
-                       "copy refined");
-            if (inter && inter->d_disp_right)
-                HIPCHK(hipMemcpyAsync(inter->d_disp_right, dr, n * 4, hipMemcpyDeviceToDevice, st), "copy dr");
-            mark(ctx, S_CONF);
-            mark(ctx, S_REFINE);
-            mark(ctx, S_POST);
-            if (fm) {
-                HIPCHK(launch_fill_median(st, *p, w, h, scratch, d_reason, gl, nullptr, d_disparity), "fill_median");
-            } else {
-                HIPCHK(launch_fill(st, *p, w, h, d_disparity, d_reason, gl, nullptr, scratch), "fill");
-                HIPCHK(launch_median(st, *p, w, h, scratch, d_disparity), "median");
+        const int marks[F_STEPS] = {S_AUX, S_COST, S_RWTA, S_CROSS, S_SGM, S_WTA, S_POST};
+        for (int k = F_COST; k <= F_WTA || (fused && k == F_POST); ++k) {
+            if (k == F_POST) {
+                mark(ctx, S_CONF);
+                mark(ctx, S_REFINE);
             }
+            mark(ctx, marks[k]);
+            if (int s = fused_step(ctx, st, p, io, plan, k, inter, nullptr)) return s;
+        }
+        if (fused) {
             mark(ctx, S_COUNT);
             return finish_timing(ctx);
         }
-        // WTA + confidence in one kernel; the LR check follows the refiner
-        apg_hip_params pw = *p;
-        pw.post_lr_check = 0;
-        HIPCHK(launch_wta_fused(st, pw, w, h, cp, u8, planes, dmin, dmax, dr, gxl, gyl, d_disparity, d_confidence,
-                                d_reason, d_reliable, inter ? inter->d_disp_wta : nullptr,
-                                inter ? inter->d_best : nullptr, inter ? inter->d_second : nullptr,
-                                inter ? inter->d_sum32 : nullptr),
-               "wta_fused");
-        if (inter && inter->d_disp_right)
-            HIPCHK(hipMemcpyAsync(inter->d_disp_right, dr, n * 4, hipMemcpyDeviceToDevice, st), "copy dr");
         mark(ctx, S_CONF);
     } else {
     mark(ctx, S_COST);
@@ -1025,6 +1088,56 @@ size_t apgk_ctx_growth_bytes(apg_hip_ctx ctx, const apg_hip_params* p, int w, in
         if (want > have) g += want - have;
     }
     return g;
+}
+
+int apgk_ctx_create_on(int device, hipStream_t st, apg_hip_ctx* out) {
+    if (!out || !st) return 8;
+    *out = nullptr;
+    apg_hip_ctx ctx = new (std::nothrow) apg_hip_ctx_s();
+    if (!ctx) return 7;
+    ctx->device = device;
+    ctx->stream = st;
+    ctx->own_stream = false;
+    *out = ctx;
+    return 0;
+}
+
+int apgk_group_prepare(apg_hip_ctx ctx, const apg_hip_pair& q) {
+    if (!ctx) return 8;
+    if (!q.d_left || !q.d_right || q.width <= 0 || q.height <= 0) return fail(ctx, 1, "empty input image(left or right)");
+    if (q.stride < q.width * q.channels || (q.channels != 1 && q.channels != 3))
+        return fail(ctx, 2, "invalid row stride (stride less than row byte length)");
+    if (int s = check_ctx(ctx, &q.params)) return s;
+    if (ctx->capturing) return fail(ctx, 8, "group workspace not prepared before stream capture");
+    if (int s = ensure_lut(ctx, &q.params)) return s;
+    const WsPlan plan = plan_workspace(q.params, q.width, q.height, ctx->lut_host[ctx->lut_cur], ctx->schedule,
+                                       nullptr, ctx->tune);
+    if (!plan.fused) return fail(ctx, 4, "the pair does not take the fused column-range schedule");
+    for (int b = 0; b < B_COUNT; ++b)
+        if (b != B_LUT && plan.bytes[b]) HIPCHK(ensure(ctx, (BufId)b, plan.bytes[b]), "alloc");
+    return 0;
+}
+
+int apgk_group_record(apg_hip_ctx ctx, const apg_hip_pair& q, int step, apgk::LaunchSink* sink) {
+    if (!ctx || !sink) return 8;
+    if (step < 0 || step >= kGroupSteps) return fail(ctx, 4, "invalid step");
+    // the LUT must already be resident: an upload now could evict the
+    // table an earlier recorded launch of this context reads
+    const LutKey key = LutKey::of(q.params);
+    int li = -1;
+    for (int i = 0; i < kLutSlots; ++i)
+        if (ctx->lut_used[i] && ctx->lut_key[i] == key) li = i;
+    if (li < 0) return fail(ctx, 4, "cost LUT not resident");
+    ctx->lut_cur = li;
+    const WsPlan plan = plan_workspace(q.params, q.width, q.height, ctx->lut_host[li], ctx->schedule, nullptr,
+                                       ctx->tune);
+    if (!plan.fused) return fail(ctx, 4, "the pair does not take the fused column-range schedule");
+    for (int b = 0; b < B_COUNT; ++b)
+        if (b != B_LUT && plan.bytes[b] > ctx->bufs[b].bytes) return fail(ctx, 4, "workspace not prepared");
+    const PairIO io{q.d_left, q.d_right, q.width, q.height, q.stride, q.channels,
+                    q.d_disparity, q.d_confidence, q.d_invalid_reason, q.d_reliable};
+    static const int kSteps[kGroupSteps] = {F_AUX, F_COST, F_RWTA, F_CROSS, F_SGM, F_WTA, F_POST};
+    return fused_step(ctx, ctx->stream, &q.params, io, plan, kSteps[step], nullptr, sink);
 }
 
 extern "C" {

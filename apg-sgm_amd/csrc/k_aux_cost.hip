@@ -79,6 +79,8 @@ struct AuxImage {
 struct AuxPair {
     AuxImage img[2];
 };
+template <>
+struct GroupByRef<AuxPair> : std::true_type {};
 
 __device__ __forceinline__ void aux_pixel(const AuxImage& im, int w, int h, int stride, int ch, int census_type,
                                           int gmin, int gmax, uint8_t (&tile)[AUX_TY + 6][AUX_TX + 8]);
@@ -125,15 +127,19 @@ __device__ __forceinline__ void byte_transpose(const uint32_t (&g)[4], uint32_t 
 }
 
 template <int CT>
-__global__ __launch_bounds__(256) void aux4_pair_kernel(AuxPair ap, int w, int h, int stride, int ch, int gmin,
-                                                        int gmax, uint32_t* __restrict__ arms, int Lmax, int tau) {
+__device__ __forceinline__ void aux4_pair_body(const uint3 bid, const AuxPair& ap, int w, int h, int stride, int ch,
+                                               int gmin, int gmax, uint32_t* __restrict__ arms, int Lmax, int tau) {
     __shared__ uint32_t tile[(AX_ROWS + 2 * AX_LMAX) * AX_S];
-    const int view = blockIdx.z;
-    const AuxImage im = ap.img[view];
+    const int view = bid.z;
+    // the pointers read as global ones (group launches read ap from memory)
+    const AuxImage& imr = ap.img[view];
+    const AuxImage im{global_ptr(imr.src), global_ptr(imr.gray), global_ptr(imr.gx), global_ptr(imr.gy),
+                      global_ptr(imr.c32), global_ptr(imr.c64), global_ptr(imr.keys), global_ptr(imr.dmin),
+                      global_ptr(imr.dmax)};
     const bool do_arms = view == 0 && arms != nullptr;
     const int rh = do_arms ? max(3, Lmax) : 3;
     const int th = AX_ROWS + 2 * rh;
-    const int x0 = blockIdx.x * AX_PX, y0 = blockIdx.y * AX_ROWS;
+    const int x0 = bid.x * AX_PX, y0 = bid.y * AX_ROWS;
     const int tid = threadIdx.x;
     // every word's load is issued before any is stored (a word wholly inside
     // the image of a 4-byte aligned gray row is one load; the edge words and
@@ -330,6 +336,19 @@ __global__ __launch_bounds__(256) void aux4_pair_kernel(AuxPair ap, int w, int h
     else for (int j = 0; j < nq; ++j) arms[pi + j] = av[j];
 }
 
+template <int CT>
+__global__ __launch_bounds__(256) void aux4_pair_kernel(AuxPair ap, int w, int h, int stride, int ch, int gmin,
+                                                        int gmax, uint32_t* __restrict__ arms, int Lmax, int tau) {
+    aux4_pair_body<CT>(block3(), ap, w, h, stride, ch, gmin, gmax, arms, Lmax, tau);
+}
+
+template <int CT>
+struct Aux4PairFn {
+    static constexpr int kThreads = 256;
+    template <typename... A>
+    __device__ static void run(const uint3 b, const A&... a) { aux4_pair_body<CT>(b, a...); }
+};
+
 __global__ __launch_bounds__(256) void aux_kernel(const uint8_t* __restrict__ src, int w, int h, int stride, int ch,
                                                  int census_type, uint8_t* __restrict__ gray,
                                                  uint8_t* __restrict__ gx, uint8_t* __restrict__ gy,
@@ -405,13 +424,18 @@ __device__ __forceinline__ void aux_pixel(const AuxImage& im, int w, int h, int 
 
 hipError_t launch_aux_pair(hipStream_t s, const apg_hip_params& p, int w, int h, int stride, int ch,
                            const AuxViews& L, const AuxViews& R, uint32_t* rkeys, int16_t* dmin, int16_t* dmax,
-                           uint32_t* arms) {
+                           uint32_t* arms, LaunchSink* sink) {
     AuxPair ap;
     ap.img[0] = AuxImage{L.src, L.gray, L.gx, L.gy, L.c32, L.c64, nullptr, dmin, dmax};
     ap.img[1] = AuxImage{R.src, R.gray, R.gx, R.gy, R.c32, R.c64, rkeys, nullptr, nullptr};
     const int Lmax = p.max_arm_length > 1 ? p.max_arm_length : 1;
     if (arms && Lmax > AX_LMAX) return hipErrorInvalidValue;
     dim3 grid((w + AX_PX - 1) / AX_PX, (h + AX_ROWS - 1) / AX_ROWS, 2), block(256);
+    if (sink)
+        return p.census_type == 1 ? sink->add<Aux4PairFn<1>>(grid, 0, ap, w, h, stride, ch, p.min_disparity,
+                                                             p.max_disparity, arms, Lmax, p.color_threshold)
+                                  : sink->add<Aux4PairFn<0>>(grid, 0, ap, w, h, stride, ch, p.min_disparity,
+                                                             p.max_disparity, arms, Lmax, p.color_threshold);
     if (p.census_type == 1)
         hipLaunchKernelGGL(aux4_pair_kernel<1>, grid, block, 0, s, ap, w, h, stride, ch, p.min_disparity,
                            p.max_disparity, arms, Lmax, p.color_threshold);

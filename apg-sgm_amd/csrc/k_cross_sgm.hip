@@ -281,17 +281,16 @@ __global__ __launch_bounds__(64) void cross_v2_kernel(int w, int h, int d0, int 
 // segments (with 180-row segments 16: 83 us, 8: 87 us, 32: 98 us)
 constexpr int CV3_PF = 16;
 
-__global__ __launch_bounds__(64) void cross_v3_kernel(int w, int h, int d0, int D, int Lmax, int seg,
-                                                      const int16_t* __restrict__ dmin,
-                                                      const int16_t* __restrict__ dmax,
-                                                      const uint32_t* __restrict__ arms,
-                                                      const uint8_t* __restrict__ tmp, uint8_t* __restrict__ cost) {
+__device__ __forceinline__ void cross_v3_body(const uint3 bid, int w, int h, int d0, int D, int Lmax, int seg,
+                                              const int16_t* __restrict__ dmin, const int16_t* __restrict__ dmax,
+                                              const uint32_t* __restrict__ arms, const uint8_t* __restrict__ tmp,
+                                              uint8_t* __restrict__ cost) {
     extern __shared__ uint32_t sh[];
     const int lane = threadIdx.x;
-    const int x = blockIdx.x;
-    const int y0 = blockIdx.y * seg, y1 = min(h, y0 + seg);
+    const int x = bid.x;
+    const int y0 = bid.y * seg, y1 = min(h, y0 + seg);
     if (y0 >= h) return;
-    const int di = blockIdx.z * 128 + 2 * lane;
+    const int di = bid.z * 128 + 2 * lane;
     const int rs = 2 * Lmax + 2;
     const int ya = max(0, y0 - Lmax), yb = min(h, y1 + Lmax), kend = y1 + Lmax, nk = kend - ya;
     uint32_t* ring = sh;                       // [rs][64] u16-pair prefix sums
@@ -398,6 +397,20 @@ __global__ __launch_bounds__(64) void cross_v3_kernel(int w, int h, int d0, int 
         load(Y, kb + 3 * CV3_PF);
     }
 }
+
+__global__ __launch_bounds__(64) void cross_v3_kernel(int w, int h, int d0, int D, int Lmax, int seg,
+                                                      const int16_t* __restrict__ dmin,
+                                                      const int16_t* __restrict__ dmax,
+                                                      const uint32_t* __restrict__ arms,
+                                                      const uint8_t* __restrict__ tmp, uint8_t* __restrict__ cost) {
+    cross_v3_body(block3(), w, h, d0, D, Lmax, seg, dmin, dmax, arms, tmp, cost);
+}
+
+struct CrossV3Fn {
+    static constexpr int kThreads = 64;
+    template <typename... A>
+    __device__ static void run(const uint3 b, const A&... a) { cross_v3_body(b, a...); }
+};
 
 // Vertical Cross pass over u8 T for per-pixel ranges (the prior's): the grid
 // of cross_v3_kernel (column, row segment, 128-wide d chunk; the prefix of
@@ -526,7 +539,7 @@ __global__ __launch_bounds__(64) void cross_v4_kernel(int w, int h, int d0, int 
 
 hipError_t launch_cross_v_t(hipStream_t s, const apg_hip_params& p, int w, int h, const int16_t* dmin,
                             const int16_t* dmax, const uint32_t* arms, const void* tmp, void* cost, bool u8,
-                            bool column_ranges, const KTune& tn) {
+                            bool column_ranges, const KTune& tn, LaunchSink* sink) {
     {
         const int D = p.max_disparity - p.min_disparity;
         const int Lmax = p.max_arm_length > 1 ? p.max_arm_length : 1;
@@ -540,11 +553,15 @@ hipError_t launch_cross_v_t(hipStream_t s, const apg_hip_params& p, int w, int h
             const int seg = (h + nseg - 1) / nseg;
             nseg = (h + seg - 1) / seg;
             const size_t shm = (size_t)(2 * Lmax + 2) * 64 * sizeof(uint32_t) + (size_t)(seg + 2 * Lmax) * 16;
+            if (sink)
+                return sink->add<CrossV3Fn>(dim3(w, nseg, dch), shm, w, h, p.min_disparity, D, Lmax, seg, dmin, dmax,
+                                            arms, (const uint8_t*)tmp, (uint8_t*)cost);
             hipLaunchKernelGGL(cross_v3_kernel, dim3(w, nseg, dch), dim3(64), shm, s, w, h, p.min_disparity, D, Lmax,
                                seg, dmin, dmax, arms, (const uint8_t*)tmp, (uint8_t*)cost);
             return hipGetLastError();
         }
     }
+    if (sink) return hipErrorNotSupported;   // the other forms have no group launch
     {
         // per-pixel ranges: the segmented grid with counted prefixes
         const int D = p.max_disparity - p.min_disparity;

@@ -265,13 +265,17 @@ __device__ __forceinline__ uint32_t grad4(uint32_t gx, uint32_t gy) { return (gx
 // below, each half over the D disparities (FULL: D == 64), so no lane idles
 // for want of disparities; each row has its own records, arms and keys.
 template <bool SYM, bool FULL, bool NOCLAMP, bool GR, bool R2>
-__global__ __launch_bounds__(64) void cost_crossh2_kernel(
-    const CostLut* __restrict__ lut, int w, int h, int d0, int D, int cost_max, int Lmax, int seg,
-    const uint8_t* __restrict__ gl, const uint8_t* __restrict__ gxl, const uint8_t* __restrict__ gyl,
-    const uint8_t* __restrict__ gr, const uint8_t* __restrict__ gxr, const uint8_t* __restrict__ gyr,
-    const uint32_t* __restrict__ c32l, const uint32_t* __restrict__ c32r, const uint64_t* __restrict__ c64l,
-    const uint64_t* __restrict__ c64r, const int16_t* __restrict__ dmin, const int16_t* __restrict__ dmax,
-    const uint32_t* __restrict__ arms, uint8_t* __restrict__ out, uint32_t* __restrict__ rkeys) {
+__device__ __forceinline__ void cost_crossh2_body(const uint3 bid, const CostLut* __restrict__ lut, int w, int h,
+                                                  int d0, int D, int cost_max, int Lmax, int seg,
+                                                  const uint8_t* __restrict__ gl, const uint8_t* __restrict__ gxl,
+                                                  const uint8_t* __restrict__ gyl, const uint8_t* __restrict__ gr,
+                                                  const uint8_t* __restrict__ gxr, const uint8_t* __restrict__ gyr,
+                                                  const uint32_t* __restrict__ c32l,
+                                                  const uint32_t* __restrict__ c32r,
+                                                  const uint64_t* __restrict__ c64l,
+                                                  const uint64_t* __restrict__ c64r, const int16_t* __restrict__ dmin,
+                                                  const int16_t* __restrict__ dmax, const uint32_t* __restrict__ arms,
+                                                  uint8_t* __restrict__ out, uint32_t* __restrict__ rkeys) {
     extern __shared__ __align__(16) unsigned char shm[];
     constexpr int NR = R2 ? 2 : 1;
     const K2Shm L = k2_layout(D, Lmax, seg, SYM, GR, NR);
@@ -292,9 +296,9 @@ __global__ __launch_bounds__(64) void cost_crossh2_kernel(
     const int lane = threadIdx.x;
     const int rsub = R2 ? lane >> 5 : 0;    // the lane's row of the wave
     const int ln = R2 ? lane & 31 : lane;   // the lane within its row's group
-    const int yw = blockIdx.y * NR;         // the wave's first row
+    const int yw = bid.y * NR;         // the wave's first row
     const bool yin = yw + rsub < h;
-    const int x0 = blockIdx.x * seg;
+    const int x0 = bid.x * seg;
     const int ncol = seg + 2 * Lmax, rs = 2 * Lmax + 2;
     const int xb = x0 - Lmax;              // column 0
     const int xs = xb - (d0 + D - 1);      // right pixel of record 0
@@ -654,12 +658,31 @@ __global__ __launch_bounds__(64) void cost_crossh2_kernel(
     }
 }
 
+template <bool SYM, bool FULL, bool NOCLAMP, bool GR, bool R2>
+__global__ __launch_bounds__(64) void cost_crossh2_kernel(
+    const CostLut* __restrict__ lut, int w, int h, int d0, int D, int cost_max, int Lmax, int seg,
+    const uint8_t* __restrict__ gl, const uint8_t* __restrict__ gxl, const uint8_t* __restrict__ gyl,
+    const uint8_t* __restrict__ gr, const uint8_t* __restrict__ gxr, const uint8_t* __restrict__ gyr,
+    const uint32_t* __restrict__ c32l, const uint32_t* __restrict__ c32r, const uint64_t* __restrict__ c64l,
+    const uint64_t* __restrict__ c64r, const int16_t* __restrict__ dmin, const int16_t* __restrict__ dmax,
+    const uint32_t* __restrict__ arms, uint8_t* __restrict__ out, uint32_t* __restrict__ rkeys) {
+    cost_crossh2_body<SYM, FULL, NOCLAMP, GR, R2>(block3(), lut, w, h, d0, D, cost_max, Lmax, seg, gl, gxl, gyl, gr,
+                                                  gxr, gyr, c32l, c32r, c64l, c64r, dmin, dmax, arms, out, rkeys);
+}
+
+template <bool SYM, bool FULL, bool NOCLAMP, bool GR, bool R2>
+struct CostCrossH2Fn {
+    static constexpr int kThreads = 64;
+    template <typename... A>
+    __device__ static void run(const uint3 b, const A&... a) { cost_crossh2_body<SYM, FULL, NOCLAMP, GR, R2>(b, a...); }
+};
+
 hipError_t launch_cost_crossh(hipStream_t s, const apg_hip_params& p, const CostLut* d_lut, int w, int h,
                               const uint8_t* gl, const uint8_t* gxl, const uint8_t* gyl, const uint8_t* gr,
                               const uint8_t* gxr, const uint8_t* gyr, const uint32_t* c32l, const uint32_t* c32r,
                               const uint64_t* c64l, const uint64_t* c64r, const int16_t* dmin, const int16_t* dmax,
                               const uint32_t* arms, void* out, bool out_u8, uint32_t* rkeys, bool column_ranges,
-                              bool clamp_free, const KTune& tn) {
+                              bool clamp_free, const KTune& tn, LaunchSink* sink) {
     const int D = p.max_disparity - p.min_disparity;
     const bool cross = p.agg_enable != 0;
     const int Lmax = cross ? (p.max_arm_length > 1 ? p.max_arm_length : 1) : 0;
@@ -676,6 +699,24 @@ hipError_t launch_cost_crossh(hipStream_t s, const apg_hip_params& p, const Cost
         const bool r2 = D <= 64;   // two rows per wave
         const K2Shm L2 = k2_layout(D, Lmax, seg, p.census_type == 1, granges, r2 ? 2 : 1);
         const bool full = r2 ? D == 64 : D % 128 == 0;
+        const dim3 grid((w + seg - 1) / seg, r2 ? (h + 1) / 2 : h);
+        if (sink && !granges) {
+            // the column-range instances as group launches
+            auto rec = [&](auto fn) {
+                return sink->add<decltype(fn)>(grid, L2.total, d_lut, w, h, p.min_disparity, D, p.cost_max, Lmax, seg,
+                                               gl, gxl, gyl, gr, gxr, gyr, c32l, c32r, c64l, c64r, dmin, dmax,
+                                               cross ? arms : nullptr, (uint8_t*)out, rkeys);
+            };
+            const bool sym = p.census_type == 1;
+#define APG_G2(S, F, N) (r2 ? rec(CostCrossH2Fn<S, F, N, false, true>{}) : rec(CostCrossH2Fn<S, F, N, false, false>{}))
+            if (clamp_free)
+                return sym ? (full ? APG_G2(true, true, true) : APG_G2(true, false, true))
+                           : (full ? APG_G2(false, true, true) : APG_G2(false, false, true));
+            return sym ? (full ? APG_G2(true, true, false) : APG_G2(true, false, false))
+                       : (full ? APG_G2(false, true, false) : APG_G2(false, false, false));
+#undef APG_G2
+        }
+        if (sink) return hipErrorNotSupported;   // per-pixel ranges: no group form
         decltype(&cost_crossh2_kernel<true, true, true, false, false>) k;
 #define APG_K2R(S, F, N, G) (r2 ? cost_crossh2_kernel<S, F, N, G, true> : cost_crossh2_kernel<S, F, N, G, false>)
 #define APG_K2(S, F, N) (granges ? APG_K2R(S, F, N, true) : APG_K2R(S, F, N, false))
@@ -692,12 +733,13 @@ hipError_t launch_cost_crossh(hipStream_t s, const apg_hip_params& p, const Cost
                 hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)L2.total);
             if (e != hipSuccess) return e;
         }
-        hipLaunchKernelGGL(k, dim3((w + seg - 1) / seg, r2 ? (h + 1) / 2 : h), dim3(64), L2.total, s, d_lut, w, h,
+        hipLaunchKernelGGL(k, grid, dim3(64), L2.total, s, d_lut, w, h,
                            p.min_disparity, D,
                            p.cost_max, Lmax, seg, gl, gxl, gyl, gr, gxr, gyr, c32l, c32r, c64l, c64r, dmin, dmax,
                            cross ? arms : nullptr, (uint8_t*)out, rkeys);
         return hipGetLastError();
     }
+    if (sink) return hipErrorNotSupported;
     const FusedShm L = fused_layout(D, Lmax);
     dim3 grid((w + F_TX - 1) / F_TX, h);
 #define APG_LAUNCH_F(S, X, T)                                                                                      \
@@ -737,16 +779,18 @@ hipError_t launch_cost_crossh(hipStream_t s, const apg_hip_params& p, const Cost
 // best d plus the legacy parabola over C(xr+d-1, d-1), C(xr+d, d),
 // C(xr+d+1, d+1), recomputed from the pixel descriptors.
 template <bool SYM>
-__global__ void rwta_finalize_kernel(const CostLut* __restrict__ lut, int w, int h, int d0, int D, int cost_max,
-                                     int use_ad, int use_grad, int subpixel, const uint8_t* __restrict__ gl,
-                                     const uint8_t* __restrict__ gxl, const uint8_t* __restrict__ gyl,
-                                     const uint8_t* __restrict__ gr, const uint8_t* __restrict__ gxr,
-                                     const uint8_t* __restrict__ gyr, const uint32_t* __restrict__ c32l,
-                                     const uint32_t* __restrict__ c32r, const uint64_t* __restrict__ c64l,
-                                     const uint64_t* __restrict__ c64r, const int16_t* __restrict__ dmin,
-                                     const int16_t* __restrict__ dmax, const uint32_t* __restrict__ rkeys,
-                                     float* __restrict__ dr) {
-    const int xr = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+__device__ __forceinline__ void rwta_finalize_body(const uint3 bid, const CostLut* __restrict__ lut, int w, int h,
+                                                   int d0, int D, int cost_max, int use_ad, int use_grad,
+                                                   int subpixel, const uint8_t* __restrict__ gl,
+                                                   const uint8_t* __restrict__ gxl, const uint8_t* __restrict__ gyl,
+                                                   const uint8_t* __restrict__ gr, const uint8_t* __restrict__ gxr,
+                                                   const uint8_t* __restrict__ gyr, const uint32_t* __restrict__ c32l,
+                                                   const uint32_t* __restrict__ c32r,
+                                                   const uint64_t* __restrict__ c64l,
+                                                   const uint64_t* __restrict__ c64r,
+                                                   const int16_t* __restrict__ dmin, const int16_t* __restrict__ dmax,
+                                                   const uint32_t* __restrict__ rkeys, float* __restrict__ dr) {
+    const int xr = bid.x * blockDim.x + threadIdx.x, y = bid.y;
     if (xr >= w) return;
     const size_t row = (size_t)y * w;
     const uint32_t k = rkeys[row + xr];
@@ -785,13 +829,49 @@ __global__ void rwta_finalize_kernel(const CostLut* __restrict__ lut, int w, int
     dr[row + xr] = dval;
 }
 
+template <bool SYM>
+__global__ __launch_bounds__(128) void rwta_finalize_kernel(const CostLut* __restrict__ lut, int w, int h, int d0,
+                                                            int D, int cost_max, int use_ad, int use_grad,
+                                                            int subpixel, const uint8_t* __restrict__ gl,
+                                                            const uint8_t* __restrict__ gxl,
+                                                            const uint8_t* __restrict__ gyl,
+                                                            const uint8_t* __restrict__ gr,
+                                                            const uint8_t* __restrict__ gxr,
+                                                            const uint8_t* __restrict__ gyr,
+                                                            const uint32_t* __restrict__ c32l,
+                                                            const uint32_t* __restrict__ c32r,
+                                                            const uint64_t* __restrict__ c64l,
+                                                            const uint64_t* __restrict__ c64r,
+                                                            const int16_t* __restrict__ dmin,
+                                                            const int16_t* __restrict__ dmax,
+                                                            const uint32_t* __restrict__ rkeys,
+                                                            float* __restrict__ dr) {
+    rwta_finalize_body<SYM>(block3(), lut, w, h, d0, D, cost_max, use_ad, use_grad, subpixel, gl, gxl, gyl, gr, gxr,
+                            gyr, c32l, c32r, c64l, c64r, dmin, dmax, rkeys, dr);
+}
+
+template <bool SYM>
+struct RwtaFinalizeFn {
+    static constexpr int kThreads = 128;
+    template <typename... A>
+    __device__ static void run(const uint3 b, const A&... a) { rwta_finalize_body<SYM>(b, a...); }
+};
+
 hipError_t launch_rwta_finalize(hipStream_t s, const apg_hip_params& p, const CostLut* d_lut, int w, int h,
                                 const uint8_t* gl, const uint8_t* gxl, const uint8_t* gyl, const uint8_t* gr,
                                 const uint8_t* gxr, const uint8_t* gyr, const uint32_t* c32l, const uint32_t* c32r,
                                 const uint64_t* c64l, const uint64_t* c64r, const int16_t* dmin, const int16_t* dmax,
-                                const uint32_t* rkeys, float* dr) {
+                                const uint32_t* rkeys, float* dr, LaunchSink* sink) {
     const int D = p.max_disparity - p.min_disparity;
     dim3 grid((w + 127) / 128, h);
+    if (sink)
+        return p.census_type == 1
+                   ? sink->add<RwtaFinalizeFn<true>>(grid, 0, d_lut, w, h, p.min_disparity, D, p.cost_max, p.use_ad,
+                                                     p.use_grad, p.post_subpixel, gl, gxl, gyl, gr, gxr, gyr, c32l,
+                                                     c32r, c64l, c64r, dmin, dmax, rkeys, dr)
+                   : sink->add<RwtaFinalizeFn<false>>(grid, 0, d_lut, w, h, p.min_disparity, D, p.cost_max, p.use_ad,
+                                                      p.use_grad, p.post_subpixel, gl, gxl, gyl, gr, gxr, gyr, c32l,
+                                                      c32r, c64l, c64r, dmin, dmax, rkeys, dr);
     if (p.census_type == 1)
         hipLaunchKernelGGL(rwta_finalize_kernel<true>, grid, dim3(128), 0, s, d_lut, w, h, p.min_disparity, D,
                            p.cost_max, p.use_ad, p.use_grad, p.post_subpixel, gl, gxl, gyl, gr, gxr, gyr, c32l, c32r,

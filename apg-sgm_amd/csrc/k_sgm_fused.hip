@@ -129,12 +129,12 @@ __device__ __forceinline__ uint32_t pack_deltas(const uint32_t (&dl)[K]) {
 }
 
 template <typename TC, int K, bool FULL>
-__global__ __launch_bounds__(64) void sgm_lines_kernel(apg_hip_params p, int w, int h, int D, int nfam,
-                                                       const uint8_t* __restrict__ gray, const TC* __restrict__ cost,
-                                                       PlaneSet planes) {
+__device__ __forceinline__ void sgm_lines_body(const uint3 bid, const apg_hip_params& p, int w, int h, int D,
+                                               int nfam, const uint8_t* __restrict__ gray,
+                                               const TC* __restrict__ cost, const PlaneSet& planes) {
     const int lane = threadIdx.x;
     // block -> (family, member): rows, then columns, then the two diagonal ray sets
-    int b = blockIdx.x, f = 0;
+    int b = bid.x, f = 0;
     {
         const int cnt[4] = {h, w, w + h - 1, w + h - 1};
         while (f < nfam - 1 && b >= cnt[f]) { b -= cnt[f]; ++f; }
@@ -232,6 +232,20 @@ __global__ __launch_bounds__(64) void sgm_lines_kernel(apg_hip_params p, int w, 
         load(YA, YB, t0 + 3 * SL_PF);
     }
 }
+
+template <typename TC, int K, bool FULL>
+__global__ __launch_bounds__(64) void sgm_lines_kernel(apg_hip_params p, int w, int h, int D, int nfam,
+                                                       const uint8_t* __restrict__ gray, const TC* __restrict__ cost,
+                                                       PlaneSet planes) {
+    sgm_lines_body<TC, K, FULL>(block3(), p, w, h, D, nfam, gray, cost, planes);
+}
+
+template <typename TC, int K, bool FULL>
+struct SgmLinesFn {
+    static constexpr int kThreads = 64;
+    template <typename... A>
+    __device__ static void run(const uint3 b, const A&... a) { sgm_lines_body<TC, K, FULL>(b, a...); }
+};
 
 // Batched-transfer form for u8 C' with D % 16 == 0, D <= 256.  Each chain
 // moves SB steps of C' (SB pixel rows) from HBM into LDS with one 16-byte
@@ -415,11 +429,11 @@ __device__ __forceinline__ void sgm_job(const apg_hip_params& p, int w, int h, i
 // measured faster at 1280x720 D=128 (more waves share each SIMD's issue
 // slots while the longest scanlines bound the duration).
 template <int K, bool FULL, bool DUAL>
-__global__ __launch_bounds__(64) void sgm_lines4_kernel(apg_hip_params p, int w, int h, int D, int nfam,
-                                                        const uint8_t* __restrict__ gray,
-                                                        const uint8_t* __restrict__ cost, PlaneSet planes) {
+__device__ __forceinline__ void sgm_lines4_body(const uint3 blk, const apg_hip_params& p, int w, int h, int D,
+                                                int nfam, const uint8_t* __restrict__ gray,
+                                                const uint8_t* __restrict__ cost, const PlaneSet& planes) {
     extern __shared__ __align__(16) uint8_t sh[];
-    const int bid = blockIdx.x;
+    const int bid = blk.x;
     if (DUAL) {
         int b = bid, f = 0;
         const int cnt[4] = {h, w, w + h - 1, w + h - 1};
@@ -432,6 +446,20 @@ __global__ __launch_bounds__(64) void sgm_lines4_kernel(apg_hip_params p, int w,
         sgm_job<K, FULL, false>(p, w, h, D, f, b, bid & 1, gray, cost, planes, sh);
     }
 }
+
+template <int K, bool FULL, bool DUAL>
+__global__ __launch_bounds__(64) void sgm_lines4_kernel(apg_hip_params p, int w, int h, int D, int nfam,
+                                                        const uint8_t* __restrict__ gray,
+                                                        const uint8_t* __restrict__ cost, PlaneSet planes) {
+    sgm_lines4_body<K, FULL, DUAL>(block3(), p, w, h, D, nfam, gray, cost, planes);
+}
+
+template <int K, bool FULL, bool DUAL>
+struct SgmLines4Fn {
+    static constexpr int kThreads = 64;
+    template <typename... A>
+    __device__ static void run(const uint3 b, const A&... a) { sgm_lines4_body<K, FULL, DUAL>(b, a...); }
+};
 
 // Half-wave form for u8 C' with D % 16 == 0, D <= 128: one wave runs both
 // chains of its scanline, chain A (forward) in lanes 0-31 and chain B
@@ -711,17 +739,31 @@ __device__ __forceinline__ void sgm_half_body(const apg_hip_params& p, int w, in
 // Scanline family f, member b of block blockIdx.x.  PAIRED: every family's
 // bit is set in planes.paired (sgm_pair_mask pairs all or none).
 template <bool FULL, int PF, bool PAIRED>
-__global__ __launch_bounds__(64) void sgm_half_kernel(apg_hip_params p, int w, int h, int D, int nfam,
-                                                      const uint8_t* __restrict__ gray,
-                                                      const uint8_t* __restrict__ cost, PlaneSet planes, int nmax) {
+__device__ __forceinline__ void sgm_half_body(const uint3 bid, const apg_hip_params& p, int w, int h, int D, int nfam,
+                                              const uint8_t* __restrict__ gray, const uint8_t* __restrict__ cost,
+                                              const PlaneSet& planes, int nmax) {
     extern __shared__ __align__(16) uint8_t sh[];
-    int b = blockIdx.x, f = 0;
+    int b = bid.x, f = 0;
     {
         const int cnt[4] = {h, w, w + h - 1, w + h - 1};
         while (f < nfam - 1 && b >= cnt[f]) { b -= cnt[f]; ++f; }
     }
     sgm_half_body<FULL, PF, PAIRED>(p, w, h, D, f, b, gray, cost, planes, nmax, sh);
 }
+
+template <bool FULL, int PF, bool PAIRED>
+__global__ __launch_bounds__(64) void sgm_half_kernel(apg_hip_params p, int w, int h, int D, int nfam,
+                                                      const uint8_t* __restrict__ gray,
+                                                      const uint8_t* __restrict__ cost, PlaneSet planes, int nmax) {
+    sgm_half_body<FULL, PF, PAIRED>(block3(), p, w, h, D, nfam, gray, cost, planes, nmax);
+}
+
+template <bool FULL, int PF, bool PAIRED>
+struct SgmHalfFn {
+    static constexpr int kThreads = 64;
+    template <typename... A>
+    __device__ static void run(const uint3 b, const A&... a) { sgm_half_body<FULL, PF, PAIRED>(b, a...); }
+};
 
 // Quarter-wave form for u8 C', 4 paths, D % 16 == 0, D <= 128: one wave runs
 // the forward and backward chains of two scanlines of one family (rows 2i,
@@ -736,9 +778,9 @@ __global__ __launch_bounds__(64) void sgm_half_kernel(apg_hip_params p, int w, i
 // most other shapes (one wave per SIMD exposes the recurrence's latency when
 // the scanlines are long or few): a launch-shape option, not the default.
 template <bool FULL, int PF>
-__global__ __launch_bounds__(64) void sgm_quad_kernel(apg_hip_params p, int w, int h, int D,
-                                                      const uint8_t* __restrict__ gray,
-                                                      const uint8_t* __restrict__ cost, PlaneSet planes, int nmax) {
+__device__ __forceinline__ void sgm_quad_body(const uint3 bid, const apg_hip_params& p, int w, int h, int D,
+                                              const uint8_t* __restrict__ gray, const uint8_t* __restrict__ cost,
+                                              const PlaneSet& planes, int nmax) {
     extern __shared__ __align__(16) uint8_t sh[];
     constexpr int RB = 128;                  // staged row bytes (>= D)
     constexpr int NC = 4;                    // chains per wave
@@ -749,8 +791,8 @@ __global__ __launch_bounds__(64) void sgm_quad_kernel(apg_hip_params p, int w, i
     const int hq = lane >> 4, hl = lane & 15;
     // block -> family (rows, then columns) and its scanline pair
     const int nrow = (h + 1) / 2;
-    const int f = (int)blockIdx.x < nrow ? 0 : 1;
-    const int sp = f == 0 ? (int)blockIdx.x : (int)blockIdx.x - nrow;
+    const int f = (int)bid.x < nrow ? 0 : 1;
+    const int sp = f == 0 ? (int)bid.x : (int)bid.x - nrow;
     const int nline = f == 0 ? h : w;
     // scanlines s0 = 2sp and s1 = 2sp + 1 (s0 again past the last one: the same
     // values are computed twice and stored twice)
@@ -927,6 +969,20 @@ __global__ __launch_bounds__(64) void sgm_quad_kernel(apg_hip_params p, int w, i
     flush(nb - 1);
 }
 
+template <bool FULL, int PF>
+__global__ __launch_bounds__(64) void sgm_quad_kernel(apg_hip_params p, int w, int h, int D,
+                                                      const uint8_t* __restrict__ gray,
+                                                      const uint8_t* __restrict__ cost, PlaneSet planes, int nmax) {
+    sgm_quad_body<FULL, PF>(block3(), p, w, h, D, gray, cost, planes, nmax);
+}
+
+template <bool FULL, int PF>
+struct SgmQuadFn {
+    static constexpr int kThreads = 64;
+    template <typename... A>
+    __device__ static void run(const uint3 b, const A&... a) { sgm_quad_body<FULL, PF>(b, a...); }
+};
+
 // batches of C' rows loaded ahead per wave: 3 or 4 measured 242 us against
 // 262 us for 2 at 1280x720 D=128 (6 no better); even, so the register set of
 // a batch's words (k & 1) is static within the PF-batch trip
@@ -944,7 +1000,7 @@ int sgm_pair_mask(const apg_hip_params& p, int D, bool cost_u8, const KTune& tn)
 
 template <typename TC>
 static hipError_t launch_lines_t(hipStream_t s, const apg_hip_params& p, int w, int h, int D, const uint8_t* gray,
-                                 const TC* cost, const PlaneSet& planes, const KTune& tn) {
+                                 const TC* cost, const PlaneSet& planes, const KTune& tn, LaunchSink* sink) {
     const int K = (D + 127) / 128;
     const int nfam = p.paths == 8 ? 4 : 2;
     const int nblk = h + w + (nfam == 4 ? 2 * (w + h - 1) : 0);
@@ -955,6 +1011,11 @@ static hipError_t launch_lines_t(hipStream_t s, const apg_hip_params& p, int w, 
         const int nmax = ((w > h ? w : h) + SB - 1) / SB * SB;
         const size_t shmq = (size_t)16 * SB * 128 + (size_t)4 * nmax * 4;
         const int nq = (h + 1) / 2 + (w + 1) / 2;
+        if (sink)
+            return full ? sink->add<SgmQuadFn<true, kHalfPF>>(dim3(nq), shmq, p, w, h, D, gray, (const uint8_t*)cost,
+                                                             planes, nmax)
+                        : sink->add<SgmQuadFn<false, kHalfPF>>(dim3(nq), shmq, p, w, h, D, gray, (const uint8_t*)cost,
+                                                              planes, nmax);
         if (shmq > 64 * 1024) {
             hipError_t e = hipFuncSetAttribute(full ? (const void*)sgm_quad_kernel<true, kHalfPF>
                                                     : (const void*)sgm_quad_kernel<false, kHalfPF>,
@@ -983,8 +1044,13 @@ static hipError_t launch_lines_t(hipStream_t s, const apg_hip_params& p, int w, 
             if (e != hipSuccess) return e;
         }
 #define APG_HALF(FF, PP)                                                                                      \
-    hipLaunchKernelGGL((sgm_half_kernel<FF, kHalfPF, PP>), dim3(nblk), dim3(64), shmh, s, p, w, h, D, nfam, gray, \
-                       (const uint8_t*)cost, planes, nmax)
+    do {                                                                                                      \
+        if (sink)                                                                                             \
+            return sink->add<SgmHalfFn<FF, kHalfPF, PP>>(dim3(nblk), shmh, p, w, h, D, nfam, gray,            \
+                                                         (const uint8_t*)cost, planes, nmax);                 \
+        hipLaunchKernelGGL((sgm_half_kernel<FF, kHalfPF, PP>), dim3(nblk), dim3(64), shmh, s, p, w, h, D, nfam, \
+                           gray, (const uint8_t*)cost, planes, nmax);                                         \
+    } while (0)
         if (planes.paired) {
             if (full) APG_HALF(true, true);
             else APG_HALF(false, true);
@@ -1000,12 +1066,21 @@ static hipError_t launch_lines_t(hipStream_t s, const apg_hip_params& p, int w, 
         const size_t shm4 = (size_t)8 * SB * 128 * K + shm;
         const int nblk4 = 2 * nblk;
 #define APG_LINES4(KK, FF)                                                                                        \
-    if (tn.sgm_dual == 1)                                                                                         \
-        hipLaunchKernelGGL((sgm_lines4_kernel<KK, FF, true>), dim3(nblk), dim3(64), shm4, s, p, w, h, D, nfam,     \
-                           gray, (const uint8_t*)cost, planes);                                                   \
-    else                                                                                                          \
-        hipLaunchKernelGGL((sgm_lines4_kernel<KK, FF, false>), dim3(nblk4), dim3(64), shm4, s, p, w, h, D, nfam,   \
-                           gray, (const uint8_t*)cost, planes)
+    do {                                                                                                          \
+        if (tn.sgm_dual == 1) {                                                                                   \
+            if (sink)                                                                                             \
+                return sink->add<SgmLines4Fn<KK, FF, true>>(dim3(nblk), shm4, p, w, h, D, nfam, gray,             \
+                                                            (const uint8_t*)cost, planes);                        \
+            hipLaunchKernelGGL((sgm_lines4_kernel<KK, FF, true>), dim3(nblk), dim3(64), shm4, s, p, w, h, D, nfam, \
+                               gray, (const uint8_t*)cost, planes);                                               \
+        } else {                                                                                                  \
+            if (sink)                                                                                             \
+                return sink->add<SgmLines4Fn<KK, FF, false>>(dim3(nblk4), shm4, p, w, h, D, nfam, gray,           \
+                                                             (const uint8_t*)cost, planes);                       \
+            hipLaunchKernelGGL((sgm_lines4_kernel<KK, FF, false>), dim3(nblk4), dim3(64), shm4, s, p, w, h, D,     \
+                               nfam, gray, (const uint8_t*)cost, planes);                                         \
+        }                                                                                                         \
+    } while (0)
         if (K == 1) {
             if (full) APG_LINES4(1, true);
             else APG_LINES4(1, false);
@@ -1017,8 +1092,15 @@ static hipError_t launch_lines_t(hipStream_t s, const apg_hip_params& p, int w, 
         return hipGetLastError();
     }
 #define APG_LINES(KK, FF)                                                                                      \
-    hipLaunchKernelGGL((sgm_lines_kernel<TC, KK, FF>), dim3(nblk), dim3(64), shm, s, p, w, h, D, nfam, gray, cost, \
-                       planes)
+    do {                                                                                                       \
+        if (sink) {                                                                                            \
+            if constexpr (sizeof(TC) == 1)                                                                     \
+                return sink->add<SgmLinesFn<TC, KK, FF>>(dim3(nblk), shm, p, w, h, D, nfam, gray, cost, planes); \
+            return hipErrorNotSupported;                                                                       \
+        }                                                                                                      \
+        hipLaunchKernelGGL((sgm_lines_kernel<TC, KK, FF>), dim3(nblk), dim3(64), shm, s, p, w, h, D, nfam, gray,   \
+                           cost, planes);                                                                      \
+    } while (0)
     if (K == 1) {
         if (full) APG_LINES(1, true);
         else APG_LINES(1, false);
@@ -1033,10 +1115,11 @@ static hipError_t launch_lines_t(hipStream_t s, const apg_hip_params& p, int w, 
 }
 
 hipError_t launch_sgm_lines(hipStream_t s, const apg_hip_params& p, int w, int h, int D, const uint8_t* gray,
-                            const void* cost, bool cost_u8, const PlaneSet& planes, const KTune& tn) {
+                            const void* cost, bool cost_u8, const PlaneSet& planes, const KTune& tn,
+                            LaunchSink* sink) {
     if (D % 2 || D > 256 || (size_t)w * h * D * (cost_u8 ? 1 : 2) >= (1ull << 31)) return hipErrorInvalidValue;
-    return cost_u8 ? launch_lines_t<uint8_t>(s, p, w, h, D, gray, (const uint8_t*)cost, planes, tn)
-                   : launch_lines_t<uint16_t>(s, p, w, h, D, gray, (const uint16_t*)cost, planes, tn);
+    return cost_u8 ? launch_lines_t<uint8_t>(s, p, w, h, D, gray, (const uint8_t*)cost, planes, tn, sink)
+                   : launch_lines_t<uint16_t>(s, p, w, h, D, gray, (const uint16_t*)cost, planes, tn, sink);
 }
 
 }  // namespace apgk

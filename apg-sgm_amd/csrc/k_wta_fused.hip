@@ -57,21 +57,21 @@ constexpr int WF_WG = 4;
 // swap of adjacent lanes; half the LDS per wave, so twice the waves per CU
 // keep loads in flight while others scan.
 template <typename TC, int NP, bool WIDE, int LP, int MUL>
-__global__ __launch_bounds__(64) void wta_fused_kernel(apg_hip_params p, int w, int h, int D, int rw,
-                                                       const TC* __restrict__ cost, PlaneSet planes,
-                                                       const int16_t* __restrict__ dmin,
-                                                       const int16_t* __restrict__ dmax, const float* __restrict__ dr,
-                                                       const uint8_t* __restrict__ gx, const uint8_t* __restrict__ gy,
-                                                       float* __restrict__ disp, float* __restrict__ conf_out,
-                                                       uint8_t* __restrict__ reason, uint8_t* __restrict__ reliable,
-                                                       float* __restrict__ disp_wta, float* __restrict__ best_out,
-                                                       float* __restrict__ second_out, uint32_t* __restrict__ sum32) {
+__device__ __forceinline__ void wta_fused_body(const uint3 bid, const apg_hip_params& p, int w, int h, int D, int rw,
+                                               const TC* __restrict__ cost, const PlaneSet& planes,
+                                               const int16_t* __restrict__ dmin, const int16_t* __restrict__ dmax,
+                                               const float* __restrict__ dr, const uint8_t* __restrict__ gx,
+                                               const uint8_t* __restrict__ gy, float* __restrict__ disp,
+                                               float* __restrict__ conf_out, uint8_t* __restrict__ reason,
+                                               uint8_t* __restrict__ reliable, float* __restrict__ disp_wta,
+                                               float* __restrict__ best_out, float* __restrict__ second_out,
+                                               uint32_t* __restrict__ sum32) {
     extern __shared__ uint32_t st[];   // [PB][rw] words
     constexpr int PB = WF_PB / LP;     // pixels per block
     const int lane = threadIdx.x;
     const int pi = lane / LP, hf = lane % LP;
     const size_t npx = (size_t)w * h;
-    const size_t p_base = (size_t)blockIdx.x * PB;
+    const size_t p_base = (size_t)bid.x * PB;
     const int npb = (int)min((size_t)PB, npx - p_base);
     const size_t pix = p_base + pi;
     const bool act = pi < npb;
@@ -262,10 +262,32 @@ __global__ __launch_bounds__(64) void wta_fused_kernel(apg_hip_params p, int w, 
     disp[pix] = dout;
 }
 
+template <typename TC, int NP, bool WIDE, int LP, int MUL>
+__global__ __launch_bounds__(64) void wta_fused_kernel(apg_hip_params p, int w, int h, int D, int rw,
+                                                       const TC* __restrict__ cost, PlaneSet planes,
+                                                       const int16_t* __restrict__ dmin,
+                                                       const int16_t* __restrict__ dmax, const float* __restrict__ dr,
+                                                       const uint8_t* __restrict__ gx, const uint8_t* __restrict__ gy,
+                                                       float* __restrict__ disp, float* __restrict__ conf_out,
+                                                       uint8_t* __restrict__ reason, uint8_t* __restrict__ reliable,
+                                                       float* __restrict__ disp_wta, float* __restrict__ best_out,
+                                                       float* __restrict__ second_out, uint32_t* __restrict__ sum32) {
+    wta_fused_body<TC, NP, WIDE, LP, MUL>(block3(), p, w, h, D, rw, cost, planes, dmin, dmax, dr, gx, gy, disp,
+                                          conf_out, reason, reliable, disp_wta, best_out, second_out, sum32);
+}
+
+template <typename TC, int NP, bool WIDE, int LP, int MUL>
+struct WtaFusedFn {
+    static constexpr int kThreads = 64;
+    template <typename... A>
+    __device__ static void run(const uint3 b, const A&... a) { wta_fused_body<TC, NP, WIDE, LP, MUL>(b, a...); }
+};
+
 hipError_t launch_wta_fused(hipStream_t s, const apg_hip_params& p, int w, int h, const void* cost, bool cost_u8,
                             const PlaneSet& planes, const int16_t* dmin, const int16_t* dmax, const float* dr,
                             const uint8_t* gx, const uint8_t* gy, float* disp, float* conf, uint8_t* reason,
-                            uint8_t* reliable, float* disp_wta, float* best, float* second, uint32_t* sum32) {
+                            uint8_t* reliable, float* disp_wta, float* best, float* second, uint32_t* sum32,
+                            LaunchSink* sink) {
     const int D = p.max_disparity - p.min_disparity;
     if (D % 2 || D > 256) return hipErrorInvalidValue;
     const size_t npx = (size_t)w * h;
@@ -277,9 +299,15 @@ hipError_t launch_wta_fused(hipStream_t s, const apg_hip_params& p, int w, int h
     const size_t shm = (size_t)(WF_PB / lp) * rw * 4;
     dim3 grid((unsigned)((npx + WF_PB / lp - 1) / (WF_PB / lp)));
 #define APG_WTAF(TC, NP, WD, MUL)                                                                              \
-    hipLaunchKernelGGL((wta_fused_kernel<TC, NP, WD, (WD ? 2 : 1), MUL>), grid, dim3(64), shm, s, p, w, h, D, rw,    \
-                       (const TC*)cost, planes, dmin, dmax, dr, gx, gy, disp, conf, reason, reliable, disp_wta, best,   \
-                       second, sum32)
+    do {                                                                                                       \
+        if (sink)                                                                                              \
+            return sink->add<WtaFusedFn<TC, NP, WD, (WD ? 2 : 1), MUL>>(                                       \
+                grid, shm, p, w, h, D, rw, (const TC*)cost, planes, dmin, dmax, dr, gx, gy, disp, conf, reason,  \
+                reliable, disp_wta, best, second, sum32);                                                      \
+        hipLaunchKernelGGL((wta_fused_kernel<TC, NP, WD, (WD ? 2 : 1), MUL>), grid, dim3(64), shm, s, p, w, h, D, \
+                           rw, (const TC*)cost, planes, dmin, dmax, dr, gx, gy, disp, conf, reason, reliable,     \
+                           disp_wta, best, second, sum32);                                                     \
+    } while (0)
     if (planes.paired) {
         // paired families keep one plane (u8 C' only): NP planes for MUL paths
         if (!cost_u8) return hipErrorInvalidValue;

@@ -425,16 +425,15 @@ __global__ __launch_bounds__(256) void median3_kernel(int w, int h, const float*
 constexpr int FM_W = 64, FM_H = 16, FM_TC = 192, FM_TR = FM_H + 4, FM_FC = FM_W + 2, FM_FR = FM_H + 2;
 constexpr int FM_GMAX = 62;
 
-__global__ __launch_bounds__(256) void fill_median_kernel(int w, int h, int max_gap, int radius,
-                                                          const float* __restrict__ in,
-                                                          const uint8_t* __restrict__ reason,
-                                                          const uint8_t* __restrict__ gray,
-                                                          const float* __restrict__ prior, float* __restrict__ out) {
+__device__ __forceinline__ void fill_median_body(const uint3 bid, int w, int h, int max_gap, int radius,
+                                                 const float* __restrict__ in, const uint8_t* __restrict__ reason,
+                                                 const uint8_t* __restrict__ gray, const float* __restrict__ prior,
+                                                 float* __restrict__ out) {
     __shared__ float s_in[FM_TR * FM_TC];
     __shared__ uint8_t s_g[FM_TR * FM_TC];
     __shared__ uint64_t s_m[FM_TR * 3];
     __shared__ float s_f[FM_FR * FM_FC];
-    const int x0 = blockIdx.x * FM_W, y0 = blockIdx.y * FM_H;
+    const int x0 = bid.x * FM_W, y0 = bid.y * FM_H;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int xb = x0 - 64, yb = y0 - 2;   // image coordinates of tile cell (0, 0)
     // 60 row segments of 64 columns, 15 per wave; all loads issued first
@@ -549,14 +548,32 @@ __global__ __launch_bounds__(256) void fill_median_kernel(int w, int h, int max_
     }
 }
 
+__global__ __launch_bounds__(256) void fill_median_kernel(int w, int h, int max_gap, int radius,
+                                                          const float* __restrict__ in,
+                                                          const uint8_t* __restrict__ reason,
+                                                          const uint8_t* __restrict__ gray,
+                                                          const float* __restrict__ prior, float* __restrict__ out) {
+    fill_median_body(block3(), w, h, max_gap, radius, in, reason, gray, prior, out);
+}
+
+struct FillMedianFn {
+    static constexpr int kThreads = 256;
+    template <typename... A>
+    __device__ static void run(const uint3 b, const A&... a) { fill_median_body(b, a...); }
+};
+
 bool fill_median_fusable(const apg_hip_params& p) {
     return p.post_median_radius >= 0 && p.post_median_radius <= 1 && p.post_max_fill_gap >= 0 &&
            p.post_max_fill_gap <= FM_GMAX;
 }
 
 hipError_t launch_fill_median(hipStream_t s, const apg_hip_params& p, int w, int h, const float* disp,
-                              const uint8_t* reason, const uint8_t* gray, const float* prior, float* out) {
+                              const uint8_t* reason, const uint8_t* gray, const float* prior, float* out,
+                              LaunchSink* sink) {
     if (!fill_median_fusable(p)) return hipErrorInvalidValue;
+    if (sink)
+        return sink->add<FillMedianFn>(dim3((w + FM_W - 1) / FM_W, (h + FM_H - 1) / FM_H), 0, w, h,
+                                       p.post_max_fill_gap, p.post_median_radius, disp, reason, gray, prior, out);
     hipLaunchKernelGGL(fill_median_kernel, dim3((w + FM_W - 1) / FM_W, (h + FM_H - 1) / FM_H), dim3(256), 0, s, w, h,
                        p.post_max_fill_gap, p.post_median_radius, disp, reason, gray, prior, out);
     return hipGetLastError();

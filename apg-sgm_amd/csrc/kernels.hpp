@@ -7,6 +7,7 @@
 #include <stdint.h>
 
 #include "../../include/apg_sgm_hip.h"
+#include "group.hpp"
 
 // Library-internal: marks a context whose stream is being captured into a
 // hipGraph, so the pipeline refuses workspace growth and LUT uploads.
@@ -14,6 +15,20 @@ void apgk_ctx_set_capturing(apg_hip_ctx ctx, bool on);
 // Library-internal: device bytes a pipeline call on (p, w, h) would add to
 // the context's workspace.
 size_t apgk_ctx_growth_bytes(apg_hip_ctx ctx, const apg_hip_params* p, int w, int h);
+
+// Library-internal, the batched form's group launches (launches that span
+// pairs): a context on an existing stream, which it does not own and which
+// has no stage timing; sizing a context's workspace and cost LUT for a pair
+// of the fused column-range schedule (not eligible: nonzero status); and the
+// launches of one of the schedule's kGroupSteps steps for that pair,
+// recorded into a sink (a launch with no group form: nonzero status).
+constexpr int kGroupSteps = 7;
+namespace apgk {
+struct LaunchSink;
+}
+int apgk_ctx_create_on(int device, hipStream_t st, apg_hip_ctx* out);
+int apgk_group_prepare(apg_hip_ctx ctx, const apg_hip_pair& q);
+int apgk_group_record(apg_hip_ctx ctx, const apg_hip_pair& q, int step, apgk::LaunchSink* sink);
 
 namespace apgk {
 
@@ -63,7 +78,7 @@ struct AuxViews {
 // view's Cross arms (only where aux_pair_builds_arms(p))
 hipError_t launch_aux_pair(hipStream_t s, const apg_hip_params& p, int w, int h, int stride, int ch,
                            const AuxViews& L, const AuxViews& R, uint32_t* rkeys, int16_t* dmin, int16_t* dmax,
-                           uint32_t* arms);
+                           uint32_t* arms, LaunchSink* sink = nullptr);
 bool aux_pair_builds_arms(const apg_hip_params& p);
 hipError_t launch_range_init(hipStream_t s, const apg_hip_params& p, int w, int h, int16_t* dmin, int16_t* dmax);
 hipError_t launch_cost(hipStream_t s, const apg_hip_params& p, const CostLut* d_lut, int w, int h,
@@ -106,9 +121,13 @@ hipError_t launch_median(hipStream_t s, const apg_hip_params& p, int w, int h, c
 // radius 0 or 1 and max_fill_gap <= 62 (fill_median_fusable)
 bool fill_median_fusable(const apg_hip_params& p);
 hipError_t launch_fill_median(hipStream_t s, const apg_hip_params& p, int w, int h, const float* disp,
-                              const uint8_t* reason, const uint8_t* gray, const float* prior, float* out);
+                              const uint8_t* reason, const uint8_t* gray, const float* prior, float* out,
+                              LaunchSink* sink = nullptr);
 
 // ---- fused schedule ----
+// Launchers given a LaunchSink record the launches that group launches
+// support (the column-range forms) instead of enqueuing them; the others are
+// enqueued on s as without one.
 // Largest quantised cost any state can take (bound from the LUT maxima);
 // <= 254 allows u8 storage of T and C' with 255 as the invalid value.
 int cost_upper_bound(const CostLut& lut, int cost_max);
@@ -118,7 +137,7 @@ bool cost_clamp_free(const CostLut& lut, int cost_max);
 // column_ranges: every pixel's range depends on its column only (dense range init)
 hipError_t launch_cross_v_t(hipStream_t s, const apg_hip_params& p, int w, int h, const int16_t* dmin,
                             const int16_t* dmax, const uint32_t* arms, const void* tmp, void* cost, bool u8,
-                            bool column_ranges, const KTune& tn = KTune());
+                            bool column_ranges, const KTune& tn = KTune(), LaunchSink* sink = nullptr);
 hipError_t launch_fill_u32(hipStream_t s, size_t n, uint32_t v, uint32_t* p);
 hipError_t launch_cost_crossh(hipStream_t s, const apg_hip_params& p, const CostLut* d_lut, int w, int h,
                               const uint8_t* gl, const uint8_t* gxl, const uint8_t* gyl, const uint8_t* gr,
@@ -126,12 +145,12 @@ hipError_t launch_cost_crossh(hipStream_t s, const apg_hip_params& p, const Cost
                               const uint64_t* c64l, const uint64_t* c64r, const int16_t* dmin, const int16_t* dmax,
                               const uint32_t* arms, void* out, bool out_u8, uint32_t* rkeys,
                               bool column_ranges, bool clamp_free = false,
-                              const KTune& tn = KTune());
+                              const KTune& tn = KTune(), LaunchSink* sink = nullptr);
 hipError_t launch_rwta_finalize(hipStream_t s, const apg_hip_params& p, const CostLut* d_lut, int w, int h,
                                 const uint8_t* gl, const uint8_t* gxl, const uint8_t* gyl, const uint8_t* gr,
                                 const uint8_t* gxr, const uint8_t* gyr, const uint32_t* c32l, const uint32_t* c32r,
                                 const uint64_t* c64l, const uint64_t* c64r, const int16_t* dmin, const int16_t* dmax,
-                                const uint32_t* rkeys, float* dr);
+                                const uint32_t* rkeys, float* dr, LaunchSink* sink = nullptr);
 // One u8 delta plane per SGM direction (2 per scanline family: forward, backward)
 struct PlaneSet {
     uint8_t* p[8];
@@ -146,6 +165,8 @@ struct PlaneSet {
     // family, the two of a plain family forward then backward.
     int paired;
 };
+template <>
+struct GroupByRef<PlaneSet> : std::true_type {};
 constexpr size_t kPlaneSpare = 256;
 // Refiner (reference src/refiner.cpp:14-98) on the device.  The engine is
 // the std::mt19937(12345) output stream; it must hold (iterations + 1) *
@@ -180,11 +201,13 @@ hipError_t launch_refine(hipStream_t st, const apg_hip_params& p, const CostLut*
 // form with max(P2, P1 + 1) <= 127), 0 for none
 int sgm_pair_mask(const apg_hip_params& p, int D, bool cost_u8, const KTune& tn);
 hipError_t launch_sgm_lines(hipStream_t s, const apg_hip_params& p, int w, int h, int D, const uint8_t* gray,
-                            const void* cost, bool cost_u8, const PlaneSet& planes, const KTune& tn = KTune());
+                            const void* cost, bool cost_u8, const PlaneSet& planes, const KTune& tn = KTune(),
+                            LaunchSink* sink = nullptr);
 hipError_t launch_wta_fused(hipStream_t s, const apg_hip_params& p, int w, int h, const void* cost, bool cost_u8,
                             const PlaneSet& planes, const int16_t* dmin, const int16_t* dmax, const float* dr,
                             const uint8_t* gx, const uint8_t* gy, float* disp, float* conf, uint8_t* reason,
-                            uint8_t* reliable, float* disp_wta, float* best, float* second, uint32_t* sum32);
+                            uint8_t* reliable, float* disp_wta, float* best, float* second, uint32_t* sum32,
+                            LaunchSink* sink = nullptr);
 
 // HighQuality prior (k_prior.hip): per-sample census candidates; the
 // support selection, cell statistics and hole filling; the per-pixel prior

@@ -208,7 +208,9 @@ def bench_batched(args):
     """Batched form against the plain per-pair path on the same 64 device-
     resident pairs: plain = apg_hip_pipeline per pair on one stream with a
     synchronize after each pair; batched = apg_hip_batch_run over `slots`
-    streams replayed as one hipGraph, launch shapes from the autotuner."""
+    streams with group launches (one launch per step and kernel instance
+    spans a slot's pairs) replayed as one hipGraph, launch shapes from the
+    autotuner; also the same slots with each pair its own kernels."""
     import apg_amd as A
     specs = batch_specs(args.pairs)
     states = sum(w * h * d for w, h, d, _ in specs)
@@ -259,6 +261,22 @@ def bench_batched(args):
         direct_ms.append((time.perf_counter() - t0) * 1e3)
     same = all(all(np.array_equal(x, y, equal_nan=True) for x, y in zip(q.outputs(), w_))
                for q, w_ in zip(pairs, want))
+    grouped, launches, single = b.group_stats()
+    # the same slots with each pair its own kernels (no group launches)
+    for q in pairs:
+        q.clear_outputs()
+    b.run(arr, graph=True, grouped=False)
+    b.sync()
+    b.run(arr, graph=True, grouped=False)
+    b.sync()
+    per_pair_ms = []
+    for _ in range(args.reps):
+        t0 = time.perf_counter()
+        b.run(arr, graph=True, grouped=False)
+        b.sync()
+        per_pair_ms.append((time.perf_counter() - t0) * 1e3)
+    same = same and all(all(np.array_equal(x, y, equal_nan=True) for x, y in zip(q.outputs(), w_))
+                        for q, w_ in zip(pairs, want))
     tuned = {}
     for q in pairs:
         t = b.tuning_for(q.params, q.w, q.h).as_tuple()
@@ -267,7 +285,7 @@ def bench_batched(args):
     b.close()
     ctx.close()
     med = lambda v: float(sorted(v)[len(v) // 2])  # noqa: E731
-    p_ms, g_ms, d_ms = med(plain_ms), med(batch_ms), med(direct_ms)
+    p_ms, g_ms, d_ms, pp_ms = med(plain_ms), med(batch_ms), med(direct_ms), med(per_pair_ms)
     value = states / (g_ms * 1e-3) / 1e6
     plain = states / (p_ms * 1e-3) / 1e6
     return {"metric": f"Mdisp/s, batched form: {len(specs)} uneven pairs (sum W*H*D per batch / batch wall time)",
@@ -277,6 +295,10 @@ def bench_batched(args):
                                    "{64,128,192,256} < W/2, Balanced, noise + 4 rectangles", "seed": SEED_BATCH,
                        "states_per_batch": states, "slots": args.slots},
             "ms_per_batch": round(g_ms, 3), "ms_per_batch_direct": round(d_ms, 3),
+            "group_launches": {"pairs_grouped": grouped, "launches": launches, "pairs_single": single},
+            "slots_per_pair_kernels": {"ms_per_batch": round(pp_ms, 3),
+                                       "value": round(states / (pp_ms * 1e-3) / 1e6, 1),
+                                       "how": "the same slots and graph, each pair its own kernel sequence"},
             "plain_per_pair": {"value": round(plain, 1), "ms_per_batch": round(p_ms, 3),
                                "how": "apg_hip_pipeline per pair, one stream, synchronize after each pair"},
             "speedup_vs_plain": round(value / plain, 3), "bit_identical_to_plain": same,

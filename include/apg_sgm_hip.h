@@ -390,18 +390,32 @@ APG_HIP_API int apg_hip_batch_synchronize(apg_hip_batch b);
 APG_HIP_API size_t apg_hip_batch_workspace_bytes(apg_hip_batch b);
 APG_HIP_API const char* apg_hip_batch_last_error(apg_hip_batch b);
 
-#define APG_HIP_BATCH_GRAPH 1   /* replay the batch as one captured hipGraph */
+#define APG_HIP_BATCH_GRAPH 1      /* replay the batch as one captured hipGraph */
+#define APG_HIP_BATCH_PER_PAIR 2   /* no group launches: each pair its own kernels */
 
 /* Runs every pair, stream-ordered after earlier work on the origin stream;
  * later work on it waits for all pairs.  Pairs are spread over the slots by
  * longest-processing-time order of W*H*D, each with the launch shapes its
- * size bucket was tuned to.  With APG_HIP_BATCH_GRAPH the whole batch is one
+ * size bucket was tuned to.  Unless APG_HIP_BATCH_PER_PAIR is set, the pairs
+ * of a slot that take the fused column-range schedule (Fast, Balanced) run
+ * as group launches: each step of the schedule is one launch per kernel
+ * instance whose grid spans every pair of the slot's chunk that uses that
+ * instance (chunks of up to 1 GiB of workspace by default; each pair of a
+ * chunk has its own workspace, kept by the batch); the other pairs run one
+ * at a time on the slot after them.  With APG_HIP_BATCH_GRAPH the whole batch is one
  * hipGraphLaunch whenever the same pair list ran before on this batch: a new
  * list runs directly once (sizing workspaces and LUTs), is then captured,
  * and replays from the next call on.  A list with a prior or refiner stage
- * (HighQuality) always runs directly.  Results are identical to
- * apg_hip_pipeline on each pair alone. */
+ * (HighQuality) always runs directly, one pair at a time.  Results are
+ * identical to apg_hip_pipeline on each pair alone. */
 APG_HIP_API int apg_hip_batch_run(apg_hip_batch b, const apg_hip_pair* pairs, int n, int flags);
+/* Workspace bytes of the pairs one slot runs as one chunk of group
+ * launches (0: 1 GiB); applies from the next new pair list. */
+APG_HIP_API int apg_hip_batch_set_group_bytes(apg_hip_batch b, size_t bytes);
+/* Of the last pair list run with group launches: pairs in group launches,
+ * the group launches, and the pairs that ran one at a time. */
+APG_HIP_API int apg_hip_batch_group_stats(apg_hip_batch b, int* grouped_pairs, int* group_launches,
+                                          int* single_pairs);
 
 /* Autotuner: for each size bucket (W/256, H/128, D, paths, Cross on/off,
  * census) of `pairs` not yet tuned, times the candidate launch shapes of

@@ -84,31 +84,82 @@ def test_alone_results_of_small_pairs_match_oracle(alone):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("slots", [1, 3, 4])
-def test_batch_direct_matches_alone(alone, device_pairs, slots):
-    print(f"batch direct slots={slots}", flush=True)
+@pytest.mark.parametrize("slots,grouped", [(1, True), (3, True), (4, True), (1, False), (4, False)])
+def test_batch_direct_matches_alone(alone, device_pairs, slots, grouped):
+    print(f"batch direct slots={slots} grouped={grouped}", flush=True)
     with A.Batch(0, slots) as b:
+        for rep in range(2):
+            for q in device_pairs:
+                q.clear_outputs()
+            b.run(device_pairs, grouped=grouped)
+            b.sync()
+            for i, q in enumerate(device_pairs):
+                _same(q.outputs(), alone[i], f"slots={slots} grouped={grouped} call {rep} pair {i}")
+
+
+# pairs of the mix whose schedule has a group form for every step: the
+# fused column-range schedule (Fast, Balanced; u8 C', arms from the aux pass)
+GROUPABLE = [i for i, (kind, w, h, D, mode, over) in enumerate(MIX) if mode in (0, 1)]
+
+
+@pytest.mark.gpu
+def test_group_launches_span_pairs(alone, device_pairs):
+    # one slot: every Fast / Balanced pair of the mix runs in group launches,
+    # seven steps each, so fewer launches than pairs x steps means launches
+    # that cover several pairs
+    print("group launch plan, 1 slot", flush=True)
+    with A.Batch(0, 1) as b:
         for q in device_pairs:
             q.clear_outputs()
         b.run(device_pairs)
         b.sync()
+        grouped, launches, single = b.group_stats()
+        print(f"grouped={grouped} launches={launches} single={single}", flush=True)
+        assert grouped + single == len(MIX)
+        assert grouped == len(GROUPABLE)
+        assert 0 < launches < 7 * grouped
         for i, q in enumerate(device_pairs):
-            _same(q.outputs(), alone[i], f"slots={slots} pair {i}")
+            _same(q.outputs(), alone[i], f"grouped pair {i}")
 
 
 @pytest.mark.gpu
-def test_batch_graph_replays_match_alone(ctx, alone, device_pairs):
+def test_group_launches_of_one_shape(ctx):
+    # eight pairs of one size and mode (one kernel instance per step): seven
+    # launches in all, each pair's outputs as alone
+    print("group launches, 8 pairs of 200x120 D=64 Balanced", flush=True)
+    pars = A.params(1, 64)
+    pairs, want = [], []
+    for i in range(8):
+        left, right = _inputs(200 + i, i % 3, 200, 120, 64)
+        g = A.run_pipeline(ctx, pars, left, right)
+        want.append((g.disparity, g.confidence, g.invalid_reason, g.reliable))
+        pairs.append(A.DevicePair(ctx, pars, left, right))
+    with A.Batch(0, 1) as b:
+        arr = b.pair_array(pairs)
+        for rep in range(3):
+            for q in pairs:
+                q.clear_outputs()
+            b.run(arr, graph=True)
+            b.sync()
+            assert b.group_stats() == (8, 7, 0)
+            for i, q in enumerate(pairs):
+                _same(q.outputs(), want[i], f"call {rep} pair {i}")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("grouped", [True, False])
+def test_batch_graph_replays_match_alone(ctx, alone, device_pairs, grouped):
     # the first graph call runs directly and captures; each later call is one
     # graph launch.  Outputs are cleared and unrelated eager work runs on
     # another stream between replays, and every replay is checked.
-    print("batch graph capture + 3 replays", flush=True)
+    print(f"batch graph capture + 3 replays, grouped={grouped}", flush=True)
     other = A.DevicePair(ctx, A.params(1, 32), *_inputs(99, 0, 96, 64, 32))
     with A.Batch(0, 4) as b:
         arr = b.pair_array(device_pairs)
         for rep in range(4):
             for q in device_pairs:
                 q.clear_outputs()
-            b.run(arr, graph=True)
+            b.run(arr, graph=True, grouped=grouped)
             b.sync()
             for i, q in enumerate(device_pairs):
                 _same(q.outputs(), alone[i], f"graph call {rep} pair {i}")
