@@ -161,7 +161,7 @@ def load():
         "apg_hip_batch_last_error": (C.c_char_p, [vp]),
         "apg_hip_batch_run": (i, [vp, C.POINTER(Pair), i, i]),
         "apg_hip_batch_group_stats": (i, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
-        "apg_hip_batch_set_group_bytes": (i, [vp, C.c_size_t]),
+        "apg_hip_batch_set_grouping": (i, [vp, C.c_size_t, C.c_size_t]),
         "apg_hip_batch_autotune": (i, [vp, C.POINTER(Pair), i, C.c_char_p]),
         "apg_hip_batch_tuning_for": (i, [vp, P, i, i, C.POINTER(Tuning)]),
         "apg_hip_server_create": (i, [i, i, sz, C.POINTER(vp)]),
@@ -417,9 +417,11 @@ class Batch:
         flags = (BATCH_GRAPH if graph else 0) | (0 if grouped else BATCH_PER_PAIR)
         self._check(self.lib.apg_hip_batch_run(self.h, arr, len(arr), flags), "batch_run")
 
-    def set_group_bytes(self, nbytes):
-        """Workspace bytes per chunk of group launches (0: the default)."""
-        self._check(self.lib.apg_hip_batch_set_group_bytes(self.h, int(nbytes)), "set_group_bytes")
+    def set_grouping(self, chunk_bytes=0, max_pair_states=0):
+        """Group launches: workspace bytes per chunk and the largest pair
+        (W*H*D) that joins them (0: the defaults)."""
+        self._check(self.lib.apg_hip_batch_set_grouping(self.h, int(chunk_bytes), int(max_pair_states)),
+                    "set_grouping")
 
     def group_stats(self):
         """(pairs in group launches, group launches, pairs run one at a time)

@@ -144,6 +144,11 @@ namespace {
 // in 1 GiB chunks; 64 pairs of 640x480 D=64 9.99 against 10.40 ms with each
 // pair its own kernels.
 constexpr size_t kGroupChunkBytes = size_t(1) << 30;
+// Largest pair (W*H*D) that joins group launches (default): a larger pair's
+// own grids fill the device, and in a group launch it would only pay the
+// group prologue.  64 uneven pairs over 4 slots: 36.7 ms with 32 M, 37.3 with
+// 48 M, 37.5 with every pair grouped, 37.3 with none.
+constexpr size_t kGroupMaxStates = size_t(32) << 20;
 
 // One group launch: its kernel instance and the offset of its chunk records
 // in the plan's device buffer.
@@ -185,6 +190,7 @@ struct apg_hip_batch_s {
     std::vector<std::vector<apg_hip_ctx>> gctx;
     GroupPlan plan;
     size_t group_bytes = kGroupChunkBytes;
+    size_t group_states = kGroupMaxStates;
 };
 
 namespace {
@@ -362,6 +368,11 @@ int build_plan(apg_hip_batch b, const apg_hip_pair* pairs, int n) {
         size_t acc = 0;
         for (int i : per[s]) {
             const apg_hip_pair& q = pairs[i];
+            const size_t states = (size_t)q.width * q.height * (size_t)(q.params.max_disparity - q.params.min_disparity);
+            if (states > b->group_states) {
+                P.single[s].push_back(i);
+                continue;
+            }
             const size_t ws = apg_hip_pipeline_workspace_bytes(&q.params, q.width, q.height);
             if (chunks.empty() || acc + ws > b->group_bytes) {
                 chunks.emplace_back();
@@ -433,7 +444,8 @@ int build_plan(apg_hip_batch b, const apg_hip_pair* pairs, int n) {
     return 0;
 }
 
-// Enqueues the plan: per slot its group launches, then its single pairs.
+// Enqueues the plan: per slot its single pairs (the larger ones, first as in
+// longest-processing-time order), then its group launches.
 int enqueue_plan(apg_hip_batch b, const apg_hip_pair* pairs) {
     const GroupPlan& P = b->plan;
     if (int s = bhip(b, hipEventRecord(b->fork, b->origin), "fork record")) return s;
@@ -443,18 +455,18 @@ int enqueue_plan(apg_hip_batch b, const apg_hip_pair* pairs) {
         apg_hip_ctx ctx = b->slots[s];
         hipStream_t st = (hipStream_t)apg_hip_stream(ctx);
         if (int e = bhip(b, hipStreamWaitEvent(st, b->fork, 0), "fork wait")) return e;
+        for (int i : P.single[s]) {
+            if (status) break;
+            set_pair_tuning(b, ctx, pairs[i]);
+            status = run_pair(ctx, pairs[i]);
+            if (status) b->last_error = "pair " + std::to_string(i) + ": " + apg_hip_last_error(ctx);
+        }
         for (const GroupLaunch& g : P.launches[s]) {
             if (status) break;
             const void* rp = (const uint8_t*)P.d_tables + g.recs_off;
             void* kargs[] = {(void*)&rp};
             status = bhip(b, hipLaunchKernel(g.kernel, dim3(g.nblocks), dim3(g.threads), kargs, g.shm, st),
                           "group launch");
-        }
-        for (int i : P.single[s]) {
-            if (status) break;
-            set_pair_tuning(b, ctx, pairs[i]);
-            status = run_pair(ctx, pairs[i]);
-            if (status) b->last_error = "pair " + std::to_string(i) + ": " + apg_hip_last_error(ctx);
         }
         hipError_t e = hipEventRecord(b->join[s], st);
         if (e == hipSuccess) e = hipStreamWaitEvent(b->origin, b->join[s], 0);
@@ -576,9 +588,10 @@ int apg_hip_batch_run(apg_hip_batch b, const apg_hip_pair* pairs, int n, int fla
     return 0;
 }
 
-int apg_hip_batch_set_group_bytes(apg_hip_batch b, size_t bytes) {
+int apg_hip_batch_set_grouping(apg_hip_batch b, size_t chunk_bytes, size_t max_pair_states) {
     if (!b) return 8;
-    b->group_bytes = bytes ? bytes : kGroupChunkBytes;
+    b->group_bytes = chunk_bytes ? chunk_bytes : kGroupChunkBytes;
+    b->group_states = max_pair_states ? max_pair_states : kGroupMaxStates;
     drop_plan(b);
     return 0;
 }

@@ -126,10 +126,12 @@ __device__ __forceinline__ void byte_transpose(const uint32_t (&g)[4], uint32_t 
     c[3] = __builtin_amdgcn_perm(y1, x1, 0x07060302u);
 }
 
+constexpr int kAuxTile = (AX_ROWS + 2 * AX_LMAX) * AX_S;   // words
+
 template <int CT>
 __device__ __forceinline__ void aux4_pair_body(const uint3 bid, const AuxPair& ap, int w, int h, int stride, int ch,
                                                int gmin, int gmax, uint32_t* __restrict__ arms, int Lmax, int tau) {
-    __shared__ uint32_t tile[(AX_ROWS + 2 * AX_LMAX) * AX_S];
+    __shared__ uint32_t tile[kAuxTile];
     const int view = bid.z;
     // the pointers read as global ones (group launches read ap from memory)
     const AuxImage& imr = ap.img[view];
@@ -217,21 +219,17 @@ __device__ __forceinline__ void aux4_pair_body(const uint3 bid, const AuxPair& a
         else for (int j = 0; j < nq; ++j) im.keys[pi + j] = 0xFFFFFFFFu;
     }
     if (im.dmin) {   // as range_kernel
-        int16_t lo[4], hi[4];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            lo[j] = (int16_t)gmin;
-            hi[j] = (int16_t)max(min(gmax, x + j + 1), gmin);
-        }
+        // [gmin, max(min(gmax, x + 1), gmin)) per pixel, computed where stored:
+        // an array indexed by the edge loop below went to LDS (4 KB per block)
+        auto hi = [&](int j) { return (uint32_t)(uint16_t)max(min(gmax, x + j + 1), gmin); };
         if (wide) {
-            *(uint2*)(im.dmin + pi) = make_uint2((uint16_t)lo[0] | ((uint32_t)(uint16_t)lo[1] << 16),
-                                                 (uint16_t)lo[2] | ((uint32_t)(uint16_t)lo[3] << 16));
-            *(uint2*)(im.dmax + pi) = make_uint2((uint16_t)hi[0] | ((uint32_t)(uint16_t)hi[1] << 16),
-                                                 (uint16_t)hi[2] | ((uint32_t)(uint16_t)hi[3] << 16));
+            const uint32_t lo2 = (uint32_t)(uint16_t)gmin * 0x10001u;
+            *(uint2*)(im.dmin + pi) = make_uint2(lo2, lo2);
+            *(uint2*)(im.dmax + pi) = make_uint2(hi(0) | hi(1) << 16, hi(2) | hi(3) << 16);
         } else {
             for (int j = 0; j < nq; ++j) {
-                im.dmin[pi + j] = lo[j];
-                im.dmax[pi + j] = hi[j];
+                im.dmin[pi + j] = (int16_t)gmin;
+                im.dmax[pi + j] = (int16_t)hi(j);
             }
         }
     }

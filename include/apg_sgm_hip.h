@@ -400,18 +400,20 @@ APG_HIP_API const char* apg_hip_batch_last_error(apg_hip_batch b);
  * of a slot that take the fused column-range schedule (Fast, Balanced) run
  * as group launches: each step of the schedule is one launch per kernel
  * instance whose grid spans every pair of the slot's chunk that uses that
- * instance (chunks of up to 1 GiB of workspace by default; each pair of a
- * chunk has its own workspace, kept by the batch); the other pairs run one
- * at a time on the slot after them.  With APG_HIP_BATCH_GRAPH the whole batch is one
+ * instance (chunks of up to 1 GiB of workspace, pairs of up to 32 M states
+ * by default; each pair of a chunk has its own workspace, kept by the
+ * batch); the other pairs run one at a time on the slot before them.  With APG_HIP_BATCH_GRAPH the whole batch is one
  * hipGraphLaunch whenever the same pair list ran before on this batch: a new
  * list runs directly once (sizing workspaces and LUTs), is then captured,
  * and replays from the next call on.  A list with a prior or refiner stage
  * (HighQuality) always runs directly, one pair at a time.  Results are
  * identical to apg_hip_pipeline on each pair alone. */
 APG_HIP_API int apg_hip_batch_run(apg_hip_batch b, const apg_hip_pair* pairs, int n, int flags);
-/* Workspace bytes of the pairs one slot runs as one chunk of group
- * launches (0: 1 GiB); applies from the next new pair list. */
-APG_HIP_API int apg_hip_batch_set_group_bytes(apg_hip_batch b, size_t bytes);
+/* Group launches: workspace bytes of the pairs one slot runs as one chunk
+ * (0: 1 GiB), and the largest pair (W*H*D states) that joins them; larger
+ * pairs fill the device alone and run with their own kernels (0: 32 M states).
+ * Applies from the next new pair list. */
+APG_HIP_API int apg_hip_batch_set_grouping(apg_hip_batch b, size_t chunk_bytes, size_t max_pair_states);
 /* Of the last pair list run with group launches: pairs in group launches,
  * the group launches, and the pairs that ran one at a time. */
 APG_HIP_API int apg_hip_batch_group_stats(apg_hip_batch b, int* grouped_pairs, int* group_launches,

@@ -123,6 +123,26 @@ def test_group_launches_span_pairs(alone, device_pairs):
 
 
 @pytest.mark.gpu
+def test_grouping_size_threshold(alone, device_pairs):
+    # pairs above the state threshold run with their own kernels, before the
+    # group launches of the others, on the same slot
+    small = [i for i in GROUPABLE if MIX[i][1] * MIX[i][2] * MIX[i][3] <= 1_000_000]
+    print(f"grouping threshold 1 M states: {len(small)} small pairs", flush=True)
+    with A.Batch(0, 1) as b:
+        b.set_grouping(0, 1_000_000)
+        for rep in range(2):
+            for q in device_pairs:
+                q.clear_outputs()
+            b.run(device_pairs, graph=rep == 1)
+            b.sync()
+            grouped, launches, single = b.group_stats()
+            assert (grouped, single) == (len(small), len(MIX) - len(small))
+            assert 0 < launches < 7 * grouped
+            for i, q in enumerate(device_pairs):
+                _same(q.outputs(), alone[i], f"threshold call {rep} pair {i}")
+
+
+@pytest.mark.gpu
 def test_group_launches_of_one_shape(ctx):
     # eight pairs of one size and mode (one kernel instance per step): seven
     # launches in all, each pair's outputs as alone
