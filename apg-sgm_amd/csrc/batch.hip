@@ -22,10 +22,7 @@
 
 #include <algorithm>
 #include <chrono>
-#include <condition_variable>
 #include <deque>
-#include <functional>
-#include <mutex>
 #include <map>
 #include <memory>
 #include <numeric>
@@ -34,6 +31,7 @@
 #include <tuple>
 #include <vector>
 
+#include "copy_pool.hpp"
 #include "kernels.hpp"
 
 namespace {
@@ -717,86 +715,6 @@ struct IoLayout {
     size_t left, right, disp, conf, reason, rel, total;
 };
 
-// Host copies between the requests' buffers and the pinned staging, split
-// over a few threads: one request moves its two images in and four result
-// planes out (about 10 bytes per pixel), which at the serving mix's rate is
-// more than one core copies.
-class CopyPool {
-public:
-    explicit CopyPool(int nthreads) {
-        for (int i = 0; i < nthreads; ++i) th_.emplace_back([this] { work(); });
-    }
-    ~CopyPool() {
-        {
-            std::lock_guard<std::mutex> g(m_);
-            stop_ = true;
-        }
-        cv_.notify_all();
-        for (auto& t : th_) t.join();
-    }
-    // Copies every (dst, src, bytes) of the list; returns when all are done.
-    void copy(const std::vector<std::tuple<void*, const void*, size_t>>& list) {
-        constexpr size_t kPiece = size_t(1) << 20;
-        std::vector<std::tuple<void*, const void*, size_t>> pieces;
-        for (const auto& c : list) {
-            uint8_t* d = (uint8_t*)std::get<0>(c);
-            const uint8_t* s = (const uint8_t*)std::get<1>(c);
-            const size_t n = std::get<2>(c);
-            for (size_t o = 0; o < n; o += kPiece) pieces.emplace_back(d + o, s + o, std::min(kPiece, n - o));
-        }
-        if (th_.empty() || pieces.size() <= 1) {
-            for (const auto& p : pieces) memcpy(std::get<0>(p), std::get<1>(p), std::get<2>(p));
-            return;
-        }
-        {
-            std::lock_guard<std::mutex> g(m_);
-            for (const auto& p : pieces) q_.push_back(p);
-            pending_ += pieces.size();
-        }
-        cv_.notify_all();
-        // the calling thread copies too, then waits for the rest
-        for (;;) {
-            std::tuple<void*, const void*, size_t> p;
-            {
-                std::unique_lock<std::mutex> g(m_);
-                if (q_.empty()) {
-                    done_cv_.wait(g, [this] { return pending_ == 0; });
-                    return;
-                }
-                p = q_.front();
-                q_.pop_front();
-            }
-            memcpy(std::get<0>(p), std::get<1>(p), std::get<2>(p));
-            finish_one();
-        }
-    }
-
-private:
-    void finish_one() {
-        std::lock_guard<std::mutex> g(m_);
-        if (--pending_ == 0) done_cv_.notify_all();
-    }
-    void work() {
-        for (;;) {
-            std::tuple<void*, const void*, size_t> p;
-            {
-                std::unique_lock<std::mutex> g(m_);
-                cv_.wait(g, [this] { return stop_ || !q_.empty(); });
-                if (q_.empty()) return;
-                p = q_.front();
-                q_.pop_front();
-            }
-            memcpy(std::get<0>(p), std::get<1>(p), std::get<2>(p));
-            finish_one();
-        }
-    }
-    std::vector<std::thread> th_;
-    std::mutex m_;
-    std::condition_variable cv_, done_cv_;
-    std::deque<std::tuple<void*, const void*, size_t>> q_;
-    size_t pending_ = 0;
-    bool stop_ = false;
-};
 
 IoLayout io_layout(const apg_hip_request& r) {
     const size_t n = (size_t)r.width * r.height;
@@ -826,7 +744,7 @@ struct apg_hip_server_s {
     std::vector<Slot> slots;
     std::deque<std::pair<apg_hip_request, uint64_t>> queue;
     std::string last_error;
-    std::unique_ptr<CopyPool> copies;
+    std::unique_ptr<apgk::CopyPool> copies;
 };
 
 namespace {
@@ -959,8 +877,7 @@ int apg_hip_server_create(int device, int slots, size_t hbm_budget_bytes, apg_hi
     s->budget = hbm_budget_bytes ? hbm_budget_bytes : ((size_t)64 << 30);
     s->slots.resize(slots);
     // staging copies: the polling thread and up to 7 more
-    const int hw = (int)std::thread::hardware_concurrency();
-    s->copies.reset(new (std::nothrow) CopyPool(std::max(0, std::min(7, hw / 2 - 1))));
+    s->copies.reset(new (std::nothrow) apgk::CopyPool(apgk::CopyPool::default_threads()));
     int st = s->copies ? (hipSetDevice(device) == hipSuccess ? 0 : 8) : 7;
     for (Slot& sl : s->slots) {
         if (st) break;

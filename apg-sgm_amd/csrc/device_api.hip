@@ -539,6 +539,20 @@ int apg_hip_free(apg_hip_ctx ctx, void* d_ptr) {
     return 0;
 }
 
+int apg_hip_host_malloc(apg_hip_ctx ctx, size_t bytes, void** h_out) {
+    if (!ctx || !h_out) return 8;
+    HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
+    HIPCHK(hipHostMalloc(h_out, bytes ? bytes : 1, hipHostMallocDefault), "hipHostMalloc");
+    return 0;
+}
+
+int apg_hip_host_free(apg_hip_ctx ctx, void* h_ptr) {
+    if (!ctx) return 8;
+    HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
+    HIPCHK(hipHostFree(h_ptr), "hipHostFree");
+    return 0;
+}
+
 int apg_hip_upload(apg_hip_ctx ctx, void* d_dst, const void* h_src, size_t bytes) {
     if (!ctx) return 8;
     HIPCHK(hipMemcpyAsync(d_dst, h_src, bytes, hipMemcpyHostToDevice, ctx->stream), "upload");

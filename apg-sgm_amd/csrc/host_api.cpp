@@ -15,6 +15,7 @@
 #include "apg_sgm/pipeline.hpp"
 #include "apg_sgm/stages.hpp"
 #include "apg_sgm_hip.h"
+#include "copy_pool.hpp"
 
 namespace apg {
 
@@ -703,6 +704,7 @@ StereoWorkspace::StereoWorkspace(ExecutionOptions options) : options_(options) {
 StereoWorkspace::~StereoWorkspace() {
     if (ctx_) {
         if (dev_io_) apg_hip_free(ctx_, dev_io_);
+        if (host_io_) apg_hip_host_free(ctx_, host_io_);
         apg_hip_ctx_destroy(ctx_);
     }
 }
@@ -736,6 +738,9 @@ void StereoWorkspace::release() {
         if (dev_io_) apg_hip_free(ctx_, dev_io_);
         dev_io_ = nullptr;
         dev_io_bytes_ = 0;
+        if (host_io_) apg_hip_host_free(ctx_, host_io_);
+        host_io_ = nullptr;
+        host_io_bytes_ = 0;
         apg_hip_release_workspace(ctx_);
     }
     last_w_ = last_h_ = last_dmax_ = 0;
@@ -743,6 +748,13 @@ void StereoWorkspace::release() {
 
 // ------------------------------------------------------------ StereoMatcher --
 namespace {
+// shared by every session of the process (copies of one call at a time
+// finish before it returns)
+apgk::CopyPool& copy_pool() {
+    static apgk::CopyPool pool(apgk::CopyPool::default_threads());
+    return pool;
+}
+
 Status status_from(int rc, const char* msg) {
     if (rc == 0) return Status::Ok();
     return Status(static_cast<StatusCode>(rc), msg ? msg : "");
@@ -780,12 +792,32 @@ Status StereoMatcher::compute_impl(const ImageView8& left, const ImageView8& rig
             ws.dev_io_bytes_ = total;
         }
         uint8_t* base = static_cast<uint8_t*>(ws.dev_io_);
+        // A workspace that keeps its memory between frames also keeps a pinned
+        // host block of the same layout: the images are copied into it and
+        // the results out of it over a few threads, and each direction is one
+        // transfer at the link's full rate.  Per-frame release copies directly
+        // (a pinned allocation per frame would cost more than it saves).
+        uint8_t* hbase = nullptr;
+        if (ws.options().memory_policy != MemoryPolicy::ReleasePerFrame) {
+            if (ws.host_io_bytes_ < total) {
+                if (ws.host_io_) apg_hip_host_free(ctx, ws.host_io_);
+                ws.host_io_ = nullptr;
+                ws.host_io_bytes_ = 0;
+                if (apg_hip_host_malloc(ctx, total, &ws.host_io_) == 0) ws.host_io_bytes_ = total;
+            }
+            hbase = static_cast<uint8_t*>(ws.host_io_);
+        }
         // Equal strides: the strided rows are uploaded as-is and the kernels
         // read through the stride. Different strides: both views are packed.
         int rc;
         if (right.stride() == left.stride()) {
-            rc = apg_hip_upload(ctx, base, left.data(), in_l);
-            if (!rc) rc = apg_hip_upload(ctx, base + o_r, right.data(), in_r);
+            if (hbase) {
+                copy_pool().copy({{hbase, left.data(), in_l}, {hbase + o_r, right.data(), in_r}});
+                rc = apg_hip_upload(ctx, base, hbase, o_r + in_r);
+            } else {
+                rc = apg_hip_upload(ctx, base, left.data(), in_l);
+                if (!rc) rc = apg_hip_upload(ctx, base + o_r, right.data(), in_r);
+            }
         } else {
             const std::vector<uint8_t> lp = pack(left), rp = pack(right);
             rc = apg_hip_upload(ctx, base, lp.data(), lp.size());
@@ -824,14 +856,28 @@ Status StereoMatcher::compute_impl(const ImageView8& left, const ImageView8& rig
                               reinterpret_cast<float*>(base + o_c), base + o_re, base + o_m,
                               any_inter ? &inter : nullptr);
         if (rc) return status_from(rc, apg_hip_last_error(ctx));
-        result.disparity = Image32f(w, h, -1.f);
-        result.confidence = Image32f(w, h, 0.f);
-        result.invalid_reason = Image8u1(w, h, 0);
-        result.reliable_mask = Image8u1(w, h, 0);
-        rc = apg_hip_download(ctx, result.disparity.data(), base + o_d, n * 4);
-        if (!rc) rc = apg_hip_download(ctx, result.confidence.data(), base + o_c, n * 4);
-        if (!rc) rc = apg_hip_download(ctx, result.invalid_reason.data(), base + o_re, n);
-        if (!rc) rc = apg_hip_download(ctx, result.reliable_mask.data(), base + o_m, n);
+        // every pixel of the four planes is written below: planes of the
+        // right size are reused as they are
+        if (result.disparity.width() != w || result.disparity.height() != h) result.disparity = Image32f(w, h, -1.f);
+        if (result.confidence.width() != w || result.confidence.height() != h)
+            result.confidence = Image32f(w, h, 0.f);
+        if (result.invalid_reason.width() != w || result.invalid_reason.height() != h)
+            result.invalid_reason = Image8u1(w, h, 0);
+        if (result.reliable_mask.width() != w || result.reliable_mask.height() != h)
+            result.reliable_mask = Image8u1(w, h, 0);
+        if (hbase) {
+            rc = apg_hip_download(ctx, hbase + o_d, base + o_d, o_m + n - o_d);
+            if (!rc)
+                copy_pool().copy({{result.disparity.data(), hbase + o_d, n * 4},
+                                  {result.confidence.data(), hbase + o_c, n * 4},
+                                  {result.invalid_reason.data(), hbase + o_re, n},
+                                  {result.reliable_mask.data(), hbase + o_m, n}});
+        } else {
+            rc = apg_hip_download(ctx, result.disparity.data(), base + o_d, n * 4);
+            if (!rc) rc = apg_hip_download(ctx, result.confidence.data(), base + o_c, n * 4);
+            if (!rc) rc = apg_hip_download(ctx, result.invalid_reason.data(), base + o_re, n);
+            if (!rc) rc = apg_hip_download(ctx, result.reliable_mask.data(), base + o_m, n);
+        }
         if (!rc && extra) {
             extra->disparity_right = Image32f(w, h, -1.f);
             rc = apg_hip_download(ctx, extra->disparity_right.data(), dr_dev->as<void>(), n * 4);

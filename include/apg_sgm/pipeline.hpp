@@ -124,6 +124,8 @@ private:
     friend class StereoMatcher;
     void* dev_io_ = nullptr;   // device input/output staging
     size_t dev_io_bytes_ = 0;
+    void* host_io_ = nullptr;  // pinned host staging of the same layout (retaining policies)
+    size_t host_io_bytes_ = 0;
 };
 
 class StereoMatcher {

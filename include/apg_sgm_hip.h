@@ -93,6 +93,11 @@ APG_HIP_API int apg_hip_free(apg_hip_ctx ctx, void* d_ptr);
 APG_HIP_API int apg_hip_upload(apg_hip_ctx ctx, void* d_dst, const void* h_src, size_t bytes);
 APG_HIP_API int apg_hip_download(apg_hip_ctx ctx, void* h_dst, const void* d_src, size_t bytes);
 APG_HIP_API int apg_hip_memset(apg_hip_ctx ctx, void* d_dst, int value, size_t bytes);
+/* Pinned (page-locked) host memory: uploads and downloads from it run at the
+ * link's full rate.  No counterpart in the reference (its buffers are host
+ * memory already); the session path stages its images and results here. */
+APG_HIP_API int apg_hip_host_malloc(apg_hip_ctx ctx, size_t bytes, void** h_out);
+APG_HIP_API int apg_hip_host_free(apg_hip_ctx ctx, void* h_ptr);
 
 /* ---- stages --------------------------------------------------------- */
 

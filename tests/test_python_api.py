@@ -133,3 +133,40 @@ def test_threads_on_one_session_are_serialised():
         outs = list(ex.map(lambda _: s.process(left, right).disparity.copy(), range(8)))
     assert all(np.array_equal(o, want) for o in outs)
     assert s.frame_count == 9
+
+
+@pytest.mark.gpu
+def test_bounded_reuse_across_sizes_and_threads():
+    # a retaining workspace stages images and results through pinned host
+    # memory that grows with the frame and is then reused; result planes of
+    # the right size are reused as well.  Frames of changing sizes, then one
+    # session per thread at once (the copy threads are shared), all against
+    # the oracle.
+    sizes = [(160, 96), (200, 80), (150, 60), (200, 80), (96, 64)]
+    frames = []
+    for i, (w, h) in enumerate(sizes):
+        left, right, _ = A.synth_pair(i % 3, w, h, 48, 300 + i)
+        o = O.pipeline(O.cfg(1, 48), left, right)
+        frames.append((left, right, o))
+    s = apg.StereoSession(mode="balanced", max_disparity=48, memory_policy="bounded_reuse",
+                          max_retained_bytes=1 << 30)
+    for i, (left, right, o) in enumerate(frames):
+        print(f"bounded reuse frame {i}: {left.shape[1]}x{left.shape[0]}", flush=True)
+        r = s.process(left, right)
+        np.testing.assert_array_equal(r.disparity, o["disparity"])
+        np.testing.assert_array_equal(r.confidence, o["confidence"])
+        np.testing.assert_array_equal(r.invalid_reason, o["invalid_reason"])
+        np.testing.assert_array_equal(r.reliable_mask, o["reliable"])
+
+    def run(i):
+        left, right, o = frames[i % len(frames)]
+        t = apg.StereoSession(mode="balanced", max_disparity=48, memory_policy="bounded_reuse",
+                              max_retained_bytes=1 << 30)
+        ok = True
+        for _ in range(3):
+            r = t.process(left, right)
+            ok = ok and np.array_equal(r.disparity, o["disparity"]) and np.array_equal(r.confidence, o["confidence"])
+        return ok
+    print("bounded reuse: 4 sessions on 4 threads", flush=True)
+    with concurrent.futures.ThreadPoolExecutor(max_workers=4) as ex:
+        assert all(ex.map(run, range(4)))
