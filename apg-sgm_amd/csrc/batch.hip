@@ -528,7 +528,13 @@ void* apg_hip_batch_stream(apg_hip_batch b) { return b ? (void*)b->origin : null
 
 int apg_hip_batch_synchronize(apg_hip_batch b) {
     if (!b) return 8;
-    return bhip(b, hipStreamSynchronize(b->origin), "batch synchronize");
+    if (int s = bhip(b, hipStreamSynchronize(b->origin), "batch synchronize")) return s;
+    // a HighQuality refiner inside a replayed graph cannot report a hand-off
+    // wait that gave up when it runs: its slot's sticky status tells here
+    for (size_t s = 0; s < b->slots.size(); ++s)
+        if (int e = apgk_ctx_take_status(b->slots[s]))
+            return bfail(b, e, "slot " + std::to_string(s) + ": " + apg_hip_last_error(b->slots[s]));
+    return 0;
 }
 
 size_t apg_hip_batch_workspace_bytes(apg_hip_batch b) {
@@ -547,10 +553,10 @@ int apg_hip_batch_run(apg_hip_batch b, const apg_hip_pair* pairs, int n, int fla
     if (n < 0 || (n > 0 && !pairs)) return bfail(b, 4, "invalid pair list");
     if (n == 0) return 0;
     if (int s = bhip(b, hipSetDevice(b->device), "hipSetDevice")) return s;
-    bool host_steps = false;   // the prior and the refiner synchronise: no capture
-    for (int i = 0; i < n; ++i) host_steps = host_steps || pairs[i].params.prior_enable || pairs[i].params.refine_enable;
-    const bool grouped = !(flags & APG_HIP_BATCH_PER_PAIR) && !host_steps;
-    if (!(flags & APG_HIP_BATCH_GRAPH) || host_steps) {
+    // (HighQuality pairs have no group form: they run one at a time on their
+    // slot, in a captured graph like the others)
+    const bool grouped = !(flags & APG_HIP_BATCH_PER_PAIR);
+    if (!(flags & APG_HIP_BATCH_GRAPH)) {
         // a direct run may grow slot workspaces, which a cached graph points into
         drop_graph(b);
         if (!grouped) return enqueue(b, pairs, n);

@@ -77,6 +77,10 @@ struct apg_hip_ctx_s {
     int cu_count = 0;
     long long wall_hz = 0;
     long long refine_wait = 0;
+    // sticky status of the refiner's banded sweep: 1 once a hand-off wait gave
+    // up; read and cleared after a direct run, or by the batch's synchronize
+    // after a replayed graph
+    uint32_t* d_status = nullptr;
 };
 
 namespace {
@@ -479,6 +483,7 @@ void apg_hip_ctx_destroy(apg_hip_ctx ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (auto& b : ctx->bufs)
         if (b.ptr) (void)hipFree(b.ptr);
+    if (ctx->d_status) (void)hipFree(ctx->d_status);
     for (int i = 0; i <= S_COUNT; ++i)
         if (ctx->ev[i]) (void)hipEventDestroy(ctx->ev[i]);
     if (ctx->stream && ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
@@ -838,7 +843,6 @@ int run_prior(apg_hip_ctx ctx, const apg_hip_params* p, int w, int h, const uint
 int run_refine(apg_hip_ctx ctx, const apg_hip_params* p, int w, int h, const uint8_t* gl, const uint8_t* gxl,
                const uint8_t* gyl, const uint8_t* gr, const uint8_t* gxr, const uint8_t* gyr, const void* cl,
                const void* cr, const int16_t* dmin, const int16_t* dmax, const uint8_t* reliable, float* disp) {
-    if (ctx->capturing) return fail(ctx, 8, "the HighQuality refiner uploads its random stream and cannot be captured");
     if (h > 4096) return fail(ctx, 6, "the refiner supports heights up to 4096");
     hipStream_t st = ctx->stream;
     const size_t n = (size_t)w * h;
@@ -847,6 +851,7 @@ int run_refine(apg_hip_ctx ctx, const apg_hip_params* p, int w, int h, const uin
     if (need > 0xFFFFFFFFull) return fail(ctx, 5, "the refiner's random stream exceeds 2^32 outputs");
     (void)n;
     if (ctx->engine_len < need || !ctx->bufs[B_ENGINE].ptr) {
+        if (ctx->capturing) return fail(ctx, 8, "the refiner's random stream was not uploaded before stream capture");
         std::vector<uint32_t> e(need);
         std::mt19937 gen(12345);
         for (auto& v : e) v = (uint32_t)gen();
@@ -868,22 +873,21 @@ int run_refine(apg_hip_ctx ctx, const apg_hip_params* p, int w, int h, const uin
         ctx->cu_count = cus > 0 ? cus : 1;
         ctx->wall_hz = khz > 0 ? (long long)khz * 1000 : 100000000LL;
     }
-    // the banded sweep's hand-offs: a bounded wait (2 s of wall clock), whose
-    // give-up is reported below; under stream capture the result cannot be
-    // read back, so the single-workgroup sweep runs instead
-    const RefineBanding bd{ctx->capturing ? 0 : 1, ctx->cu_count,
-                           ctx->refine_wait ? ctx->refine_wait : 2 * ctx->wall_hz};
-    uint32_t* band_err = nullptr;
+    if (!ctx->d_status) {
+        if (ctx->capturing) return fail(ctx, 8, "the refiner's status word was not allocated before stream capture");
+        HIPCHK(hipMalloc((void**)&ctx->d_status, 256), "alloc");
+        HIPCHK(launch_fill_u32(st, 1, 0u, ctx->d_status), "status clear");
+    }
+    // the banded sweep's hand-offs: a bounded wait (2 s of wall clock); a
+    // give-up sets the context's sticky status, read below after a direct
+    // run, or by apgk_ctx_take_status after a replayed graph
+    const RefineBanding bd{1, ctx->cu_count, ctx->refine_wait ? ctx->refine_wait : 2 * ctx->wall_hz};
+    bool banded = false;
     HIPCHK(launch_refine(st, *p, cur_lut(ctx), w, h, in, dmin, dmax, reliable, disp,
                          buf<uint32_t>(ctx, B_ENGINE), ctx->engine_len, ctx->bufs[B_REFINE].ptr,
-                         ctx->bufs[B_REFINE].bytes, bd, &band_err),
+                         ctx->bufs[B_REFINE].bytes, bd, ctx->d_status, &banded),
            "refine");
-    if (band_err) {
-        uint32_t gave_up = 0;
-        HIPCHK(hipMemcpyAsync(&gave_up, band_err, 4, hipMemcpyDeviceToHost, st), "refine status");
-        HIPCHK(hipStreamSynchronize(st), "refine status");
-        if (gave_up) return fail(ctx, 8, "refiner: a band's hand-off wait gave up");
-    }
+    if (banded && !ctx->capturing) return apgk_ctx_take_status(ctx);
     return 0;
 }
 
@@ -1088,6 +1092,17 @@ int apg_hip_get_tuning(apg_hip_ctx ctx, apg_hip_tuning* out) {
 // Library-internal (hidden): stream-capture guard used by the batch executor.
 void apgk_ctx_set_capturing(apg_hip_ctx ctx, bool on) {
     if (ctx) ctx->capturing = on;
+}
+
+int apgk_ctx_take_status(apg_hip_ctx ctx) {
+    if (!ctx || !ctx->d_status) return 0;
+    uint32_t gave_up = 0;
+    HIPCHK(hipMemcpyAsync(&gave_up, ctx->d_status, 4, hipMemcpyDeviceToHost, ctx->stream), "refine status");
+    HIPCHK(hipStreamSynchronize(ctx->stream), "refine status");
+    if (!gave_up) return 0;
+    HIPCHK(launch_fill_u32(ctx->stream, 1, 0u, ctx->d_status), "status clear");
+    HIPCHK(hipStreamSynchronize(ctx->stream), "status clear");
+    return fail(ctx, 8, "refiner: a band's hand-off wait gave up");
 }
 
 size_t apgk_ctx_growth_bytes(apg_hip_ctx ctx, const apg_hip_params* p, int w, int h) {

@@ -404,7 +404,7 @@ __global__ void refine_band_unpack_kernel(int w, int h, int fwd, int kb, const f
 
 struct BandSync {
     uint32_t* edge;     // [bands + 1][kb]: row b + 1 the tagged last-row values of band b (zeroed before the sweep)
-    uint32_t* err;      // set when a wait gave up
+    uint32_t* err;      // set when a wait gave up (sticky: cleared by the host after reading it)
     long long timeout;  // wall-clock ticks the waits may take; < 0: give up at once
 };
 
@@ -570,7 +570,7 @@ BandKernel<PD> band_kernel(int use_ad, int use_grad) {
 hipError_t launch_refine(hipStream_t st, const apg_hip_params& p, const CostLut* d_lut, int w, int h,
                          const RefineInputs& in, const int16_t* dmin, const int16_t* dmax, const uint8_t* reliable,
                          float* disp, const uint32_t* eng, size_t eng_len, void* scratch, size_t scratch_bytes,
-                         const RefineBanding& bd, uint32_t** band_err) {
+                         const RefineBanding& bd, uint32_t* status, bool* banded_out) {
     const size_t n = (size_t)w * h, nw = n + kRefineSlack;
     if (h > kSweepThreads * kSweepRows || n >= (1u << 31)) return hipErrorInvalidValue;
     if (scratch_bytes < refine_scratch_bytes(w, h)) return hipErrorInvalidValue;
@@ -601,7 +601,6 @@ hipError_t launch_refine(hipStream_t st, const apg_hip_params& p, const CostLut*
     uint2* rt = (uint2*)sp; sp += r256((n + 2 * (size_t)kBandPadCols) * 8);
     uint32_t* base = (uint32_t*)sp; sp += 256;
     uint32_t* edge = (uint32_t*)sp;
-    uint32_t* berr = edge + (size_t)(nbands + 1) * kb;
     sp += r256(((size_t)(nbands + 1) * kb + 1) * 4);
     void* temp = sp;
     size_t temp_bytes = scratch_bytes - (size_t)(sp - (uint8_t*)scratch);
@@ -621,9 +620,9 @@ hipError_t launch_refine(hipStream_t st, const apg_hip_params& p, const CostLut*
     int rs_log2 = 6;
     while ((1 << rs_log2) < D) ++rs_log2;
     const size_t bshm = 3328 + (size_t)kBandRows * ((size_t)1 << rs_log2) * 8;
-    const bool banded = bd.enable && sym && D > 0 && D <= 256 && std::abs(d0) + D + 2 * kBandSlack <= kBandPadCols &&
-                        n + 2 * (size_t)kBandPadCols < ((size_t)1 << 29) &&
-                        nbands <= bd.max_bands;
+    const bool banded = bd.enable && status && sym && D > 0 && D <= 256 &&
+                        std::abs(d0) + D + 2 * kBandSlack <= kBandPadCols &&
+                        n + 2 * (size_t)kBandPadCols < ((size_t)1 << 29) && nbands <= bd.max_bands;
     const BandKernel<kBandPD> bk = band_kernel<kBandPD>(p.use_ad, p.use_grad);
     if (banded) {
         if (bshm > 64 * 1024 &&
@@ -632,7 +631,7 @@ hipError_t launch_refine(hipStream_t st, const apg_hip_params& p, const CostLut*
             return e;
         hipLaunchKernelGGL(refine_band_terms_kernel, dim3(gb), dim3(256), 0, st, (int)n, rrec, rt);
     }
-    if (band_err) *band_err = banded ? berr : nullptr;
+    if (banded_out) *banded_out = banded;
     // a band runs whole prefetch spans: the last lane's last column is step w + 62
     const int krun = (w + kBandRows - 1 + kBandPD - 1) / kBandPD * kBandPD;
     const unsigned gs = (unsigned)((slots + 255) / 256);
@@ -650,8 +649,8 @@ hipError_t launch_refine(hipStream_t st, const apg_hip_params& p, const CostLut*
             hipLaunchKernelGGL(refine_band_static_kernel, dim3(gs), dim3(256), 0, st, d_lut, w, h, fwd, r, kb,
                                p.use_ad, p.use_grad, lrec, rrec, reliable, dmin, dmax, disp, eng,
                                (uint32_t)(eng_len - 1), base, acc, didx, slots, bra, brb);
-            if ((e = launch_fill_u32(st, (size_t)(nbands + 1) * kb + 1, 0u, edge)) != hipSuccess) return e;
-            const BandSync bs{edge, berr, bd.wait_ticks};
+            if ((e = launch_fill_u32(st, (size_t)(nbands + 1) * kb, 0u, edge)) != hipSuccess) return e;
+            const BandSync bs{edge, status, bd.wait_ticks};
             hipLaunchKernelGGL(bk, dim3(nbands), dim3(kBandRows), bshm, st, d_lut, w, h, fwd, d0, D, rs_log2, kb, krun,
                                bra, brb, rt, sres, bs);
             hipLaunchKernelGGL(refine_band_unpack_kernel, dim3(gb), dim3(256), 0, st, w, h, fwd, kb, sres, disp);

@@ -27,6 +27,10 @@ namespace apgk {
 struct LaunchSink;
 }
 int apgk_ctx_create_on(int device, hipStream_t st, apg_hip_ctx* out);
+// Library-internal: reads and clears the context's sticky refiner status
+// (after its stream's work is done): 0, or 8 when a banded sweep's hand-off
+// wait gave up since the last read.
+int apgk_ctx_take_status(apg_hip_ctx ctx);
 int apgk_group_prepare(apg_hip_ctx ctx, const apg_hip_pair& q);
 int apgk_group_record(apg_hip_ctx ctx, const apg_hip_pair& q, int step, apgk::LaunchSink* sink);
 
@@ -186,7 +190,9 @@ inline size_t refine_engine_len(const apg_hip_params& p, int w, int h) {
 }
 // The banded sweep (64-row bands, one wave each, hand-offs through HBM with
 // a bounded wait) when enable is set and it applies; wait_ticks: wall-clock
-// ticks a hand-off may wait, < 0 gives up at the first wait (for tests).
+// ticks a hand-off may wait, < 0 gives up at the first wait (for tests).  A
+// wait that gives up sets *status to 1 (never clears it; the caller reads and
+// clears it); *banded_out tells whether the banded sweep ran.
 struct RefineBanding {
     int enable;
     int max_bands;          // bands that can be resident together (one per CU)
@@ -195,7 +201,7 @@ struct RefineBanding {
 hipError_t launch_refine(hipStream_t st, const apg_hip_params& p, const CostLut* d_lut, int w, int h,
                          const RefineInputs& in, const int16_t* dmin, const int16_t* dmax, const uint8_t* reliable,
                          float* disp, const uint32_t* eng, size_t eng_len, void* scratch, size_t scratch_bytes,
-                         const RefineBanding& bd, uint32_t** band_err);
+                         const RefineBanding& bd, uint32_t* status, bool* banded_out);
 // the scanline families (bit f: 0 rows, 1 columns, 2 and 3 the diagonals)
 // whose two directions launch_sgm_lines sums into one plane (the half-wave
 // form with max(P2, P1 + 1) <= 127), 0 for none

@@ -410,9 +410,10 @@ APG_HIP_API const char* apg_hip_batch_last_error(apg_hip_batch b);
  * batch); the other pairs run one at a time on the slot before them.  With APG_HIP_BATCH_GRAPH the whole batch is one
  * hipGraphLaunch whenever the same pair list ran before on this batch: a new
  * list runs directly once (sizing workspaces and LUTs), is then captured,
- * and replays from the next call on.  A list with a prior or refiner stage
- * (HighQuality) always runs directly, one pair at a time.  Results are
- * identical to apg_hip_pipeline on each pair alone. */
+ * and replays from the next call on.  HighQuality pairs run one at a time on
+ * their slot; in a replayed graph a refiner hand-off wait that gave up is
+ * reported by apg_hip_batch_synchronize.  Results are identical to
+ * apg_hip_pipeline on each pair alone. */
 APG_HIP_API int apg_hip_batch_run(apg_hip_batch b, const apg_hip_pair* pairs, int n, int flags);
 /* Group launches: workspace bytes of the pairs one slot runs as one chunk
  * (0: 1 GiB), and the largest pair (W*H*D states) that joins them; larger

@@ -292,3 +292,31 @@ def test_server_budget_admission_with_highquality(ctx):
         left, right = _inputs(i, kind, w, h, D)
         o = O.pipeline(O.cfg_from(A.params(mode, D).as_dict()), left, right)
         _same(r.outputs(), (o["disparity"], o["confidence"], o["invalid_reason"], o["reliable"]), f"mode {mode}")
+
+
+@pytest.mark.gpu
+def test_batch_graph_with_highquality(ctx):
+    # HighQuality pairs (device prior, refiner with its banded sweep) in a
+    # captured batch beside a Balanced pair: every replay, with unrelated
+    # eager work between replays, equals each pair run alone
+    specs = [(2, 200, 150, 64, 0, 41), (2, 160, 120, 48, 1, 42), (1, 200, 150, 64, 2, 43)]
+    pairs, want = [], []
+    for mode, w, h, D, kind, seed in specs:
+        left, right, _ = A.synth_pair(kind, w, h, D, seed)
+        g = A.run_pipeline(ctx, A.params(mode, D), left, right)
+        want.append((g.disparity, g.confidence, g.invalid_reason, g.reliable))
+        pairs.append(A.DevicePair(ctx, A.params(mode, D), left, right))
+    other = A.DevicePair(ctx, A.params(1, 32), *_inputs(98, 0, 96, 64, 32))
+    with A.Batch(0, 2) as b:
+        arr = b.pair_array(pairs)
+        for rep in range(4):
+            print(f"HighQuality batch graph call {rep}", flush=True)
+            for q in pairs:
+                q.clear_outputs()
+            b.run(arr, graph=True)
+            b.sync()
+            for i, q in enumerate(pairs):
+                _same(q.outputs(), want[i], f"HighQuality graph call {rep} pair {i}")
+            with A.Batch(0, 1) as eager:
+                eager.run(eager.pair_array([other]))
+                eager.sync()
