@@ -281,49 +281,59 @@ __global__ __launch_bounds__(64) void cross_v2_kernel(int w, int h, int d0, int 
 // segments (with 180-row segments 16: 83 us, 8: 87 us, 32: 98 us)
 constexpr int CV3_PF = 16;
 
+// TWO (D <= 64): two columns per wave, a half-wave (32 lanes, 2 d each) per
+// column, so no lane idles; each column has its own window metadata.
+template <bool TWO>
 __device__ __forceinline__ void cross_v3_body(const uint3 bid, int w, int h, int d0, int D, int Lmax, int seg,
                                               const int16_t* __restrict__ dmin, const int16_t* __restrict__ dmax,
                                               const uint32_t* __restrict__ arms, const uint8_t* __restrict__ tmp,
                                               uint8_t* __restrict__ cost) {
     extern __shared__ uint32_t sh[];
     const int lane = threadIdx.x;
-    const int x = bid.x;
+    constexpr int NC = TWO ? 2 : 1;              // columns per wave
+    const int half = TWO ? lane >> 5 : 0;        // the lane's column of the wave
+    const int xw = bid.x * NC;                   // the wave's first column
+    const int x = xw + half;                     // the lane's column
     const int y0 = bid.y * seg, y1 = min(h, y0 + seg);
     if (y0 >= h) return;
-    const int di = bid.z * 128 + 2 * lane;
+    const int di = bid.z * 128 + 2 * (TWO ? lane & 31 : lane);
     const int rs = 2 * Lmax + 2;
     const int ya = max(0, y0 - Lmax), yb = min(h, y1 + Lmax), kend = y1 + Lmax, nk = kend - ya;
     uint32_t* ring = sh;                       // [rs][64] u16-pair prefix sums
-    uint4* meta = (uint4*)(sh + rs * 64);      // [nk] per step: write slot, window slots, magic
+    uint4* meta = (uint4*)(sh + rs * 64);      // [NC][nk] per step: write slot, window slots, magic
     // step k (input row k, output row y = k - Lmax): byte offsets of the ring
     // rows P(k + 1), P(y + Dn + 1), P(y - U) and ceil(2^31 / (U + Dn + 1))
-    for (int r = lane; r < nk; r += 64) {
+    for (int rr = lane; rr < NC * nk; rr += 64) {
+        const int c = TWO ? rr / nk : 0, r = rr - c * nk;
         const int k = ya + r, y = k - Lmax;
         uint4 m;
         m.x = (uint32_t)((r + 1) % rs) * 256u;
         m.y = m.z = m.w = 0u;
-        if (y >= y0) {
-            const uint32_t a = arms[(size_t)y * w + x];
+        if (y >= y0 && xw + c < w) {
+            const uint32_t a = arms[(size_t)y * w + xw + c];
             const int U = (a >> 16) & 0xFF, Dn = a >> 24;
             m.y = (uint32_t)((y + Dn + 1 - ya) % rs) * 256u;
             m.z = (uint32_t)((y - U - ya) % rs) * 256u;
             const uint32_t n = (uint32_t)(U + Dn + 1);
             m.w = (0x80000000u + n - 1u) / n;
         }
-        meta[r] = m;
+        meta[rr] = m;
     }
+    if (TWO) meta += half * nk;   // the lane's column's metadata
     ring[lane] = 0u;   // P(ya)
     __syncthreads();
-    const int lo = dmin[(size_t)y0 * w + x], hi = dmax[(size_t)y0 * w + x];
+    const int xc = min(x, w - 1);
+    const int lo = dmin[(size_t)y0 * w + xc], hi = dmax[(size_t)y0 * w + xc];
     const int dA = d0 + di;
-    const bool inA = di < D;   // D even: the pair is inside or outside as a whole
+    const bool inA = di < D && x < w;   // D even: the pair is inside or outside as a whole
     const uint32_t omask = ((inA && dA >= lo && dA < hi) ? 0u : 0xFFu) |
                            ((inA && dA + 1 >= lo && dA + 1 < hi) ? 0u : 0xFF00u);
-    const uint32_t voff = inA ? (uint32_t)di : 0u;
+    // byte offset of the lane's pair from the wave's first column
+    const uint32_t voff = inA ? (uint32_t)(di + half * D) : 0u;
     const uint32_t lofs = 4u * (uint32_t)lane;
     const auto rt = raw_rsrc(tmp);
     const auto rc = raw_rsrc(cost);
-    const uint32_t rowb = (uint32_t)w * (uint32_t)D, colb = (uint32_t)x * (uint32_t)D;
+    const uint32_t rowb = (uint32_t)w * (uint32_t)D, colb = (uint32_t)xw * (uint32_t)D;
     const int kout = y0 + Lmax;   // first step with an output row
     uint32_t acc = 0;
     auto step = [&](int k, uint32_t raw, uint4 m) {
@@ -398,18 +408,20 @@ __device__ __forceinline__ void cross_v3_body(const uint3 bid, int w, int h, int
     }
 }
 
+template <bool TWO>
 __global__ __launch_bounds__(64) void cross_v3_kernel(int w, int h, int d0, int D, int Lmax, int seg,
                                                       const int16_t* __restrict__ dmin,
                                                       const int16_t* __restrict__ dmax,
                                                       const uint32_t* __restrict__ arms,
                                                       const uint8_t* __restrict__ tmp, uint8_t* __restrict__ cost) {
-    cross_v3_body(block3(), w, h, d0, D, Lmax, seg, dmin, dmax, arms, tmp, cost);
+    cross_v3_body<TWO>(block3(), w, h, d0, D, Lmax, seg, dmin, dmax, arms, tmp, cost);
 }
 
+template <bool TWO>
 struct CrossV3Fn {
     static constexpr int kThreads = 64;
     template <typename... A>
-    __device__ static void run(const uint3 b, const A&... a) { cross_v3_body(b, a...); }
+    __device__ static void run(const uint3 b, const A&... a) { cross_v3_body<TWO>(b, a...); }
 };
 
 // Vertical Cross pass over u8 T for per-pixel ranges (the prior's): the grid
@@ -545,19 +557,30 @@ hipError_t launch_cross_v_t(hipStream_t s, const apg_hip_params& p, int w, int h
         const int Lmax = p.max_arm_length > 1 ? p.max_arm_length : 1;
         if (u8 && column_ranges && p.min_disparity >= 0 && D % 2 == 0 && (size_t)w * h * D < (1ull << 31)) {
             const int dch = (D + 127) / 128;
+            // D <= 64: two columns per wave (a half-wave each)
+            const bool two = D <= 64;
+            const int nxw = two ? (w + 1) / 2 : w;   // waves across a row
             // 1280x720 D=128, arm 17: 8 segments of 90 rows take 77 us, 4 of 180 rows 83 us,
             // 2 of 360 rows 117 us, 12 of 60 rows 83 us
             const int target = tn.cross_v_waves > 0 ? tn.cross_v_waves : 10240;
-            int nseg = (target + w * dch - 1) / (w * dch);
+            int nseg = (target + nxw * dch - 1) / (nxw * dch);
             nseg = max(1, min(nseg, (h + 63) / 64));
             const int seg = (h + nseg - 1) / nseg;
             nseg = (h + seg - 1) / seg;
-            const size_t shm = (size_t)(2 * Lmax + 2) * 64 * sizeof(uint32_t) + (size_t)(seg + 2 * Lmax) * 16;
+            const size_t shm =
+                (size_t)(2 * Lmax + 2) * 64 * sizeof(uint32_t) + (size_t)(two ? 2 : 1) * (seg + 2 * Lmax) * 16;
+            const dim3 grid(nxw, nseg, dch);
             if (sink)
-                return sink->add<CrossV3Fn>(dim3(w, nseg, dch), shm, w, h, p.min_disparity, D, Lmax, seg, dmin, dmax,
-                                            arms, (const uint8_t*)tmp, (uint8_t*)cost);
-            hipLaunchKernelGGL(cross_v3_kernel, dim3(w, nseg, dch), dim3(64), shm, s, w, h, p.min_disparity, D, Lmax,
-                               seg, dmin, dmax, arms, (const uint8_t*)tmp, (uint8_t*)cost);
+                return two ? sink->add<CrossV3Fn<true>>(grid, shm, w, h, p.min_disparity, D, Lmax, seg, dmin, dmax,
+                                                         arms, (const uint8_t*)tmp, (uint8_t*)cost)
+                           : sink->add<CrossV3Fn<false>>(grid, shm, w, h, p.min_disparity, D, Lmax, seg, dmin, dmax,
+                                                          arms, (const uint8_t*)tmp, (uint8_t*)cost);
+            if (two)
+                hipLaunchKernelGGL(cross_v3_kernel<true>, grid, dim3(64), shm, s, w, h, p.min_disparity, D, Lmax, seg,
+                                   dmin, dmax, arms, (const uint8_t*)tmp, (uint8_t*)cost);
+            else
+                hipLaunchKernelGGL(cross_v3_kernel<false>, grid, dim3(64), shm, s, w, h, p.min_disparity, D, Lmax,
+                                   seg, dmin, dmax, arms, (const uint8_t*)tmp, (uint8_t*)cost);
             return hipGetLastError();
         }
     }
