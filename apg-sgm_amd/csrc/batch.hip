@@ -640,6 +640,7 @@ int time_shapes(apg_hip_ctx ctx, const apg_hip_pair& q, const apg_hip_tuning& t,
         if (r == 0 || x.cost_ms < m.cost_ms) m.cost_ms = x.cost_ms;
         if (r == 0 || x.cross_ms < m.cross_ms) m.cross_ms = x.cross_ms;
         if (r == 0 || x.sgm_ms < m.sgm_ms) m.sgm_ms = x.sgm_ms;
+        if (r == 0 || x.wta_ms < m.wta_ms) m.wta_ms = x.wta_ms;
         if (r == 0 || x.total_ms < m.total_ms) m.total_ms = x.total_ms;
     }
     apg_hip_enable_stage_timing(ctx, 0);
@@ -647,9 +648,12 @@ int time_shapes(apg_hip_ctx ctx, const apg_hip_pair& q, const apg_hip_tuning& t,
     return 0;
 }
 
-// Coordinate search: each knob changes the launch of one stage only, so each
-// is chosen from that stage's own time with the other knobs at their
-// defaults.  A candidate must beat the built-in shape by 2% to replace it.
+// Coordinate search: each knob is chosen from the time of the stages it
+// changes, with the other knobs at their defaults: the cost segment from the
+// cost pass, the Cross-V wave count from Cross-V, the SGM wave shape from the
+// SGM plus the WTA (shapes that cannot pair a scanline family's planes leave
+// the WTA more planes to read).  A candidate must beat the built-in shape by
+// 2% to replace it.
 int tune_pair(apg_hip_ctx ctx, const apg_hip_pair& q, apg_hip_tuning* out) {
     const int reps = 3;
     apg_hip_timing base{}, t{};
@@ -664,9 +668,10 @@ int tune_pair(apg_hip_ctx ctx, const apg_hip_pair& q, apg_hip_tuning* out) {
         if (int s = time_shapes(ctx, q, apg_hip_tuning{0, 0, cv}, reps, &t)) return s;
         if (t.cross_ms < 0.98 * bv) bv = t.cross_ms, r.cross_v_waves = cv;
     }
+    bs += base.wta_ms;
     for (int sd : {1, 2, 4}) {
         if (int s = time_shapes(ctx, q, apg_hip_tuning{0, sd, 0}, reps, &t)) return s;
-        if (t.sgm_ms < 0.98 * bs) bs = t.sgm_ms, r.sgm_dual = sd;
+        if (t.sgm_ms + t.wta_ms < 0.98 * bs) bs = t.sgm_ms + t.wta_ms, r.sgm_dual = sd;
     }
     *out = r;
     return 0;
