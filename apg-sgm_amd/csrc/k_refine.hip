@@ -289,10 +289,11 @@ __global__ __launch_bounds__(kSweepThreads) void refine_sweep_kernel(const CostL
 //    the sweep never rounds.
 //  * A band's first row takes its upper neighbours from the band above's last
 //    row through `edge`: lane 63 stores each result, tagged (bit 31 set), with
-//    an agent-scope relaxed atomic store; the band below loads it, also
-//    relaxed and agent-scope, PD steps ahead of its use, and re-polls only if
-//    the tag is not there yet.  Each value is its own flag, so no fence is
-//    needed.  The wait is bounded (bs.timeout ticks of the wall clock from the
+//    an agent-scope relaxed atomic store at the index of its step; the band
+//    below loads the values of a block of PD columns with one relaxed,
+//    agent-scope load a block ahead of its use, and re-polls only if a column
+//    inside the row does not carry the tag yet.  Each value is its own flag,
+//    so no fence is needed.  The wait is bounded (bs.timeout ticks of the wall clock from the
 //    kernel's start and an iteration cap per wait); a band that gives up sets
 //    *bs.err, takes no neighbour from above for the rest of the sweep and
 //    runs to its end, so every wave exits.  A negative timeout makes every
@@ -308,10 +309,14 @@ __global__ __launch_bounds__(kSweepThreads) void refine_sweep_kernel(const CostL
 //    terms; no candidate reads them (their cost is the out-of-range 1e6).
 //    Lanes before the start of their row write the columns their window will
 //    hold anyway (RS >= 64 keeps those writes off the other columns' slots).
+//  * Records, results and the entering columns are read and written through
+//    buffer resources, the per-step offsets scalar; the host keeps every
+//    offset below 2^31 bytes.
 constexpr int kBandRows = 64;
 constexpr int kBandPD = 8;          // steps of prefetch (records, entering columns, edge values)
 constexpr int kBandSlack = 96;      // local steps past w in a band's slots: the skew (63) and two prefetch spans
 constexpr int kBandPadCols = 2048;  // right-view terms padding at both ends
+constexpr int kBandLdsBytes = kBandRows * 256 * 8 + (65 + 256 + 511) * 4;   // rings of D <= 256, LUTs
 
 __host__ __device__ constexpr int band_steps(int w) { return w + kBandSlack; }
 
@@ -403,10 +408,25 @@ __global__ void refine_band_unpack_kernel(int w, int h, int fwd, int kb, const f
 }
 
 struct BandSync {
-    uint32_t* edge;     // [bands + 1][kb]: row b + 1 the tagged last-row values of band b (zeroed before the sweep)
+    uint32_t* edge;     // [bands + 1][kb]: row b + 1 the tagged results of band b's last row, at its step index (zeroed before the sweep)
     uint32_t* err;      // set when a wait gave up (sticky: cleared by the host after reading it)
     long long timeout;  // wall-clock ticks the waits may take; < 0: give up at once
 };
+
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+
+// the median of three (v_med3_i32): min(max(v, lo), hi) when lo <= hi
+__device__ __forceinline__ int med3_i32(int v, int lo, int hi) {
+    int r;
+    asm("v_med3_i32 %0, %1, %2, %3" : "=v"(r) : "v"(v), "v"(lo), "v"(hi));
+    return r;
+}
+// c - 8 a (v_mad_i32_i24) for |a| < 2^23
+__device__ __forceinline__ int sub8x_i24(int c, int a) {
+    int r;
+    asm("v_mad_i32_i24 %0, %1, -8, %2" : "=v"(r) : "v"(a), "v"(c));
+    return r;
+}
 
 template <int PD, bool AD, bool GRAD>
 __global__ __launch_bounds__(kBandRows) void refine_band_kernel(const CostLut* __restrict__ lut, int w, int h,
@@ -415,14 +435,17 @@ __global__ __launch_bounds__(kBandRows) void refine_band_kernel(const CostLut* _
                                                                  const uint4* __restrict__ rb,
                                                                  const uint2* __restrict__ rt,
                                                                  float* __restrict__ out, BandSync bs) {
-    extern __shared__ __align__(16) unsigned char shb[];
-    float* s_lc = (float*)shb;
-    float* s_la = s_lc + 65;
-    float* s_lg = s_la + 256;
-    const int RS = 1 << rs_log2;
+    // LDS (static, so its base is a constant): row l's window ring at byte
+    // l * RS * 8, so a column's slot is one AND-OR of its byte offset, then
+    // the census, AD and gradient LUTs
+    __shared__ __align__(16) unsigned char shb[kBandLdsBytes];
     const int lane = threadIdx.x, b = blockIdx.x;
-    const int nb = gridDim.x;
-    uint2* wrow = (uint2*)(shb + 3328) + lane * RS;   // the lane's window ring
+    const uint32_t rowbase = (uint32_t)lane << (rs_log2 + 3);
+    const uint32_t mask8 = (8u << rs_log2) - 1u;
+    const uint32_t lut_c = (uint32_t)kBandRows << (rs_log2 + 3);
+    const uint32_t lut_a = lut_c + 65 * 4, lut_g = lut_a + 256 * 4;
+    auto ring = [&](int col8) { return (u32x2*)(shb + (((uint32_t)col8 & mask8) | rowbase)); };
+    auto lutf = [&](uint32_t base, uint32_t i) { return *(const float*)(shb + base + 4 * i); };
     {
         constexpr int NLUT = 65 + 256 + 511, NL = (NLUT + 63) / 64;
         float lv[NL];
@@ -430,15 +453,14 @@ __global__ __launch_bounds__(kBandRows) void refine_band_kernel(const CostLut* _
         for (int k = 0; k < NL; ++k) lv[k] = lut->census[min(lane + 64 * k, NLUT - 1)];
 #pragma unroll
         for (int k = 0; k < NL; ++k)
-            if (lane + 64 * k < NLUT) s_lc[lane + 64 * k] = lv[k];
+            if (lane + 64 * k < NLUT) *(float*)(shb + lut_c + 4 * (lane + 64 * k)) = lv[k];
     }
     const int ys = b * kBandRows + lane;
     const int yc = min(ys, h - 1);
     const int y = fwd ? yc : h - 1 - yc;
-    // right-view terms of column c of the lane's row: rtp[c] (c in [-kBandPadCols, w + kBandPadCols))
-    const uint2* rtp = rt + kBandPadCols + (size_t)y * w;
     // the window of the row's first pixel: columns [x0 - d0 - D + 1, x0 - d0]
     {
+        const uint2* rtp = rt + kBandPadCols + (size_t)y * w;
         const int x0 = fwd ? 0 : w - 1;
         const int c0 = x0 - d0 - D + 1;
         for (int j0 = 0; j0 < D; j0 += 32) {
@@ -447,73 +469,91 @@ __global__ __launch_bounds__(kBandRows) void refine_band_kernel(const CostLut* _
             for (int j = 0; j < 32; ++j) t[j] = rtp[c0 + min(j0 + j, D - 1)];
 #pragma unroll
             for (int j = 0; j < 32; ++j)
-                if (j0 + j < D) wrow[(c0 + j0 + j) & (RS - 1)] = t[j];
+                if (j0 + j < D) *ring((c0 + j0 + j) * 8) = u32x2{t[j].x, t[j].y};
         }
     }
-    // entering column at step k: fwd x - d0 with x = k - lane, backward
-    // x - d0 - D + 1 with x = w - 1 - (k - lane); its offset in rt is
-    // col_lane + k (fwd) or col_lane - k (backward)
-    const uint32_t col_lane = (uint32_t)(kBandPadCols + y * w + (fwd ? -lane - d0 : w - d0 - D + lane));
-    const uint32_t col_b = col_lane * 8u;   // the padded terms stay below 4 GB
-    const size_t band0 = (size_t)b * kb;
-    const float4* ra_b = ra + (band0 << 6);
-    const uint4* rb_b = rb + (band0 << 6);
-    float* out_b = out + (band0 << 6);
-    const uint32_t* edge_in = bs.edge + (size_t)b * kb;         // written by band b - 1
-    uint32_t* edge_out = bs.edge + (size_t)(b + 1) * kb;        // read by band b + 1
-    float4 pa[PD];
-    uint4 pb[PD];
-    uint2 pc[PD];
-    uint32_t pe[PD];
+    // The band's records, results and the right-view terms through buffer
+    // resources: per-step offsets are scalar.  The entering column at step k
+    // is x - d0 (fwd, x = k - lane) or x - d0 - D + 1 (backward,
+    // x = w - 1 - (k - lane)); its byte offset in rt is tvoff + 8k (fwd) or
+    // tvoff + 8 (kend - k) (backward), kend past the last step loaded.
+    const size_t slot0 = ((size_t)b * kb) << 6;
+    const __amdgpu_buffer_rsrc_t rsa = raw_rsrc(ra + slot0), rsb = raw_rsrc(rb + slot0);
+    const __amdgpu_buffer_rsrc_t rso = raw_rsrc(out + slot0), rst = raw_rsrc(rt);
+    const int kend = krun + PD;
+    const uint32_t col = (uint32_t)(kBandPadCols + y * w + (fwd ? -lane - d0 : w - d0 - D + lane));
+    const uint32_t tvoff = fwd ? col * 8u : (col - (uint32_t)kend) * 8u;
+    const uint32_t rvoff = (uint32_t)lane * 16u;
+    u32x4 pa[PD], pb[PD];
+    u32x2 pc[PD];
     auto issue = [&](int k, int q) {
-        pa[q] = ra_b[((size_t)k << 6) + lane];
-        pb[q] = rb_b[((size_t)k << 6) + lane];
-        pc[q] = *(const uint2*)((const char*)(fwd ? rt + k : rt - k) + col_b);
-        pe[q] = __hip_atomic_load(edge_in + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        pa[q] = buf_ld128(rsa, rvoff, (uint32_t)k * 1024u);
+        pb[q] = buf_ld128(rsb, rvoff, (uint32_t)k * 1024u);
+        pc[q] = __builtin_amdgcn_raw_buffer_load_b64(rst, (int)tvoff, (fwd ? k : kend - k) * 8, 0);
     };
+    // Upper neighbours of lane 0: the band above's lane 63 stores its result
+    // of step k at edge row b, index k, i.e. column c at index c + 63.  Lane j
+    // of one load holds column k0 + j of a block of PD steps, loaded a block
+    // ahead.
+    const uint32_t* edge_in = bs.edge + (size_t)b * kb + (kBandRows - 1);
+    uint32_t* edge_out = bs.edge + (size_t)(b + 1) * kb;
+    const int el = min(lane, PD - 1);
+    // lane 63's store of step k's result: the same instruction as an
+    // agent-scope relaxed atomic store (sc1); the other lanes' offsets lie
+    // past the resource's records, so the hardware drops their stores and no
+    // lane branches
+    const __amdgpu_buffer_rsrc_t rse = __builtin_amdgcn_make_buffer_rsrc(edge_out, 0, kb * 4, 0x00020000);
+    const uint32_t evoff = lane == kBandRows - 1 ? 0u : 0x80000000u;
+    auto edge_load = [&](int k0) {
+        return __hip_atomic_load(edge_in + k0 + el, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    };
+    uint32_t pev = edge_load(0);
 #pragma unroll
     for (int j = 0; j < PD; ++j) issue(j, j);
     const long long t_start = (long long)wall_clock64();
     bool give_up = bs.timeout < 0 && b > 0;
     int ri_prev = -1;                                  // the lane's result of the previous step, rounded
-    int x = fwd ? -lane : w - 1 + lane;               // the lane's image column at step 0
-    const int xdir = fwd ? 1 : -1;
-    const int ent = fwd ? -d0 : -d0 - D + 1;          // entering column - x
+    int x8 = (fwd ? -lane : w - 1 + lane) * 8;        // 8 x, x the lane's image column at the step
+    const int xd8 = fwd ? 8 : -8;
+    const int ent8 = (fwd ? -d0 : -d0 - D + 1) * 8;   // 8 (entering column - x)
+    const uint32_t w8 = (uint32_t)w * 8u;
     for (int k0 = 0; k0 < krun; k0 += PD) {
+        uint32_t ev = pev;
+        pev = edge_load(k0 + PD);
+        int dec = -1;
+        if (b > 0 && !give_up) {
+            // the block's columns inside the row must carry the tag (bit 31)
+            const bool need = lane < PD && k0 + lane < w;
+            if (__ballot(need && !(ev >> 31))) {
+                for (int it = 0; it < (1 << 22); ++it) {
+                    ev = __hip_atomic_load(edge_in + k0 + el, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if (!__ballot(need && !(ev >> 31)) || (long long)wall_clock64() - t_start > bs.timeout) break;
+                    __builtin_amdgcn_s_sleep(1);
+                }
+                if (__ballot(need && !(ev >> 31))) give_up = true;
+            }
+            if (!give_up) dec = (int)(ev & 0x7FFFFFFFu) - 1;
+        }
         static_for_b<PD>([&](auto j_t) {
             constexpr int q = decltype(j_t)::value;
             const int k = k0 + q;
-            const float4 A = pa[q];
-            const uint4 B = pb[q];
-            const uint2 C = pc[q];
-            // the window takes the entering column
-            wrow[(x + ent) & (RS - 1)] = C;
-            // lane 0's upper neighbour: column k of the band above's last row
-            int up0 = -1;
-            if (b > 0 && k < w && !give_up) {
-                uint32_t e = (uint32_t)__builtin_amdgcn_readfirstlane((int)pe[q]);
-                if (!(e >> 31)) {
-                    for (int it = 0; it < (1 << 22); ++it) {
-                        e = (uint32_t)__builtin_amdgcn_readfirstlane((int)__hip_atomic_load(
-                            edge_in + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-                        if ((e >> 31) || (long long)wall_clock64() - t_start > bs.timeout) break;
-                        __builtin_amdgcn_s_sleep(1);
-                    }
-                    if (!(e >> 31)) give_up = true;
-                }
-                if (e >> 31) up0 = (int)(e & 0x7FFFFFFFu) - 1;
-            }
-            const int up = __builtin_amdgcn_update_dpp(up0, ri_prev, kDppWaveShr1, 0xF, 0xF, false);
-            const float sc = A.x, sd = A.y;
-            const int sdi = __float_as_int(A.z), lo = __float_as_int(A.w), hm1 = (int)B.x;
+            const u32x4 A = pa[q], B = pb[q];
+            *ring(x8 + ent8) = pc[q];   // the window takes the entering column
+            const int up = __builtin_amdgcn_update_dpp(__builtin_amdgcn_readlane(dec, q), ri_prev, kDppWaveShr1,
+                                                       0xF, 0xF, false);
+            const float sc = __uint_as_float(A.x), sd = __uint_as_float(A.y);
+            const int sdi = (int)A.z, lo = (int)A.w, hm1 = (int)B.x;
+            // a neighbour's value clamped to the pixel's range (for swept
+            // pixels lo <= hm1; the others keep their value whatever the
+            // candidates cost)
             auto cand = [&](int nbv, float& c, int& nd) {
-                nd = min(max(nbv, lo), hm1);
-                const int xr = x - nd;
-                const uint2 t = wrow[xr & (RS - 1)];
-                float v = s_lc[__popc(B.y ^ t.x)];
-                if (AD) v += s_la[__builtin_amdgcn_sad_u8(B.z, t.y & 0xFFu, 0u)];
-                if (GRAD) v += s_lg[__builtin_amdgcn_sad_u8(B.w, t.y & 0xFFFF00u, 0u)];
-                v = (uint32_t)xr < (uint32_t)w ? v : 1e6f;
+                nd = med3_i32(nbv, lo, hm1);
+                const int xr8 = sub8x_i24(x8, nd);   // |nd| <= 2^15 (16-bit ranges)
+                const u32x2 t = *ring(xr8);
+                float v = lutf(lut_c, __popc(B.y ^ t.x));
+                if (AD) v += lutf(lut_a, __builtin_amdgcn_sad_u8(B.z, t.y & 0xFFu, 0u));
+                if (GRAD) v += lutf(lut_g, __builtin_amdgcn_sad_u8(B.w, t.y & 0xFFFF00u, 0u));
+                v = (uint32_t)xr8 < w8 ? v : 1e6f;
                 c = nbv >= 0 ? v : __builtin_inff();
             };
             float cl, cu;
@@ -529,11 +569,9 @@ __global__ __launch_bounds__(kBandRows) void refine_band_kernel(const CostLut* _
             ri = wu ? nu : ri;
             const float res = (wl || wu) ? (float)ri : sd;
             ri_prev = ri;
-            out_b[((size_t)k << 6) + lane] = res;
-            if (lane == kBandRows - 1 && b + 1 < nb && k >= kBandRows - 1 && k - (kBandRows - 1) < w)
-                __hip_atomic_store(edge_out + (k - (kBandRows - 1)), (uint32_t)(ri + 1) | 0x80000000u,
-                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            x += xdir;
+            buf_st32(rso, __float_as_uint(res), (uint32_t)lane * 4u, (uint32_t)k * 256u);
+            __builtin_amdgcn_raw_buffer_store_b32((uint32_t)(ri + 1) | 0x80000000u, rse, (int)evoff, k * 4, 16);
+            x8 += xd8;
             issue(k + PD, q);
         });
     }
@@ -619,16 +657,12 @@ hipError_t launch_refine(hipStream_t st, const apg_hip_params& p, const CostLut*
     const int D = p.max_disparity - p.min_disparity, d0 = p.min_disparity;
     int rs_log2 = 6;
     while ((1 << rs_log2) < D) ++rs_log2;
-    const size_t bshm = 3328 + (size_t)kBandRows * ((size_t)1 << rs_log2) * 8;
     const bool banded = bd.enable && status && sym && D > 0 && D <= 256 &&
                         std::abs(d0) + D + 2 * kBandSlack <= kBandPadCols &&
-                        n + 2 * (size_t)kBandPadCols < ((size_t)1 << 29) && nbands <= bd.max_bands;
+                        (n + 2 * (size_t)kBandPadCols) * 8 < ((size_t)1 << 31) &&
+                        (size_t)(kb + kBandPD) * 1024 < ((size_t)1 << 31) && nbands <= bd.max_bands;
     const BandKernel<kBandPD> bk = band_kernel<kBandPD>(p.use_ad, p.use_grad);
     if (banded) {
-        if (bshm > 64 * 1024 &&
-            (e = hipFuncSetAttribute((const void*)bk, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bshm)) !=
-                hipSuccess)
-            return e;
         hipLaunchKernelGGL(refine_band_terms_kernel, dim3(gb), dim3(256), 0, st, (int)n, rrec, rt);
     }
     if (banded_out) *banded_out = banded;
@@ -651,7 +685,7 @@ hipError_t launch_refine(hipStream_t st, const apg_hip_params& p, const CostLut*
                                (uint32_t)(eng_len - 1), base, acc, didx, slots, bra, brb);
             if ((e = launch_fill_u32(st, (size_t)(nbands + 1) * kb, 0u, edge)) != hipSuccess) return e;
             const BandSync bs{edge, status, bd.wait_ticks};
-            hipLaunchKernelGGL(bk, dim3(nbands), dim3(kBandRows), bshm, st, d_lut, w, h, fwd, d0, D, rs_log2, kb, krun,
+            hipLaunchKernelGGL(bk, dim3(nbands), dim3(kBandRows), 0, st, d_lut, w, h, fwd, d0, D, rs_log2, kb, krun,
                                bra, brb, rt, sres, bs);
             hipLaunchKernelGGL(refine_band_unpack_kernel, dim3(gb), dim3(256), 0, st, w, h, fwd, kb, sres, disp);
         } else {
