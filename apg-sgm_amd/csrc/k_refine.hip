@@ -320,24 +320,21 @@ constexpr int kBandLdsBytes = kBandRows * 256 * 8 + (65 + 256 + 511) * 4;   // r
 
 __host__ __device__ constexpr int band_steps(int w) { return w + kBandSlack; }
 
-// slot records: A = {sc, sd, sdi, lo}, B = {hi - 1, census, gray, gx << 8 | gy << 16}
-__global__ void refine_band_static_kernel(const CostLut* __restrict__ lut, int w, int h, int fwd, int r, int kb,
-                                          int use_ad, int use_grad, const uint4* __restrict__ lrec,
-                                          const uint4* __restrict__ rrec, const uint8_t* __restrict__ reliable,
-                                          const int16_t* __restrict__ dmin, const int16_t* __restrict__ dmax,
-                                          const float* __restrict__ disp, const uint32_t* __restrict__ eng,
-                                          uint32_t eng_last, const uint32_t* __restrict__ base,
-                                          const uint32_t* __restrict__ acc, const uint32_t* __restrict__ didx,
-                                          size_t nslots, float4* __restrict__ ra, uint4* __restrict__ rb) {
-    const size_t sl = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (sl >= nslots) return;
-    const int l = (int)(sl & 63);
-    const size_t bk = sl >> 6;
-    const int k = (int)(bk % (size_t)kb), b = (int)(bk / (size_t)kb);
-    const int xs = k - l, ys = b * kBandRows + l;
+// The record of band slot (xs, ys): A = {sc, sd, sdi, lo}, B = {hi - 1,
+// census, gray, gx << 8 | gy << 16}.
+__device__ __forceinline__ void band_static_record(const CostLut* __restrict__ lut, int w, int h, int fwd, int r,
+                                                   int use_ad, int use_grad, const uint4* __restrict__ lrec,
+                                                   const uint4* __restrict__ rrec,
+                                                   const uint8_t* __restrict__ reliable,
+                                                   const int16_t* __restrict__ dmin, const int16_t* __restrict__ dmax,
+                                                   const float* __restrict__ disp, const uint32_t* __restrict__ eng,
+                                                   uint32_t eng_last, const uint32_t* __restrict__ base,
+                                                   const uint32_t* __restrict__ acc,
+                                                   const uint32_t* __restrict__ didx, int xs, int ys, float4& ra,
+                                                   uint4& rb) {
     if (xs < 0 || xs >= w || ys >= h) {
-        ra[sl] = make_float4(-1.f, -1.f, __int_as_float(-1), __int_as_float(0));
-        rb[sl] = make_uint4(0u, 0u, 0u, 0u);
+        ra = make_float4(-1.f, -1.f, __int_as_float(-1), __int_as_float(0));
+        rb = make_uint4(0u, 0u, 0u, 0u);
         return;
     }
     const int x = fwd ? xs : w - 1 - xs, y = fwd ? ys : h - 1 - ys;
@@ -384,8 +381,43 @@ __global__ void refine_band_static_kernel(const CostLut* __restrict__ lut, int w
         sd = (float)bd;
         sdi = bd;
     }
-    ra[sl] = make_float4(sc, sd, __int_as_float(sdi), __int_as_float(lo));
-    rb[sl] = make_uint4((uint32_t)(hi - 1), lt.x, lt.z & 0xFFu, lt.z & 0xFFFF00u);
+    ra = make_float4(sc, sd, __int_as_float(sdi), __int_as_float(lo));
+    rb = make_uint4((uint32_t)(hi - 1), lt.x, lt.z & 0xFFu, lt.z & 0xFFFF00u);
+}
+
+// Slot records of one band (blockIdx.y), kStaticDiag consecutive steps
+// (blockIdx.x) of 256 / kStaticDiag of its rows (blockIdx.z), one slot per
+// thread: a row's pixels on consecutive lanes, so the per-pixel reads and the
+// draw-index -> accepted-offset -> engine chain are coalesced; the records
+// are written to their slots directly (staging them in LDS to write whole
+// steps measured slower: the occupancy it costs hides less of the chain's
+// latency).
+constexpr int kStaticDiag = 32;
+__global__ __launch_bounds__(256) void refine_band_static_kernel(const CostLut* __restrict__ lut, int w, int h,
+                                                                  int fwd, int r, int kb, int use_ad, int use_grad,
+                                                                  const uint4* __restrict__ lrec,
+                                                                  const uint4* __restrict__ rrec,
+                                                                  const uint8_t* __restrict__ reliable,
+                                                                  const int16_t* __restrict__ dmin,
+                                                                  const int16_t* __restrict__ dmax,
+                                                                  const float* __restrict__ disp,
+                                                                  const uint32_t* __restrict__ eng, uint32_t eng_last,
+                                                                  const uint32_t* __restrict__ base,
+                                                                  const uint32_t* __restrict__ acc,
+                                                                  const uint32_t* __restrict__ didx,
+                                                                  float4* __restrict__ ra, uint4* __restrict__ rb) {
+    const int b = blockIdx.y, k0 = blockIdx.x * kStaticDiag;
+    const int t = threadIdx.x;
+    const int j = t % kStaticDiag, l = t / kStaticDiag + (256 / kStaticDiag) * blockIdx.z;
+    float4 A;
+    uint4 B;
+    band_static_record(lut, w, h, fwd, r, use_ad, use_grad, lrec, rrec, reliable, dmin, dmax, disp, eng,
+                       eng_last, base, acc, didx, k0 + j - l, b * kBandRows + l, A, B);
+    if (k0 + j < kb) {
+        const size_t sl = ((size_t)b * kb + k0 + j) * kBandRows + l;
+        ra[sl] = A;
+        rb[sl] = B;
+    }
 }
 
 // the right view's terms (census, gray | gx << 8 | gy << 16) at [kBandPadCols + q]
@@ -668,7 +700,6 @@ hipError_t launch_refine(hipStream_t st, const apg_hip_params& p, const CostLut*
     if (banded_out) *banded_out = banded;
     // a band runs whole prefetch spans: the last lane's last column is step w + 62
     const int krun = (w + kBandRows - 1 + kBandPD - 1) / kBandPD * kBandPD;
-    const unsigned gs = (unsigned)((slots + 255) / 256);
     for (int it = 0; it < iters; ++it) {
         const int fwd = (it % 2) == 0 ? 1 : 0;
         const int r = std::max(1, p.refine_random_radius >> it);
@@ -680,9 +711,9 @@ hipError_t launch_refine(hipStream_t st, const apg_hip_params& p, const CostLut*
         if ((e = hipcub::DeviceScan::ExclusiveSum(temp, temp_bytes, aflag, apos, (int)nw, st)) != hipSuccess) return e;
         hipLaunchKernelGGL(jitter_compact_kernel, dim3(gw), dim3(256), 0, st, aflag, apos, nw, acc);
         if (banded) {
-            hipLaunchKernelGGL(refine_band_static_kernel, dim3(gs), dim3(256), 0, st, d_lut, w, h, fwd, r, kb,
-                               p.use_ad, p.use_grad, lrec, rrec, reliable, dmin, dmax, disp, eng,
-                               (uint32_t)(eng_len - 1), base, acc, didx, slots, bra, brb);
+            hipLaunchKernelGGL(refine_band_static_kernel, dim3((kb + kStaticDiag - 1) / kStaticDiag, nbands, kBandRows * kStaticDiag / 256),
+                               dim3(256), 0, st, d_lut, w, h, fwd, r, kb, p.use_ad, p.use_grad, lrec, rrec, reliable,
+                               dmin, dmax, disp, eng, (uint32_t)(eng_len - 1), base, acc, didx, bra, brb);
             if ((e = launch_fill_u32(st, (size_t)(nbands + 1) * kb, 0u, edge)) != hipSuccess) return e;
             const BandSync bs{edge, status, bd.wait_ticks};
             hipLaunchKernelGGL(bk, dim3(nbands), dim3(kBandRows), 0, st, d_lut, w, h, fwd, d0, D, rs_log2, kb, krun,
