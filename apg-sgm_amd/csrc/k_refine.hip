@@ -53,35 +53,45 @@ __device__ __forceinline__ int sweep_index(bool fwd, int w, int h, int x, int y)
     return fwd ? y * w + x : (h - 1 - y) * w + (w - 1 - x);
 }
 
-// 1 where the pixel at sweep position j draws a jitter
-__global__ void refine_mask_kernel(int w, int h, int fwd, const uint8_t* __restrict__ reliable,
-                                   const int16_t* __restrict__ dmin, const int16_t* __restrict__ dmax,
-                                   const float* __restrict__ disp, uint32_t* __restrict__ flag) {
-    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= (size_t)w * h) return;
-    const int x = (int)(q % w), y = (int)(q / w);
-    flag[sweep_index(fwd, w, h, x, y)] = (!reliable[q] && dmax[q] > dmin[q] && disp[q] >= 0.f) ? 1u : 0u;
-}
-
-// acceptance of engine outputs base .. base + n - 1 for the range s (t = 2^32 mod s)
-__global__ void jitter_accept_kernel(const uint32_t* __restrict__ eng, const uint32_t* __restrict__ base, size_t n,
-                                     uint32_t s, uint32_t t, uint32_t* __restrict__ flag) {
+// Per sweep position i < nw, one pair of flags: x = 1 where the pixel at
+// sweep position i (i < n) draws a jitter, y = 1 where engine output
+// base + i is accepted for the range s (t = 2^32 mod s).  One exclusive scan
+// of the pairs gives each pixel's draw index and each accepted output's slot.
+__global__ void refine_flags_kernel(int w, int h, int fwd, const uint8_t* __restrict__ reliable,
+                                    const int16_t* __restrict__ dmin, const int16_t* __restrict__ dmax,
+                                    const float* __restrict__ disp, const uint32_t* __restrict__ eng,
+                                    const uint32_t* __restrict__ base, size_t n, size_t nw, uint32_t s, uint32_t t,
+                                    uint2* __restrict__ flag) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    flag[i] = (uint32_t)((uint64_t)eng[*base + i] * s) >= t ? 1u : 0u;
+    if (i >= nw) return;
+    uint32_t d = 0u;
+    if (i < n) {
+        const int ys = (int)(i / (size_t)w), xs = (int)(i - (size_t)ys * w);
+        const int x = fwd ? xs : w - 1 - xs, y = fwd ? ys : h - 1 - ys;
+        const size_t q = (size_t)y * w + x;
+        d = (!reliable[q] && dmax[q] > dmin[q] && disp[q] >= 0.f) ? 1u : 0u;
+    }
+    const uint32_t a = (uint32_t)((uint64_t)eng[*base + i] * s) >= t ? 1u : 0u;
+    flag[i] = make_uint2(d, a);
 }
 
-// acc[i] -> slot pos[i] of the list of accepted offsets
-__global__ void jitter_compact_kernel(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ pos, size_t n,
+struct PairSum {
+    __host__ __device__ uint2 operator()(const uint2& a, const uint2& b) const {
+        return make_uint2(a.x + b.x, a.y + b.y);
+    }
+};
+
+// acc[i] -> slot pos[i].y of the list of accepted offsets
+__global__ void jitter_compact_kernel(const uint2* __restrict__ flag, const uint2* __restrict__ pos, size_t n,
                                       uint32_t* __restrict__ acc) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n && flag[i]) acc[pos[i]] = (uint32_t)i;
+    if (i < n && flag[i].y) acc[pos[i].y] = (uint32_t)i;
 }
 
 // after a sweep: the engine outputs it consumed end at its last accepted one
-__global__ void jitter_advance_kernel(uint32_t* __restrict__ base, const uint32_t* __restrict__ dflag,
-                                      const uint32_t* __restrict__ didx, size_t npx, const uint32_t* __restrict__ acc) {
-    const uint32_t ndraw = didx[npx - 1] + dflag[npx - 1];
+__global__ void jitter_advance_kernel(uint32_t* __restrict__ base, const uint2* __restrict__ flag,
+                                      const uint2* __restrict__ pos, size_t npx, const uint32_t* __restrict__ acc) {
+    const uint32_t ndraw = pos[npx - 1].x + flag[npx - 1].x;
     if (ndraw) *base += acc[ndraw - 1] + 1u;
 }
 
@@ -144,7 +154,7 @@ __global__ void refine_static_kernel(const CostLut* __restrict__ lut, int w, int
                                      const int16_t* __restrict__ dmin, const int16_t* __restrict__ dmax,
                                      const float* __restrict__ disp, const uint32_t* __restrict__ eng,
                                      uint32_t eng_last, const uint32_t* __restrict__ base,
-                                     const uint32_t* __restrict__ acc, const uint32_t* __restrict__ didx,
+                                     const uint32_t* __restrict__ acc, const uint2* __restrict__ dpos,
                                      SweepRec* __restrict__ out) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= w * h) return;
@@ -170,7 +180,7 @@ __global__ void refine_static_kernel(const CostLut* __restrict__ lut, int w, int
         o.st.y = (float)dc;
         o.has = kHasOwn;
         const uint32_t s = 2u * (uint32_t)r + 1u;
-        const uint32_t e = eng[min(*base + acc[didx[sweep_index(fwd, w, h, x, y)]], eng_last)];
+        const uint32_t e = eng[min(*base + acc[dpos[sweep_index(fwd, w, h, x, y)].x], eng_last)];
         const int jit = (int)(((uint64_t)e * s) >> 32) - r;
         const float dj = cur + (float)jit;
         if (!(dj < 0.f)) {
@@ -330,7 +340,7 @@ __device__ __forceinline__ void band_static_record(const CostLut* __restrict__ l
                                                    const float* __restrict__ disp, const uint32_t* __restrict__ eng,
                                                    uint32_t eng_last, const uint32_t* __restrict__ base,
                                                    const uint32_t* __restrict__ acc,
-                                                   const uint32_t* __restrict__ didx, int xs, int ys, float4& ra,
+                                                   const uint2* __restrict__ dpos, int xs, int ys, float4& ra,
                                                    uint4& rb) {
     if (xs < 0 || xs >= w || ys >= h) {
         ra = make_float4(-1.f, -1.f, __int_as_float(-1), __int_as_float(0));
@@ -362,7 +372,7 @@ __device__ __forceinline__ void band_static_record(const CostLut* __restrict__ l
             best = cost(dc);
             bd = dc;
             const uint32_t s = 2u * (uint32_t)r + 1u;
-            const uint32_t e = eng[min(*base + acc[didx[sweep_index(fwd, w, h, x, y)]], eng_last)];
+            const uint32_t e = eng[min(*base + acc[dpos[sweep_index(fwd, w, h, x, y)].x], eng_last)];
             const int jit = (int)(((uint64_t)e * s) >> 32) - r;
             const float dj = cur + (float)jit;
             if (!(dj < 0.f)) {
@@ -404,7 +414,7 @@ __global__ __launch_bounds__(256) void refine_band_static_kernel(const CostLut* 
                                                                   const uint32_t* __restrict__ eng, uint32_t eng_last,
                                                                   const uint32_t* __restrict__ base,
                                                                   const uint32_t* __restrict__ acc,
-                                                                  const uint32_t* __restrict__ didx,
+                                                                  const uint2* __restrict__ dpos,
                                                                   float4* __restrict__ ra, uint4* __restrict__ rb) {
     const int b = blockIdx.y, k0 = blockIdx.x * kStaticDiag;
     const int t = threadIdx.x;
@@ -412,7 +422,7 @@ __global__ __launch_bounds__(256) void refine_band_static_kernel(const CostLut* 
     float4 A;
     uint4 B;
     band_static_record(lut, w, h, fwd, r, use_ad, use_grad, lrec, rrec, reliable, dmin, dmax, disp, eng,
-                       eng_last, base, acc, didx, k0 + j - l, b * kBandRows + l, A, B);
+                       eng_last, base, acc, dpos, k0 + j - l, b * kBandRows + l, A, B);
     if (k0 + j < kb) {
         const size_t sl = ((size_t)b * kb + k0 + j) * kBandRows + l;
         ra[sl] = A;
@@ -615,15 +625,16 @@ __global__ __launch_bounds__(kBandRows) void refine_band_kernel(const CostLut* _
 size_t refine_scratch_bytes(int w, int h) {
     const size_t n = (size_t)w * h, nw = n + kRefineSlack;
     size_t temp = 0;
-    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, temp, (const uint32_t*)nullptr, (uint32_t*)nullptr, (int)nw);
+    (void)hipcub::DeviceScan::ExclusiveScan(nullptr, temp, (const uint2*)nullptr, (uint2*)nullptr, PairSum(),
+                                            make_uint2(0u, 0u), (int)nw);
     const size_t nbands = (size_t)(h + kBandRows - 1) / kBandRows;
     const size_t kb = (size_t)band_steps(w), slots = nbands * kb * kBandRows;
     auto r256 = [](size_t v) { return (v + 255) & ~(size_t)255; };
-    // flags, draw indices, sweep results (image or band-major), acceptance
-    // flags, their positions, accepted offsets, left and right terms, the
-    // sweep records (diagonal or band-major), padded right terms, engine base,
-    // edge values and the give-up flag, scan temp
-    return r256(2 * n * 4) + r256(std::max(n, slots) * 4) + r256(3 * nw * 4) + r256(2 * n * 16) +
+    // flag pairs and their exclusive sums (draw indices, accepted slots),
+    // sweep results (image or band-major), accepted offsets, left and right
+    // terms, the sweep records (diagonal or band-major), padded right terms,
+    // engine base, edge values and the give-up flag, scan temp
+    return r256(4 * nw * 4) + r256(std::max(n, slots) * 4) + r256(nw * 4) + r256(2 * n * 16) +
            r256(std::max(n * sizeof(SweepRec), slots * 32)) + r256((n + 2 * (size_t)kBandPadCols) * 8) + 256 +
            r256(((nbands + 1) * kb + 1) * 4) + r256(temp);
 }
@@ -653,14 +664,12 @@ hipError_t launch_refine(hipStream_t st, const apg_hip_params& p, const CostLut*
     const size_t slots = (size_t)nbands * kb * kBandRows;
     auto r256 = [](size_t v) { return (v + 255) & ~(size_t)255; };
     uint8_t* sp = (uint8_t*)scratch;
-    uint32_t* dflag = (uint32_t*)sp;
-    uint32_t* didx = dflag + n;
-    sp += r256(2 * n * 4);
+    uint2* flag = (uint2*)sp;   // (draws, accepted) per sweep position
+    uint2* pos = flag + nw;     // their exclusive sums: draw index, accepted slot
+    sp += r256(4 * nw * 4);
     float* sres = (float*)sp; sp += r256(std::max(n, slots) * 4);
-    uint32_t* aflag = (uint32_t*)sp;
-    uint32_t* apos = aflag + nw;
-    uint32_t* acc = apos + nw;
-    sp += r256(3 * nw * 4);
+    uint32_t* acc = (uint32_t*)sp;
+    sp += r256(nw * 4);
     uint4* lrec = (uint4*)sp;
     uint4* rrec = lrec + n;
     sp += r256(2 * n * 16);
@@ -705,15 +714,16 @@ hipError_t launch_refine(hipStream_t st, const apg_hip_params& p, const CostLut*
         const int r = std::max(1, p.refine_random_radius >> it);
         const uint32_t s = 2u * (uint32_t)r + 1u;
         const uint32_t t = (uint32_t)(0u - s) % s;   // 2^32 mod s
-        hipLaunchKernelGGL(refine_mask_kernel, dim3(gb), dim3(256), 0, st, w, h, fwd, reliable, dmin, dmax, disp, dflag);
-        if ((e = hipcub::DeviceScan::ExclusiveSum(temp, temp_bytes, dflag, didx, (int)n, st)) != hipSuccess) return e;
-        hipLaunchKernelGGL(jitter_accept_kernel, dim3(gw), dim3(256), 0, st, eng, base, nw, s, t, aflag);
-        if ((e = hipcub::DeviceScan::ExclusiveSum(temp, temp_bytes, aflag, apos, (int)nw, st)) != hipSuccess) return e;
-        hipLaunchKernelGGL(jitter_compact_kernel, dim3(gw), dim3(256), 0, st, aflag, apos, nw, acc);
+        hipLaunchKernelGGL(refine_flags_kernel, dim3(gw), dim3(256), 0, st, w, h, fwd, reliable, dmin, dmax, disp,
+                           eng, base, n, nw, s, t, flag);
+        if ((e = hipcub::DeviceScan::ExclusiveScan(temp, temp_bytes, flag, pos, PairSum(), make_uint2(0u, 0u), (int)nw,
+                                                   st)) != hipSuccess)
+            return e;
+        hipLaunchKernelGGL(jitter_compact_kernel, dim3(gw), dim3(256), 0, st, flag, pos, nw, acc);
         if (banded) {
             hipLaunchKernelGGL(refine_band_static_kernel, dim3((kb + kStaticDiag - 1) / kStaticDiag, nbands, kBandRows * kStaticDiag / 256),
                                dim3(256), 0, st, d_lut, w, h, fwd, r, kb, p.use_ad, p.use_grad, lrec, rrec, reliable,
-                               dmin, dmax, disp, eng, (uint32_t)(eng_len - 1), base, acc, didx, bra, brb);
+                               dmin, dmax, disp, eng, (uint32_t)(eng_len - 1), base, acc, pos, bra, brb);
             if ((e = launch_fill_u32(st, (size_t)(nbands + 1) * kb, 0u, edge)) != hipSuccess) return e;
             const BandSync bs{edge, status, bd.wait_ticks};
             hipLaunchKernelGGL(bk, dim3(nbands), dim3(kBandRows), 0, st, d_lut, w, h, fwd, d0, D, rs_log2, kb, krun,
@@ -722,12 +732,12 @@ hipError_t launch_refine(hipStream_t st, const apg_hip_params& p, const CostLut*
         } else {
             hipLaunchKernelGGL(refine_static_kernel, dim3(gb), dim3(256), 0, st, d_lut, w, h, fwd, r, sym, p.use_ad,
                                p.use_grad, lrec, rrec, reliable, dmin, dmax, disp, eng, (uint32_t)(eng_len - 1), base,
-                               acc, didx, srec);
+                               acc, pos, srec);
             hipLaunchKernelGGL(refine_sweep_kernel, dim3(1), dim3(kSweepThreads), shm, st, d_lut, w, h, fwd, sym,
                                p.use_ad, p.use_grad, srec, rrec, sres);
             hipLaunchKernelGGL(refine_unpack_kernel, dim3(gb), dim3(256), 0, st, w, h, fwd, sres, disp);
         }
-        hipLaunchKernelGGL(jitter_advance_kernel, dim3(1), dim3(1), 0, st, base, dflag, didx, n, acc);
+        hipLaunchKernelGGL(jitter_advance_kernel, dim3(1), dim3(1), 0, st, base, flag, pos, n, acc);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     return hipSuccess;
