@@ -330,6 +330,9 @@ __device__ __forceinline__ void cross_v3_body(const uint3 bid, int w, int h, int
                            ((inA && dA + 1 >= lo && dA + 1 < hi) ? 0u : 0xFF00u);
     // byte offset of the lane's pair from the wave's first column
     const uint32_t voff = inA ? (uint32_t)(di + half * D) : 0u;
+    // stores of lanes outside the volume go past the buffer's records, where
+    // the hardware drops them: no lane branches around a store
+    const uint32_t soff = inA ? voff : 0x80000000u;
     const uint32_t lofs = 4u * (uint32_t)lane;
     const auto rt = raw_rsrc(tmp);
     const auto rc = raw_rsrc(cost);
@@ -346,7 +349,7 @@ __device__ __forceinline__ void cross_v3_body(const uint3 bid, int w, int h, int
         const uint32_t s2 = as_u32(as_v2(s) + as_v2(s));
         const uint32_t qa = __umulhi(s2 & 0xFFFFu, m.w), qb = __umulhi(s2 >> 16, m.w);
         const uint32_t out = (qa | (qb << 8)) | omask;
-        if (inA) buf_st16(rc, out, voff, (uint32_t)(k - Lmax) * rowb + colb);
+        buf_st16(rc, out, soff, (uint32_t)(k - Lmax) * rowb + colb);
     };
     uint32_t X[CV3_PF], Y[CV3_PF];
     // a batch's rows past the image bottom reload its last row (their prefix
@@ -386,8 +389,7 @@ __device__ __forceinline__ void cross_v3_body(const uint3 bid, int w, int h, int
                     const uint32_t sd = as_u32(as_v2(p1) - as_v2(p0));
                     const uint32_t s2 = as_u32(as_v2(sd) + as_v2(sd));
                     const uint32_t qa = __umulhi(s2 & 0xFFFFu, m[i].w), qb = __umulhi(s2 >> 16, m[i].w);
-                    if (inA)
-                        buf_st16(rc, __builtin_amdgcn_perm(qb, qa, 0x0c0c0400u) | omask, voff,
+                    buf_st16(rc, __builtin_amdgcn_perm(qb, qa, 0x0c0c0400u) | omask, soff,
                                  (uint32_t)(kb + i - Lmax) * rowb + colb);
                 }
             }
@@ -474,6 +476,7 @@ __global__ __launch_bounds__(64) void cross_v4_kernel(int w, int h, int d0, int 
     const int dA = d0 + di;
     const bool inA = di < D;   // D even: the pair is inside or outside as a whole
     const uint32_t voff = inA ? (uint32_t)di : 0u;
+    const uint32_t soff = inA ? voff : 0x80000000u;   // past the records: the store is dropped
     const auto rt = raw_rsrc(tmp);
     const auto rc = raw_rsrc(cost);
     const uint32_t rowb = (uint32_t)w * (uint32_t)D, colb = (uint32_t)x * (uint32_t)D;
@@ -498,9 +501,13 @@ __global__ __launch_bounds__(64) void cross_v4_kernel(int w, int h, int d0, int 
         const uint32_t wc = (uint32_t)cring[s1 * 64 + lane] - (uint32_t)cring[s0 * 64 + lane];
         const int lo = (int16_t)(m.y & 0xFFFFu), hi = (int16_t)(m.y >> 16);
         const uint32_t ca = wc & 0xFFu, cb = (wc >> 8) & 0xFFu;
-        const uint32_t oa = (dA >= lo && dA < hi && ca) ? __umulhi(sw & 0xFFFFu, mag[ca]) : 0xFFu;
-        const uint32_t ob = (dA + 1 >= lo && dA + 1 < hi && cb) ? __umulhi(sw >> 16, mag[cb]) : 0xFFu;
-        if (inA) buf_st16(rc, __builtin_amdgcn_perm(ob, oa, 0x0c0c0400u), voff, (uint32_t)(k - Lmax) * rowb + colb);
+        // both magic numbers are read whatever the range test says (a count
+        // is at most the window, 2 Lmax + 1, and mag[0] = 0), so the test is a
+        // select and no lane branches
+        const uint32_t qa = __umulhi(sw & 0xFFFFu, mag[ca]), qb = __umulhi(sw >> 16, mag[cb]);
+        const uint32_t oa = (dA >= lo && dA < hi && ca) ? qa : 0xFFu;
+        const uint32_t ob = (dA + 1 >= lo && dA + 1 < hi && cb) ? qb : 0xFFu;
+        buf_st16(rc, __builtin_amdgcn_perm(ob, oa, 0x0c0c0400u), soff, (uint32_t)(k - Lmax) * rowb + colb);
     };
     uint32_t X[CV4_PF], Y[CV4_PF];
     // as in cross_v3_kernel: unclamped loads for whole batches inside the
