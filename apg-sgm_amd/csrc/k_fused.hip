@@ -498,7 +498,10 @@ __device__ __forceinline__ void cost_crossh2_body(const uint3 bid, const CostLut
                     const uint32_t cnt = (wc >> (8 * e)) & 0xFFu;
                     const uint32_t se = e ? (sw >> 16) : (sw & 0xFFFFu);
                     const bool v = d >= lo && d < hi && cnt > 0;
-                    qq[e] = v ? __umulhi(se, s_mag[cnt]) : 0xFFu;
+                    // the magic number is read on every lane (the index kept
+                    // in the table) and the test is a select: no lane branches
+                    const uint32_t qv = __umulhi(se, s_mag[min(cnt, (uint32_t)(2 * kMaxArm + 1))]);
+                    qq[e] = v ? qv : 0xFFu;
                 }
                 q0 = qq[0];
                 q1 = qq[1];
@@ -511,14 +514,16 @@ __device__ __forceinline__ void cost_crossh2_body(const uint3 bid, const CostLut
                     const int cnt = min(xhi, w - 1 + d) - max(xlo, d) + 1;
                     const uint32_t se = e ? (sw >> 16) : (sw & 0xFFFFu);
                     const bool v = d <= pxo && cnt > 0;   // d in [min, min(max, x+1)), window not empty
-                    qq[e] = v ? __umulhi(se, s_mag[cnt > 0 ? cnt : 0]) : 0xFFu;
+                    const uint32_t qv = __umulhi(se, s_mag[min(max(cnt, 0), 2 * kMaxArm + 1)]);
+                    qq[e] = v ? qv : 0xFFu;
                 }
                 q0 = qq[0];
                 q1 = qq[1];
             }
             // bytes q0, q1 (each < 256) -> one 16-bit store
-            if (act) buf_st16(rout, __builtin_amdgcn_perm(q1, q0, 0x0c0c0400u), (uint32_t)di0 + voff_row,
-                              (uint32_t)((int)roww + pxo) * (uint32_t)D);
+            // an inactive lane's offset lies past the records: the store is dropped
+            buf_st16(rout, __builtin_amdgcn_perm(q1, q0, 0x0c0c0400u), act ? (uint32_t)di0 + voff_row : 0x80000000u,
+                     (uint32_t)((int)roww + pxo) * (uint32_t)D);
         };
 
         // keys: own column (right-view WTA candidates); outp: an output column's
